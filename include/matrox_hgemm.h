@@ -1,0 +1,140 @@
+/*
+ * matrox_hgemm.h — C-ABI of the gfx950 HIP evaluator of Y = K~ W.
+ *
+ * Two layers:
+ *
+ * 1. Drop-in entry points with the exact signatures of the functions MatRox
+ *    generates (HGEMM for H2-b, symGen/HGEMM_gen.h:117-124; the HSS form,
+ *    symGen/HGEMM_v1.h:117-124).  All buffers are caller-owned HOST memory in
+ *    the reference's CDS layout; the call copies them to the GPU, evaluates in
+ *    FP64 and copies apres/wskel/uskel back, with the reference's
+ *    accumulation rules (H2-b: apres and uskel accumulate; HSS: apres and the
+ *    far part of uskel are overwritten).  The generated code bakes the loop
+ *    bounds nl, fl, cdepth and the leaf count into its body; here they are set
+ *    once by mx_hgemm_configure(), the equivalent of code generation.
+ *
+ * 2. A plan API for device-resident evaluation: mx_plan_create() uploads the
+ *    structure and blocks once (in every precision requested), mx_eval() runs
+ *    one evaluation on device buffers on a HIP stream.  W and Y use the
+ *    leaf-blocked layout of the reference's TW/apres (matroxTest-V1/hcds.h:
+ *    463-485): leaf l's rows start at row wrow[l], stored Dim_l x nrhs
+ *    column-major at element offset wrow[l]*nrhs.
+ *
+ * Every function returns an mx_status (0 = success).  mx_last_error() gives a
+ * message for the last failure on the calling thread.
+ */
+#ifndef MATROX_HGEMM_H
+#define MATROX_HGEMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum mx_status {
+    MX_OK = 0,
+    MX_ERR_ARG = -1,        /* null pointer, bad size, inconsistent arrays   */
+    MX_ERR_HIP = -2,        /* HIP runtime failure                           */
+    MX_ERR_STATE = -3,      /* call order (e.g. HGEMM before configure)      */
+    MX_ERR_UNSUPPORTED = -4 /* precision or layout not built into the plan   */
+} mx_status;
+
+/* Arithmetic of an evaluation. */
+typedef enum mx_precision {
+    MX_FP64 = 0,   /* FP64 MFMA, the drop-in path                              */
+    MX_FP32 = 1,   /* FP32 blocks and vectors, FP32 MFMA                        */
+    MX_BF16X3 = 2  /* near and far blocks as split-BF16 x3 products on MFMA
+                      with FP32 accumulation; tree transfers in FP32           */
+} mx_precision;
+
+/* Element type of W and Y in mx_eval. */
+typedef enum mx_io_type { MX_IO_F64 = 0, MX_IO_F32 = 1 } mx_io_type;
+
+/* ---- layer 1: drop-in entry points (host memory, FP64) ------------------- */
+
+/* Loop bounds the generated HGEMM bakes in: number of leaves, near
+ * row-blocks nl, far row-blocks fl and coarsened level groups cdepth
+ * (symGen/HGEMM_gen.h:126,142,180,197).  Process-wide; call before HGEMM. */
+mx_status mx_hgemm_configure(int32_t nleaf, int32_t nl, int32_t fl, int32_t cdepth);
+
+/* H2-b evaluation; replaces symGen/HGEMM_gen.h:117-238.  Argument meaning,
+ * layout and accumulation are the reference's; returns 0 on success or a
+ * negative mx_status (the reference always returns 0). */
+int32_t HGEMM(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
+              int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
+              double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
+              int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
+              int32_t *nblockSet, int32_t *nblocks, int32_t *npairx, int32_t *npairy, int32_t *fblockSet,
+              int32_t *fblocks, int32_t *fpairx, int32_t *fpairy, int32_t *wpart, int32_t *clevelset);
+
+/* HSS evaluation; replaces symGen/HGEMM_v1.h:117-405 (the reference names it
+ * HGEMM too and selects it with #define HSS; a build links this symbol under
+ * that name, see the integration notes).  Near: diagonal leaf blocks Dptr[i];
+ * far: node i with its sibling through B[Bptr[i-1]]. */
+int32_t HGEMM_HSS(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
+                  int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
+                  double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
+                  int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
+                  int32_t *wpart, int32_t *clevelset);
+
+/* ---- layer 2: plans on device memory ------------------------------------- */
+
+/* The compressed matrix in the reference layout (host arrays; copied by
+ * mx_plan_create).  Pair arrays follow the packing order of D and B.  For an
+ * HSS structure (is_hss = 1) near pair k is leaf k and far pair k couples
+ * node k+1 with its sibling, the order HGEMM_v1.h indexes.  Row offsets are
+ * in rows: wptr = wrow*nrhs and wskeloffset = skelrow*nrhs in the
+ * reference's element units (hcds.h:362-413). */
+typedef struct mx_structure {
+    int32_t nleaf, numnodes, is_hss;
+    int64_t ncount, fcount;
+    const int32_t *lchildren, *rchildren, *lm, *Ddim, *slen;
+    const int32_t *npairx, *npairy;  /* leaf indices, ncount entries */
+    const int32_t *fpairx, *fpairy;  /* node ids, fcount entries     */
+    const uint64_t *Dptr, *Bptr;
+    const int32_t *VTptr;
+    const int32_t *wrow;             /* nleaf row offsets of TW/Y blocks      */
+    const int32_t *skelrow;          /* numnodes row offsets of wskel/uskel   */
+    const double *D, *B, *VT;
+    int64_t D_len, B_len, VT_len;    /* element counts of D, B, VT            */
+} mx_structure;
+
+typedef struct mx_plan mx_plan;
+
+/* Bits of `precisions` in mx_plan_create. */
+#define MX_PLAN_FP64 (1u << MX_FP64)
+#define MX_PLAN_FP32 (1u << MX_FP32)
+#define MX_PLAN_BF16X3 (1u << MX_BF16X3)
+
+/* Validate the structure, build the per-destination task lists and upload
+ * blocks for each precision in `precisions` to the current HIP device. */
+mx_status mx_plan_create(mx_plan **plan, const mx_structure *s, uint32_t precisions);
+mx_status mx_plan_destroy(mx_plan *plan);
+
+/* Y = K~ W on device buffers (leaf-blocked, N x nrhs), enqueued on `stream`
+ * (a hipStream_t; NULL = default stream).  Y is overwritten.  The plan's
+ * skeleton work buffers are (re)allocated when nrhs grows; evaluations on one
+ * plan must not overlap. */
+mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
+                  void *stream);
+
+/* Device pointers of the skeleton buffers of the last evaluation
+ * (wskel = V^T W per node, uskel = coupled + transferred values per node),
+ * laid out at skelrow[v]*nrhs with leading dimension slen[v]; element type is
+ * double for MX_FP64 and float otherwise. */
+mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uskel);
+
+/* Copy the skeleton buffers of the last evaluation (skel_rows x nrhs
+ * elements each, layout as above) into caller device buffers on `stream`. */
+mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst, int32_t nrhs, void *stream);
+
+/* Number of rows N and of skeleton rows sum(slen) of a plan. */
+mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);
+
+const char *mx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MATROX_HGEMM_H */

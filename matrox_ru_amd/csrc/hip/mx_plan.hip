@@ -1,0 +1,897 @@
+// Plans: structure validation, per-destination task lists, device uploads and
+// the launch sequence of one evaluation.  C-ABI in include/matrox_hgemm.h.
+//
+// Evaluation order (Y = K~ W, semantics of symGen/HGEMM_gen.h:117-238):
+//   UP        one launch per tree level, leaves first:
+//             w_v = VT_v W_leaf  |  w_v = VT_v[:, :s_l] w_l + VT_v[:, s_l:] w_r
+//   FAR+DOWN  one launch per level, root's children first; each task owns one
+//             node c and sums its far couplings and its parent's transfer:
+//             u_c = beta*u_c + sum_{far (c,y)} B_cy w_y + VT_p[:, part(c)]^T u_p
+//   NEAR+LEAF one launch; each task owns one leaf block of Y:
+//             Y_x = beta*Y_x + sum_{near (x,y)} D_xy W_y + VT_x^T u_x
+// The reference runs NEAR first, then UP, FAR and DOWN with beta = 1 updates
+// (HGEMM_gen.h:125-236); fusing the transfers into the coupling and near
+// tasks gives the same sums with each output block written once per launch.
+#include "../../../include/matrox_hgemm.h"
+#include "mx_bf16x3.h"
+#include "mx_seg_gemm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mx {
+
+static thread_local std::string g_err;
+
+struct MxError : std::runtime_error {
+    mx_status code;
+    MxError(mx_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            throw MxError(MX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+static void arg_check(bool ok, const std::string &msg) {
+    if (!ok) throw MxError(MX_ERR_ARG, msg);
+}
+
+struct Launch {
+    int tile_offset;  // into the plan's tile array
+    int ntiles;
+};
+
+}  // namespace mx
+
+struct mx_plan {
+    int device = 0;
+    int nleaf = 0, numnodes = 0, is_hss = 0;
+    int64_t N = 0, skel_rows = 0;
+    std::vector<int32_t> wrow, Ddim, skelrow, slen;
+
+    // task lists: forward evaluation
+    mx::Task *d_tasks = nullptr;
+    mx::Seg *d_segs = nullptr;
+    int2 *d_tiles = nullptr;
+    std::vector<mx::Launch> launches;   // UP levels, FAR+DOWN levels, NEAR+LEAF
+    // the same tasks split for the split-BF16 path: near/far segments go to
+    // the BF16x3 kernel, transfer segments to the FP32 kernel
+    mx::Task *d_tasks_x3 = nullptr;
+    mx::Seg *d_segs_x3 = nullptr;
+    int2 *d_tiles_x3 = nullptr;
+    std::vector<mx::Launch> launches_x3;
+    std::vector<int> launch_kind_x3;    // 0: FP32 seg kernel, 1: BF16x3 kernel
+
+    // stored blocks per precision
+    double *D64 = nullptr, *B64 = nullptr, *VT64 = nullptr;
+    float *D32 = nullptr, *B32 = nullptr, *VT32 = nullptr;
+    uint16_t *Dhi = nullptr, *Dlo = nullptr, *Bhi = nullptr, *Blo = nullptr;
+    int64_t x3_elems[2] = {0, 0};   // packed split D and B elements
+    int64_t D_len = 0, B_len = 0, VT_len = 0;
+    uint32_t precisions = 0;
+
+    // work buffers
+    void *wskel = nullptr, *uskel = nullptr;
+    size_t skel_bytes = 0;
+    void *io_w = nullptr, *io_y = nullptr;   // FP32 staging when io is FP64
+    size_t io_bytes = 0;
+    int last_prec = -1;
+
+    ~mx_plan() {
+        void *ptrs[] = {d_tasks, d_segs, d_tiles, d_tasks_x3, d_segs_x3, d_tiles_x3, D64, B64, VT64, D32, B32,
+                        VT32, Dhi, Dlo, Bhi, Blo, wskel, uskel, io_w, io_y};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace mx {
+
+// Host description of the task lists of one evaluation.
+struct TaskLists {
+    std::vector<Task> tasks;
+    std::vector<Seg> segs;
+    std::vector<int2> tiles;
+    std::vector<Launch> launches;
+    std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
+    std::vector<int> task_kind;   // per task
+};
+
+struct Topology {
+    std::vector<int> parent, depth;
+    std::vector<std::vector<int>> levels;  // node ids by depth (root excluded)
+    std::vector<int> node_of_leaf;
+};
+
+static Topology topology(const mx_structure &s) {
+    Topology t;
+    const int nn = s.numnodes;
+    t.parent.assign(nn, -1);
+    t.depth.assign(nn, -1);
+    t.node_of_leaf.assign(s.nleaf, -1);
+    for (int v = 0; v < nn; ++v) {
+        int l = s.lchildren[v], r = s.rchildren[v];
+        arg_check((l == -1) == (r == -1), "node " + std::to_string(v) + " has exactly one child");
+        if (l != -1) {
+            arg_check(l > 0 && l < nn && r > 0 && r < nn && l != r, "child index out of range");
+            arg_check(t.parent[l] == -1 && t.parent[r] == -1, "node with two parents");
+            t.parent[l] = v;
+            t.parent[r] = v;
+        } else {
+            int lf = s.lm[v];
+            arg_check(lf >= 0 && lf < s.nleaf, "leaf node " + std::to_string(v) + " has no leaf index");
+            arg_check(t.node_of_leaf[lf] == -1, "leaf index used twice");
+            t.node_of_leaf[lf] = v;
+        }
+    }
+    arg_check(t.parent[0] == -1, "node 0 must be the root");
+    // depth by BFS from the root
+    std::vector<int> q{0};
+    t.depth[0] = 0;
+    for (size_t i = 0; i < q.size(); ++i) {
+        int v = q[i];
+        if (s.lchildren[v] != -1) {
+            for (int c : {s.lchildren[v], s.rchildren[v]}) {
+                t.depth[c] = t.depth[v] + 1;
+                q.push_back(c);
+            }
+        }
+    }
+    arg_check((int)q.size() == nn, "tree does not reach every node from the root");
+    int maxd = 0;
+    for (int v = 0; v < nn; ++v) maxd = std::max(maxd, t.depth[v]);
+    t.levels.assign(maxd + 1, {});
+    for (int v = 1; v < nn; ++v) t.levels[t.depth[v]].push_back(v);
+    for (int l = 0; l < s.nleaf; ++l) arg_check(t.node_of_leaf[l] != -1, "leaf index without node");
+    return t;
+}
+
+static int cols_of(const mx_structure &s, int v) {
+    if (s.lchildren[v] == -1) return s.Ddim[s.lm[v]];
+    return s.slen[s.lchildren[v]] + s.slen[s.rchildren[v]];
+}
+
+static void validate(const mx_structure &s, const Topology &t) {
+    const int nn = s.numnodes;
+    for (int l = 0; l < s.nleaf; ++l) arg_check(s.Ddim[l] >= 0, "negative leaf size");
+    for (int v = 0; v < nn; ++v) arg_check(s.slen[v] >= 0, "negative skeleton length");
+    for (int v = 1; v < nn; ++v) {
+        int64_t need = (int64_t)s.VTptr[v] + (int64_t)s.slen[v] * cols_of(s, v);
+        arg_check(s.slen[v] == 0 || (s.VTptr[v] >= 0 && need <= s.VT_len),
+                  "VT block of node " + std::to_string(v) + " exceeds VT");
+    }
+    for (int64_t k = 0; k < s.ncount; ++k) {
+        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
+        arg_check(x >= 0 && x < s.nleaf && y >= 0 && y < s.nleaf, "near pair leaf out of range");
+        int64_t need = (int64_t)s.Dptr[k] + (int64_t)s.Ddim[x] * s.Ddim[y];
+        arg_check(need <= s.D_len, "near block " + std::to_string(k) + " exceeds D");
+    }
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x, y;
+        if (s.is_hss) {
+            x = (int)k + 1;
+            arg_check(x < nn, "HSS far count exceeds numnodes-1");
+            int p = t.parent[x];
+            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+        } else {
+            x = s.fpairx[k];
+            y = s.fpairy[k];
+        }
+        arg_check(x > 0 && x < nn && y > 0 && y < nn, "far pair node out of range");
+        int64_t need = (int64_t)s.Bptr[k] + (int64_t)s.slen[x] * s.slen[y];
+        arg_check(need <= s.B_len, "far block " + std::to_string(k) + " exceeds B");
+    }
+    for (int l = 0; l < s.nleaf; ++l) arg_check(s.wrow[l] >= 0, "negative leaf row offset");
+    for (int v = 0; v < nn; ++v) arg_check(s.skelrow[v] >= 0, "negative skeleton row offset");
+}
+
+static int add_tiles(TaskLists &L, int task, int m) {
+    const int bm = L.task_kind[task] == 1 ? X3_BM : SG_BM;
+    int n = (m + bm - 1) / bm;
+    for (int i = 0; i < n; ++i) L.tiles.push_back(make_int2(task, i));
+    return n;
+}
+
+// Build the forward task lists.  beta_u / beta_y select the accumulation
+// rule of the coupling and leaf tasks.  With split = true, near and far
+// segments are put in tasks of their own (kind 1) ahead of the transfer
+// tasks (kind 0) that then accumulate into the same blocks.
+static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, bool split) {
+    TaskLists L;
+    const int nn = s.numnodes;
+    // far pairs by destination node
+    std::vector<std::vector<int64_t>> far_of(nn);
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x = s.is_hss ? (int)k + 1 : s.fpairx[k];
+        far_of[x].push_back(k);
+    }
+    auto far_partner = [&](int64_t k) {
+        if (!s.is_hss) return s.fpairy[k];
+        int x = (int)k + 1, p = t.parent[x];
+        return s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+    };
+    std::vector<std::vector<int64_t>> near_of(s.nleaf);
+    for (int64_t k = 0; k < s.ncount; ++k) near_of[s.is_hss ? (int)k : s.npairx[k]].push_back(k);
+
+    auto begin_launch = [&](int kind) {
+        L.launches.push_back(Launch{(int)L.tiles.size(), 0});
+        L.kinds.push_back(kind);
+    };
+    auto end_launch = [&]() {
+        L.launches.back().ntiles = (int)L.tiles.size() - L.launches.back().tile_offset;
+        if (L.launches.back().ntiles == 0) {
+            L.launches.pop_back();
+            L.kinds.pop_back();
+        }
+    };
+    auto new_task = [&](int64_t c_row, int m, int c_buf, int beta) {
+        Task tk{};
+        tk.c_row = c_row;
+        tk.m = m;
+        tk.c_buf = c_buf;
+        tk.seg_begin = (int)L.segs.size();
+        tk.seg_end = tk.seg_begin;
+        tk.beta = beta;
+        L.tasks.push_back(tk);
+        L.task_kind.push_back(L.kinds.back());
+        return (int)L.tasks.size() - 1;
+    };
+    auto add_seg = [&](int task, int64_t a_off, int a_buf, int a_trans, int lda, int k, int b_buf, int64_t b_row,
+                       int b_ld) {
+        if (k <= 0) return;
+        Seg sg{};
+        sg.a_off = a_off;
+        sg.a_buf = a_buf;
+        sg.a_trans = a_trans;
+        sg.lda = lda;
+        sg.k = k;
+        sg.b_buf = b_buf;
+        sg.b_row = b_row;
+        sg.b_ld = b_ld;
+        L.segs.push_back(sg);
+        L.tasks[task].seg_end = (int)L.segs.size();
+    };
+    const int maxd = (int)t.levels.size() - 1;
+    // UP: deepest level first
+    for (int d = maxd; d >= 1; --d) {
+        begin_launch(0);
+        for (int v : t.levels[d]) {
+            int sv = s.slen[v];
+            if (sv == 0) continue;
+            int tk = new_task(s.skelrow[v], sv, VEC_WSKEL, 0);
+            if (s.lchildren[v] == -1) {
+                int lf = s.lm[v];
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.Ddim[lf], VEC_W, s.wrow[lf], s.Ddim[lf]);
+            } else {
+                int l = s.lchildren[v], r = s.rchildren[v];
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.slen[l], VEC_WSKEL, s.skelrow[l], s.slen[l]);
+                add_seg(tk, (int64_t)s.VTptr[v] + (int64_t)sv * s.slen[l], MAT_VT, 0, sv, s.slen[r], VEC_WSKEL,
+                        s.skelrow[r], s.slen[r]);
+            }
+            add_tiles(L, tk, sv);
+        }
+        end_launch();
+    }
+    // transfer segment of node c from its parent p: VT_p[:, part(c)]^T u_p
+    auto add_parent = [&](int tk, int c) {
+        int p = t.parent[c];
+        if (p <= 0 || s.slen[p] == 0) return;
+        int sp = s.slen[p];
+        int64_t off = s.VTptr[p];
+        if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
+        add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, s.skelrow[p], sp);
+    };
+    auto add_far = [&](int tk, int c) {
+        for (int64_t k : far_of[c]) {
+            int y = far_partner(k);
+            add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, s.slen[c], s.slen[y], VEC_WSKEL, s.skelrow[y], s.slen[y]);
+        }
+    };
+    // FAR + DOWN: root's children first
+    for (int d = 1; d <= maxd; ++d) {
+        if (!split) {
+            begin_launch(0);
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
+                add_far(tk, c);
+                add_parent(tk, c);
+                add_tiles(L, tk, sc);
+            }
+            end_launch();
+        } else {
+            if (d == 1) {
+                // all far couplings in one launch (they only read wskel)
+                begin_launch(1);
+                for (int dd = 1; dd <= maxd; ++dd)
+                    for (int c : t.levels[dd]) {
+                        int sc = s.slen[c];
+                        if (sc == 0) continue;
+                        int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
+                        add_far(tk, c);
+                        add_tiles(L, tk, sc);
+                    }
+                end_launch();
+            }
+            begin_launch(0);
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(s.skelrow[c], sc, VEC_USKEL, 1);
+                add_parent(tk, c);
+                if (L.tasks[tk].seg_end == L.tasks[tk].seg_begin) {
+                    L.tasks.pop_back();
+                    L.task_kind.pop_back();
+                    continue;
+                }
+                add_tiles(L, tk, sc);
+            }
+            end_launch();
+        }
+    }
+    // NEAR + LEAF
+    auto add_near = [&](int tk, int lf) {
+        for (int64_t k : near_of[lf]) {
+            int y = s.is_hss ? (int)k : s.npairy[k];
+            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, s.wrow[y], s.Ddim[y]);
+        }
+    };
+    auto add_leafdown = [&](int tk, int lf) {
+        int v = t.node_of_leaf[lf];
+        int sv = s.slen[v];
+        if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, s.skelrow[v], sv);
+    };
+    if (!split) {
+        begin_launch(0);
+        for (int lf = 0; lf < s.nleaf; ++lf) {
+            if (s.Ddim[lf] == 0) continue;
+            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
+            add_near(tk, lf);
+            add_leafdown(tk, lf);
+            add_tiles(L, tk, s.Ddim[lf]);
+        }
+        end_launch();
+    } else {
+        begin_launch(1);
+        for (int lf = 0; lf < s.nleaf; ++lf) {
+            if (s.Ddim[lf] == 0) continue;
+            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
+            add_near(tk, lf);
+            add_tiles(L, tk, s.Ddim[lf]);
+        }
+        end_launch();
+        begin_launch(0);
+        for (int lf = 0; lf < s.nleaf; ++lf) {
+            if (s.Ddim[lf] == 0) continue;
+            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, 1);
+            add_leafdown(tk, lf);
+            if (L.tasks[tk].seg_end == L.tasks[tk].seg_begin) {
+                L.tasks.pop_back();
+                L.task_kind.pop_back();
+                continue;
+            }
+            add_tiles(L, tk, s.Ddim[lf]);
+        }
+        end_launch();
+    }
+    return L;
+}
+
+template <typename T>
+static T *upload(const T *h, size_t n) {
+    T *d = nullptr;
+    if (n == 0) n = 1;
+    HIPCHK(hipMalloc(&d, n * sizeof(T)));
+    if (h) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+static void upload_lists(const TaskLists &L, Task **t, Seg **s, int2 **tl) {
+    *t = upload(L.tasks.data(), L.tasks.size());
+    *s = upload(L.segs.data(), L.segs.size());
+    *tl = upload(L.tiles.data(), L.tiles.size());
+}
+
+__global__ void cvt_f64_f32(const double *__restrict__ in, float *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (float)in[i];
+}
+
+__global__ void cvt_f32_f64(const float *__restrict__ in, double *__restrict__ out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) out[i] = (double)in[i];
+}
+
+static int cvt_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 8192);
+}
+
+static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y) {
+    arg_check(s.nleaf > 0 && s.numnodes >= 1, "empty structure");
+    arg_check(s.numnodes == 2 * s.nleaf - 1, "numnodes must be 2*nleaf-1 for a binary tree");
+    arg_check(s.lchildren && s.rchildren && s.lm && s.Ddim && s.slen && s.VTptr && s.wrow && s.skelrow,
+              "null structure array");
+    arg_check(s.ncount >= 0 && s.fcount >= 0, "negative pair count");
+    arg_check(s.ncount == 0 || (s.Dptr && s.D), "near blocks missing");
+    arg_check(s.fcount == 0 || (s.Bptr && s.B), "far blocks missing");
+    arg_check(s.is_hss || s.ncount == 0 || (s.npairx && s.npairy), "near pair arrays missing");
+    arg_check(s.is_hss || s.fcount == 0 || (s.fpairx && s.fpairy), "far pair arrays missing");
+    arg_check(precisions != 0 && (precisions & ~7u) == 0, "bad precision mask");
+    Topology t = topology(s);
+    validate(s, t);
+
+    std::unique_ptr<mx_plan> P(new mx_plan());
+    HIPCHK(hipGetDevice(&P->device));
+    P->nleaf = s.nleaf;
+    P->numnodes = s.numnodes;
+    P->is_hss = s.is_hss;
+    P->wrow.assign(s.wrow, s.wrow + s.nleaf);
+    P->Ddim.assign(s.Ddim, s.Ddim + s.nleaf);
+    P->skelrow.assign(s.skelrow, s.skelrow + s.numnodes);
+    P->slen.assign(s.slen, s.slen + s.numnodes);
+    int64_t N = 0, SK = 0;
+    for (int l = 0; l < s.nleaf; ++l) N = std::max(N, (int64_t)s.wrow[l] + s.Ddim[l]);
+    for (int v = 0; v < s.numnodes; ++v) SK = std::max(SK, (int64_t)s.skelrow[v] + s.slen[v]);
+    P->N = N;
+    P->skel_rows = SK;
+    P->D_len = s.D_len;
+    P->B_len = s.B_len;
+    P->VT_len = s.VT_len;
+    P->precisions = precisions;
+
+    TaskLists L = build_forward(s, t, beta_u, beta_y, false);
+    upload_lists(L, &P->d_tasks, &P->d_segs, &P->d_tiles);
+    P->launches = L.launches;
+    TaskLists X;
+    std::vector<X3Block> xblk[2];   // packed D and B blocks
+    if (precisions & MX_PLAN_BF16X3) {
+        X = build_forward(s, t, beta_u, beta_y, true);
+        // near/far segments of BF16x3 tasks read packed split blocks: assign
+        // each stored block its packed offset (once per block)
+        std::map<int64_t, int64_t> seen[2];
+        int64_t next[2] = {0, 0};
+        for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
+            if (X.task_kind[ti] != 1) continue;
+            for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
+                Seg &sg = X.segs[si];
+                arg_check(sg.a_trans == 0 && (sg.a_buf == MAT_D || sg.a_buf == MAT_B), "bad BF16x3 segment");
+                int which = sg.a_buf == MAT_D ? 0 : 1;
+                auto it = seen[which].find(sg.a_off);
+                int64_t dst;
+                if (it == seen[which].end()) {
+                    dst = next[which];
+                    seen[which][sg.a_off] = dst;
+                    xblk[which].push_back(X3Block{sg.a_off, dst, X.tasks[ti].m, sg.k});
+                    next[which] += x3_packed_elems(X.tasks[ti].m, sg.k);
+                } else {
+                    dst = it->second;
+                }
+                sg.a_off = dst;
+                sg.lda = (sg.k + 31) / 32;
+            }
+        }
+        P->x3_elems[0] = next[0];
+        P->x3_elems[1] = next[1];
+        upload_lists(X, &P->d_tasks_x3, &P->d_segs_x3, &P->d_tiles_x3);
+        P->launches_x3 = X.launches;
+        P->launch_kind_x3 = X.kinds;
+    }
+    // stored blocks
+    double *D64 = upload(s.D, (size_t)s.D_len), *B64 = upload(s.B, (size_t)s.B_len);
+    double *VT64 = upload(s.VT, (size_t)s.VT_len);
+    if (precisions & (MX_PLAN_FP32 | MX_PLAN_BF16X3)) {
+        P->VT32 = upload<float>(nullptr, (size_t)s.VT_len);
+        cvt_f64_f32<<<cvt_grid(s.VT_len), 256>>>(VT64, P->VT32, s.VT_len);
+    }
+    if (precisions & MX_PLAN_FP32) {
+        P->D32 = upload<float>(nullptr, (size_t)s.D_len);
+        P->B32 = upload<float>(nullptr, (size_t)s.B_len);
+        cvt_f64_f32<<<cvt_grid(s.D_len), 256>>>(D64, P->D32, s.D_len);
+        cvt_f64_f32<<<cvt_grid(s.B_len), 256>>>(B64, P->B32, s.B_len);
+    }
+    if (precisions & MX_PLAN_BF16X3) {
+        uint16_t **hi[2] = {&P->Dhi, &P->Bhi}, **lo[2] = {&P->Dlo, &P->Blo};
+        double *src[2] = {D64, B64};
+        for (int w = 0; w < 2; ++w) {
+            *hi[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
+            *lo[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
+            for (size_t b0 = 0; b0 < xblk[w].size(); b0 += 65535) {
+                size_t nb = std::min<size_t>(65535, xblk[w].size() - b0);
+                X3Block *db = upload(xblk[w].data() + b0, nb);
+                pack_x3_blocks<<<dim3(64, (unsigned)nb), 256>>>(src[w], db, *hi[w], *lo[w]);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipDeviceSynchronize());
+                HIPCHK(hipFree(db));
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    if (precisions & MX_PLAN_FP64) {
+        P->D64 = D64;
+        P->B64 = B64;
+        P->VT64 = VT64;
+    } else {
+        HIPCHK(hipFree(D64));
+        HIPCHK(hipFree(B64));
+        HIPCHK(hipFree(VT64));
+    }
+    return P.release();
+}
+
+static void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
+    size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
+    if (sb > P->skel_bytes) {
+        if (P->wskel) HIPCHK(hipFree(P->wskel));
+        if (P->uskel) HIPCHK(hipFree(P->uskel));
+        P->wskel = P->uskel = nullptr;
+        HIPCHK(hipMalloc(&P->wskel, sb));
+        HIPCHK(hipMalloc(&P->uskel, sb));
+        P->skel_bytes = sb;
+    }
+    if (need_io) {
+        size_t ib = (size_t)std::max<int64_t>(P->N, 1) * nrhs * sizeof(float);
+        if (ib > P->io_bytes) {
+            if (P->io_w) HIPCHK(hipFree(P->io_w));
+            if (P->io_y) HIPCHK(hipFree(P->io_y));
+            P->io_w = P->io_y = nullptr;
+            HIPCHK(hipMalloc(&P->io_w, ib));
+            HIPCHK(hipMalloc(&P->io_y, ib));
+            P->io_bytes = ib;
+        }
+    }
+}
+
+template <typename T>
+static void run_seg_launch(const Launch &ln, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs,
+                           const Bufs<T> &b, hipStream_t st) {
+    int nnt = (nrhs + SG_BN - 1) / SG_BN;
+    int64_t total = (int64_t)ln.ntiles * nnt;
+    if (total <= 0) return;
+    if (total > INT32_MAX) throw MxError(MX_ERR_ARG, "launch too large");
+    hipLaunchKernelGGL(seg_gemm_kernel<T>, dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
+                       tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+}
+
+static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+                 hipStream_t st) {
+    arg_check(W && Y, "null W or Y");
+    arg_check(nrhs >= 1, "nrhs must be >= 1");
+    arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
+    int dev;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != P->device) throw MxError(MX_ERR_STATE, "plan was created on another device");
+    if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
+    const int64_t nel = P->N * (int64_t)nrhs;
+    if (prec == MX_FP64) {
+        if (io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
+        ensure_work(P, nrhs, sizeof(double), false);
+        Bufs<double> b;
+        b.mat[MAT_D] = P->D64;
+        b.mat[MAT_B] = P->B64;
+        b.mat[MAT_VT] = P->VT64;
+        b.vec[VEC_W] = const_cast<double *>(static_cast<const double *>(W));
+        b.vec[VEC_WSKEL] = (double *)P->wskel;
+        b.vec[VEC_USKEL] = (double *)P->uskel;
+        b.vec[VEC_Y] = (double *)Y;
+        for (size_t i = 0; i < P->launches.size(); ++i)
+            run_seg_launch<double>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+    } else {
+        ensure_work(P, nrhs, sizeof(float), io == MX_IO_F64);
+        const float *Wf;
+        float *Yf;
+        if (io == MX_IO_F64) {
+            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)P->io_w, nel);
+            Wf = (const float *)P->io_w;
+            Yf = (float *)P->io_y;
+        } else {
+            Wf = (const float *)W;
+            Yf = (float *)Y;
+        }
+        Bufs<float> b;
+        b.mat[MAT_D] = P->D32;
+        b.mat[MAT_B] = P->B32;
+        b.mat[MAT_VT] = P->VT32;
+        b.vec[VEC_W] = const_cast<float *>(Wf);
+        b.vec[VEC_WSKEL] = (float *)P->wskel;
+        b.vec[VEC_USKEL] = (float *)P->uskel;
+        b.vec[VEC_Y] = Yf;
+        if (prec == MX_FP32) {
+            for (size_t i = 0; i < P->launches.size(); ++i)
+                run_seg_launch<float>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+        } else {
+            X3Bufs xb;
+            xb.hi[MAT_D] = P->Dhi;
+            xb.lo[MAT_D] = P->Dlo;
+            xb.hi[MAT_B] = P->Bhi;
+            xb.lo[MAT_B] = P->Blo;
+            xb.hi[MAT_VT] = nullptr;
+            xb.lo[MAT_VT] = nullptr;
+            for (int i = 0; i < 4; ++i) xb.vec[i] = b.vec[i];
+            for (size_t i = 0; i < P->launches_x3.size(); ++i) {
+                if (P->launch_kind_x3[i] == 0) {
+                    run_seg_launch<float>(P->launches_x3[i], P->d_tasks_x3, P->d_segs_x3, P->d_tiles_x3, nrhs,
+                                          b, st);
+                } else {
+                    launch_bf16x3(P->launches_x3[i].ntiles, P->d_tasks_x3, P->d_segs_x3,
+                                  P->d_tiles_x3 + P->launches_x3[i].tile_offset, nrhs, xb, st);
+                }
+            }
+        }
+        if (io == MX_IO_F64) cvt_f32_f64<<<cvt_grid(nel), 256, 0, st>>>(Yf, (double *)Y, nel);
+    }
+    HIPCHK(hipGetLastError());
+    P->last_prec = prec;
+}
+
+// ---- drop-in entry points -------------------------------------------------
+
+struct CompatDims {
+    bool set = false;
+    int32_t nleaf = 0, nl = 0, fl = 0, cdepth = 0;
+};
+static CompatDims g_dims;
+
+static std::vector<int64_t> visit_blocks(const int32_t *set, const int32_t *blocks, int32_t rows) {
+    std::vector<int64_t> ks;
+    for (int i = 0; i < rows; ++i)
+        for (int j = set[i]; j < set[i + 1]; ++j)
+            for (int k = blocks[j]; k < blocks[j + 1]; ++k) ks.push_back(k);
+    return ks;
+}
+
+static int32_t compat_eval(bool hss, double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr,
+                           int32_t *VTptr, int32_t *lc, int32_t *rc, double *mrhs, double *apres, int32_t nrhs,
+                           int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel, int32_t *wso,
+                           double *uskel, int32_t *uso, int32_t *lm, int32_t *slen, int32_t *nblockSet,
+                           int32_t *nblocks, int32_t *npairx, int32_t *npairy, int32_t *fblockSet,
+                           int32_t *fblocks, int32_t *fpairx, int32_t *fpairy) {
+    if (!g_dims.set) throw MxError(MX_ERR_STATE, "call mx_hgemm_configure before HGEMM");
+    arg_check(nrhs >= 1, "nrhs must be >= 1");
+    arg_check(D && B && VT && Dptr && Bptr && VTptr && lc && rc && mrhs && apres && Ddim && wptr && uptr &&
+                  wskel && wso && uskel && uso && lm && slen,
+              "null argument");
+    const int nleaf = g_dims.nleaf, nn = 2 * nleaf - 1;
+    mx_structure s{};
+    s.nleaf = nleaf;
+    s.numnodes = nn;
+    s.is_hss = hss ? 1 : 0;
+    s.lchildren = lc;
+    s.rchildren = rc;
+    s.lm = lm;
+    s.Ddim = Ddim;
+    s.slen = slen;
+    s.VTptr = VTptr;
+    std::vector<int32_t> wrow(nleaf), skelrow(nn);
+    for (int l = 0; l < nleaf; ++l) {
+        arg_check(wptr[l] % nrhs == 0 && uptr[l] == wptr[l], "wptr/uptr must be equal multiples of nrhs");
+        wrow[l] = wptr[l] / nrhs;
+    }
+    for (int v = 0; v < nn; ++v) {
+        arg_check(wso[v] % nrhs == 0 && uso[v] == wso[v], "skeleton offsets must be equal multiples of nrhs");
+        skelrow[v] = wso[v] / nrhs;
+    }
+    s.wrow = wrow.data();
+    s.skelrow = skelrow.data();
+    std::vector<int32_t> px, py, fx, fy;
+    std::vector<uint64_t> dp, bp;
+    if (hss) {
+        s.ncount = nleaf;
+        s.fcount = nn - 1;
+        s.Dptr = Dptr;
+        s.Bptr = Bptr;
+    } else {
+        arg_check(nblockSet && nblocks && npairx && npairy && fblockSet && fblocks && fpairx && fpairy,
+                  "null blocking array");
+        for (int64_t k : visit_blocks(nblockSet, nblocks, g_dims.nl)) {
+            px.push_back(npairx[k]);
+            py.push_back(npairy[k]);
+            dp.push_back(Dptr[k]);
+        }
+        for (int64_t k : visit_blocks(fblockSet, fblocks, g_dims.fl)) {
+            fx.push_back(fpairx[k]);
+            fy.push_back(fpairy[k]);
+            bp.push_back(Bptr[k]);
+        }
+        s.ncount = (int64_t)px.size();
+        s.fcount = (int64_t)fx.size();
+        s.npairx = px.data();
+        s.npairy = py.data();
+        s.fpairx = fx.data();
+        s.fpairy = fy.data();
+        s.Dptr = dp.data();
+        s.Bptr = bp.data();
+    }
+    // buffer extents implied by the offsets
+    int64_t dl = 0, bl = 0, vl = 0;
+    for (int64_t k = 0; k < s.ncount; ++k) {
+        int x = hss ? (int)k : s.npairx[k], y = hss ? (int)k : s.npairy[k];
+        arg_check(x >= 0 && x < nleaf && y >= 0 && y < nleaf, "near pair leaf out of range");
+        dl = std::max<int64_t>(dl, (int64_t)s.Dptr[k] + (int64_t)Ddim[x] * Ddim[y]);
+    }
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x, y;
+        if (hss) {
+            x = (int)k + 1;
+            y = (x & 1) ? x + 1 : x - 1;
+        } else {
+            x = s.fpairx[k];
+            y = s.fpairy[k];
+        }
+        arg_check(x > 0 && x < nn && y > 0 && y < nn, "far pair node out of range");
+        bl = std::max<int64_t>(bl, (int64_t)s.Bptr[k] + (int64_t)slen[x] * slen[y]);
+    }
+    for (int v = 1; v < nn; ++v) {
+        arg_check((lc[v] == -1) == (rc[v] == -1), "node with one child");
+        if (lc[v] == -1) arg_check(lm[v] >= 0 && lm[v] < nleaf, "leaf map out of range");
+        else arg_check(lc[v] > 0 && lc[v] < nn && rc[v] > 0 && rc[v] < nn, "child out of range");
+        int cols = lc[v] == -1 ? Ddim[lm[v]] : slen[lc[v]] + slen[rc[v]];
+        vl = std::max<int64_t>(vl, (int64_t)VTptr[v] + (int64_t)slen[v] * cols);
+    }
+    s.D = D;
+    s.B = B;
+    s.VT = VT;
+    s.D_len = dl;
+    s.B_len = bl;
+    s.VT_len = vl;
+    int beta = hss ? 0 : 1;
+    std::unique_ptr<mx_plan> P(create_plan(s, MX_PLAN_FP64, beta, beta));
+    int64_t N = P->N, SK = P->skel_rows;
+    double *dW = upload(mrhs, (size_t)(N * nrhs));
+    double *dY = upload(apres, (size_t)(N * nrhs));
+    ensure_work(P.get(), nrhs, sizeof(double), false);
+    HIPCHK(hipMemcpy(P->uskel, uskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyHostToDevice));
+    try {
+        eval(P.get(), MX_FP64, dW, dY, nrhs, MX_IO_F64, nullptr);
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(apres, dY, (size_t)(N * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(wskel, P->wskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(uskel, P->uskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
+    } catch (...) {
+        (void)hipFree(dW);
+        (void)hipFree(dY);
+        throw;
+    }
+    HIPCHK(hipFree(dW));
+    HIPCHK(hipFree(dY));
+    return 0;
+}
+
+template <typename F>
+static int32_t guard(F &&f) {
+    try {
+        return (int32_t)f();
+    } catch (const MxError &e) {
+        g_err = e.what();
+        return (int32_t)e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return (int32_t)MX_ERR_ARG;
+    }
+}
+
+}  // namespace mx
+
+using namespace mx;
+
+extern "C" {
+
+mx_status mx_hgemm_configure(int32_t nleaf, int32_t nl, int32_t fl, int32_t cdepth) {
+    return (mx_status)guard([&]() {
+        arg_check(nleaf >= 1 && nl >= 0 && fl >= 0 && cdepth >= 0, "bad dimensions");
+        g_dims.nleaf = nleaf;
+        g_dims.nl = nl;
+        g_dims.fl = fl;
+        g_dims.cdepth = cdepth;
+        g_dims.set = true;
+        return 0;
+    });
+}
+
+int32_t HGEMM(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
+              int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
+              double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
+              int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
+              int32_t *nblockSet, int32_t *nblocks, int32_t *npairx, int32_t *npairy, int32_t *fblockSet,
+              int32_t *fblocks, int32_t *fpairx, int32_t *fpairy, int32_t *wpart, int32_t *clevelset) {
+    (void)levelset;
+    (void)idx;
+    (void)wpart;
+    (void)clevelset;
+    return guard([&]() {
+        return compat_eval(false, D, B, VT, Dptr, Bptr, VTptr, lchildren, rchildren, mrhs, apres, nrhs, Ddim,
+                           wptr, uptr, wskel, wskeloffset, uskel, uskeloffset, lm, slen, nblockSet, nblocks,
+                           npairx, npairy, fblockSet, fblocks, fpairx, fpairy);
+    });
+}
+
+int32_t HGEMM_HSS(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
+                  int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
+                  double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
+                  int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
+                  int32_t *wpart, int32_t *clevelset) {
+    (void)levelset;
+    (void)idx;
+    (void)wpart;
+    (void)clevelset;
+    return guard([&]() {
+        return compat_eval(true, D, B, VT, Dptr, Bptr, VTptr, lchildren, rchildren, mrhs, apres, nrhs, Ddim, wptr,
+                           uptr, wskel, wskeloffset, uskel, uskeloffset, lm, slen, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, nullptr, nullptr);
+    });
+}
+
+mx_status mx_plan_create(mx_plan **plan, const mx_structure *s, uint32_t precisions) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && s, "null argument");
+        *plan = nullptr;
+        *plan = create_plan(*s, precisions, 0, 0);
+        return 0;
+    });
+}
+
+mx_status mx_plan_destroy(mx_plan *plan) {
+    delete plan;
+    return MX_OK;
+}
+
+mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
+                  void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
+        eval(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uskel) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && wskel && uskel, "null argument");
+        if (plan->last_prec < 0) throw MxError(MX_ERR_STATE, "no evaluation has run on this plan");
+        *wskel = plan->wskel;
+        *uskel = plan->uskel;
+        return 0;
+    });
+}
+
+mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst, int32_t nrhs, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && wskel_dst && uskel_dst && nrhs >= 1, "bad argument");
+        if (plan->last_prec < 0) throw MxError(MX_ERR_STATE, "no evaluation has run on this plan");
+        size_t elem = plan->last_prec == MX_FP64 ? sizeof(double) : sizeof(float);
+        size_t bytes = (size_t)plan->skel_rows * nrhs * elem;
+        arg_check(bytes <= plan->skel_bytes, "nrhs exceeds the last evaluation's buffers");
+        HIPCHK(hipMemcpyAsync(wskel_dst, plan->wskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        HIPCHK(hipMemcpyAsync(uskel_dst, plan->uskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        return 0;
+    });
+}
+
+mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && N && skel_rows, "null argument");
+        *N = plan->N;
+        *skel_rows = plan->skel_rows;
+        return 0;
+    });
+}
+
+const char *mx_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,226 @@
+// Segmented grouped GEMM on MFMA (FP64 and FP32), the workhorse of every
+// evaluation phase.
+//
+// One workgroup (256 threads = 4 wave64) computes a 64 x 64 tile of one output
+// block C; each wave owns a 32 x 32 quarter as 2 x 2 MFMA 16x16x4 blocks
+// (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32).  The K loop walks the
+// task's segments in 16-deep chunks, staging the A chunk (64 x 16, padded,
+// m fastest) and the B chunk (16 x 64, k fastest, padded) in LDS with a
+// double buffer: the global loads of chunk i+1 are in flight while chunk i
+// is multiplied.  Ragged edges (block sizes that are not multiples of the
+// tile) load zeros.  The finished tile goes through LDS so that the global
+// stores (and, for beta = 1, loads) of C are 64 consecutive rows per column.
+//
+// MFMA 16x16x4 operand layout (lane l of 64): A element (row l%16, k l/16),
+// B element (k l/16, col l%16); result register r holds C(4*(l/16)+r, l%16).
+#pragma once
+#include "mx_common.h"
+
+namespace mx {
+
+typedef double d4_t __attribute__((ext_vector_type(4)));
+typedef float f4_t __attribute__((ext_vector_type(4)));
+
+template <typename T> struct MfmaAcc;
+template <> struct MfmaAcc<double> { typedef d4_t type; };
+template <> struct MfmaAcc<float> { typedef f4_t type; };
+
+__device__ __forceinline__ d4_t mfma16x16x4(double a, double b, d4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4_t mfma16x16x4(float a, float b, f4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int SG_BM = 64, SG_BN = 64, SG_BK = 16;
+constexpr int SG_AP = SG_BM + 1;   // padded row stride of the A chunk in LDS
+constexpr int SG_BP = SG_BK + 1;   // padded column stride of the B chunk in LDS
+constexpr int SG_CP = SG_BM + 1;   // padded column stride of the C tile in LDS
+constexpr int SG_THREADS = 256;
+
+template <typename T>
+struct SegCursor {
+    int seg;   // current segment
+    int k0;    // first k of the chunk within the segment
+};
+
+// Per-thread register staging of one chunk: 4 A values and 4 B values.
+template <typename T>
+struct Stage {
+    T a[4];
+    T b[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(Stage<T> &st, const Seg &sg, int k0, int m0, int M, int n0,
+                                           int nrhs, const Bufs<T> &bufs, int tid) {
+    const T *A = bufs.mat[sg.a_buf] + sg.a_off;
+    const int K = sg.k;
+    if (!sg.a_trans) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            int kk = e / SG_BM, mm = e % SG_BM;
+            int gm = m0 + mm, gk = k0 + kk;
+            st.a[r] = (gm < M && gk < K) ? A[(int64_t)gk * sg.lda + gm] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            int mm = e / SG_BK, kk = e % SG_BK;
+            int gm = m0 + mm, gk = k0 + kk;
+            st.a[r] = (gm < M && gk < K) ? A[(int64_t)gm * sg.lda + gk] : T(0);
+        }
+    }
+    const T *Bv = bufs.vec[sg.b_buf] + sg.b_row * (int64_t)nrhs;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int e = r * SG_THREADS + tid;
+        int nn = e / SG_BK, kk = e % SG_BK;
+        int gn = n0 + nn, gk = k0 + kk;
+        st.b[r] = (gn < nrhs && gk < K) ? Bv[(int64_t)gn * sg.b_ld + gk] : T(0);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(const Stage<T> &st, int a_trans, T *As, T *Bs, int tid) {
+    if (!a_trans) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            As[(e / SG_BM) * SG_AP + (e % SG_BM)] = st.a[r];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            As[(e % SG_BK) * SG_AP + (e / SG_BK)] = st.a[r];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int e = r * SG_THREADS + tid;
+        Bs[(e / SG_BK) * SG_BP + (e % SG_BK)] = st.b[r];
+    }
+}
+
+// Advance to the next 16-deep chunk of the task's segment sequence; returns
+// false when the task is exhausted.  Segments with k == 0 are skipped.
+__device__ __forceinline__ bool next_chunk(const Seg *segs, int seg_end, int &seg, int &k0) {
+    k0 += SG_BK;
+    while (seg < seg_end && k0 >= segs[seg].k) {
+        ++seg;
+        k0 = 0;
+    }
+    return seg < seg_end;
+}
+
+__device__ __forceinline__ bool first_chunk(const Seg *segs, int seg_begin, int seg_end, int &seg, int &k0) {
+    seg = seg_begin;
+    k0 = 0;
+    while (seg < seg_end && segs[seg].k <= 0) ++seg;
+    return seg < seg_end;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SG_THREADS) void seg_gemm_kernel(const Task *__restrict__ tasks,
+                                                              const Seg *__restrict__ segs,
+                                                              const int2 *__restrict__ tiles, int ntiles,
+                                                              int nrhs, int n_ntiles, Bufs<T> bufs) {
+    typedef typename MfmaAcc<T>::type acc_t;
+    constexpr int A_ELEMS = SG_BK * SG_AP;
+    constexpr int B_ELEMS = SG_BN * SG_BP;
+    constexpr int STAGE_ELEMS = A_ELEMS + B_ELEMS;
+    constexpr int SMEM_ELEMS = (2 * STAGE_ELEMS > SG_BN * SG_CP) ? 2 * STAGE_ELEMS : SG_BN * SG_CP;
+    __shared__ T smem[SMEM_ELEMS];
+
+    const int tid = threadIdx.x;
+    const int total = ntiles * n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / n_ntiles, nt = w % n_ntiles;
+    const int2 tl = tiles[tile_id];
+    const Task task = tasks[tl.x];
+    const int M = task.m;
+    const int m0 = tl.y * SG_BM, n0 = nt * SG_BN;
+
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    int seg, k0;
+    bool have = first_chunk(segs, task.seg_begin, task.seg_end, seg, k0);
+    Stage<T> st;
+    int cur = 0;
+    if (have) {
+        Seg sg = segs[seg];
+        load_chunk(st, sg, k0, m0, M, n0, nrhs, bufs, tid);
+        store_chunk(st, sg.a_trans, smem, smem + A_ELEMS, tid);
+    }
+    __syncthreads();
+    while (have) {
+        int nseg = seg, nk0 = k0;
+        bool nhave = next_chunk(segs, task.seg_end, nseg, nk0);
+        Seg nsg;
+        if (nhave) {
+            nsg = segs[nseg];
+            load_chunk(st, nsg, nk0, m0, M, n0, nrhs, bufs, tid);
+        }
+        const T *As = smem + cur * STAGE_ELEMS;
+        const T *Bs = As + A_ELEMS;
+#pragma unroll
+        for (int kk = 0; kk < SG_BK; kk += 4) {
+            T a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[(kk + lk) * SG_AP + wr * 32 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[(wc * 32 + j * 16 + lr) * SG_BP + kk + lk];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
+        }
+        if (nhave) {
+            T *Asn = smem + (cur ^ 1) * STAGE_ELEMS;
+            store_chunk(st, nsg.a_trans, Asn, Asn + A_ELEMS, tid);
+        }
+        __syncthreads();
+        cur ^= 1;
+        seg = nseg;
+        k0 = nk0;
+        have = nhave;
+    }
+
+    // C tile -> LDS (column-major, padded) -> coalesced global update
+    T *Cs = smem;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int row = wr * 32 + i * 16 + 4 * lk + r;
+                int col = wc * 32 + j * 16 + lr;
+                Cs[col * SG_CP + row] = acc[i][j][r];
+            }
+    __syncthreads();
+    T *C = bufs.vec[task.c_buf] + task.c_row * (int64_t)nrhs;
+#pragma unroll 4
+    for (int e = tid; e < SG_BM * SG_BN; e += SG_THREADS) {
+        int col = e / SG_BM, row = e % SG_BM;
+        int gm = m0 + row, gn = n0 + col;
+        if (gm < M && gn < nrhs) {
+            T v = Cs[col * SG_CP + row];
+            T *p = C + (int64_t)gn * M + gm;
+            *p = task.beta ? *p + v : v;
+        }
+    }
+}
+
+}  // namespace mx

@@ -1,0 +1,226 @@
+"""Python binding of the HIP evaluator (include/matrox_hgemm.h).
+
+``Plan`` uploads an ``HMatrix`` (matrox_ru_amd.host) once and evaluates
+Y = K~ W on device tensors; ``hgemm``/``hgemm_hss`` call the drop-in entry
+points with the reference's exact argument lists on host arrays
+(symGen/HGEMM_gen.h:117-124, symGen/HGEMM_v1.h:117-124).
+
+torch is used only for device memory and streams.  There is no fallback:
+without the built HIP library every call raises.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _native
+
+FP64, FP32, BF16X3 = 0, 1, 2
+IO_F64, IO_F32 = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+
+
+class MxStructure(ctypes.Structure):
+    _fields_ = [("nleaf", I32), ("numnodes", I32), ("is_hss", I32),
+                ("ncount", I64), ("fcount", I64),
+                ("lchildren", P), ("rchildren", P), ("lm", P), ("Ddim", P), ("slen", P),
+                ("npairx", P), ("npairy", P), ("fpairx", P), ("fpairy", P),
+                ("Dptr", P), ("Bptr", P), ("VTptr", P), ("wrow", P), ("skelrow", P),
+                ("D", P), ("B", P), ("VT", P),
+                ("D_len", I64), ("B_len", I64), ("VT_len", I64)]
+
+
+class MxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mx status {code}: {msg}")
+        self.code = code
+
+
+_configured = False
+
+
+def lib():
+    global _configured
+    L = _native.hgemm_lib()
+    if not _configured:
+        L.mx_last_error.restype = ctypes.c_char_p
+        L.mx_hgemm_configure.argtypes = [I32, I32, I32, I32]
+        L.mx_hgemm_configure.restype = ctypes.c_int
+        L.HGEMM.argtypes = [P] * 12 + [I32] + [P] * 19
+        L.HGEMM.restype = I32
+        L.HGEMM_HSS.argtypes = [P] * 12 + [I32] + [P] * 11
+        L.HGEMM_HSS.restype = I32
+        L.mx_plan_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(MxStructure), ctypes.c_uint32]
+        L.mx_plan_create.restype = ctypes.c_int
+        L.mx_plan_destroy.argtypes = [P]
+        L.mx_plan_destroy.restype = ctypes.c_int
+        L.mx_eval.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
+        L.mx_eval.restype = ctypes.c_int
+        L.mx_plan_skeletons.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
+        L.mx_plan_skeletons.restype = ctypes.c_int
+        L.mx_plan_copy_skeletons.argtypes = [P, P, P, I32, P]
+        L.mx_plan_copy_skeletons.restype = ctypes.c_int
+        L.mx_plan_dims.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]
+        L.mx_plan_dims.restype = ctypes.c_int
+        _configured = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise MxError(rc, lib().mx_last_error().decode())
+
+
+def _np(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def structure_of(hm):
+    """mx_structure over an HMatrix; returns (struct, keepalive list)."""
+    a = hm.arrays
+    keep = []
+
+    def ptr(name, dt):
+        arr = _np(a[name], dt)
+        keep.append(arr)
+        return arr.ctypes.data
+
+    s = MxStructure()
+    s.nleaf = int(hm.scalars["nleaf"])
+    s.numnodes = int(hm.scalars["numnodes"])
+    s.is_hss = int(hm.scalars["is_hss"])
+    s.ncount = len(a["npairx"])
+    s.fcount = len(a["fpairx"])
+    for f in ("lchildren", "rchildren", "lm", "Ddim", "slen", "npairx", "npairy", "fpairx", "fpairy", "VTptr",
+              "wrow", "skelrow"):
+        setattr(s, f, ptr(f, np.int32))
+    s.Dptr = ptr("Dptr", np.uint64)
+    s.Bptr = ptr("Bptr", np.uint64)
+    s.D = ptr("D", np.float64)
+    s.B = ptr("B", np.float64)
+    s.VT = ptr("VT", np.float64)
+    s.D_len, s.B_len, s.VT_len = len(a["D"]), len(a["B"]), len(a["VT"])
+    return s, keep
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Plan:
+    """Device-resident compressed matrix (mx_plan)."""
+
+    def __init__(self, hm, precisions=("fp64",)):
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISIONS[p]
+        s, keep = structure_of(hm)
+        h = P()
+        _check(lib().mx_plan_create(ctypes.byref(h), ctypes.byref(s), mask))
+        del keep
+        self._h = h
+        self.N = int(hm.scalars["n"])
+        n, sk = I64(), I64()
+        _check(lib().mx_plan_dims(h, ctypes.byref(n), ctypes.byref(sk)))
+        self.rows = n.value
+        self.skel_rows = sk.value
+        self.precisions = tuple(precisions)
+
+    def eval(self, W, Y, nrhs, precision="fp64", stream=None):
+        """Y = K~ W; W and Y are contiguous CUDA tensors of N*nrhs elements
+        (leaf-blocked layout), float64 or float32 (io type follows W)."""
+        import torch
+        assert W.is_cuda and Y.is_cuda and W.is_contiguous() and Y.is_contiguous()
+        assert W.numel() == self.rows * nrhs and Y.numel() == self.rows * nrhs
+        assert W.dtype == Y.dtype and W.dtype in (torch.float64, torch.float32)
+        io = IO_F64 if W.dtype == torch.float64 else IO_F32
+        _check(lib().mx_eval(self._h, PRECISIONS[precision], W.data_ptr(), Y.data_ptr(), nrhs, io,
+                             _stream_handle(stream)))
+        return Y
+
+    def skeletons(self, nrhs, dtype, stream=None):
+        """Copies of (wskel, uskel) of the last evaluation as CUDA tensors."""
+        import torch
+        ws = torch.empty(self.skel_rows * nrhs, dtype=dtype, device="cuda")
+        us = torch.empty_like(ws)
+        _check(lib().mx_plan_copy_skeletons(self._h, ws.data_ptr(), us.data_ptr(), nrhs, _stream_handle(stream)))
+        return ws, us
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def configure(nleaf, nl, fl, cdepth):
+    """mx_hgemm_configure: the loop bounds the generated HGEMM bakes in."""
+    _check(lib().mx_hgemm_configure(nleaf, nl, fl, cdepth))
+
+
+def _hp(x):
+    return x.ctypes.data
+
+
+def hgemm(hm, TW, nrhs, apres=None, uskel=None):
+    """Drop-in H2-b HGEMM on host arrays; returns (apres, wskel, uskel)."""
+    a = hm.arrays
+    configure(int(hm.scalars["nleaf"]), int(hm.scalars["nl"]), int(hm.scalars["fl"]), int(hm.scalars["cdepth"]))
+    N = hm.N
+    sk = int(hm.scalars["skel_rows"]) * nrhs
+    TW = _np(TW, np.float64)
+    Y = np.zeros(N * nrhs) if apres is None else _np(apres, np.float64).copy()
+    ws = np.zeros(max(sk, 1))
+    us = np.zeros(max(sk, 1)) if uskel is None else _np(uskel, np.float64).copy()
+    wptr = hm.wptr(nrhs)
+    so = hm.skeloffset(nrhs)
+    arrs = {k: _np(a[k], np.int32) for k in ("VTptr", "lchildren", "rchildren", "levelset", "idx", "Ddim", "lm",
+                                              "slen", "nblockSet", "nblocks", "npairx", "npairy", "fblockSet",
+                                              "fblocks", "fpairx", "fpairy", "wpart", "clevelset")}
+    D, B, VT = _np(a["D"], np.float64), _np(a["B"], np.float64), _np(a["VT"], np.float64)
+    Dptr, Bptr = _np(a["Dptr"], np.uint64), _np(a["Bptr"], np.uint64)
+    rc = lib().HGEMM(_hp(D), _hp(B), _hp(VT), _hp(Dptr), _hp(Bptr), _hp(arrs["VTptr"]), _hp(arrs["lchildren"]),
+                     _hp(arrs["rchildren"]), _hp(arrs["levelset"]), _hp(arrs["idx"]), _hp(TW), _hp(Y), nrhs,
+                     _hp(arrs["Ddim"]), _hp(wptr), _hp(wptr), _hp(ws), _hp(so), _hp(us), _hp(so), _hp(arrs["lm"]),
+                     _hp(arrs["slen"]), _hp(arrs["nblockSet"]), _hp(arrs["nblocks"]), _hp(arrs["npairx"]),
+                     _hp(arrs["npairy"]), _hp(arrs["fblockSet"]), _hp(arrs["fblocks"]), _hp(arrs["fpairx"]),
+                     _hp(arrs["fpairy"]), _hp(arrs["wpart"]), _hp(arrs["clevelset"]))
+    _check(rc)
+    return Y, ws[:sk], us[:sk]
+
+
+def hgemm_hss(hm, TW, nrhs, apres=None, uskel=None):
+    """Drop-in HSS HGEMM (HGEMM_v1.h signature) on host arrays."""
+    a = hm.arrays
+    configure(int(hm.scalars["nleaf"]), 0, 0, int(hm.scalars["cdepth"]))
+    N = hm.N
+    sk = int(hm.scalars["skel_rows"]) * nrhs
+    TW = _np(TW, np.float64)
+    Y = np.zeros(N * nrhs) if apres is None else _np(apres, np.float64).copy()
+    ws = np.zeros(max(sk, 1))
+    us = np.zeros(max(sk, 1)) if uskel is None else _np(uskel, np.float64).copy()
+    wptr = hm.wptr(nrhs)
+    so = hm.skeloffset(nrhs)
+    arrs = {k: _np(a[k], np.int32) for k in ("VTptr", "lchildren", "rchildren", "levelset", "idx", "Ddim", "lm",
+                                              "slen", "wpart", "clevelset")}
+    D, B, VT = _np(a["D"], np.float64), _np(a["B"], np.float64), _np(a["VT"], np.float64)
+    Dptr, Bptr = _np(a["Dptr"], np.uint64), _np(a["Bptr"], np.uint64)
+    rc = lib().HGEMM_HSS(_hp(D), _hp(B), _hp(VT), _hp(Dptr), _hp(Bptr), _hp(arrs["VTptr"]),
+                         _hp(arrs["lchildren"]), _hp(arrs["rchildren"]), _hp(arrs["levelset"]), _hp(arrs["idx"]),
+                         _hp(TW), _hp(Y), nrhs, _hp(arrs["Ddim"]), _hp(wptr), _hp(wptr), _hp(ws), _hp(so), _hp(us),
+                         _hp(so), _hp(arrs["lm"]), _hp(arrs["slen"]), _hp(arrs["wpart"]), _hp(arrs["clevelset"]))
+    _check(rc)
+    return Y, ws[:sk], us[:sk]

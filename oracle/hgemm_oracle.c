@@ -1,0 +1,373 @@
+/*
+ * oracle/hgemm_oracle.c -- CPU restatement of the MatRox evaluation phase.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Only the tests, the smoke check and the CPU
+ * baseline leg of the benchmark may load this library, and only as the
+ * checker (or the timed CPU baseline); the product path never calls it.
+ *
+ * Contents (each function cites the reference code it restates):
+ *   orc_hgemm_h2      symGen/HGEMM_gen.h:117-238      H2-b evaluation (blocked near/far,
+ *                                                    coarsened up/down sweeps)
+ *   orc_hgemm_hss     symGen/HGEMM_v1.h:117-405       HSS evaluation (diagonal near, sibling far)
+ *   orc_dense_rows    matroxTest-V1/Util.h:182-215,303-308  dense K(rows,:) W, direct distances
+ *   orc_acc           matroxTest-V1/Util.h:276-332    the `acc` column (computeError)
+ *   orc_flops         matroxTest-V1/compression.h:1079-1130  evaluation flop count
+ *   orc_assemble      (no reference counterpart)       dense K~ from the blocks through
+ *                                                    explicit nested bases: an independent
+ *                                                    route used to pin orc_hgemm_h2
+ *
+ * The dense products are FP64 column-major GEMMs written out here (alpha = 1,
+ * beta in {0,1}, A optionally transposed), the operation the reference hands
+ * to MKL cblas_dgemm.  Parallel loops follow the reference's OpenMP loops,
+ * which are race-free by construction of the row-blocking.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* C (m x n, ldc) = beta*C + op(A) * B ; op(A) = A (m x k, lda) or A^T (A is k x m, lda). */
+static void gemm(int transA, int m, int n, int k, const double *A, int lda, const double *Bm, int ldb,
+                 double beta, double *C, int ldc) {
+    if (m <= 0 || n <= 0) return;
+    if (beta == 0.0) {
+        for (int j = 0; j < n; ++j) memset(C + (size_t)j * ldc, 0, sizeof(double) * (size_t)m);
+    }
+    if (k <= 0) return;
+    if (!transA) {
+        int j = 0;
+        for (; j + 4 <= n; j += 4) {
+            double *c0 = C + (size_t)j * ldc, *c1 = c0 + ldc, *c2 = c1 + ldc, *c3 = c2 + ldc;
+            const double *b0 = Bm + (size_t)j * ldb, *b1 = b0 + ldb, *b2 = b1 + ldb, *b3 = b2 + ldb;
+            for (int p = 0; p < k; ++p) {
+                const double *a = A + (size_t)p * lda;
+                const double x0 = b0[p], x1 = b1[p], x2 = b2[p], x3 = b3[p];
+                for (int i = 0; i < m; ++i) {
+                    const double ai = a[i];
+                    c0[i] += ai * x0;
+                    c1[i] += ai * x1;
+                    c2[i] += ai * x2;
+                    c3[i] += ai * x3;
+                }
+            }
+        }
+        for (; j < n; ++j) {
+            double *c = C + (size_t)j * ldc;
+            const double *b = Bm + (size_t)j * ldb;
+            for (int p = 0; p < k; ++p) {
+                const double *a = A + (size_t)p * lda;
+                const double x = b[p];
+                for (int i = 0; i < m; ++i) c[i] += a[i] * x;
+            }
+        }
+    } else {
+        for (int j = 0; j < n; ++j) {
+            const double *b = Bm + (size_t)j * ldb;
+            double *c = C + (size_t)j * ldc;
+            for (int i = 0; i < m; ++i) {
+                const double *a = A + (size_t)i * lda;
+                double s = 0.0;
+                for (int p = 0; p < k; ++p) s += a[p] * b[p];
+                c[i] += s;
+            }
+        }
+    }
+}
+
+/* Upward step for node v (HGEMM_gen.h:150-176): leaf -> w_v = VT_v W_leaf;
+ * internal -> w_v = VT_v[:, :s_l] w_l + VT_v[:, s_l:] w_r. */
+static void up_node(int v, const double *VT, const int32_t *VTptr, const int32_t *lc, const int32_t *rc,
+                    const double *mrhs, int nrhs, const int32_t *Ddim, const int32_t *wptr, double *wskel,
+                    const int32_t *wso, const int32_t *lm, const int32_t *slen) {
+    if (lc[v] == -1) {
+        gemm(0, slen[v], nrhs, Ddim[lm[v]], VT + VTptr[v], slen[v], mrhs + wptr[lm[v]], Ddim[lm[v]], 0.0,
+             wskel + wso[v], slen[v]);
+    } else {
+        int l = lc[v], r = rc[v];
+        gemm(0, slen[v], nrhs, slen[l], VT + VTptr[v], slen[v], wskel + wso[l], slen[l], 0.0, wskel + wso[v],
+             slen[v]);
+        gemm(0, slen[v], nrhs, slen[r], VT + VTptr[v] + (size_t)slen[v] * slen[l], slen[v], wskel + wso[r],
+             slen[r], 1.0, wskel + wso[v], slen[v]);
+    }
+}
+
+/* Downward step for node v (HGEMM_gen.h:208-234): leaf -> Y_leaf += VT_v^T u_v;
+ * internal -> u_l += VT_v[:, :s_l]^T u_v, u_r += VT_v[:, s_l:]^T u_v. */
+static void down_node(int v, const double *VT, const int32_t *VTptr, const int32_t *lc, const int32_t *rc,
+                      double *apres, int nrhs, const int32_t *Ddim, const int32_t *uptr, double *uskel,
+                      const int32_t *uso, const int32_t *lm, const int32_t *slen) {
+    if (lc[v] == -1) {
+        gemm(1, Ddim[lm[v]], nrhs, slen[v], VT + VTptr[v], slen[v], uskel + uso[v], slen[v], 1.0,
+             apres + uptr[lm[v]], Ddim[lm[v]]);
+    } else {
+        int l = lc[v], r = rc[v];
+        gemm(1, slen[l], nrhs, slen[v], VT + VTptr[v], slen[v], uskel + uso[v], slen[v], 1.0, uskel + uso[l],
+             slen[l]);
+        gemm(1, slen[r], nrhs, slen[v], VT + VTptr[v] + (size_t)slen[v] * slen[l], slen[v], uskel + uso[v],
+             slen[v], 1.0, uskel + uso[r], slen[r]);
+    }
+}
+
+static void up_sweep(const double *VT, const int32_t *VTptr, const int32_t *lc, const int32_t *rc,
+                     const int32_t *idx, const double *mrhs, int nrhs, const int32_t *Ddim, const int32_t *wptr,
+                     double *wskel, const int32_t *wso, const int32_t *lm, const int32_t *slen,
+                     const int32_t *wpart, const int32_t *clevelset, int cdepth) {
+    for (int c = 0; c < cdepth; ++c) {
+#pragma omp parallel for schedule(dynamic)
+        for (int p = clevelset[c]; p < clevelset[c + 1]; ++p)
+            for (int j = wpart[p]; j < wpart[p + 1]; ++j)
+                up_node(idx[j], VT, VTptr, lc, rc, mrhs, nrhs, Ddim, wptr, wskel, wso, lm, slen);
+    }
+}
+
+static void down_sweep(const double *VT, const int32_t *VTptr, const int32_t *lc, const int32_t *rc,
+                       const int32_t *idx, double *apres, int nrhs, const int32_t *Ddim, const int32_t *uptr,
+                       double *uskel, const int32_t *uso, const int32_t *lm, const int32_t *slen,
+                       const int32_t *wpart, const int32_t *clevelset, int cdepth) {
+    for (int c = cdepth - 1; c >= 0; --c) {
+#pragma omp parallel for schedule(dynamic)
+        for (int p = clevelset[c]; p < clevelset[c + 1]; ++p)
+            for (int j = wpart[p + 1] - 1; j >= wpart[p]; --j)
+                down_node(idx[j], VT, VTptr, lc, rc, apres, nrhs, Ddim, uptr, uskel, uso, lm, slen);
+    }
+}
+
+/* H2-b evaluation, symGen/HGEMM_gen.h:117-238.  The generated code bakes nl,
+ * fl and cdepth into its loop bounds; here they are arguments.  apres and
+ * uskel accumulate (beta = 1), wskel is overwritten. */
+int32_t orc_hgemm_h2(const double *D, const double *B, const double *VT, const uint64_t *Dptr,
+                     const uint64_t *Bptr, const int32_t *VTptr, const int32_t *lchildren,
+                     const int32_t *rchildren, const int32_t *levelset, const int32_t *idx,
+                     const double *mrhs, double *apres, int32_t nrhs, const int32_t *Ddim,
+                     const int32_t *wptr, const int32_t *uptr, double *wskel, const int32_t *wskeloffset,
+                     double *uskel, const int32_t *uskeloffset, const int32_t *lm, const int32_t *slen,
+                     const int32_t *nblockSet, const int32_t *nblocks, const int32_t *npairx,
+                     const int32_t *npairy, const int32_t *fblockSet, const int32_t *fblocks,
+                     const int32_t *fpairx, const int32_t *fpairy, const int32_t *wpart,
+                     const int32_t *clevelset, int32_t nl, int32_t fl, int32_t cdepth) {
+    (void)levelset;
+    /* NEAR (:125-141) */
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < nl; ++i)
+        for (int j = nblockSet[i]; j < nblockSet[i + 1]; ++j)
+            for (int k = nblocks[j]; k < nblocks[j + 1]; ++k) {
+                int x = npairx[k], y = npairy[k];
+                gemm(0, Ddim[x], nrhs, Ddim[y], D + Dptr[k], Ddim[x], mrhs + wptr[y], Ddim[y], 1.0,
+                     apres + uptr[x], Ddim[x]);
+            }
+    /* UP (:142-178) */
+    up_sweep(VT, VTptr, lchildren, rchildren, idx, mrhs, nrhs, Ddim, wptr, wskel, wskeloffset, lm, slen, wpart,
+             clevelset, cdepth);
+    /* FAR (:179-195) */
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < fl; ++i)
+        for (int j = fblockSet[i]; j < fblockSet[i + 1]; ++j)
+            for (int k = fblocks[j]; k < fblocks[j + 1]; ++k) {
+                int x = fpairx[k], y = fpairy[k];
+                gemm(0, slen[x], nrhs, slen[y], B + Bptr[k], slen[x], wskel + wskeloffset[y], slen[y], 1.0,
+                     uskel + uskeloffset[x], slen[x]);
+            }
+    /* DOWN (:196-236) */
+    down_sweep(VT, VTptr, lchildren, rchildren, idx, apres, nrhs, Ddim, uptr, uskel, uskeloffset, lm, slen, wpart,
+               clevelset, cdepth);
+    return 0;
+}
+
+/* HSS evaluation, symGen/HGEMM_v1.h:117-405: near is the diagonal block of
+ * each leaf with beta = 0 (:125-133); far couples node i with its sibling
+ * (i even -> i-1, odd -> i+1) through B[Bptr[i-1]] with beta = 0 (:254-271). */
+int32_t orc_hgemm_hss(const double *D, const double *B, const double *VT, const uint64_t *Dptr,
+                      const uint64_t *Bptr, const int32_t *VTptr, const int32_t *lchildren,
+                      const int32_t *rchildren, const int32_t *levelset, const int32_t *idx,
+                      const double *mrhs, double *apres, int32_t nrhs, const int32_t *Ddim,
+                      const int32_t *wptr, const int32_t *uptr, double *wskel, const int32_t *wskeloffset,
+                      double *uskel, const int32_t *uskeloffset, const int32_t *lm, const int32_t *slen,
+                      const int32_t *wpart, const int32_t *clevelset, int32_t nleaf, int32_t numnodes,
+                      int32_t cdepth) {
+    (void)levelset;
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < nleaf; ++i)
+        gemm(0, Ddim[i], nrhs, Ddim[i], D + Dptr[i], Ddim[i], mrhs + wptr[i], Ddim[i], 0.0, apres + uptr[i],
+             Ddim[i]);
+    up_sweep(VT, VTptr, lchildren, rchildren, idx, mrhs, nrhs, Ddim, wptr, wskel, wskeloffset, lm, slen, wpart,
+             clevelset, cdepth);
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 1; i < numnodes; ++i) {
+        int sib = (i & 1) == 0 ? i - 1 : i + 1;
+        gemm(0, slen[i], nrhs, slen[sib], B + Bptr[i - 1], slen[i], wskel + wskeloffset[sib], slen[sib], 0.0,
+             uskel + uskeloffset[i], slen[i]);
+    }
+    down_sweep(VT, VTptr, lchildren, rchildren, idx, apres, nrhs, Ddim, uptr, uskel, uskeloffset, lm, slen, wpart,
+               clevelset, cdepth);
+    return 0;
+}
+
+/* Dense K(rows, :) W with directly computed squared distances
+ * (Fsubmatrix of matroxTest-V1/Util.h:182-215 and the dgemm of :303-308).
+ * W is n x nrhs column-major in original point order; out is nrows x nrhs. */
+int orc_dense_rows(const double *X, int32_t n, int32_t d, double h, const int32_t *rows, int32_t nrows,
+                   const double *W, int32_t nrhs, double *out) {
+    if (!X || !rows || !W || !out || n < 1 || d < 1 || nrows < 0 || nrhs < 0) return -1;
+    const double s = 1.0 / (2.0 * h * h);
+    memset(out, 0, sizeof(double) * (size_t)nrows * nrhs);
+#pragma omp parallel
+    {
+        double *krow = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(dynamic)
+        for (int r = 0; r < nrows; ++r) {
+            const double *a = X + (size_t)rows[r] * d;
+            for (int j = 0; j < n; ++j) {
+                const double *b = X + (size_t)j * d;
+                double k2 = 0.0;
+                for (int p = 0; p < d; ++p) {
+                    double t = b[p] - a[p];
+                    k2 += t * t;
+                }
+                krow[j] = exp(-k2 * s);
+            }
+            for (int c = 0; c < nrhs; ++c) {
+                const double *w = W + (size_t)c * n;
+                double acc = 0.0;
+                for (int j = 0; j < n; ++j) acc += krow[j] * w[j];
+                out[(size_t)c * nrows + r] = acc;
+            }
+        }
+        free(krow);
+    }
+    return 0;
+}
+
+/* computeError (matroxTest-V1/Util.h:276-332): the first ntest (= 10) points
+ * of leaf 0 (lids[0 .. len), len = Dim[0]); mean over those rows of
+ * |y_row - u_row|_2 / |y_row|_2, where U is the leaf-blocked output whose
+ * block 0 has leading dimension len.  The reference fixes h = 5 (Util.h:301);
+ * h is an argument here. */
+double orc_acc(const int32_t *lids, int32_t len, int32_t nrhs, int32_t n, const double *X, int32_t d,
+               double h, const double *W, const double *U, int32_t ntest) {
+    if (ntest > len) ntest = len;
+    if (ntest <= 0) return 0.0;
+    double *res = (double *)malloc(sizeof(double) * (size_t)ntest * nrhs);
+    orc_dense_rows(X, n, d, h, lids, ntest, W, nrhs, res);
+    double av = 0.0;
+    for (int i = 0; i < ntest; ++i) {
+        double e = 0.0, nr = 0.0;
+        for (int j = 0; j < nrhs; ++j) {
+            double y = res[(size_t)j * ntest + i], u = U[(size_t)j * len + i];
+            e += (y - u) * (y - u);
+            nr += y * y;
+        }
+        av += sqrt(e) / sqrt(nr);
+    }
+    free(res);
+    return av / ntest;
+}
+
+/* computeFlops (matroxTest-V1/compression.h:1079-1130), computed from the
+ * structure arrays: 2 * (sum_leaf 2 s Dim n + sum_internal 2 s (s_l + s_r) n)
+ * + sum_near 2 Dx Dy n + sum_far 2 sx sy n, root excluded.  Leaves are the
+ * nodes with lchildren == -1. */
+double orc_flops(int32_t numnodes, const int32_t *lchildren, const int32_t *rchildren, const int32_t *lm,
+                 const int32_t *Ddim, const int32_t *slen, int64_t ncount, const int32_t *npairx,
+                 const int32_t *npairy, int64_t fcount, const int32_t *fpairx, const int32_t *fpairy,
+                 int32_t nrhs) {
+    double tree = 0.0, inter = 0.0;
+    for (int v = 1; v < numnodes; ++v) {
+        if (lchildren[v] == -1) tree += 2.0 * slen[v] * (double)nrhs * Ddim[lm[v]];
+        else tree += 2.0 * slen[v] * (double)(slen[lchildren[v]] + slen[rchildren[v]]) * nrhs;
+    }
+    for (int64_t k = 0; k < ncount; ++k) inter += 2.0 * Ddim[npairx[k]] * (double)Ddim[npairy[k]] * nrhs;
+    for (int64_t k = 0; k < fcount; ++k) inter += 2.0 * slen[fpairx[k]] * (double)slen[fpairy[k]] * nrhs;
+    return 2.0 * tree + inter;
+}
+
+/* Dense K~ (N x N, leaf-blocked row/column order, column-major) assembled
+ * from the blocks: near pair (x, y) places D_k at rows(x) x rows(y); far pair
+ * (x, y) adds U_x B_k U_y^T, where the nested basis is U_leaf = VT_leaf^T and
+ * U_v = [U_l VT_v[:, :s_l]^T ; U_r VT_v[:, s_l:]^T].  This uses no sweep and
+ * no skeleton buffers, so it checks orc_hgemm_h2 from a different route.
+ * rowoff[l] is the first row of leaf l (wptr / nrhs). */
+int orc_assemble(int32_t N, int32_t numnodes, const int32_t *lchildren, const int32_t *rchildren,
+                 const int32_t *lm, const int32_t *Ddim, const int32_t *rowoff, const int32_t *slen,
+                 const double *D, const uint64_t *Dptr, int64_t ncount, const int32_t *npairx,
+                 const int32_t *npairy, const double *B, const uint64_t *Bptr, int64_t fcount,
+                 const int32_t *fpairx, const int32_t *fpairy, const double *VT, const int32_t *VTptr,
+                 double *K) {
+    memset(K, 0, sizeof(double) * (size_t)N * N);
+    /* rows of every node (leaf order within) and its basis U_v (rows x s_v) */
+    int32_t **rows = (int32_t **)calloc(numnodes, sizeof(int32_t *));
+    int32_t *nrows = (int32_t *)calloc(numnodes, sizeof(int32_t));
+    double **U = (double **)calloc(numnodes, sizeof(double *));
+    /* post-order over node ids: children have larger ids in BFS numbering, so
+     * walking ids downward visits children before parents */
+    for (int v = numnodes - 1; v >= 0; --v) {
+        if (lchildren[v] == -1) {
+            int l = lm[v];
+            nrows[v] = Ddim[l];
+            rows[v] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(Ddim[l] > 0 ? Ddim[l] : 1));
+            for (int i = 0; i < Ddim[l]; ++i) rows[v][i] = rowoff[l] + i;
+            if (v > 0) {
+                int s = slen[v];
+                U[v] = (double *)calloc((size_t)nrows[v] * (s > 0 ? s : 1), sizeof(double));
+                /* U = VT^T : (Dim x s), VT is s x Dim col-major */
+                for (int j = 0; j < s; ++j)
+                    for (int i = 0; i < nrows[v]; ++i) U[v][(size_t)j * nrows[v] + i] = VT[VTptr[v] + (size_t)i * s + j];
+            }
+        } else {
+            int l = lchildren[v], r = rchildren[v];
+            nrows[v] = nrows[l] + nrows[r];
+            rows[v] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nrows[v] > 0 ? nrows[v] : 1));
+            memcpy(rows[v], rows[l], sizeof(int32_t) * (size_t)nrows[l]);
+            memcpy(rows[v] + nrows[l], rows[r], sizeof(int32_t) * (size_t)nrows[r]);
+            if (v > 0) {
+                int s = slen[v], sl = slen[l], sr = slen[r];
+                U[v] = (double *)calloc((size_t)nrows[v] * (s > 0 ? s : 1), sizeof(double));
+                /* U_v = [U_l T[:sl, :]; U_r T[sl:, :]] with T = VT_v^T ((sl+sr) x s);
+                 * VT_v is s x (sl+sr) col-major, so T(p, j) = VT[p*s + j] */
+                double *T = (double *)malloc(sizeof(double) * (size_t)(sl + sr) * (s > 0 ? s : 1));
+                for (int j = 0; j < s; ++j)
+                    for (int p = 0; p < sl + sr; ++p) T[(size_t)j * (sl + sr) + p] = VT[VTptr[v] + (size_t)p * s + j];
+                gemm(0, nrows[l], s, sl, U[l], nrows[l], T, sl + sr, 0.0, U[v], nrows[v]);
+                gemm(0, nrows[r], s, sr, U[r], nrows[r], T + sl, sl + sr, 0.0, U[v] + nrows[l], nrows[v]);
+                free(T);
+            }
+        }
+    }
+    for (int64_t k = 0; k < ncount; ++k) {
+        int x = npairx[k], y = npairy[k];
+        const double *Dk = D + Dptr[k];
+        for (int j = 0; j < Ddim[y]; ++j)
+            for (int i = 0; i < Ddim[x]; ++i)
+                K[(size_t)(rowoff[y] + j) * N + rowoff[x] + i] += Dk[(size_t)j * Ddim[x] + i];
+    }
+    for (int64_t k = 0; k < fcount; ++k) {
+        int x = fpairx[k], y = fpairy[k];
+        int sx = slen[x], sy = slen[y];
+        if (sx == 0 || sy == 0) continue;
+        /* T1 = U_x B (nx x sy); block = T1 U_y^T (nx x ny) */
+        double *T1 = (double *)malloc(sizeof(double) * (size_t)nrows[x] * sy);
+        gemm(0, nrows[x], sy, sx, U[x], nrows[x], B + Bptr[k], sx, 0.0, T1, nrows[x]);
+        for (int j = 0; j < nrows[y]; ++j) {
+            int col = rows[y][j];
+            for (int i = 0; i < nrows[x]; ++i) {
+                double s = 0.0;
+                for (int p = 0; p < sy; ++p) s += T1[(size_t)p * nrows[x] + i] * U[y][(size_t)p * nrows[y] + j];
+                K[(size_t)col * N + rows[x][i]] += s;
+            }
+        }
+        free(T1);
+    }
+    for (int v = 0; v < numnodes; ++v) {
+        free(rows[v]);
+        free(U[v]);
+    }
+    free(rows);
+    free(nrows);
+    free(U);
+    return 0;
+}
+
+/* Plain column-major GEMM exported for the checks of single stages. */
+void orc_gemm(int32_t transA, int32_t m, int32_t n, int32_t k, const double *A, int32_t lda, const double *Bm,
+              int32_t ldb, double beta, double *C, int32_t ldc) {
+    gemm(transA, m, n, k, A, lda, Bm, ldb, beta, C, ldc);
+}

@@ -1,0 +1,170 @@
+"""ctypes view of the CPU oracle (oracle/hgemm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, always as the checker or the timed CPU
+baseline, never as part of the product path.
+
+Functions take an ``HMatrix`` (matrox_ru_amd.host) and numpy arrays and call
+the C restatements:
+  hgemm      -> orc_hgemm_h2 / orc_hgemm_hss  (symGen/HGEMM_gen.h:117-238, HGEMM_v1.h:117-405)
+  dense_rows -> orc_dense_rows                (matroxTest-V1/Util.h:182-215, 303-308)
+  acc        -> orc_acc                       (matroxTest-V1/Util.h:276-332)
+  flops      -> orc_flops                     (matroxTest-V1/compression.h:1079-1130)
+  assemble   -> orc_assemble                  (dense K~ through nested bases)
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int32
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB):
+        build()
+    L = ctypes.CDLL(_LIB)
+    L.orc_hgemm_h2.restype = I32
+    L.orc_hgemm_h2.argtypes = [P] * 12 + [I32] + [P] * 19 + [I32, I32, I32]
+    L.orc_hgemm_hss.restype = I32
+    L.orc_hgemm_hss.argtypes = [P] * 12 + [I32] + [P] * 11 + [I32, I32, I32]
+    L.orc_dense_rows.restype = ctypes.c_int
+    L.orc_dense_rows.argtypes = [P, I32, I32, F64, P, I32, P, I32, P]
+    L.orc_acc.restype = F64
+    L.orc_acc.argtypes = [P, I32, I32, I32, P, I32, F64, P, P, I32]
+    L.orc_flops.restype = F64
+    L.orc_flops.argtypes = [I32, P, P, P, P, P, I64, P, P, I64, P, P, I32]
+    L.orc_assemble.restype = ctypes.c_int
+    L.orc_assemble.argtypes = [I32, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, I64, P, P, P, P, P]
+    L.orc_gemm.restype = None
+    L.orc_gemm.argtypes = [I32, I32, I32, I32, P, I32, P, I32, F64, P, I32]
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def hgemm(hm, TW, nrhs, apres=None, uskel=None, variant=None):
+    """Run the restated reference evaluation on leaf-blocked TW (flat, N*nrhs).
+
+    variant: "h2" (HGEMM_gen.h) or "hss" (HGEMM_v1.h); default picks "hss" for
+    an HSS structure.  apres/uskel are the accumulators (zero if None), as the
+    reference driver pre-zeroes them (main.cpp:47-48, hcds.h:389-392).
+    Returns (apres, wskel, uskel).
+    """
+    L = lib()
+    a = hm.arrays
+    N = hm.N
+    if variant is None:
+        variant = "hss" if hm.scalars["is_hss"] else "h2"
+    TW = _c(TW, np.float64)
+    assert TW.size == N * nrhs
+    Y = np.zeros(N * nrhs) if apres is None else _c(apres, np.float64).copy()
+    sk = int(hm.scalars["skel_rows"]) * nrhs
+    wskel = np.zeros(max(sk, 1))
+    us = np.zeros(max(sk, 1)) if uskel is None else _c(uskel, np.float64).copy()
+    wptr = hm.wptr(nrhs)
+    so = hm.skeloffset(nrhs)
+    keep = [wptr, so]
+    common = [_p(a["D"]), _p(a["B"]), _p(a["VT"]), _p(a["Dptr"]), _p(a["Bptr"]), _p(a["VTptr"]),
+              _p(a["lchildren"]), _p(a["rchildren"]), _p(a["levelset"]), _p(a["idx"]), _p(TW), _p(Y)]
+    mid = [_p(a["Ddim"]), _p(wptr), _p(wptr), _p(wskel), _p(so), _p(us), _p(so), _p(a["lm"]), _p(a["slen"])]
+    if variant == "h2":
+        rc = L.orc_hgemm_h2(*common, nrhs, *mid,
+                            _p(a["nblockSet"]), _p(a["nblocks"]), _p(a["npairx"]), _p(a["npairy"]),
+                            _p(a["fblockSet"]), _p(a["fblocks"]), _p(a["fpairx"]), _p(a["fpairy"]),
+                            _p(a["wpart"]), _p(a["clevelset"]),
+                            int(hm.scalars["nl"]), int(hm.scalars["fl"]), int(hm.scalars["cdepth"]))
+    elif variant == "hss":
+        rc = L.orc_hgemm_hss(*common, nrhs, *mid, _p(a["wpart"]), _p(a["clevelset"]),
+                             int(hm.scalars["nleaf"]), int(hm.scalars["numnodes"]),
+                             int(hm.scalars["cdepth"]))
+    else:
+        raise ValueError(variant)
+    del keep
+    assert rc == 0
+    return Y, wskel[:sk], us[:sk]
+
+
+def dense_rows(X, h, rows, W):
+    """Dense K(rows, :) W; W is N x nrhs (any order), returns nrows x nrhs."""
+    X = _c(X, np.float64)
+    n, d = X.shape
+    rows = _c(rows, np.int32)
+    Wf = np.asfortranarray(W, dtype=np.float64)
+    nrhs = Wf.shape[1]
+    out = np.empty((len(rows), nrhs), dtype=np.float64, order="F")
+    rc = lib().orc_dense_rows(_p(X), n, d, h, _p(rows), len(rows), Wf.ctypes.data, nrhs, out.ctypes.data)
+    assert rc == 0
+    return out
+
+
+def acc(hm, X, h, W, Y, ntest=10):
+    """computeError on the leaf-blocked output Y (flat) against dense K W."""
+    X = _c(X, np.float64)
+    n, d = X.shape
+    Wf = np.asfortranarray(W, dtype=np.float64)
+    nrhs = Wf.shape[1]
+    Y = _c(Y, np.float64)
+    lids = hm.arrays["lids"]
+    return lib().orc_acc(_p(lids), int(hm.arrays["Ddim"][0]), nrhs, n, _p(X), d, h, Wf.ctypes.data, _p(Y),
+                         ntest)
+
+
+def flops(hm, nrhs):
+    a = hm.arrays
+    return lib().orc_flops(int(hm.scalars["numnodes"]), _p(a["lchildren"]), _p(a["rchildren"]), _p(a["lm"]),
+                           _p(a["Ddim"]), _p(a["slen"]), len(a["npairx"]), _p(a["npairx"]), _p(a["npairy"]),
+                           len(a["fpairx"]), _p(a["fpairx"]), _p(a["fpairy"]), nrhs)
+
+
+def assemble(hm):
+    """Dense K~ (N x N, leaf-blocked order, Fortran) from the blocks."""
+    a = hm.arrays
+    N = hm.N
+    K = np.empty((N, N), dtype=np.float64, order="F")
+    rowoff = _c(a["wrow"], np.int32)
+    rc = lib().orc_assemble(N, int(hm.scalars["numnodes"]), _p(a["lchildren"]), _p(a["rchildren"]), _p(a["lm"]),
+                            _p(a["Ddim"]), _p(rowoff), _p(a["slen"]), _p(a["D"]), _p(a["Dptr"]),
+                            len(a["npairx"]), _p(a["npairx"]), _p(a["npairy"]), _p(a["B"]), _p(a["Bptr"]),
+                            len(a["fpairx"]), _p(a["fpairx"]), _p(a["fpairy"]), _p(a["VT"]), _p(a["VTptr"]),
+                            K.ctypes.data)
+    assert rc == 0
+    return K
+
+
+def gemm(A, Bm, transA=False, C=None, beta=0.0):
+    """Column-major GEMM restated in C (single-stage checks)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    Bm = np.asfortranarray(Bm, dtype=np.float64)
+    m = A.shape[1] if transA else A.shape[0]
+    k = A.shape[0] if transA else A.shape[1]
+    n = Bm.shape[1]
+    if C is None:
+        C = np.zeros((m, n), order="F")
+    else:
+        C = np.array(C, dtype=np.float64, order="F", copy=True)
+    lib().orc_gemm(int(transA), m, n, k, A.ctypes.data, A.shape[0], Bm.ctypes.data, Bm.shape[0], beta,
+                   C.ctypes.data, m)
+    return C

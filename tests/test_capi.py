@@ -1,0 +1,65 @@
+"""The C-ABI libraries load on a CPU host and export every function their
+headers declare (no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from matrox_ru_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+HEADERS = {
+    "matrox_host.h": _native.HOST_LIB,
+    "matrox_hgemm.h": _native.HGEMM_LIB,
+}
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text):
+        name = m.group(1)
+        start = text.rfind("\n", 0, m.start()) + 1
+        line = text[start:m.start()]
+        if name in ("defined", "sizeof", "if", "while", "for", "return") or line.strip().startswith("#"):
+            continue
+        # a declaration: return type (and optional pointer) precedes the name on the line
+        if re.match(r"^\s*(const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*$", line):
+            names.add(name)
+    return names
+
+
+@pytest.mark.parametrize("header", sorted(HEADERS))
+def test_header_symbols_exported(header):
+    names = declared_functions(os.path.join(ROOT, "include", header))
+    assert len(names) >= 5
+    lib = ctypes.CDLL(HEADERS[header])
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_expected_entry_points_present():
+    names = declared_functions(os.path.join(ROOT, "include", "matrox_hgemm.h"))
+    for n in ("HGEMM", "HGEMM_HSS", "mx_hgemm_configure", "mx_plan_create", "mx_eval", "mx_plan_destroy"):
+        assert n in names
+
+
+def test_configure_rejects_bad_dims_without_gpu():
+    lib = ctypes.CDLL(_native.HGEMM_LIB)
+    lib.mx_last_error.restype = ctypes.c_char_p
+    assert lib.mx_hgemm_configure(-1, 0, 0, 0) == -1
+    assert b"bad" in lib.mx_last_error()
+
+
+def test_hgemm_before_configure_fails_cleanly():
+    # a fresh process state cannot be guaranteed here, so only check that a
+    # null argument list is rejected before any device work
+    lib = ctypes.CDLL(_native.HGEMM_LIB)
+    lib.mx_hgemm_configure(4, 2, 2, 1)
+    args = [None] * 12 + [4] + [None] * 19
+    lib.HGEMM.restype = ctypes.c_int32
+    assert lib.HGEMM(*args) == -1
