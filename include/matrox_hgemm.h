@@ -129,6 +129,32 @@ mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uske
  * elements each, layout as above) into caller device buffers on `stream`. */
 mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst, int32_t nrhs, void *stream);
 
+/* ---- per-result checks and measurement ------------------------------------ */
+
+/* Attach the points (host, N x d row-major, in LEAF order: row j is point
+ * lids[j]) and the Gaussian bandwidth h, enabling mx_dense_rows. */
+mx_status mx_plan_set_points(mx_plan *plan, const double *X_leaf, int32_t d, double h);
+
+/* Dense FP64 reference rows: out (device, nrows x nrhs column-major) =
+ * K(rows, :) W with K(i,j) = exp(-|x_i - x_j|^2 / (2 h^2)) evaluated directly
+ * (matroxTest-V1/Util.h:182-215, 303-308).  rows are host leaf-order row
+ * indices; W is the FP64 leaf-blocked input on device. */
+mx_status mx_dense_rows(mx_plan *plan, const int32_t *rows, int32_t nrows, const double *W, int32_t nrhs,
+                        double *out, void *stream);
+
+/* Launch timing: when enabled, every kernel launch of mx_eval is bracketed
+ * by HIP events on the evaluation stream.  mx_plan_launch_times waits for the
+ * last evaluation and returns, per launch, its duration in ms and its kernel
+ * kind (MX_KIND_*); *count is the number of launches (<= cap filled). */
+enum { MX_KIND_SEG_F64 = 0, MX_KIND_SEG_F32 = 1, MX_KIND_BF16X3 = 2, MX_KIND_CONVERT = 3 };
+mx_status mx_plan_set_profiling(mx_plan *plan, int32_t enable);
+mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t cap, int32_t *count);
+
+/* Algorithmic flops of one evaluation with nrhs columns (the count of
+ * matroxTest-V1/compression.h:1079-1130), split by kernel kind of the given
+ * precision: flops[MX_KIND_*] (4 entries); returns the total in *total. */
+mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, double *total);
+
 /* Number of rows N and of skeleton rows sum(slen) of a plan. */
 mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);
 

@@ -29,17 +29,19 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 struct X3Bufs {
-    const uint16_t *hi[3];   // packed split blocks (D, B; VT unused)
+    const uint16_t *hi[3];   // packed split A operands from D, B, VT
     const uint16_t *lo[3];
     float *vec[4];           // W, wskel, uskel, Y (FP32)
 };
 
-// A block to pack: source offset (FP64 elements, column-major M x K),
-// destination offset in packed elements.
+// An A operand to pack: op(A) is M x K; source element (i, k) is
+// src[src_off + k*lda + i] (trans = 0) or src[src_off + i*lda + k] (trans = 1),
+// FP64; destination offset in packed elements.
 struct alignas(16) X3Block {
     int64_t src_off;
     int64_t dst_off;
     int32_t M, K;
+    int32_t lda, trans;
 };
 
 constexpr int X3_BM = 128, X3_BN = 128, X3_BK = 32;

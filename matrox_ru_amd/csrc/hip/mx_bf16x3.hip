@@ -31,7 +31,9 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             int k = kb + e;
-            double x = (row < bk.M && k < bk.K) ? src[bk.src_off + (int64_t)k * bk.M + row] : 0.0;
+            double x = 0.0;
+            if (row < bk.M && k < bk.K)
+                x = bk.trans ? src[bk.src_off + (int64_t)row * bk.lda + k] : src[bk.src_off + (int64_t)k * bk.lda + row];
             uint16_t h = bf16_rne((float)x);
             hi[dst + e] = h;
             lo[dst + e] = bf16_rne((float)(x - (double)bf16_f(h)));

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -65,19 +66,19 @@ struct mx_plan {
     mx::Seg *d_segs = nullptr;
     int2 *d_tiles = nullptr;
     std::vector<mx::Launch> launches;   // UP levels, FAR+DOWN levels, NEAR+LEAF
-    // the same tasks split for the split-BF16 path: near/far segments go to
-    // the BF16x3 kernel, transfer segments to the FP32 kernel
+    // the same tasks for the split-BF16x3 kernel (128-row tiles, packed A)
     mx::Task *d_tasks_x3 = nullptr;
     mx::Seg *d_segs_x3 = nullptr;
     int2 *d_tiles_x3 = nullptr;
     std::vector<mx::Launch> launches_x3;
-    std::vector<int> launch_kind_x3;    // 0: FP32 seg kernel, 1: BF16x3 kernel
+    std::vector<int> launch_kind_x3;    // 1 for every launch (BF16x3 kernel)
 
     // stored blocks per precision
     double *D64 = nullptr, *B64 = nullptr, *VT64 = nullptr;
     float *D32 = nullptr, *B32 = nullptr, *VT32 = nullptr;
-    uint16_t *Dhi = nullptr, *Dlo = nullptr, *Bhi = nullptr, *Blo = nullptr;
-    int64_t x3_elems[2] = {0, 0};   // packed split D and B elements
+    uint16_t *xhi[3] = {nullptr, nullptr, nullptr};   // packed split D, B, VT operands
+    uint16_t *xlo[3] = {nullptr, nullptr, nullptr};
+    int64_t x3_elems[3] = {0, 0, 0};
     int64_t D_len = 0, B_len = 0, VT_len = 0;
     uint32_t precisions = 0;
 
@@ -88,11 +89,26 @@ struct mx_plan {
     size_t io_bytes = 0;
     int last_prec = -1;
 
+    // algorithmic flops per nrhs column of each launch (both task lists)
+    std::vector<double> launch_flops, launch_flops_x3;
+
+    // points for the dense reference rows
+    double *X_leaf = nullptr;
+    int pts_d = 0;
+    double pts_h = 0.0;
+
+    // launch timing
+    int profiling = 0;
+    std::vector<hipEvent_t> ev;       // 2 per launch
+    std::vector<int> ev_kind;
+    int ev_used = 0;
+
     ~mx_plan() {
         void *ptrs[] = {d_tasks, d_segs, d_tiles, d_tasks_x3, d_segs_x3, d_tiles_x3, D64, B64, VT64, D32, B32,
-                        VT32, Dhi, Dlo, Bhi, Blo, wskel, uskel, io_w, io_y};
+                        VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
 };
 
@@ -106,7 +122,22 @@ struct TaskLists {
     std::vector<Launch> launches;
     std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
     std::vector<int> task_kind;   // per task
+    std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
 };
+
+static void compute_launch_flops(TaskLists &L) {
+    L.flops.assign(L.launches.size(), 0.0);
+    for (size_t li = 0; li < L.launches.size(); ++li) {
+        int prev = -1;
+        for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
+            int task = L.tiles[t].x;
+            if (task == prev) continue;   // tiles of one task are consecutive
+            prev = task;
+            const Task &tk = L.tasks[task];
+            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) L.flops[li] += 2.0 * tk.m * (double)L.segs[sg].k;
+        }
+    }
+}
 
 struct Topology {
     std::vector<int> parent, depth;
@@ -203,11 +234,11 @@ static int add_tiles(TaskLists &L, int task, int m) {
     return n;
 }
 
-// Build the forward task lists.  beta_u / beta_y select the accumulation
-// rule of the coupling and leaf tasks.  With split = true, near and far
-// segments are put in tasks of their own (kind 1) ahead of the transfer
-// tasks (kind 0) that then accumulate into the same blocks.
-static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, bool split) {
+// Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
+// in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
+// beta_u / beta_y select the accumulation rule of the coupling and leaf
+// tasks (0 for mx_eval, the reference's rules for the drop-in entry points).
+static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind) {
     TaskLists L;
     const int nn = s.numnodes;
     // far pairs by destination node
@@ -224,7 +255,7 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
     std::vector<std::vector<int64_t>> near_of(s.nleaf);
     for (int64_t k = 0; k < s.ncount; ++k) near_of[s.is_hss ? (int)k : s.npairx[k]].push_back(k);
 
-    auto begin_launch = [&](int kind) {
+    auto begin_launch = [&]() {
         L.launches.push_back(Launch{(int)L.tiles.size(), 0});
         L.kinds.push_back(kind);
     };
@@ -244,7 +275,7 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         tk.seg_end = tk.seg_begin;
         tk.beta = beta;
         L.tasks.push_back(tk);
-        L.task_kind.push_back(L.kinds.back());
+        L.task_kind.push_back(kind);
         return (int)L.tasks.size() - 1;
     };
     auto add_seg = [&](int task, int64_t a_off, int a_buf, int a_trans, int lda, int k, int b_buf, int64_t b_row,
@@ -265,7 +296,7 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
     const int maxd = (int)t.levels.size() - 1;
     // UP: deepest level first
     for (int d = maxd; d >= 1; --d) {
-        begin_launch(0);
+        begin_launch();
         for (int v : t.levels[d]) {
             int sv = s.slen[v];
             if (sv == 0) continue;
@@ -283,109 +314,43 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         }
         end_launch();
     }
-    // transfer segment of node c from its parent p: VT_p[:, part(c)]^T u_p
-    auto add_parent = [&](int tk, int c) {
-        int p = t.parent[c];
-        if (p <= 0 || s.slen[p] == 0) return;
-        int sp = s.slen[p];
-        int64_t off = s.VTptr[p];
-        if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
-        add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, s.skelrow[p], sp);
-    };
-    auto add_far = [&](int tk, int c) {
-        for (int64_t k : far_of[c]) {
-            int y = far_partner(k);
-            add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, s.slen[c], s.slen[y], VEC_WSKEL, s.skelrow[y], s.slen[y]);
-        }
-    };
-    // FAR + DOWN: root's children first
+    // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
     for (int d = 1; d <= maxd; ++d) {
-        if (!split) {
-            begin_launch(0);
-            for (int c : t.levels[d]) {
-                int sc = s.slen[c];
-                if (sc == 0) continue;
-                int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
-                add_far(tk, c);
-                add_parent(tk, c);
-                add_tiles(L, tk, sc);
+        begin_launch();
+        for (int c : t.levels[d]) {
+            int sc = s.slen[c];
+            if (sc == 0) continue;
+            int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
+            for (int64_t k : far_of[c]) {
+                int y = far_partner(k);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, s.skelrow[y], s.slen[y]);
             }
-            end_launch();
-        } else {
-            if (d == 1) {
-                // all far couplings in one launch (they only read wskel)
-                begin_launch(1);
-                for (int dd = 1; dd <= maxd; ++dd)
-                    for (int c : t.levels[dd]) {
-                        int sc = s.slen[c];
-                        if (sc == 0) continue;
-                        int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
-                        add_far(tk, c);
-                        add_tiles(L, tk, sc);
-                    }
-                end_launch();
+            int p = t.parent[c];
+            if (p > 0 && s.slen[p] > 0) {
+                int sp = s.slen[p];
+                int64_t off = s.VTptr[p];
+                if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
+                add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, s.skelrow[p], sp);
             }
-            begin_launch(0);
-            for (int c : t.levels[d]) {
-                int sc = s.slen[c];
-                if (sc == 0) continue;
-                int tk = new_task(s.skelrow[c], sc, VEC_USKEL, 1);
-                add_parent(tk, c);
-                if (L.tasks[tk].seg_end == L.tasks[tk].seg_begin) {
-                    L.tasks.pop_back();
-                    L.task_kind.pop_back();
-                    continue;
-                }
-                add_tiles(L, tk, sc);
-            }
-            end_launch();
+            add_tiles(L, tk, sc);
         }
+        end_launch();
     }
-    // NEAR + LEAF
-    auto add_near = [&](int tk, int lf) {
+    // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x
+    begin_launch();
+    for (int lf = 0; lf < s.nleaf; ++lf) {
+        if (s.Ddim[lf] == 0) continue;
+        int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
         for (int64_t k : near_of[lf]) {
             int y = s.is_hss ? (int)k : s.npairy[k];
             add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, s.wrow[y], s.Ddim[y]);
         }
-    };
-    auto add_leafdown = [&](int tk, int lf) {
         int v = t.node_of_leaf[lf];
         int sv = s.slen[v];
         if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, s.skelrow[v], sv);
-    };
-    if (!split) {
-        begin_launch(0);
-        for (int lf = 0; lf < s.nleaf; ++lf) {
-            if (s.Ddim[lf] == 0) continue;
-            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
-            add_near(tk, lf);
-            add_leafdown(tk, lf);
-            add_tiles(L, tk, s.Ddim[lf]);
-        }
-        end_launch();
-    } else {
-        begin_launch(1);
-        for (int lf = 0; lf < s.nleaf; ++lf) {
-            if (s.Ddim[lf] == 0) continue;
-            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
-            add_near(tk, lf);
-            add_tiles(L, tk, s.Ddim[lf]);
-        }
-        end_launch();
-        begin_launch(0);
-        for (int lf = 0; lf < s.nleaf; ++lf) {
-            if (s.Ddim[lf] == 0) continue;
-            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, 1);
-            add_leafdown(tk, lf);
-            if (L.tasks[tk].seg_end == L.tasks[tk].seg_begin) {
-                L.tasks.pop_back();
-                L.task_kind.pop_back();
-                continue;
-            }
-            add_tiles(L, tk, s.Ddim[lf]);
-        }
-        end_launch();
+        add_tiles(L, tk, s.Ddim[lf]);
     }
+    end_launch();
     return L;
 }
 
@@ -454,39 +419,48 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     P->VT_len = s.VT_len;
     P->precisions = precisions;
 
-    TaskLists L = build_forward(s, t, beta_u, beta_y, false);
+    TaskLists L = build_forward(s, t, beta_u, beta_y, 0);
+    compute_launch_flops(L);
     upload_lists(L, &P->d_tasks, &P->d_segs, &P->d_tiles);
     P->launches = L.launches;
-    TaskLists X;
-    std::vector<X3Block> xblk[2];   // packed D and B blocks
+    P->launch_flops = L.flops;
+    std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     if (precisions & MX_PLAN_BF16X3) {
-        X = build_forward(s, t, beta_u, beta_y, true);
-        // near/far segments of BF16x3 tasks read packed split blocks: assign
-        // each stored block its packed offset (once per block)
-        std::map<int64_t, int64_t> seen[2];
-        int64_t next[2] = {0, 0};
+        TaskLists X = build_forward(s, t, beta_u, beta_y, 1);
+        // every A operand is read from packed split-BF16 fragments: assign each
+        // distinct (buffer, offset, orientation, shape) its packed offset
+        std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
+        int64_t next[3] = {0, 0, 0};
         for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
-            if (X.task_kind[ti] != 1) continue;
             for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
                 Seg &sg = X.segs[si];
-                arg_check(sg.a_trans == 0 && (sg.a_buf == MAT_D || sg.a_buf == MAT_B), "bad BF16x3 segment");
-                int which = sg.a_buf == MAT_D ? 0 : 1;
-                auto it = seen[which].find(sg.a_off);
+                const int w = sg.a_buf, m = X.tasks[ti].m;
+                auto key = std::make_tuple(sg.a_off, sg.a_trans, sg.lda, m, sg.k);
+                auto it = seen[w].find(key);
                 int64_t dst;
-                if (it == seen[which].end()) {
-                    dst = next[which];
-                    seen[which][sg.a_off] = dst;
-                    xblk[which].push_back(X3Block{sg.a_off, dst, X.tasks[ti].m, sg.k});
-                    next[which] += x3_packed_elems(X.tasks[ti].m, sg.k);
+                if (it == seen[w].end()) {
+                    dst = next[w];
+                    seen[w][key] = dst;
+                    X3Block xb{};
+                    xb.src_off = sg.a_off;
+                    xb.dst_off = dst;
+                    xb.M = m;
+                    xb.K = sg.k;
+                    xb.lda = sg.lda;
+                    xb.trans = sg.a_trans;
+                    xblk[w].push_back(xb);
+                    next[w] += x3_packed_elems(m, sg.k);
                 } else {
                     dst = it->second;
                 }
                 sg.a_off = dst;
+                sg.a_trans = 0;
                 sg.lda = (sg.k + 31) / 32;
             }
         }
-        P->x3_elems[0] = next[0];
-        P->x3_elems[1] = next[1];
+        for (int w = 0; w < 3; ++w) P->x3_elems[w] = next[w];
+        compute_launch_flops(X);
+        P->launch_flops_x3 = X.flops;
         upload_lists(X, &P->d_tasks_x3, &P->d_segs_x3, &P->d_tiles_x3);
         P->launches_x3 = X.launches;
         P->launch_kind_x3 = X.kinds;
@@ -494,26 +468,23 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     // stored blocks
     double *D64 = upload(s.D, (size_t)s.D_len), *B64 = upload(s.B, (size_t)s.B_len);
     double *VT64 = upload(s.VT, (size_t)s.VT_len);
-    if (precisions & (MX_PLAN_FP32 | MX_PLAN_BF16X3)) {
+    if (precisions & MX_PLAN_FP32) {
         P->VT32 = upload<float>(nullptr, (size_t)s.VT_len);
         cvt_f64_f32<<<cvt_grid(s.VT_len), 256>>>(VT64, P->VT32, s.VT_len);
-    }
-    if (precisions & MX_PLAN_FP32) {
         P->D32 = upload<float>(nullptr, (size_t)s.D_len);
         P->B32 = upload<float>(nullptr, (size_t)s.B_len);
         cvt_f64_f32<<<cvt_grid(s.D_len), 256>>>(D64, P->D32, s.D_len);
         cvt_f64_f32<<<cvt_grid(s.B_len), 256>>>(B64, P->B32, s.B_len);
     }
     if (precisions & MX_PLAN_BF16X3) {
-        uint16_t **hi[2] = {&P->Dhi, &P->Bhi}, **lo[2] = {&P->Dlo, &P->Blo};
-        double *src[2] = {D64, B64};
-        for (int w = 0; w < 2; ++w) {
-            *hi[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
-            *lo[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
+        double *src[3] = {D64, B64, VT64};
+        for (int w = 0; w < 3; ++w) {
+            P->xhi[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
+            P->xlo[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
             for (size_t b0 = 0; b0 < xblk[w].size(); b0 += 65535) {
                 size_t nb = std::min<size_t>(65535, xblk[w].size() - b0);
                 X3Block *db = upload(xblk[w].data() + b0, nb);
-                pack_x3_blocks<<<dim3(64, (unsigned)nb), 256>>>(src[w], db, *hi[w], *lo[w]);
+                pack_x3_blocks<<<dim3(64, (unsigned)nb), 256>>>(src[w], db, P->xhi[w], P->xlo[w]);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipDeviceSynchronize());
                 HIPCHK(hipFree(db));
@@ -568,6 +539,24 @@ static void run_seg_launch(const Launch &ln, const Task *tasks, const Seg *segs,
                        tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
 }
 
+static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
+    if (!P->profiling) return;
+    if (begin) {
+        size_t need = 2 * (size_t)P->ev_used + 2;
+        while (P->ev.size() < need) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            P->ev.push_back(e);
+        }
+        if ((int)P->ev_kind.size() <= P->ev_used) P->ev_kind.resize(P->ev_used + 1);
+        P->ev_kind[P->ev_used] = kind;
+        HIPCHK(hipEventRecord(P->ev[2 * P->ev_used], st));
+    } else {
+        HIPCHK(hipEventRecord(P->ev[2 * P->ev_used + 1], st));
+        ++P->ev_used;
+    }
+}
+
 static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
                  hipStream_t st) {
     arg_check(W && Y, "null W or Y");
@@ -578,6 +567,7 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
     if (dev != P->device) throw MxError(MX_ERR_STATE, "plan was created on another device");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const int64_t nel = P->N * (int64_t)nrhs;
+    P->ev_used = 0;
     if (prec == MX_FP64) {
         if (io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
         ensure_work(P, nrhs, sizeof(double), false);
@@ -589,14 +579,19 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         b.vec[VEC_WSKEL] = (double *)P->wskel;
         b.vec[VEC_USKEL] = (double *)P->uskel;
         b.vec[VEC_Y] = (double *)Y;
-        for (size_t i = 0; i < P->launches.size(); ++i)
+        for (size_t i = 0; i < P->launches.size(); ++i) {
+            prof_mark(P, st, MX_KIND_SEG_F64, true);
             run_seg_launch<double>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+            prof_mark(P, st, MX_KIND_SEG_F64, false);
+        }
     } else {
         ensure_work(P, nrhs, sizeof(float), io == MX_IO_F64);
         const float *Wf;
         float *Yf;
         if (io == MX_IO_F64) {
+            prof_mark(P, st, MX_KIND_CONVERT, true);
             cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)P->io_w, nel);
+            prof_mark(P, st, MX_KIND_CONVERT, false);
             Wf = (const float *)P->io_w;
             Yf = (float *)P->io_y;
         } else {
@@ -612,31 +607,108 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         b.vec[VEC_USKEL] = (float *)P->uskel;
         b.vec[VEC_Y] = Yf;
         if (prec == MX_FP32) {
-            for (size_t i = 0; i < P->launches.size(); ++i)
+            for (size_t i = 0; i < P->launches.size(); ++i) {
+                prof_mark(P, st, MX_KIND_SEG_F32, true);
                 run_seg_launch<float>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+                prof_mark(P, st, MX_KIND_SEG_F32, false);
+            }
         } else {
             X3Bufs xb;
-            xb.hi[MAT_D] = P->Dhi;
-            xb.lo[MAT_D] = P->Dlo;
-            xb.hi[MAT_B] = P->Bhi;
-            xb.lo[MAT_B] = P->Blo;
-            xb.hi[MAT_VT] = nullptr;
-            xb.lo[MAT_VT] = nullptr;
+            for (int w = 0; w < 3; ++w) {
+                xb.hi[w] = P->xhi[w];
+                xb.lo[w] = P->xlo[w];
+            }
             for (int i = 0; i < 4; ++i) xb.vec[i] = b.vec[i];
             for (size_t i = 0; i < P->launches_x3.size(); ++i) {
-                if (P->launch_kind_x3[i] == 0) {
-                    run_seg_launch<float>(P->launches_x3[i], P->d_tasks_x3, P->d_segs_x3, P->d_tiles_x3, nrhs,
-                                          b, st);
-                } else {
-                    launch_bf16x3(P->launches_x3[i].ntiles, P->d_tasks_x3, P->d_segs_x3,
-                                  P->d_tiles_x3 + P->launches_x3[i].tile_offset, nrhs, xb, st);
-                }
+                prof_mark(P, st, MX_KIND_BF16X3, true);
+                launch_bf16x3(P->launches_x3[i].ntiles, P->d_tasks_x3, P->d_segs_x3,
+                              P->d_tiles_x3 + P->launches_x3[i].tile_offset, nrhs, xb, st);
+                prof_mark(P, st, MX_KIND_BF16X3, false);
             }
         }
-        if (io == MX_IO_F64) cvt_f32_f64<<<cvt_grid(nel), 256, 0, st>>>(Yf, (double *)Y, nel);
+        if (io == MX_IO_F64) {
+            prof_mark(P, st, MX_KIND_CONVERT, true);
+            cvt_f32_f64<<<cvt_grid(nel), 256, 0, st>>>(Yf, (double *)Y, nel);
+            prof_mark(P, st, MX_KIND_CONVERT, false);
+        }
     }
     HIPCHK(hipGetLastError());
     P->last_prec = prec;
+}
+
+// Kr (nrows x N, column-major) = exp(-|x_rows[r] - x_j|^2 / (2 h^2)), points in
+// leaf order; distances summed directly as in matroxTest-V1/Util.h:189-201.
+__global__ void dense_kernel_rows(const double *__restrict__ X, int64_t N, int d, double h,
+                                  const int32_t *__restrict__ rows, int nrows, double *__restrict__ Kr) {
+    const double s = 1.0 / (2.0 * h * h);
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < N * nrows;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = e / nrows;
+        int r = (int)(e % nrows);
+        const double *a = X + (int64_t)rows[r] * d;
+        const double *b = X + j * d;
+        double k2 = 0.0;
+        for (int p = 0; p < d; ++p) {
+            double t = b[p] - a[p];
+            k2 += t * t;
+        }
+        Kr[e] = exp(-k2 * s);
+    }
+}
+
+static void dense_rows(mx_plan *P, const int32_t *rows, int nrows, const double *W, int nrhs, double *out,
+                       hipStream_t st) {
+    if (!P->X_leaf) throw MxError(MX_ERR_STATE, "mx_plan_set_points has not been called");
+    arg_check(rows && W && out && nrows >= 1 && nrhs >= 1, "bad argument");
+    for (int r = 0; r < nrows; ++r) arg_check(rows[r] >= 0 && rows[r] < P->N, "row index out of range");
+    TaskLists L;
+    Task tk{};
+    tk.c_row = 0;
+    tk.m = nrows;
+    tk.c_buf = VEC_Y;
+    tk.seg_begin = 0;
+    tk.beta = 0;
+    for (int l = 0; l < P->nleaf; ++l) {
+        if (P->Ddim[l] == 0) continue;
+        Seg sg{};
+        sg.a_off = (int64_t)P->wrow[l] * nrows;
+        sg.a_buf = MAT_D;
+        sg.a_trans = 0;
+        sg.lda = nrows;
+        sg.k = P->Ddim[l];
+        sg.b_buf = VEC_W;
+        sg.b_row = P->wrow[l];
+        sg.b_ld = P->Ddim[l];
+        L.segs.push_back(sg);
+    }
+    tk.seg_end = (int)L.segs.size();
+    L.tasks.push_back(tk);
+    for (int i = 0; i < (nrows + SG_BM - 1) / SG_BM; ++i) L.tiles.push_back(make_int2(0, i));
+    Task *dt = nullptr;
+    Seg *ds = nullptr;
+    int2 *dtl = nullptr;
+    int32_t *drows = nullptr;
+    double *Kr = nullptr;
+    try {
+        upload_lists(L, &dt, &ds, &dtl);
+        drows = upload(rows, (size_t)nrows);
+        HIPCHK(hipMalloc(&Kr, sizeof(double) * (size_t)nrows * P->N));
+        int64_t tot = (int64_t)nrows * P->N;
+        dense_kernel_rows<<<cvt_grid(tot), 256, 0, st>>>(P->X_leaf, P->N, P->pts_d, P->pts_h, drows, nrows, Kr);
+        Bufs<double> b{};
+        b.mat[MAT_D] = Kr;
+        b.vec[VEC_W] = const_cast<double *>(W);
+        b.vec[VEC_Y] = out;
+        Launch ln{0, (int)L.tiles.size()};
+        run_seg_launch<double>(ln, dt, ds, dtl, nrhs, b, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (...) {
+        for (void *p : {(void *)dt, (void *)ds, (void *)dtl, (void *)drows, (void *)Kr})
+            if (p) (void)hipFree(p);
+        throw;
+    }
+    for (void *p : {(void *)dt, (void *)ds, (void *)dtl, (void *)drows, (void *)Kr}) HIPCHK(hipFree(p));
 }
 
 // ---- drop-in entry points -------------------------------------------------
@@ -879,6 +951,67 @@ mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst
         arg_check(bytes <= plan->skel_bytes, "nrhs exceeds the last evaluation's buffers");
         HIPCHK(hipMemcpyAsync(wskel_dst, plan->wskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
         HIPCHK(hipMemcpyAsync(uskel_dst, plan->uskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_points(mx_plan *plan, const double *X_leaf, int32_t d, double h) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && X_leaf && d >= 1 && h > 0, "bad argument");
+        if (plan->X_leaf) HIPCHK(hipFree(plan->X_leaf));
+        plan->X_leaf = nullptr;
+        plan->X_leaf = upload(X_leaf, (size_t)plan->N * d);
+        plan->pts_d = d;
+        plan->pts_h = h;
+        return 0;
+    });
+}
+
+mx_status mx_dense_rows(mx_plan *plan, const int32_t *rows, int32_t nrows, const double *W, int32_t nrhs,
+                        double *out, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        dense_rows(plan, rows, nrows, W, nrhs, out, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_profiling(mx_plan *plan, int32_t enable) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        plan->profiling = enable ? 1 : 0;
+        plan->ev_used = 0;
+        return 0;
+    });
+}
+
+mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t cap, int32_t *count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && count && cap >= 0 && (cap == 0 || (ms && kind)), "bad argument");
+        *count = plan->ev_used;
+        if (plan->ev_used == 0) return 0;
+        HIPCHK(hipEventSynchronize(plan->ev[2 * plan->ev_used - 1]));
+        for (int i = 0; i < plan->ev_used && i < cap; ++i) {
+            HIPCHK(hipEventElapsedTime(&ms[i], plan->ev[2 * i], plan->ev[2 * i + 1]));
+            kind[i] = plan->ev_kind[i];
+        }
+        return 0;
+    });
+}
+
+mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, double *total) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && flops && total && nrhs >= 0, "bad argument");
+        for (int i = 0; i < 4; ++i) flops[i] = 0.0;
+        if (prec == MX_BF16X3) {
+            arg_check(!plan->launch_flops_x3.empty() || plan->launches_x3.empty(), "no BF16x3 lists");
+            for (size_t i = 0; i < plan->launch_flops_x3.size(); ++i)
+                flops[plan->launch_kind_x3[i] ? MX_KIND_BF16X3 : MX_KIND_SEG_F32] += plan->launch_flops_x3[i] * nrhs;
+        } else {
+            int k = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
+            for (double f : plan->launch_flops) flops[k] += f * nrhs;
+        }
+        *total = flops[0] + flops[1] + flops[2] + flops[3];
         return 0;
     });
 }

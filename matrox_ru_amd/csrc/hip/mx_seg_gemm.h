@@ -12,7 +12,9 @@
 // stores (and, for beta = 1, loads) of C are 64 consecutive rows per column.
 //
 // MFMA 16x16x4 operand layout (lane l of 64): A element (row l%16, k l/16),
-// B element (k l/16, col l%16); result register r holds C(4*(l/16)+r, l%16).
+// B element (k l/16, col l%16).  Result register r holds C(4*(l/16)+r, l%16)
+// for the FP32 instruction but C(l/16+4*r, l%16) for the FP64 one (measured
+// with a lane-layout probe on gfx950).
 #pragma once
 #include "mx_common.h"
 
@@ -22,8 +24,15 @@ typedef double d4_t __attribute__((ext_vector_type(4)));
 typedef float f4_t __attribute__((ext_vector_type(4)));
 
 template <typename T> struct MfmaAcc;
-template <> struct MfmaAcc<double> { typedef d4_t type; };
-template <> struct MfmaAcc<float> { typedef f4_t type; };
+template <> struct MfmaAcc<double> {
+    typedef d4_t type;
+    // row of result register r within the 16x16 block, for lane group g = l/16
+    static __device__ __forceinline__ int row(int g, int r) { return g + 4 * r; }
+};
+template <> struct MfmaAcc<float> {
+    typedef f4_t type;
+    static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
 
 __device__ __forceinline__ d4_t mfma16x16x4(double a, double b, d4_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -205,7 +214,7 @@ __global__ __launch_bounds__(SG_THREADS) void seg_gemm_kernel(const Task *__rest
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                int row = wr * 32 + i * 16 + 4 * lk + r;
+                int row = wr * 32 + i * 16 + MfmaAcc<T>::row(lk, r);
                 int col = wc * 32 + j * 16 + lr;
                 Cs[col * SG_CP + row] = acc[i][j][r];
             }

@@ -15,6 +15,7 @@ import numpy as np
 from . import _native
 
 FP64, FP32, BF16X3 = 0, 1, 2
+KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -65,6 +66,16 @@ def lib():
         L.mx_plan_copy_skeletons.restype = ctypes.c_int
         L.mx_plan_dims.argtypes = [P, ctypes.POINTER(I64), ctypes.POINTER(I64)]
         L.mx_plan_dims.restype = ctypes.c_int
+        L.mx_plan_set_points.argtypes = [P, P, I32, ctypes.c_double]
+        L.mx_plan_set_points.restype = ctypes.c_int
+        L.mx_dense_rows.argtypes = [P, P, I32, P, I32, P, P]
+        L.mx_dense_rows.restype = ctypes.c_int
+        L.mx_plan_set_profiling.argtypes = [P, I32]
+        L.mx_plan_set_profiling.restype = ctypes.c_int
+        L.mx_plan_launch_times.argtypes = [P, P, P, I32, ctypes.POINTER(I32)]
+        L.mx_plan_launch_times.restype = ctypes.c_int
+        L.mx_plan_flops.argtypes = [P, ctypes.c_int, I32, P, ctypes.POINTER(ctypes.c_double)]
+        L.mx_plan_flops.restype = ctypes.c_int
         _configured = True
     return L
 
@@ -133,6 +144,8 @@ class Plan:
         self.rows = n.value
         self.skel_rows = sk.value
         self.precisions = tuple(precisions)
+        self._wrow = np.asarray(hm.arrays["wrow"], dtype=np.int64)
+        self._ddim = np.asarray(hm.arrays["Ddim"], dtype=np.int64)
 
     def eval(self, W, Y, nrhs, precision="fp64", stream=None):
         """Y = K~ W; W and Y are contiguous CUDA tensors of N*nrhs elements
@@ -153,6 +166,53 @@ class Plan:
         us = torch.empty_like(ws)
         _check(lib().mx_plan_copy_skeletons(self._h, ws.data_ptr(), us.data_ptr(), nrhs, _stream_handle(stream)))
         return ws, us
+
+    def set_points(self, X_leaf, h):
+        """Points in leaf order (row j = point lids[j]) and bandwidth for dense_rows."""
+        X_leaf = _np(X_leaf, np.float64)
+        assert X_leaf.shape[0] == self.rows
+        _check(lib().mx_plan_set_points(self._h, X_leaf.ctypes.data, X_leaf.shape[1], float(h)))
+
+    def dense_rows(self, rows, W, nrhs, stream=None):
+        """Dense FP64 K(rows, :) W on the GPU; rows are leaf-order row ids.
+        Returns a CUDA tensor (nrows, nrhs)."""
+        import torch
+        rows = _np(rows, np.int32)
+        assert W.is_cuda and W.dtype == torch.float64 and W.numel() == self.rows * nrhs
+        out = torch.empty(nrhs * len(rows), dtype=torch.float64, device=W.device)
+        _check(lib().mx_dense_rows(self._h, rows.ctypes.data, len(rows), W.data_ptr(), nrhs, out.data_ptr(),
+                                   _stream_handle(stream)))
+        return out.view(nrhs, len(rows)).t()
+
+    def rows_of(self, Y, rows, nrhs):
+        """Rows (leaf-order ids) of a leaf-blocked N x nrhs device result, as (nrows, nrhs)."""
+        import torch
+        rows = np.asarray(rows, dtype=np.int64)
+        wrow = self._wrow
+        dim = self._ddim
+        leaf = np.searchsorted(wrow, rows, side="right") - 1
+        base = wrow[leaf] * nrhs + (rows - wrow[leaf])
+        idx = base[:, None] + np.arange(nrhs)[None, :] * dim[leaf][:, None]
+        return Y[torch.from_numpy(idx).to(Y.device)]
+
+    def set_profiling(self, enable=True):
+        _check(lib().mx_plan_set_profiling(self._h, int(enable)))
+
+    def launch_times(self):
+        """[(kind_name, ms)] of the launches of the last evaluation."""
+        cap = 4096
+        ms = (ctypes.c_float * cap)()
+        kd = (I32 * cap)()
+        cnt = I32()
+        _check(lib().mx_plan_launch_times(self._h, ms, kd, cap, ctypes.byref(cnt)))
+        return [(KIND_NAMES[kd[i]], float(ms[i])) for i in range(min(cnt.value, cap))]
+
+    def flops(self, precision, nrhs):
+        """(total, {kind_name: flops}) algorithmic flops of one evaluation."""
+        f = (ctypes.c_double * 4)()
+        tot = ctypes.c_double()
+        _check(lib().mx_plan_flops(self._h, PRECISIONS[precision], nrhs, f, ctypes.byref(tot)))
+        return tot.value, {KIND_NAMES[i]: f[i] for i in range(4) if f[i] > 0}
 
     def close(self):
         if getattr(self, "_h", None):

@@ -160,3 +160,45 @@ def test_repeated_and_interleaved_evaluations(gpu):
             outs[key] = y
     Yo, _, _ = oracle.hgemm(hm, hm.transform(host.normal((hm.N, 8), seed=408)), 8)
     assert rel(outs[(8, "fp64")], Yo) <= 1e-12
+
+
+def test_dense_rows_matches_oracle(gpu):
+    """GPU dense reference rows (used to report each result's error) against
+    the oracle's directly computed K(rows, :) W."""
+    import torch
+    _say("[dense rows]")
+    X, hm = make_problem(**CASES["h2b_ragged"])
+    plan = hgemm.Plan(hm)
+    perm = hm.lids[: hm.N]
+    plan.set_points(X[perm], 1.2)
+    nrhs = 33
+    W = host.normal((hm.N, nrhs), seed=500)
+    Wd = torch.from_numpy(hm.transform(W)).cuda()
+    rows = np.array([0, 1, 7, 150, 2999, 3000, 1234], dtype=np.int32)   # leaf-order ids
+    got = plan.dense_rows(rows, Wd, nrhs).cpu().numpy()
+    ref = oracle.dense_rows(X, 1.2, perm[rows], W)
+    assert rel(got, ref) <= 1e-12
+    # the row extraction helper addresses the leaf-blocked layout
+    Y = torch.empty_like(Wd)
+    plan.eval(Wd, Y, nrhs)
+    Yo, _, _ = oracle.hgemm(hm, hm.transform(W), nrhs)
+    Yrows = plan.rows_of(Y, rows, nrhs).cpu().numpy()
+    assert rel(Yrows, hm.untransform(Yo, nrhs)[perm[rows]]) <= 1e-12
+
+
+def test_flops_and_launch_timing(gpu):
+    import torch
+    _say("[flops/timing]")
+    X, hm = make_problem(**CASES["h2b_small"])
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"))
+    for prec in ("fp64", "bf16x3"):
+        tot, parts = plan.flops(prec, 64)
+        assert tot == pytest.approx(oracle.flops(hm, 64), rel=1e-12)
+    W = torch.from_numpy(hm.transform(host.normal((hm.N, 64), seed=1))).cuda()
+    Y = torch.empty_like(W)
+    plan.set_profiling(True)
+    plan.eval(W, Y, 64, precision="bf16x3")
+    times = plan.launch_times()
+    kinds = {k for k, _ in times}
+    assert kinds == {"bf16x3", "convert"}
+    assert all(ms > 0 for _, ms in times)
