@@ -136,10 +136,15 @@ def main():
     plan.set_profiling(True)
     per_kind = {}
     nprof = 3
+    per_launch = None
     for _ in range(nprof):
         plan.eval(Wd, Ymx, nrhs, precision=prec)
-        for k, ms in plan.launch_times():
+        lt = plan.launch_times()
+        per_launch = [ms for _, ms in lt] if per_launch is None else [a + ms for a, (_, ms) in zip(per_launch, lt)]
+        for k, ms in lt:
             per_kind.setdefault(k, []).append(ms)
+    log("[bench] per-launch ms: " + " ".join(f"{x / nprof:.3f}" for x in per_launch))
+    log("[bench] per-launch GFLOP: " + " ".join(f"{f / 1e9:.1f}" for f in plan.launch_flops(prec, nrhs)))
     plan.set_profiling(False)
     kind_total = {k: sum(v) / nprof for k, v in per_kind.items()}
     kind_launches = {k: len(v) // nprof for k, v in per_kind.items()}

@@ -122,7 +122,9 @@ mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int3
 /* Device pointers of the skeleton buffers of the last evaluation
  * (wskel = V^T W per node, uskel = coupled + transferred values per node),
  * laid out at skelrow[v]*nrhs with leading dimension slen[v]; element type is
- * double for MX_FP64 and float otherwise. */
+ * double for MX_FP64 and float for MX_FP32.  After an MX_BF16X3 evaluation
+ * the skeletons exist only as split panels and this returns
+ * MX_ERR_UNSUPPORTED. */
 mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uskel);
 
 /* Copy the skeleton buffers of the last evaluation (skel_rows x nrhs
@@ -154,6 +156,11 @@ mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t 
  * matroxTest-V1/compression.h:1079-1130), split by kernel kind of the given
  * precision: flops[MX_KIND_*] (4 entries); returns the total in *total. */
 mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, double *total);
+
+/* Algorithmic flops of each launch of one evaluation (in launch order, the
+ * order mx_plan_launch_times reports kernel launches after the conversion). */
+mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, int32_t cap,
+                               int32_t *count);
 
 /* Number of rows N and of skeleton rows sum(slen) of a plan. */
 mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);

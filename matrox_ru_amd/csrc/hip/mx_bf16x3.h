@@ -1,21 +1,29 @@
-// Split-BF16 x3 products on MFMA with FP32 accumulation, for the near-field
-// (D W) and coupling (B w) segments.
+// Split-BF16 x3 products on MFMA with FP32 accumulation: the mixed-precision
+// form of every block product of an evaluation (near D W, coupling B w,
+// upward VT w, downward VT^T u).
 //
 // A value x is split as x_hi = bf16(x), x_lo = bf16(x - x_hi); a product is
-// taken as a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, ~2^-18
+// taken as a_lo b_hi + a_hi b_lo + a_hi b_hi (the a_lo b_lo term, ~2^-18
 // relative, is dropped), each term on v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation.  Per product the relative error is a few 2^-18.
+// accumulation.
 //
-// Stored blocks (A operands) are split once, when the plan is built, and
-// packed in MFMA fragment order: for a block of M x K (column-major source),
-// rows are padded to 16 and K to 32; fragment (mi, ki) holds the 16 x 32
-// sub-block as 64 lanes x 8 consecutive k, so a wave loads an A fragment with
-// one 16-byte load per lane straight from global memory (L2), no LDS.
-// Column operands (W or wskel, FP32) are staged per 32-deep k chunk in LDS,
-// split once per workgroup, and read as fragments with 16-byte LDS loads.
+// Both operands live in HBM already split and in MFMA fragment order, so the
+// main loop is 16-byte loads straight into registers and MFMAs:
+//
+// * A operands (stored blocks, and transposed views of VT) are packed when
+//   the plan is built: op(A) (M x K) is cut into 16 x 32 fragments; fragment
+//   (mi, ki) at ((mi * ceil(K/32)) + ki) * 512 holds 64 lanes x 8 consecutive
+//   k (rows padded to 16 and K to 32 with zeros).
+// * B operands (W, wskel, uskel) are "split panels": a block of R rows and
+//   nrhs columns occupies ceil(R/32)*32 padded rows; element (k, j) sits at
+//   prow*nrhs + ((k/32)*nrhs + j)*32 + k%32, so the 16 x 32 fragment of
+//   columns j..j+15 is 1 KB contiguous.  W is split into panels once per
+//   evaluation; wskel/uskel panels are written, split, by the epilogue of the
+//   launch that produces them.
 //
 // Workgroup: 256 threads = 4 waves as 2 x 2, tile 128 x 128, each wave
-// 64 x 64 = 4 x 4 fragments (48 MFMA per k chunk).
+// 64 x 64 = 4 x 4 fragments (48 MFMA per 32-deep chunk); the fragments of
+// chunk c+1 are loaded while chunk c is multiplied.
 // MFMA 16x16x32 bf16 operand layout (lane l): A(row l%16, k 8*(l/16)+e),
 // B(k 8*(l/16)+e, col l%16), e = 0..7; result register r holds
 // C(4*(l/16)+r, l%16).
@@ -27,12 +35,7 @@ namespace mx {
 typedef float x3_f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-
-struct X3Bufs {
-    const uint16_t *hi[3];   // packed split A operands from D, B, VT
-    const uint16_t *lo[3];
-    float *vec[4];           // W, wskel, uskel, Y (FP32)
-};
+typedef uint32_t u32x2_t __attribute__((ext_vector_type(2)));
 
 // An A operand to pack: op(A) is M x K; source element (i, k) is
 // src[src_off + k*lda + i] (trans = 0) or src[src_off + i*lda + k] (trans = 1),
@@ -44,14 +47,35 @@ struct alignas(16) X3Block {
     int32_t lda, trans;
 };
 
+// A leaf block of W to split into its panel.
+struct alignas(16) X3Rows {
+    int64_t src_row;    // first row in the leaf-blocked input
+    int64_t prow;       // first padded row of its panel
+    int32_t R;          // rows
+    int32_t pad;
+};
+
+struct X3Args {
+    const Task *tasks;
+    const Seg *segs;
+    const int2 *tiles;
+    int ntiles, nrhs, n_ntiles, y_f64;
+    const uint16_t *ahi[3];   // packed A operands from D, B, VT
+    const uint16_t *alo[3];
+    uint16_t *phi[3];         // split panels of W, wskel, uskel (VecBuf 0..2)
+    uint16_t *plo[3];
+    void *Y;                  // output in the reference layout, FP32 or FP64
+};
+
 constexpr int X3_BM = 128, X3_BN = 128, X3_BK = 32;
-constexpr int X3_BKP = X3_BK + 8;   // padded k stride of the LDS column chunk (80 B)
+constexpr int X3_MAXSEG = 256;      // segments per task the kernel holds in LDS
 constexpr int X3_CP = X3_BM + 1;    // padded column stride of the output staging
 constexpr int X3_THREADS = 256;
 
 __host__ __device__ inline int64_t x3_packed_elems(int M, int K) {
     return (int64_t)((M + 15) / 16) * ((K + 31) / 32) * 512;
 }
+__host__ __device__ inline int64_t x3_pad_rows(int R) { return ((int64_t)R + 31) / 32 * 32; }
 
 __device__ __forceinline__ uint16_t bf16_rne(float x) {
     uint32_t u = __float_as_uint(x);
@@ -62,19 +86,15 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 
 __device__ __forceinline__ float bf16_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-// hi/lo split of FP64 values (elementwise; used for checks and small buffers)
-__global__ void split_bf16_f64(const double *__restrict__ in, uint16_t *__restrict__ hi,
-                               uint16_t *__restrict__ lo, int64_t n);
-
 // Pack blocks[blockIdx.y] into fragment order (hi and lo).
 __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__restrict__ blocks,
                                uint16_t *__restrict__ hi, uint16_t *__restrict__ lo);
 
-void launch_bf16x3(int ntiles, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs, const X3Bufs &b,
-                   hipStream_t st);
+// Split the leaf blocks of W (FP64 or FP32, leaf-blocked reference layout)
+// into panels.
+void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_blocks, int nrhs, uint16_t *hi,
+                       uint16_t *lo, hipStream_t st);
+
+void launch_x3(const X3Args &a, hipStream_t st);
 
 }  // namespace mx

@@ -1,19 +1,7 @@
-// Split-BF16 x3 kernels (layout and arithmetic described in mx_bf16x3.h).
+// Split-BF16 x3 kernels (layouts and arithmetic described in mx_bf16x3.h).
 #include "mx_bf16x3.h"
 
 namespace mx {
-
-__global__ void split_bf16_f64(const double *__restrict__ in, uint16_t *__restrict__ hi,
-                               uint16_t *__restrict__ lo, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) {
-        double x = in[i];
-        uint16_t h = bf16_rne((float)x);
-        hi[i] = h;
-        lo[i] = bf16_rne((float)(x - (double)bf16_f(h)));
-    }
-}
 
 __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__restrict__ blocks,
                                uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
@@ -41,41 +29,210 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
     }
 }
 
-__device__ __forceinline__ bf16x8_t ld_frag_global(const uint16_t *p) {
-    u32x4_t v = *reinterpret_cast<const u32x4_t *>(p);
-    return __builtin_bit_cast(bf16x8_t, v);
+// One thread per padded element of a leaf block: consecutive threads read
+// consecutive rows of one column (coalesced) and write consecutive k of one
+// 32-row chunk of the panel; rows past R are written as zeros.
+template <typename TW>
+__global__ void x3_split_w_kernel(const TW *__restrict__ W, const X3Rows *__restrict__ blocks, int nrhs,
+                                  uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+    const X3Rows b = blocks[blockIdx.y];
+    const int Rp = (b.R + 31) / 32 * 32;
+    const int64_t n = (int64_t)Rp * nrhs;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int col = (int)(t / Rp), k = (int)(t % Rp);
+        const double x = k < b.R ? (double)W[b.src_row * nrhs + (int64_t)col * b.R + k] : 0.0;
+        const uint16_t h = bf16_rne((float)x);
+        const int64_t o = b.prow * nrhs + ((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31);
+        hi[o] = h;
+        lo[o] = bf16_rne((float)(x - (double)bf16_f(h)));
+    }
 }
 
-__device__ __forceinline__ bf16x8_t ld_frag_lds(const uint16_t *p) {
-    u32x4_t v = *reinterpret_cast<const u32x4_t *>(p);
-    return __builtin_bit_cast(bf16x8_t, v);
+void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nblocks, int nrhs, uint16_t *hi,
+                       uint16_t *lo, hipStream_t st) {
+    dim3 grid(64, (unsigned)nblocks);
+    if (w_f64)
+        hipLaunchKernelGGL(x3_split_w_kernel<double>, grid, dim3(256), 0, st, (const double *)W, rows, nrhs, hi, lo);
+    else
+        hipLaunchKernelGGL(x3_split_w_kernel<float>, grid, dim3(256), 0, st, (const float *)W, rows, nrhs, hi, lo);
 }
 
-// One tile of C (128 x 128) += sum over the task's segments of A_seg * B_seg,
-// A packed split-BF16, B FP32 column blocks split on the fly.
-__global__ __launch_bounds__(X3_THREADS) void bf16x3_kernel(const Task *__restrict__ tasks,
-                                                            const Seg *__restrict__ segs,
-                                                            const int2 *__restrict__ tiles, int ntiles,
-                                                            int nrhs, int n_ntiles, X3Bufs bufs) {
-    constexpr int BS_ELEMS = X3_BN * X3_BKP;   // one of hi/lo
-    constexpr int LDS_BYTES_B = 2 * BS_ELEMS * 2;
-    constexpr int LDS_BYTES_C = X3_BN * X3_CP * 4;
-    constexpr int LDS_BYTES = LDS_BYTES_B > LDS_BYTES_C ? LDS_BYTES_B : LDS_BYTES_C;
-    __shared__ __attribute__((aligned(16))) unsigned char smem[LDS_BYTES];
-    uint16_t *Bhi = reinterpret_cast<uint16_t *>(smem);
-    uint16_t *Blo = Bhi + BS_ELEMS;
+__device__ __forceinline__ bf16x8_t ld16(const uint16_t *p) {
+    return __builtin_bit_cast(bf16x8_t, *reinterpret_cast<const u32x4_t *>(p));
+}
+
+__device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Main loop: X3_NST-stage pipeline through LDS.  Per 32-deep chunk the
+// workgroup needs 32 fragments of 1 KB (8 A row groups and 8 B column groups,
+// hi and lo); each wave moves 8 of them with global_load_lds_dwordx4, the
+// gfx950 direct global->LDS load (one instruction per fragment, lane l
+// writing bytes 16l..16l+15, no VGPRs).  Loads run X3_NST-1 chunks ahead of
+// the MFMAs.  Waves 0-1 load A row groups 0-3 / 4-7, waves 2-3 B column
+// groups 0-3 / 4-7.  Row groups past the block and columns past nrhs load a
+// valid neighbour (clamped) and are masked or discarded.
+constexpr int X3_NST = 4;
+constexpr int X3_FRAG = 512;              // uint16 per fragment (1 KB)
+constexpr int X3_STAGE = 32 * X3_FRAG;    // uint16 per stage (32 KB)
+
+struct X3Cursor {
+    int seg, c;
+};
+
+// next 32-deep chunk of the task's segment sequence (empty segments skipped)
+__device__ __forceinline__ bool x3_next(const Seg *segs, int seg_end, X3Cursor &cu) {
+    ++cu.c;
+    while (cu.seg < seg_end && cu.c * 32 >= segs[cu.seg].k) {
+        ++cu.seg;
+        cu.c = 0;
+    }
+    return cu.seg < seg_end;
+}
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+typedef const __attribute__((address_space(1))) void glb_void_t;
+
+__device__ __forceinline__ void lds_dma16(const uint16_t *src, uint16_t *dst_wave) {
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 16, 0, 0);
+}
+
+__device__ __forceinline__ void x3_issue(const X3Args &a, const Seg &sg, int c, uint16_t *stage, int wv, int lane,
+                                         int mtile16, int nfm, int n0) {
+    if (wv < 2) {
+        const int nfk = (sg.k + 31) / 32;
+        const uint16_t *Ah = a.ahi[sg.a_buf] + sg.a_off + lane * 8;
+        const uint16_t *Al = a.alo[sg.a_buf] + sg.a_off + lane * 8;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int rg = wv * 4 + t;
+            const int mi = min(mtile16 + rg, nfm - 1);
+            const int64_t off = ((int64_t)mi * nfk + c) * X3_FRAG;
+            lds_dma16(Ah + off, stage + (rg * 2 + 0) * X3_FRAG);
+            lds_dma16(Al + off, stage + (rg * 2 + 1) * X3_FRAG);
+        }
+    } else {
+        const int64_t pb = sg.b_row * (int64_t)a.nrhs + (int64_t)c * a.nrhs * 32 + 8 * (lane >> 4);
+        const uint16_t *Bh = a.phi[sg.b_buf] + pb;
+        const uint16_t *Bl = a.plo[sg.b_buf] + pb;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int cg = (wv - 2) * 4 + t;
+            const int col = min(n0 + cg * 16 + (lane & 15), a.nrhs - 1);
+            lds_dma16(Bh + (int64_t)col * 32, stage + (16 + cg * 2 + 0) * X3_FRAG);
+            lds_dma16(Bl + (int64_t)col * 32, stage + (16 + cg * 2 + 1) * X3_FRAG);
+        }
+    }
+}
+
+// wait until at most n of this wave's LDS-DMA loads are outstanding (n in 0..16 step 8)
+__device__ __forceinline__ void wait_dma(int n) {
+    if (n >= 16) __builtin_amdgcn_s_waitcnt(0x4F70);
+    else if (n >= 8) __builtin_amdgcn_s_waitcnt(0xF78);
+    else __builtin_amdgcn_s_waitcnt(0xF70);
+}
+
+// workgroup barrier without the memory fences of __syncthreads (which would
+// wait for every outstanding prefetch); the signal fences keep the compiler
+// from moving LDS accesses across it
+__device__ __forceinline__ void wg_barrier() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+// The 16 fragments a wave multiplies per chunk (its 4 A row groups and 4 B
+// column groups, hi and lo) are 8 KB runs at abase and bbase (LDS byte
+// offsets).  They are read in one asm block that waits for its own reads:
+// LDS reads the compiler can see would be ordered after every outstanding
+// LDS-DMA load, which would drain the pipeline each chunk.  Outputs are
+// early-clobber so no result register aliases an address operand.
+__device__ __forceinline__ void lds_read_frags(uint32_t abase, uint32_t bbase, u32x4_t (&A)[8], u32x4_t (&B)[8]) {
+    asm volatile(
+        "ds_read_b128 %0, %16 offset:0\n\t"
+        "ds_read_b128 %1, %16 offset:1024\n\t"
+        "ds_read_b128 %2, %16 offset:2048\n\t"
+        "ds_read_b128 %3, %16 offset:3072\n\t"
+        "ds_read_b128 %4, %16 offset:4096\n\t"
+        "ds_read_b128 %5, %16 offset:5120\n\t"
+        "ds_read_b128 %6, %16 offset:6144\n\t"
+        "ds_read_b128 %7, %16 offset:7168\n\t"
+        "ds_read_b128 %8, %17 offset:0\n\t"
+        "ds_read_b128 %9, %17 offset:1024\n\t"
+        "ds_read_b128 %10, %17 offset:2048\n\t"
+        "ds_read_b128 %11, %17 offset:3072\n\t"
+        "ds_read_b128 %12, %17 offset:4096\n\t"
+        "ds_read_b128 %13, %17 offset:5120\n\t"
+        "ds_read_b128 %14, %17 offset:6144\n\t"
+        "ds_read_b128 %15, %17 offset:7168\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(A[6]), "=&v"(A[7]), "=&v"(B[0]), "=&v"(B[1]), "=&v"(B[2]), "=&v"(B[3]), "=&v"(B[4]), "=&v"(B[5]), "=&v"(B[6]), "=&v"(B[7])
+        : "v"(abase), "v"(bbase)
+        : "memory");
+}
+
+__device__ __forceinline__ uint32_t lds_offset(const uint16_t *p) {
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint16_t *)p;
+}
+
+// Fragments of the current chunk from one stage, 48 MFMAs.
+__device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *stage, int lane, int wr, int wc,
+                                           int mtile16, int nfm) {
+    const uint32_t base = lds_offset(stage) + lane * 16;
+    u32x4_t A[8], B[8];
+    lds_read_frags(base + wr * 8192, base + 16384 + wc * 8192, A, B);
+    bf16x8_t ah[4], al[4], bh[4], bl[4];
+    const u32x4_t z = u32x4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const bool ok = mtile16 + wr * 4 + i < nfm;
+        ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
+        al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
+        bh[i] = __builtin_bit_cast(bf16x8_t, B[2 * i]);
+        bl[i] = __builtin_bit_cast(bf16x8_t, B[2 * i + 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+        }
+}
+
+__global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
+    // four pipeline stages as distinct LDS objects: with the loop unrolled by
+    // four every access names its stage, so the compiler can tell the DMA
+    // writes into one stage from the reads of another and does not make the
+    // reads wait for all outstanding loads
+    __shared__ __attribute__((aligned(16))) uint16_t st0[X3_STAGE];
+    __shared__ __attribute__((aligned(16))) uint16_t st1[X3_STAGE];
+    __shared__ __attribute__((aligned(16))) uint16_t st2[X3_STAGE];
+    __shared__ __attribute__((aligned(16))) uint16_t st3[X3_STAGE];
+    // the task's segment list, read from LDS inside the pipeline (a vector
+    // load there would wait for every outstanding LDS-DMA load)
+    __shared__ Seg sseg[X3_MAXSEG];
 
     const int tid = threadIdx.x;
-    const int total = ntiles * n_ntiles;
+    const int total = a.ntiles * a.n_ntiles;
     const int w = xcd_remap(blockIdx.x, total);
-    const int tile_id = w / n_ntiles, nt = w % n_ntiles;
-    const int2 tl = tiles[tile_id];
-    const Task task = tasks[tl.x];
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
     const int M = task.m;
     const int m0 = tl.y * X3_BM, n0 = nt * X3_BN;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = wv >> 1, wc = wv & 1;
     const int nfm = (M + 15) / 16;
+    const int mtile16 = m0 / 16;
+
+    const int nseg = task.seg_end - task.seg_begin;   // <= X3_MAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += X3_THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+    int nch = 0;   // 32-deep chunks of the task
+    for (int si = 0; si < nseg; ++si) nch += (sseg[si].k + 31) / 32;
 
     x3_f4 acc[4][4];
 #pragma unroll
@@ -83,111 +240,93 @@ __global__ __launch_bounds__(X3_THREADS) void bf16x3_kernel(const Task *__restri
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
 
-    // column-chunk loader: thread t covers column t/2, k half (t%2)*16
-    const int bcol = tid >> 1, bkh = (tid & 1) * 16;
-    const int gcol = n0 + bcol;
+    X3Cursor ld{0, -1};
+    int issued = 0;
+    auto issue = [&](uint16_t *stage) {
+        if (issued < nch) {
+            x3_next(sseg, nseg, ld);
+            x3_issue(a, sseg[ld.seg], ld.c, stage, wv, lane, mtile16, nfm, n0);
+            ++issued;
+        }
+    };
+    // one pipeline step: wait for chunk c, refill the stage freed by chunk
+    // c-1 with chunk c+3, multiply chunk c
+    auto step = [&](int c, const uint16_t *cur, uint16_t *refill) {
+        wait_dma(8 * (issued - c - 1));
+        wg_barrier();
+        issue(refill);
+        x3_compute(acc, cur, lane, wr, wc, mtile16, nfm);
+    };
+    issue(st0);
+    issue(st1);
+    issue(st2);
+    for (int c = 0; c < nch; c += 4) {
+        step(c, st0, st3);
+        if (c + 1 < nch) step(c + 1, st1, st0);
+        if (c + 2 < nch) step(c + 2, st2, st1);
+        if (c + 3 < nch) step(c + 3, st3, st2);
+    }
+    __syncthreads();
 
-    for (int si = task.seg_begin; si < task.seg_end; ++si) {
-        const Seg sg = segs[si];
-        const int K = sg.k;
-        const int nfk = (K + 31) / 32;
-        const uint16_t *Ahi = bufs.hi[sg.a_buf] + sg.a_off;
-        const uint16_t *Alo = bufs.lo[sg.a_buf] + sg.a_off;
-        const float *Bv = bufs.vec[sg.b_buf] + sg.b_row * (int64_t)nrhs;
-        for (int kt = 0; kt < nfk; ++kt) {
-            // stage the 32 x 128 column chunk, split into hi/lo
-            {
-                float x[16];
-                const int kb = kt * 32 + bkh;
-                const float *src = Bv + (int64_t)gcol * sg.b_ld + kb;
+    if (task.c_buf == VEC_Y) {
+        // each wave stages its 64 x 64 quarter in its own stage buffer and
+        // stores 64 consecutive rows per column
+        uint16_t *mine = wv == 0 ? st0 : wv == 1 ? st1 : wv == 2 ? st2 : st3;
+        float *Cs = reinterpret_cast<float *>(mine);
+        constexpr int CP = 65;
 #pragma unroll
-                for (int e = 0; e < 16; ++e) x[e] = (gcol < nrhs && kb + e < K) ? src[e] : 0.f;
-                uint16_t h[16], l[16];
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int e = 0; e < 16; ++e) {
-                    h[e] = bf16_rne(x[e]);
-                    l[e] = bf16_rne(x[e] - bf16_f(h[e]));
-                }
-                u32x4_t *dh = reinterpret_cast<u32x4_t *>(Bhi + bcol * X3_BKP + bkh);
-                u32x4_t *dl = reinterpret_cast<u32x4_t *>(Blo + bcol * X3_BKP + bkh);
+            for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    u32x4_t vh, vl;
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        vh[c] = (uint32_t)h[q * 8 + 2 * c] | ((uint32_t)h[q * 8 + 2 * c + 1] << 16);
-                        vl[c] = (uint32_t)l[q * 8 + 2 * c] | ((uint32_t)l[q * 8 + 2 * c + 1] << 16);
-                    }
-                    dh[q] = vh;
-                    dl[q] = vl;
-                }
+                for (int r = 0; r < 4; ++r)
+                    Cs[(j * 16 + (lane & 15)) * CP + i * 16 + 4 * (lane >> 4) + r] = acc[i][j][r];
+        __builtin_amdgcn_wave_barrier();
+        const int64_t base = task.c_row * (int64_t)a.nrhs;
+        const int rb = m0 + wr * 64, cb = n0 + wc * 64;
+        for (int e = lane; e < 64 * 64; e += 64) {
+            int col = e >> 6, row = e & 63;
+            int gm = rb + row, gn = cb + col;
+            if (gm < M && gn < a.nrhs) {
+                float v = Cs[col * CP + row];
+                int64_t o = base + (int64_t)gn * M + gm;
+                if (a.y_f64) static_cast<double *>(a.Y)[o] = (double)v;
+                else static_cast<float *>(a.Y)[o] = v;
             }
-            // A fragments of this wave's 4 row groups
-            bf16x8_t ahi[4], alo[4];
+        }
+    } else {
+        // split-panel output (rows up to the padded height; padding rows are 0)
+        const int Mp = (M + 31) / 32 * 32;
+        uint16_t *Ph = a.phi[task.c_buf] + task.c_row * (int64_t)a.nrhs;
+        uint16_t *Pl = a.plo[task.c_buf] + task.c_row * (int64_t)a.nrhs;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                int mi = (m0 + wr * 64) / 16 + i;
-                if (mi < nfm) {
-                    int64_t off = ((int64_t)mi * nfk + kt) * 512 + lane * 8;
-                    ahi[i] = ld_frag_global(Ahi + off);
-                    alo[i] = ld_frag_global(Alo + off);
-                } else {
-                    ahi[i] = __builtin_bit_cast(bf16x8_t, u32x4_t{0, 0, 0, 0});
-                    alo[i] = ahi[i];
-                }
-            }
-            __syncthreads();
-            bf16x8_t bhi[4], blo[4];
+        for (int i = 0; i < 4; ++i) {
+            const int r0 = m0 + wr * 64 + i * 16 + 4 * (lane >> 4);
+            if (r0 >= Mp) continue;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                int col = wc * 64 + j * 16 + (lane & 15);
-                int kk = 8 * (lane >> 4);
-                bhi[j] = ld_frag_lds(Bhi + col * X3_BKP + kk);
-                blo[j] = ld_frag_lds(Blo + col * X3_BKP + kk);
-            }
+                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+                if (col >= a.nrhs) continue;
+                uint16_t h[4], l[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    acc[i][j] = mfma_bf16(alo[i], bhi[j], acc[i][j]);
-                    acc[i][j] = mfma_bf16(ahi[i], blo[j], acc[i][j]);
-                    acc[i][j] = mfma_bf16(ahi[i], bhi[j], acc[i][j]);
+                for (int r = 0; r < 4; ++r) {
+                    h[r] = bf16_rne(acc[i][j][r]);
+                    l[r] = bf16_rne(acc[i][j][r] - bf16_f(h[r]));
                 }
-            __syncthreads();
-        }
-    }
-
-    float *Cs = reinterpret_cast<float *>(smem);
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int row = wr * 64 + i * 16 + 4 * (lane >> 4) + r;
-                int col = wc * 64 + j * 16 + (lane & 15);
-                Cs[col * X3_CP + row] = acc[i][j][r];
+                const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
+                *reinterpret_cast<u32x2_t *>(Ph + o) =
+                    u32x2_t{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+                *reinterpret_cast<u32x2_t *>(Pl + o) =
+                    u32x2_t{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
             }
-    __syncthreads();
-    float *C = bufs.vec[task.c_buf] + task.c_row * (int64_t)nrhs;
-    for (int e = tid; e < X3_BM * X3_BN; e += X3_THREADS) {
-        int col = e / X3_BM, row = e % X3_BM;
-        int gm = m0 + row, gn = n0 + col;
-        if (gm < M && gn < nrhs) {
-            float v = Cs[col * X3_CP + row];
-            float *p = C + (int64_t)gn * M + gm;
-            *p = task.beta ? *p + v : v;
         }
     }
 }
 
-void launch_bf16x3(int ntiles, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs, const X3Bufs &b,
-                   hipStream_t st) {
-    int nnt = (nrhs + X3_BN - 1) / X3_BN;
-    int64_t total = (int64_t)ntiles * nnt;
+void launch_x3(const X3Args &a, hipStream_t st) {
+    int64_t total = (int64_t)a.ntiles * a.n_ntiles;
     if (total <= 0) return;
-    hipLaunchKernelGGL(bf16x3_kernel, dim3((unsigned)total), dim3(X3_THREADS), 0, st, tasks, segs, tiles, ntiles,
-                       nrhs, nnt, b);
+    hipLaunchKernelGGL(x3_kernel, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
 }
 
 }  // namespace mx

@@ -72,6 +72,11 @@ struct mx_plan {
     int2 *d_tiles_x3 = nullptr;
     std::vector<mx::Launch> launches_x3;
     std::vector<int> launch_kind_x3;    // 1 for every launch (BF16x3 kernel)
+    std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
+    int64_t wp_rows = 0, sp_rows = 0;
+    mx::X3Rows *d_wrows = nullptr;
+    uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
+    size_t panel_bytes = 0;
 
     // stored blocks per precision
     double *D64 = nullptr, *B64 = nullptr, *VT64 = nullptr;
@@ -105,7 +110,8 @@ struct mx_plan {
 
     ~mx_plan() {
         void *ptrs[] = {d_tasks, d_segs, d_tiles, d_tasks_x3, d_segs_x3, d_tiles_x3, D64, B64, VT64, D32, B32,
-                        VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf};
+                        VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
+                        d_wrows, panels};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -238,7 +244,11 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
 // beta_u / beta_y select the accumulation rule of the coupling and leaf
 // tasks (0 for mx_eval, the reference's rules for the drop-in entry points).
-static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind) {
+// wrow_in / skel_in give the row offsets of W blocks and skeleton blocks in
+// the buffers the kernel reads (reference layout for kind 0, split panels for
+// kind 1); Y always uses the reference layout.
+static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind,
+                               const int64_t *wrow_in, const int64_t *skel_in) {
     TaskLists L;
     const int nn = s.numnodes;
     // far pairs by destination node
@@ -300,15 +310,15 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         for (int v : t.levels[d]) {
             int sv = s.slen[v];
             if (sv == 0) continue;
-            int tk = new_task(s.skelrow[v], sv, VEC_WSKEL, 0);
+            int tk = new_task(skel_in[v], sv, VEC_WSKEL, 0);
             if (s.lchildren[v] == -1) {
                 int lf = s.lm[v];
-                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.Ddim[lf], VEC_W, s.wrow[lf], s.Ddim[lf]);
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.Ddim[lf], VEC_W, wrow_in[lf], s.Ddim[lf]);
             } else {
                 int l = s.lchildren[v], r = s.rchildren[v];
-                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.slen[l], VEC_WSKEL, s.skelrow[l], s.slen[l]);
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.slen[l], VEC_WSKEL, skel_in[l], s.slen[l]);
                 add_seg(tk, (int64_t)s.VTptr[v] + (int64_t)sv * s.slen[l], MAT_VT, 0, sv, s.slen[r], VEC_WSKEL,
-                        s.skelrow[r], s.slen[r]);
+                        skel_in[r], s.slen[r]);
             }
             add_tiles(L, tk, sv);
         }
@@ -320,17 +330,17 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         for (int c : t.levels[d]) {
             int sc = s.slen[c];
             if (sc == 0) continue;
-            int tk = new_task(s.skelrow[c], sc, VEC_USKEL, beta_u);
+            int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
             for (int64_t k : far_of[c]) {
                 int y = far_partner(k);
-                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, s.skelrow[y], s.slen[y]);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
             }
             int p = t.parent[c];
             if (p > 0 && s.slen[p] > 0) {
                 int sp = s.slen[p];
                 int64_t off = s.VTptr[p];
                 if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
-                add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, s.skelrow[p], sp);
+                add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
             }
             add_tiles(L, tk, sc);
         }
@@ -343,11 +353,11 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
         for (int64_t k : near_of[lf]) {
             int y = s.is_hss ? (int)k : s.npairy[k];
-            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, s.wrow[y], s.Ddim[y]);
+            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
         }
         int v = t.node_of_leaf[lf];
         int sv = s.slen[v];
-        if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, s.skelrow[v], sv);
+        if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, skel_in[v], sv);
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
@@ -419,14 +429,33 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     P->VT_len = s.VT_len;
     P->precisions = precisions;
 
-    TaskLists L = build_forward(s, t, beta_u, beta_y, 0);
+    std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
+    TaskLists L = build_forward(s, t, beta_u, beta_y, 0, wrow64.data(), skel64.data());
     compute_launch_flops(L);
     upload_lists(L, &P->d_tasks, &P->d_segs, &P->d_tiles);
     P->launches = L.launches;
     P->launch_flops = L.flops;
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     if (precisions & MX_PLAN_BF16X3) {
-        TaskLists X = build_forward(s, t, beta_u, beta_y, 1);
+        // split panels: padded row offsets of W leaf blocks and skeleton blocks
+        P->wprow.assign(s.nleaf, 0);
+        P->sprow.assign(s.numnodes, 0);
+        int64_t r = 0;
+        for (int l = 0; l < s.nleaf; ++l) {
+            P->wprow[l] = r;
+            r += x3_pad_rows(s.Ddim[l]);
+        }
+        P->wp_rows = r;
+        r = 0;
+        for (int v = 0; v < s.numnodes; ++v) {
+            P->sprow[v] = r;
+            r += x3_pad_rows(s.slen[v]);
+        }
+        P->sp_rows = r;
+        std::vector<X3Rows> wr(s.nleaf);
+        for (int l = 0; l < s.nleaf; ++l) wr[l] = X3Rows{s.wrow[l], P->wprow[l], s.Ddim[l], 0};
+        P->d_wrows = upload(wr.data(), wr.size());
+        TaskLists X = build_forward(s, t, beta_u, beta_y, 1, P->wprow.data(), P->sprow.data());
         // every A operand is read from packed split-BF16 fragments: assign each
         // distinct (buffer, offset, orientation, shape) its packed offset
         std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
@@ -458,6 +487,10 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
                 sg.lda = (sg.k + 31) / 32;
             }
         }
+        for (size_t ti = 0; ti < X.tasks.size(); ++ti)
+            if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
+                throw MxError(MX_ERR_UNSUPPORTED, "a block has more than " + std::to_string(X3_MAXSEG) +
+                                                      " interactions; the BF16x3 kernel cannot hold its list");
         for (int w = 0; w < 3; ++w) P->x3_elems[w] = next[w];
         compute_launch_flops(X);
         P->launch_flops_x3 = X.flops;
@@ -528,6 +561,16 @@ static void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
     }
 }
 
+static void ensure_panels(mx_plan *P, int nrhs) {
+    size_t bytes = ((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t);
+    if (bytes > P->panel_bytes) {
+        if (P->panels) HIPCHK(hipFree(P->panels));
+        P->panels = nullptr;
+        HIPCHK(hipMalloc(&P->panels, bytes));
+        P->panel_bytes = bytes;
+    }
+}
+
 template <typename T>
 static void run_seg_launch(const Launch &ln, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs,
                            const Bufs<T> &b, hipStream_t st) {
@@ -584,7 +627,7 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
             run_seg_launch<double>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
             prof_mark(P, st, MX_KIND_SEG_F64, false);
         }
-    } else {
+    } else if (prec == MX_FP32) {
         ensure_work(P, nrhs, sizeof(float), io == MX_IO_F64);
         const float *Wf;
         float *Yf;
@@ -606,30 +649,46 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         b.vec[VEC_WSKEL] = (float *)P->wskel;
         b.vec[VEC_USKEL] = (float *)P->uskel;
         b.vec[VEC_Y] = Yf;
-        if (prec == MX_FP32) {
-            for (size_t i = 0; i < P->launches.size(); ++i) {
-                prof_mark(P, st, MX_KIND_SEG_F32, true);
-                run_seg_launch<float>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
-                prof_mark(P, st, MX_KIND_SEG_F32, false);
-            }
-        } else {
-            X3Bufs xb;
-            for (int w = 0; w < 3; ++w) {
-                xb.hi[w] = P->xhi[w];
-                xb.lo[w] = P->xlo[w];
-            }
-            for (int i = 0; i < 4; ++i) xb.vec[i] = b.vec[i];
-            for (size_t i = 0; i < P->launches_x3.size(); ++i) {
-                prof_mark(P, st, MX_KIND_BF16X3, true);
-                launch_bf16x3(P->launches_x3[i].ntiles, P->d_tasks_x3, P->d_segs_x3,
-                              P->d_tiles_x3 + P->launches_x3[i].tile_offset, nrhs, xb, st);
-                prof_mark(P, st, MX_KIND_BF16X3, false);
-            }
+        for (size_t i = 0; i < P->launches.size(); ++i) {
+            prof_mark(P, st, MX_KIND_SEG_F32, true);
+            run_seg_launch<float>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+            prof_mark(P, st, MX_KIND_SEG_F32, false);
         }
         if (io == MX_IO_F64) {
             prof_mark(P, st, MX_KIND_CONVERT, true);
             cvt_f32_f64<<<cvt_grid(nel), 256, 0, st>>>(Yf, (double *)Y, nel);
             prof_mark(P, st, MX_KIND_CONVERT, false);
+        }
+    } else {
+        ensure_panels(P, nrhs);
+        const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
+        X3Args xa{};
+        xa.nrhs = nrhs;
+        xa.n_ntiles = (nrhs + X3_BN - 1) / X3_BN;
+        xa.y_f64 = io == MX_IO_F64;
+        xa.Y = Y;
+        xa.tasks = P->d_tasks_x3;
+        xa.segs = P->d_segs_x3;
+        for (int w = 0; w < 3; ++w) {
+            xa.ahi[w] = P->xhi[w];
+            xa.alo[w] = P->xlo[w];
+        }
+        uint16_t *pw = P->panels;
+        xa.phi[VEC_W] = pw;
+        xa.plo[VEC_W] = pw + wp;
+        xa.phi[VEC_WSKEL] = pw + 2 * wp;
+        xa.plo[VEC_WSKEL] = pw + 2 * wp + sp;
+        xa.phi[VEC_USKEL] = pw + 2 * wp + 2 * sp;
+        xa.plo[VEC_USKEL] = pw + 2 * wp + 3 * sp;
+        prof_mark(P, st, MX_KIND_CONVERT, true);
+        launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
+        prof_mark(P, st, MX_KIND_CONVERT, false);
+        for (size_t i = 0; i < P->launches_x3.size(); ++i) {
+            xa.tiles = P->d_tiles_x3 + P->launches_x3[i].tile_offset;
+            xa.ntiles = P->launches_x3[i].ntiles;
+            prof_mark(P, st, MX_KIND_BF16X3, true);
+            launch_x3(xa, st);
+            prof_mark(P, st, MX_KIND_BF16X3, false);
         }
     }
     HIPCHK(hipGetLastError());
@@ -936,6 +995,8 @@ mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uske
     return (mx_status)guard([&]() {
         arg_check(plan && wskel && uskel, "null argument");
         if (plan->last_prec < 0) throw MxError(MX_ERR_STATE, "no evaluation has run on this plan");
+        if (plan->last_prec == MX_BF16X3)
+            throw MxError(MX_ERR_UNSUPPORTED, "split-BF16x3 evaluations keep skeletons as split panels");
         *wskel = plan->wskel;
         *uskel = plan->uskel;
         return 0;
@@ -946,6 +1007,8 @@ mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst
     return (mx_status)guard([&]() {
         arg_check(plan && wskel_dst && uskel_dst && nrhs >= 1, "bad argument");
         if (plan->last_prec < 0) throw MxError(MX_ERR_STATE, "no evaluation has run on this plan");
+        if (plan->last_prec == MX_BF16X3)
+            throw MxError(MX_ERR_UNSUPPORTED, "split-BF16x3 evaluations keep skeletons as split panels");
         size_t elem = plan->last_prec == MX_FP64 ? sizeof(double) : sizeof(float);
         size_t bytes = (size_t)plan->skel_rows * nrhs * elem;
         arg_check(bytes <= plan->skel_bytes, "nrhs exceeds the last evaluation's buffers");
@@ -1012,6 +1075,17 @@ mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, do
             for (double f : plan->launch_flops) flops[k] += f * nrhs;
         }
         *total = flops[0] + flops[1] + flops[2] + flops[3];
+        return 0;
+    });
+}
+
+mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, int32_t cap,
+                               int32_t *count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && count && cap >= 0 && (cap == 0 || flops), "bad argument");
+        const std::vector<double> &f = prec == MX_BF16X3 ? plan->launch_flops_x3 : plan->launch_flops;
+        *count = (int32_t)f.size();
+        for (int32_t i = 0; i < (int32_t)f.size() && i < cap; ++i) flops[i] = f[i] * nrhs;
         return 0;
     });
 }

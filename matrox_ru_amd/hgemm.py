@@ -76,6 +76,8 @@ def lib():
         L.mx_plan_launch_times.restype = ctypes.c_int
         L.mx_plan_flops.argtypes = [P, ctypes.c_int, I32, P, ctypes.POINTER(ctypes.c_double)]
         L.mx_plan_flops.restype = ctypes.c_int
+        L.mx_plan_launch_flops.argtypes = [P, ctypes.c_int, I32, P, I32, ctypes.POINTER(I32)]
+        L.mx_plan_launch_flops.restype = ctypes.c_int
         _configured = True
     return L
 
@@ -213,6 +215,14 @@ class Plan:
         tot = ctypes.c_double()
         _check(lib().mx_plan_flops(self._h, PRECISIONS[precision], nrhs, f, ctypes.byref(tot)))
         return tot.value, {KIND_NAMES[i]: f[i] for i in range(4) if f[i] > 0}
+
+    def launch_flops(self, precision, nrhs):
+        """Algorithmic flops of each kernel launch of one evaluation."""
+        cap = 4096
+        f = (ctypes.c_double * cap)()
+        cnt = I32()
+        _check(lib().mx_plan_launch_flops(self._h, PRECISIONS[precision], nrhs, f, cap, ctypes.byref(cnt)))
+        return [f[i] for i in range(min(cnt.value, cap))]
 
     def close(self):
         if getattr(self, "_h", None):
