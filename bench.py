@@ -161,6 +161,13 @@ def main():
                 "flops_per_launch": dom_flops_per_launch}
     if dom == "bf16x3":
         roofline["frac_effective_x3"] = round(achieved / (peak / 3), 5)
+    tfile = os.path.join(ROOT, "profiles", "traffic_h2b_n65536.txt")
+    if nrhs == WORKLOAD["nrhs"] and os.path.exists(tfile):
+        kv = dict(l.strip().split("=", 1) for l in open(tfile) if "=" in l and not l.startswith("#"))
+        if kv.get("kernel") == dom:
+            roofline["traffic"] = float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
+            roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/x3_kernel_h2b_n65536.md"
+            roofline["algorithmic_bytes_per_launch"] = round(4.67e9 / kind_launches[dom], -6)
     kinds_report = {k: {"ms_per_eval": round(kind_total[k], 4), "launches": kind_launches[k],
                         "tflops": round(parts.get(k, 0.0) / (kind_total[k] * 1e-3) / 1e12, 3)
                         if parts.get(k) else None} for k in kind_total}

@@ -8,7 +8,7 @@
 // accumulation.
 //
 // Both operands live in HBM already split and in MFMA fragment order, so the
-// main loop is 16-byte loads straight into registers and MFMAs:
+// main loop only moves 1 KB fragments and issues MFMAs:
 //
 // * A operands (stored blocks, and transposed views of VT) are packed when
 //   the plan is built: op(A) (M x K) is cut into 16 x 32 fragments; fragment
@@ -22,8 +22,8 @@
 //   launch that produces them.
 //
 // Workgroup: 256 threads = 4 waves as 2 x 2, tile 128 x 128, each wave
-// 64 x 64 = 4 x 4 fragments (48 MFMA per 32-deep chunk); the fragments of
-// chunk c+1 are loaded while chunk c is multiplied.
+// 64 x 64 = 4 x 4 fragments (48 MFMA per 32-deep chunk); fragments move
+// global -> LDS by direct LDS-DMA loads ahead of the MFMAs (see the kernel).
 // MFMA 16x16x32 bf16 operand layout (lane l): A(row l%16, k 8*(l/16)+e),
 // B(k 8*(l/16)+e, col l%16), e = 0..7; result register r holds
 // C(4*(l/16)+r, l%16).
@@ -95,6 +95,7 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
 void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_blocks, int nrhs, uint16_t *hi,
                        uint16_t *lo, hipStream_t st);
 
-void launch_x3(const X3Args &a, hipStream_t st);
+// stages: 2 or 4 pipeline stages (see the kernel)
+void launch_x3(const X3Args &a, int stages, hipStream_t st);
 
 }  // namespace mx

@@ -65,15 +65,16 @@ __device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Main loop: X3_NST-stage pipeline through LDS.  Per 32-deep chunk the
+// Main loop: a pipeline of NST stages through LDS.  Per 32-deep chunk the
 // workgroup needs 32 fragments of 1 KB (8 A row groups and 8 B column groups,
 // hi and lo); each wave moves 8 of them with global_load_lds_dwordx4, the
 // gfx950 direct global->LDS load (one instruction per fragment, lane l
-// writing bytes 16l..16l+15, no VGPRs).  Loads run X3_NST-1 chunks ahead of
-// the MFMAs.  Waves 0-1 load A row groups 0-3 / 4-7, waves 2-3 B column
-// groups 0-3 / 4-7.  Row groups past the block and columns past nrhs load a
-// valid neighbour (clamped) and are masked or discarded.
-constexpr int X3_NST = 4;
+// writing bytes 16l..16l+15, no VGPRs).  Loads run NST-1 chunks ahead of the
+// MFMAs.  Waves 0-1 load A row groups 0-3 / 4-7, waves 2-3 B column groups
+// 0-3 / 4-7.  Row groups past the block and columns past nrhs load a valid
+// neighbour (clamped) and are masked or discarded.  With NST = 2 (64 KB of
+// stages) two workgroups share a CU, so one's LDS reads overlap the other's
+// MFMAs; this measured faster than NST = 4 with one workgroup per CU.
 constexpr int X3_FRAG = 512;              // uint16 per fragment (1 KB)
 constexpr int X3_STAGE = 32 * X3_FRAG;    // uint16 per stage (32 KB)
 
@@ -202,15 +203,11 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *s
         }
 }
 
+// NST-stage pipeline (NST = 2: 64 KB of stages, two workgroups per CU; NST
+// = 4: 128 KB, one workgroup per CU and three chunks of prefetch).
+template <int NST>
 __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
-    // four pipeline stages as distinct LDS objects: with the loop unrolled by
-    // four every access names its stage, so the compiler can tell the DMA
-    // writes into one stage from the reads of another and does not make the
-    // reads wait for all outstanding loads
-    __shared__ __attribute__((aligned(16))) uint16_t st0[X3_STAGE];
-    __shared__ __attribute__((aligned(16))) uint16_t st1[X3_STAGE];
-    __shared__ __attribute__((aligned(16))) uint16_t st2[X3_STAGE];
-    __shared__ __attribute__((aligned(16))) uint16_t st3[X3_STAGE];
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * X3_STAGE];
     // the task's segment list, read from LDS inside the pipeline (a vector
     // load there would wait for every outstanding LDS-DMA load)
     __shared__ Seg sseg[X3_MAXSEG];
@@ -242,54 +239,48 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
 
     X3Cursor ld{0, -1};
     int issued = 0;
-    auto issue = [&](uint16_t *stage) {
+    auto issue = [&]() {
         if (issued < nch) {
             x3_next(sseg, nseg, ld);
-            x3_issue(a, sseg[ld.seg], ld.c, stage, wv, lane, mtile16, nfm, n0);
+            x3_issue(a, sseg[ld.seg], ld.c, stg + (issued % NST) * X3_STAGE, wv, lane, mtile16, nfm, n0);
             ++issued;
         }
     };
-    // one pipeline step: wait for chunk c, refill the stage freed by chunk
-    // c-1 with chunk c+3, multiply chunk c
-    auto step = [&](int c, const uint16_t *cur, uint16_t *refill) {
+#pragma unroll
+    for (int p = 0; p < NST - 1; ++p) issue();
+    for (int c = 0; c < nch; ++c) {
+        // chunk c has landed (this wave's part, then everyone's), and every
+        // wave is done with the stage chunk c-1 used, which chunk c+NST-1 refills
         wait_dma(8 * (issued - c - 1));
         wg_barrier();
-        issue(refill);
-        x3_compute(acc, cur, lane, wr, wc, mtile16, nfm);
-    };
-    issue(st0);
-    issue(st1);
-    issue(st2);
-    for (int c = 0; c < nch; c += 4) {
-        step(c, st0, st3);
-        if (c + 1 < nch) step(c + 1, st1, st0);
-        if (c + 2 < nch) step(c + 2, st2, st1);
-        if (c + 3 < nch) step(c + 3, st3, st2);
+        issue();
+        x3_compute(acc, stg + (c % NST) * X3_STAGE, lane, wr, wc, mtile16, nfm);
     }
     __syncthreads();
 
     if (task.c_buf == VEC_Y) {
-        // each wave stages its 64 x 64 quarter in its own stage buffer and
-        // stores 64 consecutive rows per column
-        uint16_t *mine = wv == 0 ? st0 : wv == 1 ? st1 : wv == 2 ? st2 : st3;
-        float *Cs = reinterpret_cast<float *>(mine);
-        constexpr int CP = 65;
+        // each wave stages its 64 x 64 quarter (16 KB; rows XOR-swizzled by
+        // column so the 16 lanes of a write hit distinct banks) and stores 64
+        // consecutive rows per column
+        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    Cs[(j * 16 + (lane & 15)) * CP + i * 16 + 4 * (lane >> 4) + r] = acc[i][j][r];
+                for (int r = 0; r < 4; ++r) {
+                    const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + r;
+                    Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[i][j][r];
+                }
         __builtin_amdgcn_wave_barrier();
         const int64_t base = task.c_row * (int64_t)a.nrhs;
         const int rb = m0 + wr * 64, cb = n0 + wc * 64;
         for (int e = lane; e < 64 * 64; e += 64) {
-            int col = e >> 6, row = e & 63;
-            int gm = rb + row, gn = cb + col;
+            const int col = e >> 6, row = e & 63;
+            const int gm = rb + row, gn = cb + col;
             if (gm < M && gn < a.nrhs) {
-                float v = Cs[col * CP + row];
-                int64_t o = base + (int64_t)gn * M + gm;
+                const float v = Cs[col * 64 + (row ^ ((col & 15) << 2))];
+                const int64_t o = base + (int64_t)gn * M + gm;
                 if (a.y_f64) static_cast<double *>(a.Y)[o] = (double)v;
                 else static_cast<float *>(a.Y)[o] = v;
             }
@@ -323,10 +314,11 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
     }
 }
 
-void launch_x3(const X3Args &a, hipStream_t st) {
+void launch_x3(const X3Args &a, int stages, hipStream_t st) {
     int64_t total = (int64_t)a.ntiles * a.n_ntiles;
     if (total <= 0) return;
-    hipLaunchKernelGGL(x3_kernel, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
+    if (stages == 4) hipLaunchKernelGGL(x3_kernel<4>, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
+    else hipLaunchKernelGGL(x3_kernel<2>, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
 }
 
 }  // namespace mx

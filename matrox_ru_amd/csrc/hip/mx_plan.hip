@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -74,6 +75,7 @@ struct mx_plan {
     std::vector<int> launch_kind_x3;    // 1 for every launch (BF16x3 kernel)
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
+    int x3_stages = 2;                  // pipeline depth of the BF16x3 kernel (MX_X3_STAGES=2|4)
     mx::X3Rows *d_wrows = nullptr;
     uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
     size_t panel_bytes = 0;
@@ -428,6 +430,7 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     P->B_len = s.B_len;
     P->VT_len = s.VT_len;
     P->precisions = precisions;
+    if (const char *e = getenv("MX_X3_STAGES")) P->x3_stages = atoi(e) == 4 ? 4 : 2;
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     TaskLists L = build_forward(s, t, beta_u, beta_y, 0, wrow64.data(), skel64.data());
@@ -687,7 +690,7 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
             xa.tiles = P->d_tiles_x3 + P->launches_x3[i].tile_offset;
             xa.ntiles = P->launches_x3[i].ntiles;
             prof_mark(P, st, MX_KIND_BF16X3, true);
-            launch_x3(xa, st);
+            launch_x3(xa, P->x3_stages, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
     }
