@@ -106,6 +106,20 @@ int mxh_gaussian_mixture(double *X, int64_t n, int32_t d, int32_t ncomp, double 
 int mxh_normal(double *out, int64_t count, uint64_t seed);
 int mxh_normal_f32(float *out, int64_t count, uint64_t seed);
 
+/* The reference's on-disk structure (one raw array per file: levelset.bin,
+ * idx.bin, clevelset.bin, wpart.bin, cidx.bin, lchildren.bin, rchildren.bin,
+ * nblockset.bin, nblocks.bin, nxval.bin, nyval.bin, fblockset.bin,
+ * fblocks.bin, fxval.bin, fyval.bin, dim.bin, leafmap.bin, lids.bin,
+ * lidsoffset.bin, lidslen.bin, NN.bin, distNN.bin), as written by
+ * savetree2disk/SaveSampling (sympiler/HTree.cpp:234-249, 469-542) and read by
+ * readHtree (matroxTest-V1/HTree.h:76-260).  mxh_save_bin writes the set of a
+ * handle into an existing directory; mxh_load_bin reads a set (for example
+ * one written by the reference), checks it, and compresses and packs it with
+ * cfg over the points X; the saved pair order, level sets and partitions are
+ * kept.  An HSS set has no blocking files. */
+int mxh_save_bin(const mxh_handle *h, const char *dir);
+mxh_handle *mxh_load_bin(const char *dir, const double *X, int32_t n, int32_t d, const mxh_config *cfg);
+
 const char *mxh_last_error(void);
 
 #ifdef __cplusplus

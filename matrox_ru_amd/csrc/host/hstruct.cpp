@@ -978,6 +978,231 @@ static Handle *build(const double *X, int n, int d, const mxh_config *cfg, bool 
     return H.release();
 }
 
+
+// ---------------------------------------------------------------------------
+// The reference's on-disk structure: raw int32 (and double) arrays, one file
+// per array, written by savetree2disk / SaveSampling (sympiler/HTree.cpp:
+// 234-249, 469-542) and read by readHtree (matroxTest-V1/HTree.h:76-260).
+static void write_raw(const std::string &path, const void *data, size_t bytes) {
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    size_t w = bytes ? fwrite(data, 1, bytes, f) : 0;
+    fclose(f);
+    if (w != bytes) throw std::runtime_error("short write to " + path);
+}
+
+template <class T>
+static bool read_raw(const std::string &path, std::vector<T> &out) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    if (sz < 0 || sz % (long)sizeof(T) != 0) {
+        fclose(f);
+        throw std::runtime_error(path + ": size is not a whole number of elements");
+    }
+    out.resize((size_t)sz / sizeof(T));
+    size_t r = sz ? fread(out.data(), 1, (size_t)sz, f) : 0;
+    fclose(f);
+    if (r != (size_t)sz) throw std::runtime_error("short read from " + path);
+    return true;
+}
+
+static void save_bin(const Handle &H, const std::string &dir) {
+    auto W = [&](const char *name, const std::vector<int32_t> &v) {
+        write_raw(dir + "/" + name, v.data(), v.size() * sizeof(int32_t));
+    };
+    auto &I = H.i32;
+    W("levelset.bin", I.at("levelset"));
+    W("idx.bin", I.at("idx_level"));
+    W("clevelset.bin", I.at("clevelset"));
+    W("wpart.bin", I.at("wpart"));
+    W("cidx.bin", I.at("idx"));
+    W("lchildren.bin", I.at("lchildren"));
+    W("rchildren.bin", I.at("rchildren"));
+    if (!H.is_hss) {
+        W("nblockset.bin", I.at("nblockSet"));
+        W("nblocks.bin", I.at("nblocks"));
+        W("nxval.bin", I.at("npairx"));
+        W("nyval.bin", I.at("npairy"));
+        W("fblockset.bin", I.at("fblockSet"));
+        W("fblocks.bin", I.at("fblocks"));
+        W("fxval.bin", I.at("fpairx"));
+        W("fyval.bin", I.at("fpairy"));
+    } else {
+        W("nxval.bin", I.at("flat_nx"));
+        W("nyval.bin", I.at("flat_ny"));
+        W("fxval.bin", I.at("flat_fx"));
+        W("fyval.bin", I.at("flat_fy"));
+    }
+    W("dim.bin", I.at("Ddim"));
+    W("leafmap.bin", I.at("lm"));
+    W("lids.bin", I.at("lids"));
+    W("lidsoffset.bin", I.at("lidsoffset"));
+    W("lidslen.bin", I.at("lidslen"));
+    std::vector<int32_t> nn(H.NN.size());
+    std::vector<double> dist(H.NN.size());
+    for (size_t i = 0; i < H.NN.size(); ++i) {
+        nn[i] = H.NN[i].second;
+        dist[i] = H.NN[i].first;
+    }
+    W("NN.bin", nn);
+    write_raw(dir + "/distNN.bin", dist.data(), dist.size() * sizeof(double));
+}
+
+// Rebuild a handle from a saved structure (the reference's file set) and the
+// points, then compress and pack it.  The blocked pair arrays, level sets and
+// coarsened partitions are taken from the files as they are, so D and B are
+// packed in the saved pair order.
+static Handle *load_bin(const std::string &dir, const double *X, int n, int d, const mxh_config &cfg) {
+    std::unique_ptr<Handle> H(new Handle());
+    H->cfg = cfg;
+    validate(cfg, n, d);
+    H->n = n;
+    H->d = d;
+    H->X.assign(X, X + (size_t)n * d);
+    H->f64["X"] = H->X;
+    auto R = [&](const char *name, std::vector<int32_t> &v, bool required) {
+        bool ok = read_raw(dir + "/" + name, v);
+        if (!ok && required) throw std::runtime_error(std::string("missing ") + name);
+        return ok;
+    };
+    std::vector<int32_t> dim, lm, lids, lo, ll, lc, rc, nx, ny, fx, fy, nbs, nbk, fbs, fbk, cl, wp, cidx, lvl, idx,
+        nnidx;
+    R("dim.bin", dim, true);
+    R("leafmap.bin", lm, true);
+    R("lids.bin", lids, true);
+    R("lidsoffset.bin", lo, true);
+    R("lidslen.bin", ll, true);
+    R("lchildren.bin", lc, true);
+    R("rchildren.bin", rc, true);
+    R("nxval.bin", nx, true);
+    R("nyval.bin", ny, true);
+    R("fxval.bin", fx, true);
+    R("fyval.bin", fy, true);
+    bool blocked = R("nblockset.bin", nbs, false);
+    if (blocked) {
+        R("nblocks.bin", nbk, true);
+        R("fblockset.bin", fbs, true);
+        R("fblocks.bin", fbk, true);
+    }
+    R("clevelset.bin", cl, true);
+    R("wpart.bin", wp, true);
+    R("cidx.bin", cidx, true);
+    R("levelset.bin", lvl, true);
+    R("idx.bin", idx, true);
+    R("NN.bin", nnidx, true);
+    std::vector<double> dist;
+    if (!read_raw(dir + "/distNN.bin", dist)) throw std::runtime_error("missing distNN.bin");
+
+    const int L = (int)dim.size();
+    int depth = 0;
+    while ((1 << depth) < L) ++depth;
+    if (L < 2 || (1 << depth) != L) throw std::runtime_error("leaf count must be a power of two (perfect tree)");
+    const int nn = 2 * L - 1, f = L - 1;
+    if ((int)lc.size() != nn || (int)rc.size() != nn || (int)lm.size() != nn || (int)lo.size() != nn ||
+        (int)ll.size() != nn)
+        throw std::runtime_error("per-node arrays must have 2*nleaf-1 entries");
+    for (int v = 0; v < nn; ++v) {
+        bool leaf = v >= f;
+        if (leaf ? (lc[v] != -1 || rc[v] != -1 || lm[v] != v - f) : (lc[v] != 2 * v + 1 || rc[v] != 2 * v + 2))
+            throw std::runtime_error("tree is not a perfect binary tree in BFS numbering");
+    }
+    H->depth = depth;
+    H->nleaf = L;
+    H->numnodes = nn;
+    H->pts.assign(nn, {});
+    std::vector<char> seen(n, 0);
+    for (int v = 1; v < nn; ++v) {
+        if (lo[v] < 0 || ll[v] < 0 || (size_t)lo[v] + ll[v] > lids.size())
+            throw std::runtime_error("lids offsets out of range");
+        H->pts[v].assign(lids.begin() + lo[v], lids.begin() + lo[v] + ll[v]);
+        for (int p : H->pts[v])
+            if (p < 0 || p >= n) throw std::runtime_error("point index out of range");
+    }
+    for (int l = 0; l < L; ++l) {
+        if ((int)H->pts[f + l].size() != dim[l]) throw std::runtime_error("dim.bin disagrees with lidslen");
+        for (int p : H->pts[f + l]) {
+            if (seen[p]) throw std::runtime_error("point in two leaves");
+            seen[p] = 1;
+        }
+    }
+    for (int p = 0; p < n; ++p)
+        if (!seen[p]) throw std::runtime_error("lids is not a permutation of the points");
+    H->pts[0].resize(n);
+    for (int p = 0; p < n; ++p) H->pts[0][p] = p;
+    H->sflevel.assign(depth + 1, {});
+    for (int l = 0; l <= depth; ++l) {
+        int dl = depth - l;
+        for (int v = (1 << dl) - 1; v < (1 << (dl + 1)) - 1; ++v) H->sflevel[l].push_back(v);
+    }
+    if (nx.size() != ny.size() || fx.size() != fy.size()) throw std::runtime_error("pair arrays differ in length");
+    H->nearn.assign(nn, {});
+    H->farn.assign(nn, {});
+    for (size_t k = 0; k < nx.size(); ++k) {
+        if (nx[k] < 0 || nx[k] >= L || ny[k] < 0 || ny[k] >= L) throw std::runtime_error("near pair out of range");
+        H->nearn[f + nx[k]].push_back(f + ny[k]);
+    }
+    for (size_t k = 0; k < fx.size(); ++k) {
+        if (fx[k] <= 0 || fx[k] >= nn || fy[k] <= 0 || fy[k] >= nn) throw std::runtime_error("far pair out of range");
+        H->farn[fx[k]].push_back(fy[k]);
+    }
+    for (int v = 0; v < nn; ++v) {
+        std::sort(H->nearn[v].begin(), H->nearn[v].end());
+        std::sort(H->farn[v].begin(), H->farn[v].end());
+    }
+    H->is_hss = true;
+    for (int v = f; v < nn; ++v)
+        if (H->nearn[v].size() != 1) H->is_hss = false;
+    // coarsened partitions from clevelset / wpart / cidx
+    if (cl.empty() || wp.empty() || (int)cidx.size() != nn - 1) throw std::runtime_error("bad coarsening arrays");
+    H->postw.assign(cl.size() - 1, {});
+    for (size_t g = 0; g + 1 < cl.size(); ++g)
+        for (int p = cl[g]; p < cl[g + 1]; ++p) {
+            if (p < 0 || p + 1 >= (int)wp.size()) throw std::runtime_error("clevelset out of range");
+            H->postw[g].push_back({});
+            for (int j = wp[p]; j < wp[p + 1]; ++j) H->postw[g].back().push_back(cidx[j]);
+        }
+    // neighbour table
+    if (nnidx.size() != dist.size() || nnidx.empty() || nnidx.size() % (size_t)n)
+        throw std::runtime_error("NN.bin / distNN.bin do not hold n*k entries");
+    H->knn_k = (int)(nnidx.size() / n);
+    H->NN.resize(nnidx.size());
+    for (size_t i = 0; i < nnidx.size(); ++i) H->NN[i] = {dist[i], nnidx[i]};
+    int saved = omp_get_max_threads();
+    if (cfg.nthreads > 0) omp_set_num_threads(cfg.nthreads);
+    try {
+        emit_structure(*H);
+        auto &I = H->i32;
+        I["levelset"] = lvl;
+        I["idx_level"] = idx;
+        I["clevelset"] = cl;
+        I["wpart"] = wp;
+        I["idx"] = cidx;
+        if (blocked && !H->is_hss) {
+            I["nblockSet"] = nbs;
+            I["nblocks"] = nbk;
+            I["npairx"] = nx;
+            I["npairy"] = ny;
+            I["fblockSet"] = fbs;
+            I["fblocks"] = fbk;
+            I["fpairx"] = fx;
+            I["fpairy"] = fy;
+            H->scal["nl"] = (int64_t)nbs.size() - 1;
+            H->scal["fl"] = (int64_t)fbs.size() - 1;
+        }
+        compress(*H);
+        balance_partitions(*H);
+        emit_cds(*H);
+    } catch (...) {
+        omp_set_num_threads(saved);
+        throw;
+    }
+    omp_set_num_threads(saved);
+    return H.release();
+}
+
 }  // namespace mxh
 
 using mxh::Handle;
@@ -1171,6 +1396,34 @@ int mxh_normal_f32(float *out, int64_t count, uint64_t seed) {
         if (2 * i + 1 < count) out[2 * i + 1] = (float)z1;
     }
     return 0;
+}
+
+int mxh_save_bin(const mxh_handle *h, const char *dir) {
+    if (!h || !dir) return -1;
+    try {
+        mxh::save_bin(*h, dir);
+        return 0;
+    } catch (const std::exception &e) {
+        mxh::g_last_error = e.what();
+        return -3;
+    }
+}
+
+mxh_handle *mxh_load_bin(const char *dir, const double *X, int32_t n, int32_t d, const mxh_config *cfg) {
+    if (!dir || !X || !cfg) {
+        mxh::g_last_error = "null argument";
+        return nullptr;
+    }
+    try {
+        Handle *h = mxh::load_bin(dir, X, n, d, *cfg);
+        mxh_handle *out = new mxh_handle();
+        static_cast<Handle &>(*out) = std::move(*h);
+        delete h;
+        return out;
+    } catch (const std::exception &e) {
+        mxh::g_last_error = e.what();
+        return nullptr;
+    }
 }
 
 const char *mxh_last_error(void) { return mxh::g_last_error.c_str(); }

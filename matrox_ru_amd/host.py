@@ -64,6 +64,10 @@ def _lib():
         lib.mxh_normal.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64]
         lib.mxh_normal_f32.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64]
         lib.mxh_last_error.restype = ctypes.c_char_p
+        lib.mxh_save_bin.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+        lib.mxh_save_bin.restype = ctypes.c_int
+        lib.mxh_load_bin.restype = ctypes.c_void_p
+        lib.mxh_load_bin.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, P(MxhConfig)]
         _configured = True
     return lib
 
@@ -215,3 +219,33 @@ def normal(shape, seed, dtype=np.float64, order="F"):
     else:
         raise TypeError("normal: float64 or float32 only")
     return flat.reshape(shape, order=order)
+
+
+def save_bin(hm, directory):
+    """Write the structure of ``hm`` as the reference's .bin file set
+    (savetree2disk / SaveSampling, sympiler/HTree.cpp:234-249, 469-542)."""
+    import os
+    os.makedirs(directory, exist_ok=True)
+    rc = _lib().mxh_save_bin(hm._handle_for_perm(), os.fsencode(directory))
+    if rc != 0:
+        raise OSError(f"save_bin failed: {_lib().mxh_last_error().decode()}")
+
+
+def load_bin(directory, X, cfg=None, **kw):
+    """Read a .bin structure set (as readHtree, matroxTest-V1/HTree.h:76-260),
+    then compress and pack it over the points X."""
+    import os
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    if cfg is None:
+        cfg = default_config(**kw)
+    else:
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+    lib = _lib()
+    h = lib.mxh_load_bin(os.fsencode(directory), X.ctypes.data, n, d, ctypes.byref(cfg))
+    if not h:
+        raise ValueError(f"load_bin failed: {lib.mxh_last_error().decode()}")
+    hm = _collect(h, keep_handle=True)
+    hm.config = {f: getattr(cfg, f) for f, _ in MxhConfig._fields_}
+    return hm
