@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain 1-buffer copy of this size (HBM-counter calibration)")
     ap.add_argument("--skip-checks", action="store_true", help="skip the dense-row error checks")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     return ap.parse_args()
 
 
@@ -112,6 +113,7 @@ def main():
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
+    plan.set_graph(not args.no_graph)
     ms64 = timed("fp64", Y64, max(3, args.steps // 4), 1)
     log(f"[bench] fp64 {ms64:.3f} ms/eval")
     msmx = timed(prec, Ymx, args.steps, args.warmup)
@@ -240,6 +242,7 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x3-f32acc" if prec == "bf16x3" else "f32",
         "data": "synthetic (splitmix64/Box-Muller Gaussian mixture points, N(0,1) W)",
+        "graph": not args.no_graph,
         "config": {"workload": wl["name"], "N": hm.N, "d": wl["d"], "nrhs": nrhs, "leaf": wl["leaf"],
                    "max_rank": wl["rank"], "tol": wl["tol"], "budget": wl["budget"], "h": wl["h"],
                    "nleaf": int(hm.nleaf), "near_blocks": int(hm.ncount), "far_blocks": int(hm.fcount),

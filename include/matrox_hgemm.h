@@ -131,6 +131,13 @@ mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uske
  * elements each, layout as above) into caller device buffers on `stream`. */
 mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst, int32_t nrhs, void *stream);
 
+/* Replay evaluations from captured hipGraphs: with enable != 0, the first
+ * mx_eval with a given (precision, io, nrhs, W, Y) captures its launch
+ * sequence and later calls with the same arguments replay it (a few graphs
+ * are kept per plan).  While profiling is on, evaluations run eagerly.
+ * enable == 0 drops the captured graphs. */
+mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable);
+
 /* ---- per-result checks and measurement ------------------------------------ */
 
 /* Attach the points (host, N x d row-major, in LEAF order: row j is point

@@ -104,6 +104,18 @@ struct mx_plan {
     int pts_d = 0;
     double pts_h = 0.0;
 
+    // captured evaluations (hipGraph) keyed by their arguments
+    struct GraphEntry {
+        int prec, io, nrhs;
+        const void *W;
+        void *Y;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    int use_graph = 0;
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+
     // launch timing
     int profiling = 0;
     std::vector<hipEvent_t> ev;       // 2 per launch
@@ -117,6 +129,7 @@ struct mx_plan {
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
     }
 };
 
@@ -698,6 +711,67 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
     P->last_prec = prec;
 }
 
+// Evaluation through a captured hipGraph: the launch sequence of one
+// (precision, I/O type, nrhs, W, Y) is captured once and replayed; work
+// buffers are sized before the capture and the sequence holds no memsets.
+// Up to four graphs are kept per plan (least recently used is dropped).
+static void eval_graph(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+                       hipStream_t st) {
+    for (auto &g : P->graphs) {
+        if (g.prec == prec && g.io == io && g.nrhs == nrhs && g.W == W && g.Y == Y) {
+            g.used = ++P->graph_clock;
+            HIPCHK(hipGraphLaunch(g.exec, st));
+            P->last_prec = prec;
+            return;
+        }
+    }
+    if (prec == MX_BF16X3) ensure_panels(P, nrhs);
+    else ensure_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
+    const int prof = P->profiling;
+    P->profiling = 0;
+    hipStream_t cs = st;
+    bool own = false;
+    if (cs == nullptr) {   // the legacy default stream cannot be captured
+        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        own = true;
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    try {
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            eval(P, prec, W, Y, nrhs, io, cs);
+        } catch (...) {
+            hipGraph_t g2 = nullptr;
+            (void)hipStreamEndCapture(cs, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(cs, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        graph = nullptr;
+    } catch (...) {
+        P->profiling = prof;
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own) (void)hipStreamDestroy(cs);
+        throw;
+    }
+    P->profiling = prof;
+    if (own) HIPCHK(hipStreamDestroy(cs));
+    if (P->graphs.size() >= 4) {
+        auto lru = std::min_element(P->graphs.begin(), P->graphs.end(),
+                                    [](const mx_plan::GraphEntry &a, const mx_plan::GraphEntry &b) {
+                                        return a.used < b.used;
+                                    });
+        HIPCHK(hipGraphExecDestroy(lru->exec));
+        P->graphs.erase(lru);
+    }
+    P->graphs.push_back(mx_plan::GraphEntry{prec, io, nrhs, W, Y, exec, ++P->graph_clock});
+    HIPCHK(hipGraphLaunch(exec, st));
+    P->last_prec = prec;
+}
+
 // Kr (nrows x N, column-major) = exp(-|x_rows[r] - x_j|^2 / (2 h^2)), points in
 // leaf order; distances summed directly as in matroxTest-V1/Util.h:189-201.
 __global__ void dense_kernel_rows(const double *__restrict__ X, int64_t N, int d, double h,
@@ -989,7 +1063,9 @@ mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int3
     return (mx_status)guard([&]() {
         arg_check(plan != nullptr, "null plan");
         arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
-        eval(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        arg_check(W && Y && nrhs >= 1, "bad W, Y or nrhs");
+        if (plan->use_graph && !plan->profiling) eval_graph(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        else eval(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
         return 0;
     });
 }
@@ -1038,6 +1114,18 @@ mx_status mx_dense_rows(mx_plan *plan, const int32_t *rows, int32_t nrows, const
     return (mx_status)guard([&]() {
         arg_check(plan != nullptr, "null plan");
         dense_rows(plan, rows, nrows, W, nrhs, out, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        plan->use_graph = enable ? 1 : 0;
+        if (!enable) {
+            for (auto &g : plan->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
+            plan->graphs.clear();
+        }
         return 0;
     });
 }

@@ -70,6 +70,8 @@ def lib():
         L.mx_plan_set_points.restype = ctypes.c_int
         L.mx_dense_rows.argtypes = [P, P, I32, P, I32, P, P]
         L.mx_dense_rows.restype = ctypes.c_int
+        L.mx_plan_set_graph.argtypes = [P, I32]
+        L.mx_plan_set_graph.restype = ctypes.c_int
         L.mx_plan_set_profiling.argtypes = [P, I32]
         L.mx_plan_set_profiling.restype = ctypes.c_int
         L.mx_plan_launch_times.argtypes = [P, P, P, I32, ctypes.POINTER(I32)]
@@ -196,6 +198,10 @@ class Plan:
         base = wrow[leaf] * nrhs + (rows - wrow[leaf])
         idx = base[:, None] + np.arange(nrhs)[None, :] * dim[leaf][:, None]
         return Y[torch.from_numpy(idx).to(Y.device)]
+
+    def set_graph(self, enable=True):
+        """Replay evaluations from captured hipGraphs (mx_plan_set_graph)."""
+        _check(lib().mx_plan_set_graph(self._h, int(enable)))
 
     def set_profiling(self, enable=True):
         _check(lib().mx_plan_set_profiling(self._h, int(enable)))

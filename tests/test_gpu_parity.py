@@ -202,3 +202,36 @@ def test_flops_and_launch_timing(gpu):
     kinds = {k for k, _ in times}
     assert kinds == {"bf16x3", "convert"}
     assert all(ms > 0 for _, ms in times)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_graph_replays_match_eager(gpu, prec):
+    """Captured evaluations replayed several times, with eager evaluations and
+    synchronisations in between and W changed in place, equal eager results."""
+    import torch
+    _say(f"[graph] {prec}")
+    X, hm = make_problem(**CASES["h2b_small"])
+    plan = hgemm.Plan(hm, precisions=(prec,))
+    nrhs = 80
+    dt = torch.float64
+    Ws = [torch.from_numpy(hm.transform(host.normal((hm.N, nrhs), seed=600 + i))).to("cuda", dt) for i in range(3)]
+    eager = []
+    for Wi in Ws:
+        Y = torch.empty_like(Wi)
+        plan.eval(Wi, Y, nrhs, precision=prec)
+        torch.cuda.synchronize()
+        eager.append(Y.clone())
+    other = hgemm.Plan(hm, precisions=(prec,))       # eager work between replays
+    plan.set_graph(True)
+    W = Ws[0].clone()
+    Yg = torch.full_like(W, float("nan"))
+    for rep in range(5):
+        i = rep % 3
+        W.copy_(Ws[i])                                # new contents, same buffer
+        plan.eval(W, Yg, nrhs, precision=prec)        # captured on first use, replayed after
+        torch.cuda.synchronize()
+        assert torch.equal(Yg, eager[i]), (prec, rep)
+        Ye = torch.empty_like(W)
+        other.eval(Ws[(i + 1) % 3], Ye, nrhs, precision=prec)
+        torch.cuda.synchronize()
+        assert torch.equal(Ye, eager[(i + 1) % 3])
