@@ -102,10 +102,13 @@ typedef struct mx_structure {
 
 typedef struct mx_plan mx_plan;
 
-/* Bits of `precisions` in mx_plan_create. */
+/* Bits of `precisions` in mx_plan_create.  MX_PLAN_ADJOINT also builds the
+ * transposed task lists (and, with MX_PLAN_BF16X3, the packed transposed D
+ * and B) used by mx_eval_adjoint. */
 #define MX_PLAN_FP64 (1u << MX_FP64)
 #define MX_PLAN_FP32 (1u << MX_FP32)
 #define MX_PLAN_BF16X3 (1u << MX_BF16X3)
+#define MX_PLAN_ADJOINT (1u << 8)
 
 /* Validate the structure, build the per-destination task lists and upload
  * blocks for each precision in `precisions` to the current HIP device. */
@@ -118,6 +121,42 @@ mx_status mx_plan_destroy(mx_plan *plan);
  * plan must not overlap. */
 mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
                   void *stream);
+
+/* Adjoint evaluation dW = K~^T dY (same layouts and rules as mx_eval): the
+ * upward pass of dY, couplings grouped by their column node through B^T,
+ * downward pass, near blocks grouped by their column leaf through D^T.  This
+ * is the reference evaluation applied to the transposed structure (pair
+ * lists swapped, D and B transposed, the same VT).  Needs MX_PLAN_ADJOINT. */
+mx_status mx_eval_adjoint(mx_plan *plan, mx_precision prec, const void *dY, void *dW, int32_t nrhs, mx_io_type io,
+                          void *stream);
+
+/* Backward pass of Y = K~ W (no counterpart in the reference, which has no
+ * derivatives): dW = K~^T dY as mx_eval_adjoint, and, for each non-NULL
+ * dD / dB / dVT (device arrays of D_len / B_len / VT_len elements of the I/O
+ * type, every element written), the gradients of the stored blocks,
+ *     dD_k = dY_x W_y^T,  dB_k = y^_x w^_y^T,
+ *     leaf dVT_v = g^_v W_v^T + u^_v dY_v^T,
+ *     internal dVT_v = [g^_v w^_l^T + u^_v y^_l^T,  g^_v w^_r^T + u^_v y^_r^T],
+ * where w^, u^ are the skeletons of the forward evaluation of W and
+ * y^ = V^T-upward pass of dY, g^ = dL/dw^ those of the adjoint evaluation.
+ * wskel / uskel: the forward skeletons as mx_plan_copy_skeletons writes them
+ * (double for MX_FP64, float for MX_FP32); NULL takes the plan's own, valid
+ * when the plan's last forward evaluation was of this W in this precision.
+ * prec is MX_FP64 (FP64 I/O) or MX_FP32 (either I/O type).  Needs
+ * MX_PLAN_ADJOINT. */
+mx_status mx_backward(mx_plan *plan, mx_precision prec, const void *W, const void *dY, void *dW, int32_t nrhs,
+                      mx_io_type io, const void *wskel, const void *uskel, void *dD, void *dB, void *dVT,
+                      void *stream);
+
+/* Replace the stored blocks of a plan (same structure and lengths) from
+ * device FP64 arrays, re-deriving every precision built into it; NULL keeps
+ * that buffer.  No counterpart in the reference (its blocks are fixed once
+ * compressed); used to differentiate through the blocks. */
+mx_status mx_plan_set_blocks(mx_plan *plan, const double *D, const double *B, const double *VT, void *stream);
+
+/* Algorithmic flops of the block gradients of mx_backward with nrhs columns
+ * (2*m*n*nrhs per product). */
+mx_status mx_plan_grad_flops(const mx_plan *plan, int32_t nrhs, double *flops);
 
 /* Device pointers of the skeleton buffers of the last evaluation
  * (wskel = V^T W per node, uskel = coupled + transferred values per node),
@@ -151,11 +190,19 @@ mx_status mx_plan_set_points(mx_plan *plan, const double *X_leaf, int32_t d, dou
 mx_status mx_dense_rows(mx_plan *plan, const int32_t *rows, int32_t nrows, const double *W, int32_t nrhs,
                         double *out, void *stream);
 
-/* Launch timing: when enabled, every kernel launch of mx_eval is bracketed
+/* Launch timing: when enabled, every kernel launch of mx_eval, mx_eval_adjoint
+ * and mx_backward is bracketed
  * by HIP events on the evaluation stream.  mx_plan_launch_times waits for the
  * last evaluation and returns, per launch, its duration in ms and its kernel
  * kind (MX_KIND_*); *count is the number of launches (<= cap filled). */
-enum { MX_KIND_SEG_F64 = 0, MX_KIND_SEG_F32 = 1, MX_KIND_BF16X3 = 2, MX_KIND_CONVERT = 3 };
+enum {
+    MX_KIND_SEG_F64 = 0,
+    MX_KIND_SEG_F32 = 1,
+    MX_KIND_BF16X3 = 2,
+    MX_KIND_CONVERT = 3,
+    MX_KIND_GRAD_F64 = 4,
+    MX_KIND_GRAD_F32 = 5
+};
 mx_status mx_plan_set_profiling(mx_plan *plan, int32_t enable);
 mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t cap, int32_t *count);
 

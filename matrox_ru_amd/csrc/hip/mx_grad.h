@@ -1,0 +1,58 @@
+// Gradients of the stored blocks for the backward pass of Y = K~ W.
+//
+// Every block gradient is a sum of NT products of two column blocks,
+//     dA = sum_s  X_s * Z_s^T        (X_s: m x nrhs, Z_s: n x nrhs),
+// contracted over the nrhs columns.  With dY the upstream gradient,
+// w^ / u^ the forward skeleton blocks and y^ = dL/du^, g^ = dL/dw^ the
+// skeleton blocks of the adjoint evaluation of dY:
+//     near pair (x, y):   dD  = dY_x  W_y^T
+//     far pair (x, y):    dB  = y^_x  w^_y^T
+//     leaf v:             dVT = g^_v  W_v^T  + u^_v dY_v^T
+//     internal v (l, r):  dVT[:, :s_l] = g^_v w^_l^T + u^_v y^_l^T,
+//                         dVT[:, s_l:] = g^_v w^_r^T + u^_v y^_r^T.
+// (The forward reads VT in the upward pass, w^_v = VT_v [W_v or w^_l; w^_r],
+// and in the downward pass, [Y_v or u^_l; u^_r] += VT_v^T u^_v,
+// symGen/HGEMM_gen.h:142-178, 196-236.)
+//
+// Column blocks sit in leaf-blocked (N x nrhs) or skeleton (skeleton rows x
+// nrhs) buffers: element (i, k) of the block starting at row r with m rows is
+// buf[r*nrhs + k*m + i].  Each output block is written once (no
+// accumulation), so one launch covers every task without write conflicts.
+#pragma once
+#include "mx_common.h"
+
+namespace mx {
+
+// operand buffers of the gradient products
+enum GradBuf : int32_t { GB_W = 0, GB_DY = 1, GB_WHAT = 2, GB_UHAT = 3, GB_YHAT = 4, GB_GHAT = 5, GB_COUNT = 6 };
+
+struct alignas(16) GTask {
+    int64_t c_off;      // element offset of the output block in its gradient buffer
+    int32_t m, n;       // output block is m x n
+    int32_t ldc;        // its leading dimension
+    int32_t c_buf;      // MatBuf of the output (D, B or VT)
+    int32_t seg_begin, seg_end;
+};
+
+struct alignas(16) GSeg {
+    int64_t x_row;      // first row of X_s (m rows) in buffer x_buf
+    int64_t z_row;      // first row of Z_s (n rows) in buffer z_buf
+    int32_t x_buf, z_buf;
+    int32_t x_ld, z_ld; // row counts of the two column blocks (their leading dimensions)
+};
+
+template <typename T, typename O>
+struct GradArgs {
+    const GTask *tasks;
+    const GSeg *segs;
+    const int4 *tiles;        // (task, row tile, column tile, unused)
+    int ntiles, nrhs;
+    const T *src[GB_COUNT];
+    O *out[3];                // dD, dB, dVT
+};
+
+// One launch over every tile of the gradient task list.
+template <typename T, typename O>
+void launch_grad(const GradArgs<T, O> &a, hipStream_t st);
+
+}  // namespace mx

@@ -14,6 +14,7 @@
 // tasks gives the same sums with each output block written once per launch.
 #include "../../../include/matrox_hgemm.h"
 #include "mx_bf16x3.h"
+#include "mx_grad.h"
 #include "mx_seg_gemm.h"
 
 #include <hip/hip_runtime.h>
@@ -54,6 +55,27 @@ struct Launch {
     int ntiles;
 };
 
+// The task lists of one evaluation direction (forward Y = K~ W or adjoint
+// dW = K~^T dY) for one kernel kind (0: segmented FP64/FP32 GEMM, 1: BF16x3).
+struct Program {
+    Task *tasks = nullptr;
+    Seg *segs = nullptr;
+    int2 *tiles = nullptr;
+    std::vector<Launch> launches;
+    std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
+    bool built = false;
+    void release() {
+        if (tasks) (void)hipFree(tasks);
+        if (segs) (void)hipFree(segs);
+        if (tiles) (void)hipFree(tiles);
+        tasks = nullptr;
+        segs = nullptr;
+        tiles = nullptr;
+    }
+};
+
+enum { DIR_FWD = 0, DIR_ADJ = 1 };
+
 }  // namespace mx
 
 struct mx_plan {
@@ -62,17 +84,10 @@ struct mx_plan {
     int64_t N = 0, skel_rows = 0;
     std::vector<int32_t> wrow, Ddim, skelrow, slen;
 
-    // task lists: forward evaluation
-    mx::Task *d_tasks = nullptr;
-    mx::Seg *d_segs = nullptr;
-    int2 *d_tiles = nullptr;
-    std::vector<mx::Launch> launches;   // UP levels, FAR+DOWN levels, NEAR+LEAF
-    // the same tasks for the split-BF16x3 kernel (128-row tiles, packed A)
-    mx::Task *d_tasks_x3 = nullptr;
-    mx::Seg *d_segs_x3 = nullptr;
-    int2 *d_tiles_x3 = nullptr;
-    std::vector<mx::Launch> launches_x3;
-    std::vector<int> launch_kind_x3;    // 1 for every launch (BF16x3 kernel)
+    // task lists: prog[direction][kind]; each is UP levels, FAR+DOWN levels,
+    // NEAR+LEAF (kind 1: 128-row tiles and packed split A operands)
+    mx::Program prog[2][2];
+    bool adjoint = false;
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
     int x3_stages = 2;                  // pipeline depth of the BF16x3 kernel (MX_X3_STAGES=2|4)
@@ -89,15 +104,25 @@ struct mx_plan {
     int64_t D_len = 0, B_len = 0, VT_len = 0;
     uint32_t precisions = 0;
 
-    // work buffers
+    // gradient products of the stored blocks (plans built with MX_PLAN_ADJOINT)
+    mx::GTask *g_tasks = nullptr;
+    mx::GSeg *g_segs = nullptr;
+    int4 *g_tiles = nullptr;
+    int g_ntiles = 0;
+    double g_flops = 0.0;               // algorithmic flops per nrhs column
+    // packing lists of the split-BF16 operands, kept to repack new blocks
+    mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
+    int64_t n_xblk[3] = {0, 0, 0};
+
+    // work buffers: skeletons of the forward (w^, u^) and of the adjoint
+    // (y^ = dL/du^, g^ = dL/dw^) evaluations
     void *wskel = nullptr, *uskel = nullptr;
-    size_t skel_bytes = 0;
-    void *io_w = nullptr, *io_y = nullptr;   // FP32 staging when io is FP64
-    size_t io_bytes = 0;
+    void *wskel_adj = nullptr, *uskel_adj = nullptr;
+    size_t skel_bytes = 0, skel_adj_bytes = 0;
+    void *io_w = nullptr, *io_y = nullptr, *io_g = nullptr;   // FP32 staging when io is FP64
+    size_t io_bytes = 0, io_g_bytes = 0;
     int last_prec = -1;
 
-    // algorithmic flops per nrhs column of each launch (both task lists)
-    std::vector<double> launch_flops, launch_flops_x3;
 
     // points for the dense reference rows
     double *X_leaf = nullptr;
@@ -106,7 +131,7 @@ struct mx_plan {
 
     // captured evaluations (hipGraph) keyed by their arguments
     struct GraphEntry {
-        int prec, io, nrhs;
+        int dir, prec, io, nrhs;
         const void *W;
         void *Y;
         hipGraphExec_t exec;
@@ -118,14 +143,18 @@ struct mx_plan {
 
     // launch timing
     int profiling = 0;
+    int keep_prof = 0;                // set while one call records several evaluations
     std::vector<hipEvent_t> ev;       // 2 per launch
     std::vector<int> ev_kind;
     int ev_used = 0;
 
     ~mx_plan() {
-        void *ptrs[] = {d_tasks, d_segs, d_tiles, d_tasks_x3, d_segs_x3, d_tiles_x3, D64, B64, VT64, D32, B32,
+        for (auto &pd : prog)
+            for (auto &pk : pd) pk.release();
+        void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
-                        d_wrows, panels};
+                        d_wrows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
+                        wskel_adj, uskel_adj, io_g};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -262,23 +291,24 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // wrow_in / skel_in give the row offsets of W blocks and skeleton blocks in
 // the buffers the kernel reads (reference layout for kind 0, split panels for
 // kind 1); Y always uses the reference layout.
-static TaskLists build_forward(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind,
-                               const int64_t *wrow_in, const int64_t *skel_in) {
+static TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir,
+                             const int64_t *wrow_in, const int64_t *skel_in) {
     TaskLists L;
     const int nn = s.numnodes;
-    // far pairs by destination node
-    std::vector<std::vector<int64_t>> far_of(nn);
-    for (int64_t k = 0; k < s.fcount; ++k) {
-        int x = s.is_hss ? (int)k + 1 : s.fpairx[k];
-        far_of[x].push_back(k);
-    }
-    auto far_partner = [&](int64_t k) {
+    // far pair k couples row node fx(k) with column node fy(k) through B_k
+    auto fx = [&](int64_t k) { return s.is_hss ? (int)k + 1 : s.fpairx[k]; };
+    auto fy = [&](int64_t k) {
         if (!s.is_hss) return s.fpairy[k];
         int x = (int)k + 1, p = t.parent[x];
         return s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
     };
-    std::vector<std::vector<int64_t>> near_of(s.nleaf);
-    for (int64_t k = 0; k < s.ncount; ++k) near_of[s.is_hss ? (int)k : s.npairx[k]].push_back(k);
+    auto nx = [&](int64_t k) { return s.is_hss ? (int)k : s.npairx[k]; };
+    auto ny = [&](int64_t k) { return s.is_hss ? (int)k : s.npairy[k]; };
+    // pairs grouped by destination: the row node (forward) or the column
+    // node (adjoint, where the block acts transposed)
+    std::vector<std::vector<int64_t>> far_of(nn), near_of(s.nleaf);
+    for (int64_t k = 0; k < s.fcount; ++k) far_of[dir == DIR_FWD ? fx(k) : fy(k)].push_back(k);
+    for (int64_t k = 0; k < s.ncount; ++k) near_of[dir == DIR_FWD ? nx(k) : ny(k)].push_back(k);
 
     auto begin_launch = [&]() {
         L.launches.push_back(Launch{(int)L.tiles.size(), 0});
@@ -347,8 +377,14 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
             if (sc == 0) continue;
             int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
             for (int64_t k : far_of[c]) {
-                int y = far_partner(k);
-                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
+                if (dir == DIR_FWD) {
+                    int y = fy(k);
+                    add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
+                } else {   // B_k^T, B_k stored s_x x s_y
+                    int x = fx(k);
+                    add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 1, s.slen[x], s.slen[x], VEC_WSKEL, skel_in[x],
+                            s.slen[x]);
+                }
             }
             int p = t.parent[c];
             if (p > 0 && s.slen[p] > 0) {
@@ -367,8 +403,13 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
         if (s.Ddim[lf] == 0) continue;
         int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
         for (int64_t k : near_of[lf]) {
-            int y = s.is_hss ? (int)k : s.npairy[k];
-            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
+            if (dir == DIR_FWD) {
+                int y = ny(k);
+                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
+            } else {   // D_k^T, D_k stored Dim_x x Dim_y
+                int x = nx(k);
+                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 1, s.Ddim[x], s.Ddim[x], VEC_W, wrow_in[x], s.Ddim[x]);
+            }
         }
         int v = t.node_of_leaf[lf];
         int sv = s.slen[v];
@@ -379,12 +420,101 @@ static TaskLists build_forward(const mx_structure &s, const Topology &t, int bet
     return L;
 }
 
+// Gradient products of every stored block (see mx_grad.h): one task per
+// near block, far block, leaf VT and half of an internal VT, tiled 64 x 64.
+struct GradLists {
+    std::vector<GTask> tasks;
+    std::vector<GSeg> segs;
+    std::vector<int4> tiles;
+    double flops = 0.0;   // per nrhs column
+};
+
+static GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
+    GradLists G;
+    auto task = [&](int64_t c_off, int m, int n, int ldc, int c_buf) {
+        GTask k{};
+        k.c_off = c_off;
+        k.m = m;
+        k.n = n;
+        k.ldc = ldc;
+        k.c_buf = c_buf;
+        k.seg_begin = k.seg_end = (int)G.segs.size();
+        G.tasks.push_back(k);
+        return (int)G.tasks.size() - 1;
+    };
+    auto seg = [&](int tk, int x_buf, int64_t x_row, int x_ld, int z_buf, int64_t z_row, int z_ld) {
+        GSeg g{};
+        g.x_buf = x_buf;
+        g.x_row = x_row;
+        g.x_ld = x_ld;
+        g.z_buf = z_buf;
+        g.z_row = z_row;
+        g.z_ld = z_ld;
+        G.segs.push_back(g);
+        G.tasks[tk].seg_end = (int)G.segs.size();
+        G.flops += 2.0 * x_ld * z_ld;
+    };
+    auto tiles = [&](int tk) {
+        const GTask &k = G.tasks[tk];
+        for (int i = 0; i < (k.m + 63) / 64; ++i)
+            for (int j = 0; j < (k.n + 63) / 64; ++j) G.tiles.push_back(make_int4(tk, i, j, 0));
+    };
+    for (int64_t k = 0; k < s.ncount; ++k) {
+        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
+        int mx_ = s.Ddim[x], ny_ = s.Ddim[y];
+        if (mx_ == 0 || ny_ == 0) continue;
+        int tk = task((int64_t)s.Dptr[k], mx_, ny_, mx_, MAT_D);
+        seg(tk, GB_DY, s.wrow[x], mx_, GB_W, s.wrow[y], ny_);
+        tiles(tk);
+    }
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x, y;
+        if (s.is_hss) {
+            x = (int)k + 1;
+            int p = t.parent[x];
+            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+        } else {
+            x = s.fpairx[k];
+            y = s.fpairy[k];
+        }
+        int sx = s.slen[x], sy = s.slen[y];
+        if (sx == 0 || sy == 0) continue;
+        int tk = task((int64_t)s.Bptr[k], sx, sy, sx, MAT_B);
+        seg(tk, GB_YHAT, s.skelrow[x], sx, GB_WHAT, s.skelrow[y], sy);
+        tiles(tk);
+    }
+    for (int v = 1; v < s.numnodes; ++v) {
+        int sv = s.slen[v];
+        if (sv == 0) continue;
+        if (s.lchildren[v] == -1) {
+            int lf = s.lm[v], dm = s.Ddim[lf];
+            if (dm == 0) continue;
+            int tk = task(s.VTptr[v], sv, dm, sv, MAT_VT);
+            seg(tk, GB_GHAT, s.skelrow[v], sv, GB_W, s.wrow[lf], dm);
+            seg(tk, GB_UHAT, s.skelrow[v], sv, GB_DY, s.wrow[lf], dm);
+            tiles(tk);
+        } else {
+            int64_t off = s.VTptr[v];
+            for (int c : {s.lchildren[v], s.rchildren[v]}) {
+                int sc = s.slen[c];
+                if (sc > 0) {
+                    int tk = task(off, sv, sc, sv, MAT_VT);
+                    seg(tk, GB_GHAT, s.skelrow[v], sv, GB_WHAT, s.skelrow[c], sc);
+                    seg(tk, GB_UHAT, s.skelrow[v], sv, GB_YHAT, s.skelrow[c], sc);
+                    tiles(tk);
+                }
+                off += (int64_t)sv * sc;
+            }
+        }
+    }
+    return G;
+}
+
 template <typename T>
 static T *upload(const T *h, size_t n) {
     T *d = nullptr;
-    if (n == 0) n = 1;
-    HIPCHK(hipMalloc(&d, n * sizeof(T)));
-    if (h) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+    if (h && n > 0) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
     return d;
 }
 
@@ -411,6 +541,18 @@ static int cvt_grid(int64_t n) {
     return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 8192);
 }
 
+// Pack the split-BF16 operands of D, B and VT from FP64 sources.
+static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
+    for (int w = 0; w < 3; ++w) {
+        if (!src[w]) continue;
+        for (int64_t b0 = 0; b0 < P->n_xblk[w]; b0 += 65535) {
+            int64_t nb = std::min<int64_t>(65535, P->n_xblk[w] - b0);
+            pack_x3_blocks<<<dim3(64, (unsigned)nb), 256, 0, st>>>(src[w], P->d_xblk[w] + b0, P->xhi[w], P->xlo[w]);
+            HIPCHK(hipGetLastError());
+        }
+    }
+}
+
 static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y) {
     arg_check(s.nleaf > 0 && s.numnodes >= 1, "empty structure");
     arg_check(s.numnodes == 2 * s.nleaf - 1, "numnodes must be 2*nleaf-1 for a binary tree");
@@ -421,7 +563,7 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     arg_check(s.fcount == 0 || (s.Bptr && s.B), "far blocks missing");
     arg_check(s.is_hss || s.ncount == 0 || (s.npairx && s.npairy), "near pair arrays missing");
     arg_check(s.is_hss || s.fcount == 0 || (s.fpairx && s.fpairy), "far pair arrays missing");
-    arg_check(precisions != 0 && (precisions & ~7u) == 0, "bad precision mask");
+    arg_check((precisions & 7u) != 0 && (precisions & ~(7u | MX_PLAN_ADJOINT)) == 0, "bad precision mask");
     Topology t = topology(s);
     validate(s, t);
 
@@ -442,15 +584,21 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     P->D_len = s.D_len;
     P->B_len = s.B_len;
     P->VT_len = s.VT_len;
-    P->precisions = precisions;
+    P->precisions = precisions & 7u;
+    P->adjoint = (precisions & MX_PLAN_ADJOINT) != 0;
     if (const char *e = getenv("MX_X3_STAGES")) P->x3_stages = atoi(e) == 4 ? 4 : 2;
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
-    TaskLists L = build_forward(s, t, beta_u, beta_y, 0, wrow64.data(), skel64.data());
-    compute_launch_flops(L);
-    upload_lists(L, &P->d_tasks, &P->d_segs, &P->d_tiles);
-    P->launches = L.launches;
-    P->launch_flops = L.flops;
+    const int ndir = P->adjoint ? 2 : 1;
+    for (int dir = 0; dir < ndir; ++dir) {
+        TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, wrow64.data(), skel64.data());
+        compute_launch_flops(L);
+        Program &pg = P->prog[dir][0];
+        upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
+        pg.launches = L.launches;
+        pg.flops = L.flops;
+        pg.built = true;
+    }
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     if (precisions & MX_PLAN_BF16X3) {
         // split panels: padded row offsets of W leaf blocks and skeleton blocks
@@ -471,48 +619,60 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         std::vector<X3Rows> wr(s.nleaf);
         for (int l = 0; l < s.nleaf; ++l) wr[l] = X3Rows{s.wrow[l], P->wprow[l], s.Ddim[l], 0};
         P->d_wrows = upload(wr.data(), wr.size());
-        TaskLists X = build_forward(s, t, beta_u, beta_y, 1, P->wprow.data(), P->sprow.data());
         // every A operand is read from packed split-BF16 fragments: assign each
-        // distinct (buffer, offset, orientation, shape) its packed offset
+        // distinct (buffer, offset, orientation, shape) its packed offset,
+        // shared by the forward and adjoint lists
         std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
         int64_t next[3] = {0, 0, 0};
-        for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
-            for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
-                Seg &sg = X.segs[si];
-                const int w = sg.a_buf, m = X.tasks[ti].m;
-                auto key = std::make_tuple(sg.a_off, sg.a_trans, sg.lda, m, sg.k);
-                auto it = seen[w].find(key);
-                int64_t dst;
-                if (it == seen[w].end()) {
-                    dst = next[w];
-                    seen[w][key] = dst;
-                    X3Block xb{};
-                    xb.src_off = sg.a_off;
-                    xb.dst_off = dst;
-                    xb.M = m;
-                    xb.K = sg.k;
-                    xb.lda = sg.lda;
-                    xb.trans = sg.a_trans;
-                    xblk[w].push_back(xb);
-                    next[w] += x3_packed_elems(m, sg.k);
-                } else {
-                    dst = it->second;
+        for (int dir = 0; dir < ndir; ++dir) {
+            TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, P->wprow.data(), P->sprow.data());
+            for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
+                if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
+                    throw MxError(MX_ERR_UNSUPPORTED, "a block has more than " + std::to_string(X3_MAXSEG) +
+                                                          " interactions; the BF16x3 kernel cannot hold its list");
+                for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
+                    Seg &sg = X.segs[si];
+                    const int w = sg.a_buf, m = X.tasks[ti].m;
+                    auto key = std::make_tuple(sg.a_off, sg.a_trans, sg.lda, m, sg.k);
+                    auto it = seen[w].find(key);
+                    int64_t dst;
+                    if (it == seen[w].end()) {
+                        dst = next[w];
+                        seen[w][key] = dst;
+                        X3Block xb{};
+                        xb.src_off = sg.a_off;
+                        xb.dst_off = dst;
+                        xb.M = m;
+                        xb.K = sg.k;
+                        xb.lda = sg.lda;
+                        xb.trans = sg.a_trans;
+                        xblk[w].push_back(xb);
+                        next[w] += x3_packed_elems(m, sg.k);
+                    } else {
+                        dst = it->second;
+                    }
+                    sg.a_off = dst;
+                    sg.a_trans = 0;
+                    sg.lda = (sg.k + 31) / 32;
                 }
-                sg.a_off = dst;
-                sg.a_trans = 0;
-                sg.lda = (sg.k + 31) / 32;
             }
+            compute_launch_flops(X);
+            Program &pg = P->prog[dir][1];
+            upload_lists(X, &pg.tasks, &pg.segs, &pg.tiles);
+            pg.launches = X.launches;
+            pg.flops = X.flops;
+            pg.built = true;
         }
-        for (size_t ti = 0; ti < X.tasks.size(); ++ti)
-            if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
-                throw MxError(MX_ERR_UNSUPPORTED, "a block has more than " + std::to_string(X3_MAXSEG) +
-                                                      " interactions; the BF16x3 kernel cannot hold its list");
         for (int w = 0; w < 3; ++w) P->x3_elems[w] = next[w];
-        compute_launch_flops(X);
-        P->launch_flops_x3 = X.flops;
-        upload_lists(X, &P->d_tasks_x3, &P->d_segs_x3, &P->d_tiles_x3);
-        P->launches_x3 = X.launches;
-        P->launch_kind_x3 = X.kinds;
+    }
+    if (P->adjoint) {
+        GradLists G = build_grad_lists(s, t);
+        P->g_tasks = upload(G.tasks.data(), G.tasks.size());
+        P->g_segs = upload(G.segs.data(), G.segs.size());
+        P->g_tiles = upload(G.tiles.data(), G.tiles.size());
+        if (G.tiles.size() > (size_t)INT32_MAX) throw MxError(MX_ERR_ARG, "gradient launch too large");
+        P->g_ntiles = (int)G.tiles.size();
+        P->g_flops = G.flops;
     }
     // stored blocks
     double *D64 = upload(s.D, (size_t)s.D_len), *B64 = upload(s.B, (size_t)s.B_len);
@@ -530,15 +690,10 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         for (int w = 0; w < 3; ++w) {
             P->xhi[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
             P->xlo[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
-            for (size_t b0 = 0; b0 < xblk[w].size(); b0 += 65535) {
-                size_t nb = std::min<size_t>(65535, xblk[w].size() - b0);
-                X3Block *db = upload(xblk[w].data() + b0, nb);
-                pack_x3_blocks<<<dim3(64, (unsigned)nb), 256>>>(src[w], db, P->xhi[w], P->xlo[w]);
-                HIPCHK(hipGetLastError());
-                HIPCHK(hipDeviceSynchronize());
-                HIPCHK(hipFree(db));
-            }
+            P->d_xblk[w] = upload(xblk[w].data(), xblk[w].size());
+            P->n_xblk[w] = (int64_t)xblk[w].size();
         }
+        pack_all(P.get(), src, nullptr);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
@@ -552,6 +707,28 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         HIPCHK(hipFree(VT64));
     }
     return P.release();
+}
+
+static void ensure_adj_work(mx_plan *P, int nrhs, size_t elem) {
+    size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
+    if (sb > P->skel_adj_bytes) {
+        if (P->wskel_adj) HIPCHK(hipFree(P->wskel_adj));
+        if (P->uskel_adj) HIPCHK(hipFree(P->uskel_adj));
+        P->wskel_adj = P->uskel_adj = nullptr;
+        HIPCHK(hipMalloc(&P->wskel_adj, sb));
+        HIPCHK(hipMalloc(&P->uskel_adj, sb));
+        P->skel_adj_bytes = sb;
+    }
+}
+
+static void ensure_io_g(mx_plan *P, int nrhs) {
+    size_t ib = (size_t)std::max<int64_t>(P->N, 1) * nrhs * sizeof(float);
+    if (ib > P->io_g_bytes) {
+        if (P->io_g) HIPCHK(hipFree(P->io_g));
+        P->io_g = nullptr;
+        HIPCHK(hipMalloc(&P->io_g, ib));
+        P->io_g_bytes = ib;
+    }
 }
 
 static void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
@@ -616,7 +793,7 @@ static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
     }
 }
 
-static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
                  hipStream_t st) {
     arg_check(W && Y, "null W or Y");
     arg_check(nrhs >= 1, "nrhs must be >= 1");
@@ -625,26 +802,34 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
     HIPCHK(hipGetDevice(&dev));
     if (dev != P->device) throw MxError(MX_ERR_STATE, "plan was created on another device");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
+    if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+    const Program &pg0 = P->prog[dir][0];
+    const Program &pg1 = P->prog[dir][1];
     const int64_t nel = P->N * (int64_t)nrhs;
-    P->ev_used = 0;
+    if (prec != MX_BF16X3 && dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
+    void *wsk = dir == DIR_ADJ ? P->wskel_adj : P->wskel;
+    void *usk = dir == DIR_ADJ ? P->uskel_adj : P->uskel;
+    if (!P->keep_prof) P->ev_used = 0;
     if (prec == MX_FP64) {
         if (io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
         ensure_work(P, nrhs, sizeof(double), false);
+        if (dir == DIR_FWD) wsk = P->wskel, usk = P->uskel;
         Bufs<double> b;
         b.mat[MAT_D] = P->D64;
         b.mat[MAT_B] = P->B64;
         b.mat[MAT_VT] = P->VT64;
         b.vec[VEC_W] = const_cast<double *>(static_cast<const double *>(W));
-        b.vec[VEC_WSKEL] = (double *)P->wskel;
-        b.vec[VEC_USKEL] = (double *)P->uskel;
+        b.vec[VEC_WSKEL] = (double *)wsk;
+        b.vec[VEC_USKEL] = (double *)usk;
         b.vec[VEC_Y] = (double *)Y;
-        for (size_t i = 0; i < P->launches.size(); ++i) {
+        for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F64, true);
-            run_seg_launch<double>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+            run_seg_launch<double>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, st);
             prof_mark(P, st, MX_KIND_SEG_F64, false);
         }
     } else if (prec == MX_FP32) {
         ensure_work(P, nrhs, sizeof(float), io == MX_IO_F64);
+        if (dir == DIR_FWD) wsk = P->wskel, usk = P->uskel;
         const float *Wf;
         float *Yf;
         if (io == MX_IO_F64) {
@@ -662,12 +847,12 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         b.mat[MAT_B] = P->B32;
         b.mat[MAT_VT] = P->VT32;
         b.vec[VEC_W] = const_cast<float *>(Wf);
-        b.vec[VEC_WSKEL] = (float *)P->wskel;
-        b.vec[VEC_USKEL] = (float *)P->uskel;
+        b.vec[VEC_WSKEL] = (float *)wsk;
+        b.vec[VEC_USKEL] = (float *)usk;
         b.vec[VEC_Y] = Yf;
-        for (size_t i = 0; i < P->launches.size(); ++i) {
+        for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F32, true);
-            run_seg_launch<float>(P->launches[i], P->d_tasks, P->d_segs, P->d_tiles, nrhs, b, st);
+            run_seg_launch<float>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, st);
             prof_mark(P, st, MX_KIND_SEG_F32, false);
         }
         if (io == MX_IO_F64) {
@@ -683,8 +868,8 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         xa.n_ntiles = (nrhs + X3_BN - 1) / X3_BN;
         xa.y_f64 = io == MX_IO_F64;
         xa.Y = Y;
-        xa.tasks = P->d_tasks_x3;
-        xa.segs = P->d_segs_x3;
+        xa.tasks = pg1.tasks;
+        xa.segs = pg1.segs;
         for (int w = 0; w < 3; ++w) {
             xa.ahi[w] = P->xhi[w];
             xa.alo[w] = P->xlo[w];
@@ -699,34 +884,155 @@ static void eval(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs
         prof_mark(P, st, MX_KIND_CONVERT, true);
         launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
         prof_mark(P, st, MX_KIND_CONVERT, false);
-        for (size_t i = 0; i < P->launches_x3.size(); ++i) {
-            xa.tiles = P->d_tiles_x3 + P->launches_x3[i].tile_offset;
-            xa.ntiles = P->launches_x3[i].ntiles;
+        for (size_t i = 0; i < pg1.launches.size(); ++i) {
+            xa.tiles = pg1.tiles + pg1.launches[i].tile_offset;
+            xa.ntiles = pg1.launches[i].ntiles;
             prof_mark(P, st, MX_KIND_BF16X3, true);
             launch_x3(xa, P->x3_stages, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
     }
     HIPCHK(hipGetLastError());
-    P->last_prec = prec;
+    if (dir == DIR_FWD) P->last_prec = prec;
+}
+
+// Backward pass of Y = K~ W for the upstream gradient dY: dW = K~^T dY by the
+// adjoint evaluation, then (when requested) the block gradients from the
+// forward skeletons (w^, u^: caller copies, or the plan's own buffers) and the
+// adjoint ones (y^, g^).  FP64 or FP32 compute; outputs in the I/O type.
+static void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
+                     const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
+    arg_check(W && dY && dW, "null W, dY or dW");
+    arg_check(prec == MX_FP64 || prec == MX_FP32, "block gradients are computed in FP64 or FP32");
+    arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
+    if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 backward needs FP64 I/O");
+    if (!P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+    const bool grads = dD || dB || dVT;
+    const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
+    if (grads && (!wskel || !uskel)) {
+        if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
+        if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes)
+            throw MxError(MX_ERR_STATE, "the last forward evaluation had fewer columns");
+        wskel = P->wskel;
+        uskel = P->uskel;
+    }
+    if (!P->keep_prof) P->ev_used = 0;
+    P->keep_prof = 1;
+    try {
+        eval(P, DIR_ADJ, prec, dY, dW, nrhs, io, st);
+    } catch (...) {
+        P->keep_prof = 0;
+        throw;
+    }
+    P->keep_prof = 0;
+    if (!grads) return;
+    const int64_t nel = P->N * (int64_t)nrhs;
+    const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : MX_KIND_GRAD_F32;
+    if (prec == MX_FP64) {
+        GradArgs<double, double> g{};
+        g.tasks = P->g_tasks;
+        g.segs = P->g_segs;
+        g.tiles = P->g_tiles;
+        g.ntiles = P->g_ntiles;
+        g.nrhs = nrhs;
+        g.src[GB_W] = (const double *)W;
+        g.src[GB_DY] = (const double *)dY;
+        g.src[GB_WHAT] = (const double *)wskel;
+        g.src[GB_UHAT] = (const double *)uskel;
+        g.src[GB_YHAT] = (const double *)P->wskel_adj;
+        g.src[GB_GHAT] = (const double *)P->uskel_adj;
+        g.out[MAT_D] = (double *)dD;
+        g.out[MAT_B] = (double *)dB;
+        g.out[MAT_VT] = (double *)dVT;
+        prof_mark(P, st, kind, true);
+        launch_grad(g, st);
+        prof_mark(P, st, kind, false);
+    } else {
+        const float *Wf = (const float *)W, *dYf = (const float *)dY;
+        if (io == MX_IO_F64) {   // the adjoint left dY (FP32) in io_w; W goes to io_g
+            ensure_io_g(P, nrhs);
+            prof_mark(P, st, MX_KIND_CONVERT, true);
+            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)P->io_g, nel);
+            prof_mark(P, st, MX_KIND_CONVERT, false);
+            Wf = (const float *)P->io_g;
+            dYf = (const float *)P->io_w;
+        }
+        auto fill = [&](auto &g) {
+            g.tasks = P->g_tasks;
+            g.segs = P->g_segs;
+            g.tiles = P->g_tiles;
+            g.ntiles = P->g_ntiles;
+            g.nrhs = nrhs;
+            g.src[GB_W] = Wf;
+            g.src[GB_DY] = dYf;
+            g.src[GB_WHAT] = (const float *)wskel;
+            g.src[GB_UHAT] = (const float *)uskel;
+            g.src[GB_YHAT] = (const float *)P->wskel_adj;
+            g.src[GB_GHAT] = (const float *)P->uskel_adj;
+        };
+        prof_mark(P, st, kind, true);
+        if (io == MX_IO_F64) {
+            GradArgs<float, double> g{};
+            fill(g);
+            g.out[MAT_D] = (double *)dD;
+            g.out[MAT_B] = (double *)dB;
+            g.out[MAT_VT] = (double *)dVT;
+            launch_grad(g, st);
+        } else {
+            GradArgs<float, float> g{};
+            fill(g);
+            g.out[MAT_D] = (float *)dD;
+            g.out[MAT_B] = (float *)dB;
+            g.out[MAT_VT] = (float *)dVT;
+            launch_grad(g, st);
+        }
+        prof_mark(P, st, kind, false);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Replace the stored blocks (same structure) and re-derive every precision
+// built into the plan.  Captured graphs stay valid: they read the same
+// buffers.
+static void set_blocks(mx_plan *P, const double *D, const double *B, const double *VT, hipStream_t st) {
+    const double *src[3] = {D, B, VT};
+    const int64_t len[3] = {P->D_len, P->B_len, P->VT_len};
+    double *d64[3] = {P->D64, P->B64, P->VT64};
+    float *d32[3] = {P->D32, P->B32, P->VT32};
+    for (int w = 0; w < 3; ++w) {
+        if (!src[w] || len[w] == 0) continue;
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, src[w]) != hipSuccess || at.type != hipMemoryTypeDevice) {
+            (void)hipGetLastError();
+            throw MxError(MX_ERR_ARG, "stored blocks must be device arrays");
+        }
+        if (d64[w]) HIPCHK(hipMemcpyAsync(d64[w], src[w], (size_t)len[w] * sizeof(double), hipMemcpyDeviceToDevice, st));
+        if (d32[w]) cvt_f64_f32<<<cvt_grid(len[w]), 256, 0, st>>>(src[w], d32[w], len[w]);
+    }
+    if (P->precisions & MX_PLAN_BF16X3) pack_all(P, src, st);
+    HIPCHK(hipGetLastError());
 }
 
 // Evaluation through a captured hipGraph: the launch sequence of one
 // (precision, I/O type, nrhs, W, Y) is captured once and replayed; work
 // buffers are sized before the capture and the sequence holds no memsets.
 // Up to four graphs are kept per plan (least recently used is dropped).
-static void eval_graph(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
                        hipStream_t st) {
     for (auto &g : P->graphs) {
-        if (g.prec == prec && g.io == io && g.nrhs == nrhs && g.W == W && g.Y == Y) {
+        if (g.dir == dir && g.prec == prec && g.io == io && g.nrhs == nrhs && g.W == W && g.Y == Y) {
             g.used = ++P->graph_clock;
             HIPCHK(hipGraphLaunch(g.exec, st));
-            P->last_prec = prec;
+            if (dir == DIR_FWD) P->last_prec = prec;
             return;
         }
     }
-    if (prec == MX_BF16X3) ensure_panels(P, nrhs);
-    else ensure_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
+    if (prec == MX_BF16X3) {
+        ensure_panels(P, nrhs);
+    } else {
+        ensure_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
+        if (dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
+    }
     const int prof = P->profiling;
     P->profiling = 0;
     hipStream_t cs = st;
@@ -740,7 +1046,7 @@ static void eval_graph(mx_plan *P, mx_precision prec, const void *W, void *Y, in
     try {
         HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         try {
-            eval(P, prec, W, Y, nrhs, io, cs);
+            eval(P, dir, prec, W, Y, nrhs, io, cs);
         } catch (...) {
             hipGraph_t g2 = nullptr;
             (void)hipStreamEndCapture(cs, &g2);
@@ -767,7 +1073,7 @@ static void eval_graph(mx_plan *P, mx_precision prec, const void *W, void *Y, in
         HIPCHK(hipGraphExecDestroy(lru->exec));
         P->graphs.erase(lru);
     }
-    P->graphs.push_back(mx_plan::GraphEntry{prec, io, nrhs, W, Y, exec, ++P->graph_clock});
+    P->graphs.push_back(mx_plan::GraphEntry{dir, prec, io, nrhs, W, Y, exec, ++P->graph_clock});
     HIPCHK(hipGraphLaunch(exec, st));
     P->last_prec = prec;
 }
@@ -965,7 +1271,7 @@ static int32_t compat_eval(bool hss, double *D, double *B, double *VT, uint64_t 
     ensure_work(P.get(), nrhs, sizeof(double), false);
     HIPCHK(hipMemcpy(P->uskel, uskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyHostToDevice));
     try {
-        eval(P.get(), MX_FP64, dW, dY, nrhs, MX_IO_F64, nullptr);
+        eval(P.get(), DIR_FWD, MX_FP64, dW, dY, nrhs, MX_IO_F64, nullptr);
         HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipMemcpy(apres, dY, (size_t)(N * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(wskel, P->wskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
@@ -1064,8 +1370,47 @@ mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int3
         arg_check(plan != nullptr, "null plan");
         arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
         arg_check(W && Y && nrhs >= 1, "bad W, Y or nrhs");
-        if (plan->use_graph && !plan->profiling) eval_graph(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
-        else eval(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        if (plan->use_graph && !plan->profiling) eval_graph(plan, DIR_FWD, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        else eval(plan, DIR_FWD, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_eval_adjoint(mx_plan *plan, mx_precision prec, const void *dY, void *dW, int32_t nrhs, mx_io_type io,
+                          void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
+        arg_check(dY && dW && nrhs >= 1, "bad dY, dW or nrhs");
+        if (plan->use_graph && !plan->profiling) eval_graph(plan, DIR_ADJ, prec, dY, dW, nrhs, io, (hipStream_t)stream);
+        else eval(plan, DIR_ADJ, prec, dY, dW, nrhs, io, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_backward(mx_plan *plan, mx_precision prec, const void *W, const void *dY, void *dW, int32_t nrhs,
+                      mx_io_type io, const void *wskel, const void *uskel, void *dD, void *dB, void *dVT,
+                      void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr && nrhs >= 1, "null plan or bad nrhs");
+        backward(plan, prec, W, dY, dW, nrhs, io, wskel, uskel, dD, dB, dVT, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_blocks(mx_plan *plan, const double *D, const double *B, const double *VT, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        set_blocks(plan, D, B, VT, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_plan_grad_flops(const mx_plan *plan, int32_t nrhs, double *flops) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && flops && nrhs >= 0, "bad argument");
+        if (!plan->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+        *flops = plan->g_flops * nrhs;
         return 0;
     });
 }
@@ -1158,12 +1503,11 @@ mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, do
         arg_check(plan && flops && total && nrhs >= 0, "bad argument");
         for (int i = 0; i < 4; ++i) flops[i] = 0.0;
         if (prec == MX_BF16X3) {
-            arg_check(!plan->launch_flops_x3.empty() || plan->launches_x3.empty(), "no BF16x3 lists");
-            for (size_t i = 0; i < plan->launch_flops_x3.size(); ++i)
-                flops[plan->launch_kind_x3[i] ? MX_KIND_BF16X3 : MX_KIND_SEG_F32] += plan->launch_flops_x3[i] * nrhs;
+            if (!plan->prog[DIR_FWD][1].built) throw MxError(MX_ERR_UNSUPPORTED, "no BF16x3 lists in the plan");
+            for (double f : plan->prog[DIR_FWD][1].flops) flops[MX_KIND_BF16X3] += f * nrhs;
         } else {
             int k = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
-            for (double f : plan->launch_flops) flops[k] += f * nrhs;
+            for (double f : plan->prog[DIR_FWD][0].flops) flops[k] += f * nrhs;
         }
         *total = flops[0] + flops[1] + flops[2] + flops[3];
         return 0;
@@ -1174,7 +1518,7 @@ mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t n
                                int32_t *count) {
     return (mx_status)guard([&]() {
         arg_check(plan && count && cap >= 0 && (cap == 0 || flops), "bad argument");
-        const std::vector<double> &f = prec == MX_BF16X3 ? plan->launch_flops_x3 : plan->launch_flops;
+        const std::vector<double> &f = plan->prog[DIR_FWD][prec == MX_BF16X3 ? 1 : 0].flops;
         *count = (int32_t)f.size();
         for (int32_t i = 0; i < (int32_t)f.size() && i < cap; ++i) flops[i] = f[i] * nrhs;
         return 0;

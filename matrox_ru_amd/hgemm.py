@@ -15,7 +15,8 @@ import numpy as np
 from . import _native
 
 FP64, FP32, BF16X3 = 0, 1, 2
-KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert"}
+KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32"}
+PLAN_ADJOINT = 1 << 8
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -60,6 +61,14 @@ def lib():
         L.mx_plan_destroy.restype = ctypes.c_int
         L.mx_eval.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
         L.mx_eval.restype = ctypes.c_int
+        L.mx_eval_adjoint.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
+        L.mx_eval_adjoint.restype = ctypes.c_int
+        L.mx_backward.argtypes = [P, ctypes.c_int, P, P, P, I32, ctypes.c_int, P, P, P, P, P, P]
+        L.mx_backward.restype = ctypes.c_int
+        L.mx_plan_set_blocks.argtypes = [P, P, P, P, P]
+        L.mx_plan_set_blocks.restype = ctypes.c_int
+        L.mx_plan_grad_flops.argtypes = [P, I32, ctypes.POINTER(ctypes.c_double)]
+        L.mx_plan_grad_flops.restype = ctypes.c_int
         L.mx_plan_skeletons.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
         L.mx_plan_skeletons.restype = ctypes.c_int
         L.mx_plan_copy_skeletons.argtypes = [P, P, P, I32, P]
@@ -133,8 +142,8 @@ def _stream_handle(stream):
 class Plan:
     """Device-resident compressed matrix (mx_plan)."""
 
-    def __init__(self, hm, precisions=("fp64",)):
-        mask = 0
+    def __init__(self, hm, precisions=("fp64",), adjoint=False):
+        mask = PLAN_ADJOINT if adjoint else 0
         for p in precisions:
             mask |= 1 << PRECISIONS[p]
         s, keep = structure_of(hm)
@@ -148,6 +157,8 @@ class Plan:
         self.rows = n.value
         self.skel_rows = sk.value
         self.precisions = tuple(precisions)
+        self.adjoint = bool(adjoint)
+        self.block_lens = (len(hm.arrays["D"]), len(hm.arrays["B"]), len(hm.arrays["VT"]))
         self._wrow = np.asarray(hm.arrays["wrow"], dtype=np.int64)
         self._ddim = np.asarray(hm.arrays["Ddim"], dtype=np.int64)
 
@@ -162,6 +173,54 @@ class Plan:
         _check(lib().mx_eval(self._h, PRECISIONS[precision], W.data_ptr(), Y.data_ptr(), nrhs, io,
                              _stream_handle(stream)))
         return Y
+
+    def eval_adjoint(self, dY, dW, nrhs, precision="fp64", stream=None):
+        """dW = K~^T dY (mx_eval_adjoint); same layouts and dtypes as eval."""
+        import torch
+        assert dY.is_cuda and dW.is_cuda and dY.is_contiguous() and dW.is_contiguous()
+        assert dY.numel() == self.rows * nrhs and dW.numel() == self.rows * nrhs
+        assert dY.dtype == dW.dtype and dY.dtype in (torch.float64, torch.float32)
+        io = IO_F64 if dY.dtype == torch.float64 else IO_F32
+        _check(lib().mx_eval_adjoint(self._h, PRECISIONS[precision], dY.data_ptr(), dW.data_ptr(), nrhs, io,
+                                     _stream_handle(stream)))
+        return dW
+
+    def backward(self, W, dY, nrhs, precision="fp64", skeletons=None, blocks=(True, True, True), stream=None):
+        """mx_backward: returns (dW, dD, dB, dVT); a block gradient is None
+        when its flag in ``blocks`` is False.  ``skeletons`` = (wskel, uskel)
+        of the forward evaluation of W (Plan.skeletons), or None for the
+        plan's own."""
+        import torch
+        assert W.is_cuda and dY.is_cuda and W.is_contiguous() and dY.is_contiguous()
+        assert W.dtype == dY.dtype and W.numel() == self.rows * nrhs and dY.numel() == self.rows * nrhs
+        io = IO_F64 if W.dtype == torch.float64 else IO_F32
+        dW = torch.empty_like(dY)
+        outs = [torch.empty(n, dtype=W.dtype, device=W.device) if f else None
+                for f, n in zip(blocks, self.block_lens)]
+        ptr = [o.data_ptr() if o is not None else None for o in outs]
+        ws = us = None
+        if skeletons is not None:
+            ws, us = (t.data_ptr() for t in skeletons)
+        _check(lib().mx_backward(self._h, PRECISIONS[precision], W.data_ptr(), dY.data_ptr(), dW.data_ptr(), nrhs,
+                                 io, ws, us, ptr[0], ptr[1], ptr[2], _stream_handle(stream)))
+        return (dW, *outs)
+
+    def set_blocks(self, D=None, B=None, VT=None, stream=None):
+        """Replace stored blocks from FP64 CUDA tensors (mx_plan_set_blocks)."""
+        import torch
+        ps = []
+        for t, n in zip((D, B, VT), self.block_lens):
+            if t is None:
+                ps.append(None)
+                continue
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n
+            ps.append(t.data_ptr())
+        _check(lib().mx_plan_set_blocks(self._h, ps[0], ps[1], ps[2], _stream_handle(stream)))
+
+    def grad_flops(self, nrhs):
+        f = ctypes.c_double()
+        _check(lib().mx_plan_grad_flops(self._h, nrhs, ctypes.byref(f)))
+        return f.value
 
     def skeletons(self, nrhs, dtype, stream=None):
         """Copies of (wskel, uskel) of the last evaluation as CUDA tensors."""

@@ -129,6 +129,62 @@ class HMatrix:
             raise RuntimeError("mxh_untransform failed")
         return W
 
+    def transposed(self):
+        """The structure of K~^T in the same layout: pair lists swapped, every
+        near block D_k and coupling block B_k transposed in place (same
+        offsets, since a transposed block has the same size), the same VT.
+        Near and far blocking become one block each (nblockSet = fblockSet =
+        [0, 1]) because the row-disjointness the reference's blocking
+        guarantees (ClusterTree.cpp:951-1089) does not survive the swap.
+        HSS: node i's coupling becomes B_sib(i)^T, the same s_i x s_sib size
+        at Bptr[i-1] (HGEMM_v1.h:254-271)."""
+        a = {k: np.array(v, copy=True) for k, v in self.arrays.items()}
+        s = dict(self.scalars)
+        Ddim, slen = a["Ddim"], a["slen"]
+
+        def tr_blocks(src, ptr, rdim, cdim):
+            out = src.copy()
+            for k in range(len(ptr)):
+                r, c, o = int(rdim[k]), int(cdim[k]), int(ptr[k])
+                blk = src[o:o + r * c].reshape(c, r)       # column-major r x c
+                out[o:o + r * c] = blk.T.reshape(-1)       # column-major c x r
+            return out
+
+        if s["is_hss"]:
+            nleaf = int(s["nleaf"])
+            leafs = np.arange(nleaf)
+            a["D"] = tr_blocks(self.arrays["D"], a["Dptr"], Ddim[leafs], Ddim[leafs])
+            numnodes = int(s["numnodes"])
+            par = np.empty(numnodes, dtype=np.int64)
+            for v in range(numnodes):
+                if a["lchildren"][v] >= 0:
+                    par[a["lchildren"][v]] = v
+                    par[a["rchildren"][v]] = v
+            B0, B = self.arrays["B"], a["B"]
+            for i in range(1, numnodes):
+                p = par[i]
+                sib = a["rchildren"][p] if a["lchildren"][p] == i else a["lchildren"][p]
+                si, ss = int(slen[i]), int(slen[sib])
+                o_src, o_dst = int(a["Bptr"][sib - 1]), int(a["Bptr"][i - 1])
+                blk = B0[o_src:o_src + ss * si].reshape(si, ss)   # B_sib, ss x si column-major
+                B[o_dst:o_dst + si * ss] = blk.T.reshape(-1)
+        else:
+            nx, ny = a["npairx"], a["npairy"]
+            fx, fy = a["fpairx"], a["fpairy"]
+            a["D"] = tr_blocks(self.arrays["D"], a["Dptr"], Ddim[nx], Ddim[ny])
+            a["B"] = tr_blocks(self.arrays["B"], a["Bptr"], slen[fx], slen[fy])
+            a["npairx"], a["npairy"] = ny.copy(), nx.copy()
+            a["fpairx"], a["fpairy"] = fy.copy(), fx.copy()
+            a["nblockSet"] = np.array([0, 1], dtype=np.int32)
+            a["nblocks"] = np.array([0, len(nx)], dtype=np.int32)
+            a["fblockSet"] = np.array([0, 1], dtype=np.int32)
+            a["fblocks"] = np.array([0, len(fx)], dtype=np.int32)
+            s["nl"], s["fl"] = 1, 1
+        t = HMatrix(scalars=s, arrays=a, config=dict(self.config))
+        if "_owner" in self.__dict__:
+            t.__dict__["_owner"] = self.__dict__["_owner"]
+        return t
+
     def _handle_for_perm(self):
         o = self.__dict__.get("_owner")
         if o is None or not o.h:

@@ -44,7 +44,8 @@ def test_header_symbols_exported(header):
 
 def test_expected_entry_points_present():
     names = declared_functions(os.path.join(ROOT, "include", "matrox_hgemm.h"))
-    for n in ("HGEMM", "HGEMM_HSS", "mx_hgemm_configure", "mx_plan_create", "mx_eval", "mx_plan_destroy"):
+    for n in ("HGEMM", "HGEMM_HSS", "mx_hgemm_configure", "mx_plan_create", "mx_eval", "mx_plan_destroy",
+              "mx_eval_adjoint", "mx_backward", "mx_plan_set_blocks"):
         assert n in names
 
 

@@ -3,7 +3,7 @@
 import numpy as np
 import pytest
 
-from conftest import make_problem
+from conftest import make_problem, rel
 from matrox_ru_amd import host
 
 
@@ -193,3 +193,20 @@ def test_synthetic_generators_are_deterministic():
     assert np.allclose(a.mean(0), 0, atol=1e-12) and np.allclose(a.std(0), 1, atol=1e-12)
     z = host.normal((200000,), seed=5)
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
+
+
+@pytest.mark.parametrize("budget", [0.0, 0.25])
+def test_transposed_structure_is_the_transpose(budget):
+    """The transposed CDS (pair lists swapped, D/B transposed, same VT)
+    assembles to K~^T, and the oracle on it satisfies <K~x, y> = <x, K~^T y>."""
+    from oracle import oracle
+    X, hm = make_problem(n=900, d=4, leaf=50, rank=30, tol=1e-7, budget=budget, h=1.1)
+    T = hm.transposed()
+    K = oracle.assemble(hm)
+    assert rel(oracle.assemble(T), K.T) <= 1e-15
+    nrhs = 3
+    x = host.normal((hm.N * nrhs,), seed=3)
+    y = host.normal((hm.N * nrhs,), seed=4)
+    Kx, _, _ = oracle.hgemm(hm, x, nrhs)
+    Kty, _, _ = oracle.hgemm(T, y, nrhs)
+    assert abs(float(Kx @ y) - float(x @ Kty)) <= 1e-13 * np.linalg.norm(Kx) * np.linalg.norm(y)
