@@ -1,21 +1,26 @@
 #!/usr/bin/env python3
 """HGEMM evaluation benchmark on one MI355X.
 
-Workload (H2-b, the headline configuration): N = 65,536 points in d = 28 from
-a 16-component Gaussian mixture (per-component diagonal sigma in [0.5, 2]),
-standardised; Gaussian kernel h = 1.5, leaf 256, rank cap 256, ID tolerance
-1e-5, near budget 0.03; W is 65,536 x 1,024, N(0,1), FP64.  Synthetic data
-from the package's splitmix64/Box-Muller generators (points seed 1002,
-W seed 2002, sampled rows seed 3002).
+Default workload (H2-b, the headline configuration): N = 65,536 points in
+d = 28 from a 16-component Gaussian mixture (per-component diagonal sigma in
+[0.5, 2]), standardised; Gaussian kernel h = 1.5, leaf 256, rank cap 256, ID
+tolerance 1e-5, near budget 0.03; W is 65,536 x 1,024, N(0,1), FP64.
+Synthetic data from the package's splitmix64/Box-Muller generators (points
+seed 1002, W seed 2002, sampled rows seed 3002).
 
-A step is one evaluation Y = K~ W in the mixed form (split-BF16x3 near and
-far products with FP32 accumulation, FP32 tree transfers), with W and Y in
-FP64 on the device (the FP64->FP32 input and FP32->FP64 output conversions
-are inside the timed region).  The plain FP64 evaluation is timed the same
-way for the speedup.  The compressed matrix is built on the host before the
-timed region, as in the reference driver (matroxTest-V1/main.cpp:84-108).
+A step is one evaluation Y = K~ W in the mixed form (split-BF16x3 products
+with FP32 accumulation on MFMA), with W and Y in FP64 on the device (the
+input split and the FP64 output are inside the timed region).  The plain
+FP64 evaluation is timed the same way for the speedup, and the adjoint
+dW = K~^T dY and the backward pass with block gradients are timed beside it.
+The compressed matrix is built on the host before the timed region, as in
+the reference driver (matroxTest-V1/main.cpp:84-108).
 
-Prints one JSON line (rank 0, N=1).
+Other workloads (--workload): hss_n16384 (plain HSS FP64, leaf 128, nrhs 256)
+and adjoint_n32768 (FP32 forward and backward with block gradients, nrhs
+128, checked against FP64 central differences on 64 directions).
+
+Prints one JSON line (N=1).
 """
 import argparse
 import json
@@ -29,9 +34,20 @@ sys.path.insert(0, ROOT)
 
 MI355X_PEAK = {"bf16": 2500.0, "fp32": 157.3, "fp64": 78.6, "hbm_gbs": 8000.0}
 
-WORKLOAD = dict(name="h2b_gmm16_n65536_d28", n=65536, d=28, ncomp=16, sig_lo=0.5, sig_hi=2.0, diag=True,
-                h=1.5, leaf=256, rank=256, tol=1e-5, budget=0.03, nrhs=1024,
-                seed_points=1002, seed_w=2002, seed_rows=3002, check_rows=1000)
+HEADLINE_METRIC = "HGEMM eval TFLOP/s+ms, mixed vs FP64 plain, N=64K nrhs=1024, rel err<=1e-5"
+
+WORKLOADS = {
+    "h2b_n65536": dict(name="h2b_gmm16_n65536_d28", n=65536, d=28, ncomp=16, sig_lo=0.5, sig_hi=2.0, diag=True,
+                       h=1.5, leaf=256, rank=256, tol=1e-5, budget=0.03, nrhs=1024, precision="bf16x3",
+                       seed_points=1002, seed_w=2002, seed_rows=3002, check_rows=1000),
+    "hss_n16384": dict(name="hss_gmm8_n16384_d18", n=16384, d=18, ncomp=8, sig_lo=1.0, sig_hi=1.0, diag=False,
+                       h=1.0, leaf=128, rank=128, tol=1e-5, budget=0.0, nrhs=256, precision="bf16x3",
+                       seed_points=1001, seed_w=2001, seed_rows=3001, check_rows=1000),
+    "adjoint_n32768": dict(name="adjoint_gmm32_n32768_d54", n=32768, d=54, ncomp=32, sig_lo=1.0, sig_hi=1.0,
+                           diag=False, h=2.0, leaf=256, rank=192, tol=1e-5, budget=0.03, nrhs=128,
+                           precision="fp32", seed_points=1004, seed_w=2004, seed_rows=3004, check_rows=1000,
+                           seed_target=4004, seed_dirs=5004),
+}
 
 
 def log(*a):
@@ -43,15 +59,235 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nrhs", type=int, default=WORKLOAD["nrhs"])
-    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--workload", default="h2b_n65536", choices=sorted(WORKLOADS))
+    ap.add_argument("--nrhs", type=int, default=0, help="override the workload's nrhs")
+    ap.add_argument("--precision", default="", choices=["", "bf16x3", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nrhs", type=int, default=128)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain 1-buffer copy of this size (HBM-counter calibration)")
     ap.add_argument("--skip-checks", action="store_true", help="skip the dense-row error checks")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and backward timings")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     return ap.parse_args()
+
+
+class Timer:
+    """Mean time of a callable over `steps` calls after `warmup` calls, with
+    events on the stream the evaluations are enqueued on."""
+
+    def __init__(self, torch, stream):
+        self.torch = torch
+        self.stream = stream
+
+    def __call__(self, fn, steps, warmup):
+        torch = self.torch
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        for _ in range(steps):
+            fn()
+        e1.record(self.stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+
+def build_matrix(wl, host):
+    t0 = time.time()
+    X = host.gaussian_mixture(wl["n"], wl["d"], wl["ncomp"], sig_lo=wl["sig_lo"], sig_hi=wl["sig_hi"],
+                              diag=wl["diag"], seed=wl["seed_points"])
+    hm = host.build_hmatrix(X, leaf_size=wl["leaf"], max_rank=wl["rank"], tol=wl["tol"], budget=wl["budget"],
+                            h=wl["h"])
+    t_build = time.time() - t0
+    log(f"[bench] host build {t_build:.1f}s: nleaf={hm.nleaf} ncount={hm.ncount} fcount={hm.fcount} "
+        f"ranks {hm.slen[1:].min()}..{hm.slen[1:].max()} hss={hm.is_hss}")
+    return X, hm, t_build
+
+
+def launch_profile(plan, run, prec, nrhs, parts, nprof=3):
+    """Per-launch durations (HIP events on the evaluation stream) -> per-kind
+    totals and the roofline entry of the dominant kernel."""
+    plan.set_profiling(True)
+    per_kind = {}
+    per_launch = None
+    for _ in range(nprof):
+        run()
+        lt = plan.launch_times()
+        per_launch = [ms for _, ms in lt] if per_launch is None else [a + ms for a, (_, ms) in zip(per_launch, lt)]
+        for k, ms in lt:
+            per_kind.setdefault(k, []).append(ms)
+    plan.set_profiling(False)
+    log("[bench] per-launch ms: " + " ".join(f"{x / nprof:.3f}" for x in per_launch))
+    log("[bench] per-launch GFLOP: " + " ".join(f"{f / 1e9:.1f}" for f in plan.launch_flops(prec, nrhs)))
+    kind_total = {k: sum(v) / nprof for k, v in per_kind.items()}
+    kind_launches = {k: len(v) // nprof for k, v in per_kind.items()}
+    dom = max((k for k in kind_total if k != "convert"), key=lambda k: kind_total[k])
+    dom_flops = parts.get(dom, 0.0)
+    dom_avg_ms = kind_total[dom] / max(1, kind_launches[dom])
+    dom_flops_per_launch = dom_flops / max(1, kind_launches[dom])
+    achieved = dom_flops_per_launch / (dom_avg_ms * 1e-3) / 1e12
+    peak = {"bf16x3": MI355X_PEAK["bf16"], "seg_gemm_f32": MI355X_PEAK["fp32"],
+            "seg_gemm_f64": MI355X_PEAK["fp64"]}[dom]
+    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
+                "launches_per_eval": kind_launches[dom], "avg_launch_ms": round(dom_avg_ms, 5),
+                "flops_per_launch": dom_flops_per_launch}
+    if dom == "bf16x3":
+        roofline["frac_effective_x3"] = round(achieved / (peak / 3), 5)
+    kinds = {k: {"ms_per_eval": round(kind_total[k], 4), "launches": kind_launches[k],
+                 "tflops": round(parts.get(k, 0.0) / (kind_total[k] * 1e-3) / 1e12, 3) if parts.get(k) else None}
+             for k in kind_total}
+    log(f"[bench] kernels: {kinds}")
+    return roofline, kinds
+
+
+def dense_checks(plan, hm, wl, Wd, nrhs, results, np, torch):
+    rng = np.random.default_rng(wl["seed_rows"])
+    rows = np.sort(rng.choice(hm.N, min(wl["check_rows"], hm.N), replace=False)).astype(np.int32)
+    dense = plan.dense_rows(rows, Wd, nrhs)
+    nd = torch.linalg.vector_norm(dense)
+    checks = {}
+    arows = np.arange(min(10, int(hm.Ddim[0])), dtype=np.int32)
+    da = plan.dense_rows(arows, Wd, nrhs)
+    for name, Y in results.items():
+        r = plan.rows_of(Y, rows, nrhs).double()
+        checks[f"rel_err_{name}_vs_dense"] = float(torch.linalg.vector_norm(r - dense) / nd)
+        # `acc` (computeError, matroxTest-V1/Util.h:276-332): first 10 points of leaf 0
+        ya = plan.rows_of(Y, arows, nrhs).double()
+        checks[f"acc_{name}"] = float(((torch.linalg.vector_norm(ya - da, dim=1) /
+                                        torch.linalg.vector_norm(da, dim=1)).mean()))
+    log(f"[bench] checks: {checks}")
+    return checks
+
+
+def cpu_baseline(hm, wl, nrhs, ns, host):
+    """The oracle (restated reference evaluation, OpenMP) on the host cores,
+    on a bounded sample: the same compressed matrix with ns columns."""
+    from oracle import oracle
+    Ws = host.normal((hm.N, ns), seed=wl["seed_w"] + 1)
+    TWs = hm.transform(Ws)
+    oracle.hgemm(hm, TWs, ns)
+    ts = []
+    for _ in range(3):
+        t1 = time.time()
+        oracle.hgemm(hm, TWs, ns)
+        ts.append(time.time() - t1)
+    tcpu = statistics.median(ts)
+    fl = oracle.flops(hm, ns)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu = {"value": round(fl / tcpu / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+           "sample": f"oracle FP64 evaluation (restated HGEMM_gen.h sweeps, OpenMP, own column-major GEMM) of the "
+                     f"same compressed matrix with nrhs={ns} instead of {nrhs}; median of 3 runs = {tcpu:.3f} s",
+           "ms_per_eval_scaled_to_nrhs": round(tcpu * 1e3 * nrhs / ns, 1)}
+    log(f"[bench] cpu baseline: {cpu}")
+    return cpu
+
+
+def adjoint_timings(plan, timer, Wd, nrhs, prec, ms_fwd_mixed, ms_fwd_64, steps, torch):
+    """Adjoint dW = K~^T dY (mixed and FP64) and the backward pass with block
+    gradients (FP32 and FP64), on the same matrix and sizes as the forward."""
+    dY = Wd   # any N x nrhs input; the timing does not depend on its values
+    G = torch.empty_like(Wd)
+    Y = torch.empty_like(Wd)
+    out = {}
+    ms_adj = timer(lambda: plan.eval_adjoint(dY, G, nrhs, prec), steps, 2)
+    G64 = torch.empty_like(Wd)
+    ms_adj64 = timer(lambda: plan.eval_adjoint(dY, G64, nrhs, "fp64"), max(3, steps // 4), 1)
+    d = G - G64
+    out["rel_err_adjoint_mixed_vs_fp64"] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(G64))
+    del d
+    out.update({"adjoint_mixed_ms": round(ms_adj, 4), "adjoint_mixed_ratio_vs_forward": round(ms_adj / ms_fwd_mixed, 3),
+                "adjoint_fp64_ms": round(ms_adj64, 4), "adjoint_fp64_ratio_vs_forward": round(ms_adj64 / ms_fwd_64, 3)})
+    # backward with block gradients: forward skeletons from a forward evaluation
+    grad_flops = plan.grad_flops(nrhs)
+    for gp in ("fp32", "fp64"):
+        if gp not in plan.precisions:
+            continue
+        plan.eval(Wd, Y, nrhs, gp)
+        ws, us = plan.skeletons(nrhs, torch.float64 if gp == "fp64" else torch.float32)
+        ms_b = timer(lambda: plan.backward(Wd, dY, nrhs, gp, skeletons=(ws, us)), max(3, steps // 4), 1)
+        ms_fwd = timer(lambda: plan.eval(Wd, Y, nrhs, gp), max(3, steps // 4), 1)
+        plan.set_profiling(True)
+        plan.backward(Wd, dY, nrhs, gp, skeletons=(ws, us))
+        lt = plan.launch_times()
+        plan.set_profiling(False)
+        ms_g = sum(ms for k, ms in lt if k.startswith("grad"))
+        out[f"backward_{gp}_ms"] = round(ms_b, 4)
+        out[f"forward_{gp}_ms"] = round(ms_fwd, 4)
+        out[f"backward_{gp}_ratio_vs_forward"] = round(ms_b / ms_fwd, 3)
+        out[f"block_grad_{gp}_ms"] = round(ms_g, 4)
+        out[f"block_grad_{gp}_tflops"] = round(grad_flops / (ms_g * 1e-3) / 1e12, 3) if ms_g > 0 else None
+        del ws, us
+    out["block_grad_tflop"] = round(grad_flops / 1e12, 5)
+    log(f"[bench] adjoint: {out}")
+    return out
+
+
+def fd_check(plan, hm, wl, nrhs, torch, np, host, ndir=64, eps=1e-6):
+    """Gradients of L = 1/2|K~W - T|^2 (W, T FP32; FP32 backward) against FP64
+    central differences on ndir random directions (a quarter each in W, D, B,
+    VT), and dW against the oracle on the transposed structure."""
+    from matrox_ru_amd.autograd import hgemm_apply
+    from oracle import oracle
+    dev = torch.device("cuda:0")
+    W32 = torch.from_numpy(host.normal((hm.N * nrhs,), seed=wl["seed_w"], dtype=np.float32)).to(dev)
+    T32 = torch.from_numpy(host.normal((hm.N * nrhs,), seed=wl["seed_target"], dtype=np.float32)).to(dev)
+    D0 = torch.from_numpy(np.ascontiguousarray(hm.D)).to(dev)
+    B0 = torch.from_numpy(np.ascontiguousarray(hm.B)).to(dev)
+    VT0 = torch.from_numpy(np.ascontiguousarray(hm.VT)).to(dev)
+    res = {}
+    grads = {}
+    for gp in ("fp32", "fp64"):
+        Wg = (W32 if gp == "fp32" else W32.double()).clone().requires_grad_()
+        D, B, VT = (t.clone().requires_grad_() for t in (D0, B0, VT0))
+        Y = hgemm_apply(plan, Wg, D, B, VT, nrhs=nrhs, precision=gp)
+        L = 0.5 * ((Y - T32.to(Y.dtype)) ** 2).sum()
+        L.backward()
+        grads[gp] = [Wg.grad.double(), D.grad, B.grad, VT.grad]
+        if gp == "fp32":
+            dY = (Y.detach().double() - T32.double()).cpu().numpy()
+    Wo, _, _ = oracle.hgemm(hm.transposed(), dY, nrhs)
+    res["rel_err_dW_fp32_vs_oracle_transposed"] = float(np.linalg.norm(grads["fp32"][0].cpu().numpy() - Wo)
+                                                        / np.linalg.norm(Wo))
+    W64, T64 = W32.double(), T32.double()
+    lens = [hm.N * nrhs, len(hm.D), len(hm.B), len(hm.VT)]
+    an = {"fp32": [], "fp64": []}
+    fd = []
+    group = []
+    for i in range(ndir):
+        g = i % 4
+        if lens[g] == 0:
+            continue
+        r = torch.from_numpy(host.normal((lens[g],), seed=wl["seed_dirs"] + i)).to(dev)
+        r /= r.norm()
+        v = [torch.zeros(n, dtype=torch.float64, device=dev) if j != g else r for j, n in enumerate(lens)]
+        Ys = []
+        for s in (1.0, -1.0):
+            plan.set_blocks(D0 + s * eps * v[1], B0 + s * eps * v[2], VT0 + s * eps * v[3])
+            Y = torch.empty_like(W64)
+            plan.eval(W64 + s * eps * v[0], Y, nrhs, "fp64")
+            Ys.append(Y)
+        fd.append(0.5 * float(((Ys[0] - Ys[1]) * (Ys[0] + Ys[1] - 2 * T64)).sum()) / (2 * eps))
+        for gp in an:
+            an[gp].append(float(torch.dot(grads[gp][g], r)))
+        group.append(g)
+    plan.set_blocks(D0, B0, VT0)
+    fd = np.array(fd)
+    group = np.array(group)
+    for gp in an:
+        a = np.array(an[gp])
+        res[f"fd_rel_err_{gp}"] = float(np.linalg.norm(a - fd) / np.linalg.norm(fd))
+        for g, nm in enumerate(["W", "D", "B", "VT"]):
+            m = group == g
+            if m.any():
+                res[f"fd_rel_err_{gp}_{nm}"] = float(np.linalg.norm(a[m] - fd[m]) / np.linalg.norm(fd[m]))
+    res["fd_directions"] = int(len(fd))
+    res["fd_eps"] = eps
+    log(f"[bench] fd check: {res}")
+    return res
 
 
 def main():
@@ -72,22 +308,17 @@ def main():
             break
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda:0")
-    wl = dict(WORKLOAD)
-    nrhs = args.nrhs
+    wl = dict(WORKLOADS[args.workload])
+    nrhs = args.nrhs or wl["nrhs"]
+    prec = args.precision or wl["precision"]
+    adjoint_wl = args.workload == "adjoint_n32768"
 
-    t0 = time.time()
-    X = host.gaussian_mixture(wl["n"], wl["d"], wl["ncomp"], sig_lo=wl["sig_lo"], sig_hi=wl["sig_hi"],
-                              diag=wl["diag"], seed=wl["seed_points"])
-    hm = host.build_hmatrix(X, leaf_size=wl["leaf"], max_rank=wl["rank"], tol=wl["tol"], budget=wl["budget"],
-                            h=wl["h"])
-    t_build = time.time() - t0
-    log(f"[bench] host build {t_build:.1f}s: nleaf={hm.nleaf} ncount={hm.ncount} fcount={hm.fcount} "
-        f"ranks {hm.slen[1:].min()}..{hm.slen[1:].max()}")
+    X, hm, t_build = build_matrix(wl, host)
     W = host.normal((hm.N, nrhs), seed=wl["seed_w"])
     TW = hm.transform(W)
     del W
-    prec = args.precision
-    plan = hgemm.Plan(hm, precisions=("fp64", prec))
+    precs = ["fp64", prec] + (["fp32"] if prec != "fp32" and not args.no_adjoint else [])
+    plan = hgemm.Plan(hm, precisions=tuple(precs), adjoint=not args.no_adjoint or adjoint_wl)
     perm = hm.lids[: hm.N]
     plan.set_points(X[perm], wl["h"])
     Wd = torch.from_numpy(TW).to(dev)
@@ -99,104 +330,50 @@ def main():
     log(f"[bench] plan ready; algorithmic flops/eval = {flops64 / 1e12:.4f} TFLOP")
 
     stream = torch.cuda.current_stream()
-
-    def timed(precision, Y, steps, warmup):
-        for _ in range(warmup):
-            plan.eval(Wd, Y, nrhs, precision=precision)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        for _ in range(steps):
-            plan.eval(Wd, Y, nrhs, precision=precision)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
-
+    timer = Timer(torch, stream)
+    # autotuner: schedule and pipeline depth per direction, on this nrhs
+    tuning = {}
+    for p in plan.precisions:
+        ms = plan.autotune(p, nrhs, "f64", reps=2)
+        tuning[p] = {d: plan.get_schedule(d, p) for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
+        tuning[p]["ms"] = [round(m, 4) for m in ms]
+    log(f"[bench] autotuned: {tuning}")
     plan.set_graph(not args.no_graph)
-    ms64 = timed("fp64", Y64, max(3, args.steps // 4), 1)
+    ms64 = timer(lambda: plan.eval(Wd, Y64, nrhs, precision="fp64"), max(3, args.steps // 4), 1)
     log(f"[bench] fp64 {ms64:.3f} ms/eval")
-    msmx = timed(prec, Ymx, args.steps, args.warmup)
+    msmx = timer(lambda: plan.eval(Wd, Ymx, nrhs, precision=prec), args.steps, args.warmup)
     log(f"[bench] {prec} {msmx:.3f} ms/eval")
-    # FP32 I/O variant of the mixed path (no conversions)
+    # FP32 I/O variant of the mixed path (no FP64 conversions)
     W32 = Wd.float()
     Y32 = torch.empty_like(W32)
-    for _ in range(2):
-        plan.eval(W32, Y32, nrhs, precision=prec)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(args.steps):
-        plan.eval(W32, Y32, nrhs, precision=prec)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms32io = e0.elapsed_time(e1) / args.steps
-    del W32, Y32
+    ms32io = timer(lambda: plan.eval(W32, Y32, nrhs, precision=prec), args.steps, 2)
+    log(f"[bench] {prec} with FP32 I/O {ms32io:.3f} ms/eval")
+    del W32
 
-    # per-launch timing on the evaluation stream (separate pass)
-    plan.set_profiling(True)
-    per_kind = {}
-    nprof = 3
-    per_launch = None
-    for _ in range(nprof):
-        plan.eval(Wd, Ymx, nrhs, precision=prec)
-        lt = plan.launch_times()
-        per_launch = [ms for _, ms in lt] if per_launch is None else [a + ms for a, (_, ms) in zip(per_launch, lt)]
-        for k, ms in lt:
-            per_kind.setdefault(k, []).append(ms)
-    log("[bench] per-launch ms: " + " ".join(f"{x / nprof:.3f}" for x in per_launch))
-    log("[bench] per-launch GFLOP: " + " ".join(f"{f / 1e9:.1f}" for f in plan.launch_flops(prec, nrhs)))
-    plan.set_profiling(False)
-    kind_total = {k: sum(v) / nprof for k, v in per_kind.items()}
-    kind_launches = {k: len(v) // nprof for k, v in per_kind.items()}
-    dom = max((k for k in kind_total if k != "convert"), key=lambda k: kind_total[k])
-    dom_flops = parts.get(dom, 0.0)
-    dom_avg_ms = kind_total[dom] / max(1, kind_launches[dom])
-    dom_flops_per_launch = dom_flops / max(1, kind_launches[dom])
-    achieved = dom_flops_per_launch / (dom_avg_ms * 1e-3) / 1e12
-    peak = {"bf16x3": MI355X_PEAK["bf16"], "seg_gemm_f32": MI355X_PEAK["fp32"],
-            "seg_gemm_f64": MI355X_PEAK["fp64"]}[dom]
-    roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
-                "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
-                "launches_per_eval": kind_launches[dom], "avg_launch_ms": round(dom_avg_ms, 5),
-                "flops_per_launch": dom_flops_per_launch}
-    if dom == "bf16x3":
-        roofline["frac_effective_x3"] = round(achieved / (peak / 3), 5)
+    roofline, kinds_report = launch_profile(plan, lambda: plan.eval(Wd, Ymx, nrhs, precision=prec), prec, nrhs,
+                                            parts)
     tfile = os.path.join(ROOT, "profiles", "traffic_h2b_n65536.txt")
-    if nrhs == WORKLOAD["nrhs"] and os.path.exists(tfile):
+    if args.workload == "h2b_n65536" and nrhs == wl["nrhs"] and os.path.exists(tfile):
         kv = dict(l.strip().split("=", 1) for l in open(tfile) if "=" in l and not l.startswith("#"))
-        if kv.get("kernel") == dom:
+        if kv.get("kernel") == roofline["kernel"]:
             roofline["traffic"] = float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
             roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/x3_kernel_h2b_n65536.md"
-            roofline["algorithmic_bytes_per_launch"] = round(4.67e9 / kind_launches[dom], -6)
-    kinds_report = {k: {"ms_per_eval": round(kind_total[k], 4), "launches": kind_launches[k],
-                        "tflops": round(parts.get(k, 0.0) / (kind_total[k] * 1e-3) / 1e12, 3)
-                        if parts.get(k) else None} for k in kind_total}
-    log(f"[bench] kernels: {kinds_report}")
+            roofline["algorithmic_bytes_per_launch"] = round(4.67e9 / roofline["launches_per_eval"], -6)
 
-    # errors
     d = Ymx - Y64
     err_vs64 = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(Y64))
     del d
+    log(f"[bench] rel err mixed vs fp64 {err_vs64:.3e}")
     checks = {}
     if not args.skip_checks:
-        rng = np.random.default_rng(wl["seed_rows"])
-        rows = np.sort(rng.choice(hm.N, wl["check_rows"], replace=False)).astype(np.int32)
-        dense = plan.dense_rows(rows, Wd, nrhs)
-        r64 = plan.rows_of(Y64, rows, nrhs)
-        rmx = plan.rows_of(Ymx, rows, nrhs)
-        nd = torch.linalg.vector_norm(dense)
-        checks["rel_err_fp64_vs_dense"] = float(torch.linalg.vector_norm(r64 - dense) / nd)
-        checks["rel_err_mixed_vs_dense"] = float(torch.linalg.vector_norm(rmx - dense) / nd)
-        # `acc` (computeError): first 10 points of leaf 0, mean relative row error
-        arows = np.arange(min(10, int(hm.Ddim[0])), dtype=np.int32)
-        da = plan.dense_rows(arows, Wd, nrhs)
-        for name, Y in (("acc_fp64", Y64), ("acc_mixed", Ymx)):
-            ya = plan.rows_of(Y, arows, nrhs)
-            checks[name] = float(((torch.linalg.vector_norm(ya - da, dim=1) /
-                                   torch.linalg.vector_norm(da, dim=1)).mean()))
-        log(f"[bench] checks: {checks}")
+        checks = dense_checks(plan, hm, wl, Wd, nrhs, {"fp64": Y64, "mixed": Ymx}, np, torch)
+    del Y32
+
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_timings(plan, timer, Wd, nrhs, prec, msmx, ms64, args.steps, torch)
+        if adjoint_wl:
+            adjoint.update(fd_check(plan, hm, wl, nrhs, torch, np, host))
 
     if args.calib_copy_mib > 0:
         nb = args.calib_copy_mib * (1 << 20) // 8
@@ -207,30 +384,12 @@ def main():
         log(f"[bench] calibration copy of {args.calib_copy_mib} MiB done")
         del a, b
 
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import oracle
-        ns = args.cpu_sample_nrhs
-        Ws = host.normal((hm.N, ns), seed=wl["seed_w"] + 1)
-        TWs = hm.transform(Ws)
-        oracle.hgemm(hm, TWs, ns)
-        ts = []
-        for _ in range(3):
-            t1 = time.time()
-            oracle.hgemm(hm, TWs, ns)
-            ts.append(time.time() - t1)
-        tcpu = statistics.median(ts)
-        fl = oracle.flops(hm, ns)
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        cpu = {"value": round(fl / tcpu / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-               "sample": f"oracle FP64 evaluation (restated HGEMM_gen.h sweeps, OpenMP, own column-major "
-                         f"GEMM) of the same compressed matrix with nrhs={ns} instead of {nrhs}; median of 3 "
-                         f"runs = {tcpu:.3f} s",
-               "ms_per_eval_scaled_to_nrhs": round(tcpu * 1e3 * nrhs / ns, 1)}
-        log(f"[bench] cpu baseline: {cpu}")
+    cpu = None if args.no_cpu_baseline else cpu_baseline(hm, wl, nrhs, min(args.cpu_sample_nrhs, nrhs), host)
 
+    metric = HEADLINE_METRIC if args.workload == "h2b_n65536" else \
+        f"HGEMM eval TFLOP/s+ms, {prec} vs FP64 plain, {wl['name']} nrhs={nrhs}"
     out = {
-        "metric": "HGEMM eval TFLOP/s+ms, mixed vs FP64 plain, N=64K nrhs=1024, rel err<=1e-5",
+        "metric": metric,
         "value": round(flopsmx / (msmx * 1e-3) / 1e12, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -246,7 +405,8 @@ def main():
         "config": {"workload": wl["name"], "N": hm.N, "d": wl["d"], "nrhs": nrhs, "leaf": wl["leaf"],
                    "max_rank": wl["rank"], "tol": wl["tol"], "budget": wl["budget"], "h": wl["h"],
                    "nleaf": int(hm.nleaf), "near_blocks": int(hm.ncount), "far_blocks": int(hm.fcount),
-                   "mean_rank": round(float(hm.slen[1:].mean()), 2), "io": "f64 (conversions timed)"},
+                   "hss": bool(hm.is_hss), "mean_rank": round(float(hm.slen[1:].mean()), 2),
+                   "io": "f64 (conversions timed)"},
         "algorithmic_tflop_per_eval": round(flops64 / 1e12, 5),
         "fp64_ms": round(ms64, 4),
         "fp64_tflops": round(flops64 / (ms64 * 1e-3) / 1e12, 3),
@@ -255,6 +415,8 @@ def main():
         "mixed_ms_f32_io": round(ms32io, 4),
         "rel_err_mixed_vs_fp64": err_vs64,
         **checks,
+        "adjoint": adjoint,
+        "tuning": tuning,
         "kernels": kinds_report,
         "roofline": roofline,
         "cpu_baseline": cpu,

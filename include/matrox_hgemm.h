@@ -177,6 +177,27 @@ mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst
  * enable == 0 drops the captured graphs. */
 mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable);
 
+/* Schedules of the coupling and downward phases (per direction and kernel
+ * kind): MX_SCHED_FUSED runs one launch per tree level computing
+ * u_c = sum_far B w + VT_p^T u_p; MX_SCHED_SPLIT runs the couplings of all
+ * levels in one launch, then the parent transfers level by level.  Both give
+ * the same result up to summation order.  Defaults: fused forward, split
+ * adjoint.  x3_stages: BF16x3 pipeline depth, 2 or 4 (0 keeps the current). */
+enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1 };
+enum { MX_DIR_FORWARD = 0, MX_DIR_ADJOINT = 1 };
+mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
+                               int32_t x3_stages);
+mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
+                               int32_t *x3_stages);
+
+/* Autotuner: times every schedule (and, for MX_BF16X3, both pipeline
+ * depths) of each direction built into the plan with `reps` evaluations of
+ * nrhs columns on scratch buffers it allocates and frees, and keeps the
+ * fastest.  best_ms (optional, 2 entries) receives the chosen per-direction
+ * times.  Synchronises `stream`; drops captured graphs. */
+mx_status mx_plan_autotune(mx_plan *plan, mx_precision prec, int32_t nrhs, mx_io_type io, int32_t reps,
+                           float *best_ms, void *stream);
+
 /* ---- per-result checks and measurement ------------------------------------ */
 
 /* Attach the points (host, N x d row-major, in LEAF order: row j is point

@@ -281,8 +281,13 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
             if (gm < M && gn < a.nrhs) {
                 const float v = Cs[col * 64 + (row ^ ((col & 15) << 2))];
                 const int64_t o = base + (int64_t)gn * M + gm;
-                if (a.y_f64) static_cast<double *>(a.Y)[o] = (double)v;
-                else static_cast<float *>(a.Y)[o] = v;
+                if (a.y_f64) {
+                    double *y = static_cast<double *>(a.Y) + o;
+                    *y = task.beta ? *y + (double)v : (double)v;
+                } else {
+                    float *y = static_cast<float *>(a.Y) + o;
+                    *y = task.beta ? *y + v : v;
+                }
             }
         }
     } else {
@@ -298,13 +303,23 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
             for (int j = 0; j < 4; ++j) {
                 const int col = n0 + wc * 64 + j * 16 + (lane & 15);
                 if (col >= a.nrhs) continue;
+                const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
+                float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+                if (task.beta) {   // accumulate onto the panel's current value
+                    const u32x2_t oh = *reinterpret_cast<const u32x2_t *>(Ph + o);
+                    const u32x2_t ol = *reinterpret_cast<const u32x2_t *>(Pl + o);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t wh = oh[r >> 1], wl = ol[r >> 1];
+                        v[r] += bf16_f((uint16_t)(wh >> (16 * (r & 1)))) + bf16_f((uint16_t)(wl >> (16 * (r & 1))));
+                    }
+                }
                 uint16_t h[4], l[4];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    h[r] = bf16_rne(acc[i][j][r]);
-                    l[r] = bf16_rne(acc[i][j][r] - bf16_f(h[r]));
+                    h[r] = bf16_rne(v[r]);
+                    l[r] = bf16_rne(v[r] - bf16_f(h[r]));
                 }
-                const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
                 *reinterpret_cast<u32x2_t *>(Ph + o) =
                     u32x2_t{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
                 *reinterpret_cast<u32x2_t *>(Pl + o) =

@@ -76,6 +76,15 @@ struct Program {
 
 enum { DIR_FWD = 0, DIR_ADJ = 1 };
 
+// Schedules of the coupling and downward phases:
+//  SCHED_FUSED: one launch per level, u_c = sum_far B w + VT_p[:, part]^T u_p;
+//  SCHED_SPLIT: one launch for the couplings of every level (they only need
+//      the upward results), then one launch per level adding the transfer
+//      from the parent (beta = 1).  Long coupling lists at the top of the
+//      tree (the adjoint's column-grouped lists) then run beside everything
+//      else instead of in a nearly empty launch.
+enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, NSCHED = 2 };
+
 }  // namespace mx
 
 struct mx_plan {
@@ -84,9 +93,11 @@ struct mx_plan {
     int64_t N = 0, skel_rows = 0;
     std::vector<int32_t> wrow, Ddim, skelrow, slen;
 
-    // task lists: prog[direction][kind]; each is UP levels, FAR+DOWN levels,
-    // NEAR+LEAF (kind 1: 128-row tiles and packed split A operands)
-    mx::Program prog[2][2];
+    // task lists: prog[direction][kind][schedule]; each is UP levels, the
+    // coupling and downward launches, NEAR+LEAF (kind 1: 128-row tiles and
+    // packed split A operands)
+    mx::Program prog[2][2][mx::NSCHED];
+    int sched[2][2] = {{mx::SCHED_FUSED, mx::SCHED_FUSED}, {mx::SCHED_SPLIT, mx::SCHED_SPLIT}};
     bool adjoint = false;
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
@@ -150,7 +161,8 @@ struct mx_plan {
 
     ~mx_plan() {
         for (auto &pd : prog)
-            for (auto &pk : pd) pk.release();
+            for (auto &pk : pd)
+                for (auto &ps : pk) ps.release();
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
                         d_wrows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
@@ -177,12 +189,12 @@ struct TaskLists {
 
 static void compute_launch_flops(TaskLists &L) {
     L.flops.assign(L.launches.size(), 0.0);
+    std::vector<int> seen(L.tasks.size(), -1);
     for (size_t li = 0; li < L.launches.size(); ++li) {
-        int prev = -1;
         for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
             int task = L.tiles[t].x;
-            if (task == prev) continue;   // tiles of one task are consecutive
-            prev = task;
+            if (seen[task] == (int)li) continue;   // count each task once per launch
+            seen[task] = (int)li;
             const Task &tk = L.tasks[task];
             for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) L.flops[li] += 2.0 * tk.m * (double)L.segs[sg].k;
         }
@@ -284,6 +296,34 @@ static int add_tiles(TaskLists &L, int task, int m) {
     return n;
 }
 
+// Longest-first order of the tiles of every launch, dealt over the 8 XCDs.
+// xcd_remap hands XCD x the x-th contiguous eighth of a launch's work items,
+// so the sorted tiles are dealt round-robin into eight groups laid out one
+// after the other: every XCD gets a similar mix and starts with its longest
+// tiles, which keeps the tail of a launch short when task lengths vary (the
+// adjoint's column-grouped near and coupling lists).
+static void order_tiles(TaskLists &L, int bk) {
+    for (const Launch &ln : L.launches) {
+        std::vector<std::pair<int64_t, int2>> v;
+        v.reserve(ln.ntiles);
+        for (int i = 0; i < ln.ntiles; ++i) {
+            int2 tl = L.tiles[ln.tile_offset + i];
+            const Task &tk = L.tasks[tl.x];
+            int64_t cost = 2;   // prologue and epilogue
+            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
+            v.push_back({cost, tl});
+        }
+        std::stable_sort(v.begin(), v.end(),
+                         [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
+                             return a.first > b.first;
+                         });
+        const int ng = 8;
+        int o = ln.tile_offset;
+        for (int g = 0; g < ng; ++g)
+            for (int i = g; i < ln.ntiles; i += ng) L.tiles[o++] = v[i].second;
+    }
+}
+
 // Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
 // in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
 // beta_u / beta_y select the accumulation rule of the coupling and leaf
@@ -292,7 +332,7 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // the buffers the kernel reads (reference layout for kind 0, split panels for
 // kind 1); Y always uses the reference layout.
 static TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir,
-                             const int64_t *wrow_in, const int64_t *skel_in) {
+                             int sched, const int64_t *wrow_in, const int64_t *skel_in) {
     TaskLists L;
     const int nn = s.numnodes;
     // far pair k couples row node fx(k) with column node fy(k) through B_k
@@ -369,33 +409,64 @@ static TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_
         }
         end_launch();
     }
-    // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
-    for (int d = 1; d <= maxd; ++d) {
-        begin_launch();
-        for (int c : t.levels[d]) {
-            int sc = s.slen[c];
-            if (sc == 0) continue;
-            int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
-            for (int64_t k : far_of[c]) {
-                if (dir == DIR_FWD) {
-                    int y = fy(k);
-                    add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
-                } else {   // B_k^T, B_k stored s_x x s_y
-                    int x = fx(k);
-                    add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 1, s.slen[x], s.slen[x], VEC_WSKEL, skel_in[x],
-                            s.slen[x]);
-                }
+    auto add_far = [&](int tk, int c) {
+        const int sc = s.slen[c];
+        for (int64_t k : far_of[c]) {
+            if (dir == DIR_FWD) {
+                int y = fy(k);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
+            } else {   // B_k^T, B_k stored s_x x s_y
+                int x = fx(k);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 1, s.slen[x], s.slen[x], VEC_WSKEL, skel_in[x], s.slen[x]);
             }
-            int p = t.parent[c];
-            if (p > 0 && s.slen[p] > 0) {
-                int sp = s.slen[p];
-                int64_t off = s.VTptr[p];
-                if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
-                add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
-            }
-            add_tiles(L, tk, sc);
         }
+    };
+    auto add_down = [&](int tk, int c) {   // VT_p[:, part(c)]^T u_p; false when there is none
+        int p = t.parent[c];
+        if (p <= 0 || s.slen[p] == 0) return false;
+        int sp = s.slen[p];
+        int64_t off = s.VTptr[p];
+        if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
+        add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
+        return true;
+    };
+    if (sched == SCHED_FUSED) {
+        // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
+        for (int d = 1; d <= maxd; ++d) {
+            begin_launch();
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
+                add_far(tk, c);
+                add_down(tk, c);
+                add_tiles(L, tk, sc);
+            }
+            end_launch();
+        }
+    } else {
+        // FAR for every level at once, then DOWN level by level (beta = 1)
+        begin_launch();
+        for (int d = 1; d <= maxd; ++d)
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
+                add_far(tk, c);
+                add_tiles(L, tk, sc);
+            }
         end_launch();
+        for (int d = 2; d <= maxd; ++d) {
+            begin_launch();
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, 1);
+                if (add_down(tk, c)) add_tiles(L, tk, sc);
+                else L.tasks.pop_back();
+            }
+            end_launch();
+        }
     }
     // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x
     begin_launch();
@@ -417,6 +488,7 @@ static TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
+    order_tiles(L, kind == 1 ? X3_BK : SG_BK);
     return L;
 }
 
@@ -590,15 +662,16 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     const int ndir = P->adjoint ? 2 : 1;
-    for (int dir = 0; dir < ndir; ++dir) {
-        TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, wrow64.data(), skel64.data());
-        compute_launch_flops(L);
-        Program &pg = P->prog[dir][0];
-        upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
-        pg.launches = L.launches;
-        pg.flops = L.flops;
-        pg.built = true;
-    }
+    for (int dir = 0; dir < ndir; ++dir)
+        for (int sc = 0; sc < NSCHED; ++sc) {
+            TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
+            compute_launch_flops(L);
+            Program &pg = P->prog[dir][0][sc];
+            upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
+            pg.launches = L.launches;
+            pg.flops = L.flops;
+            pg.built = true;
+        }
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     if (precisions & MX_PLAN_BF16X3) {
         // split panels: padded row offsets of W leaf blocks and skeleton blocks
@@ -624,8 +697,9 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         // shared by the forward and adjoint lists
         std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
         int64_t next[3] = {0, 0, 0};
-        for (int dir = 0; dir < ndir; ++dir) {
-            TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, P->wprow.data(), P->sprow.data());
+        for (int dsc = 0; dsc < ndir * NSCHED; ++dsc) {
+            const int dir = dsc / NSCHED, sc = dsc % NSCHED;
+            TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, sc, P->wprow.data(), P->sprow.data());
             for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
                 if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
                     throw MxError(MX_ERR_UNSUPPORTED, "a block has more than " + std::to_string(X3_MAXSEG) +
@@ -657,7 +731,7 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
                 }
             }
             compute_launch_flops(X);
-            Program &pg = P->prog[dir][1];
+            Program &pg = P->prog[dir][1][sc];
             upload_lists(X, &pg.tasks, &pg.segs, &pg.tiles);
             pg.launches = X.launches;
             pg.flops = X.flops;
@@ -803,8 +877,8 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
     if (dev != P->device) throw MxError(MX_ERR_STATE, "plan was created on another device");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
-    const Program &pg0 = P->prog[dir][0];
-    const Program &pg1 = P->prog[dir][1];
+    const Program &pg0 = P->prog[dir][0][P->sched[dir][0]];
+    const Program &pg1 = P->prog[dir][1][P->sched[dir][1]];
     const int64_t nel = P->N * (int64_t)nrhs;
     if (prec != MX_BF16X3 && dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
     void *wsk = dir == DIR_ADJ ? P->wskel_adj : P->wskel;
@@ -1011,6 +1085,94 @@ static void set_blocks(mx_plan *P, const double *D, const double *B, const doubl
     }
     if (P->precisions & MX_PLAN_BF16X3) pack_all(P, src, st);
     HIPCHK(hipGetLastError());
+}
+
+static void drop_graphs(mx_plan *P) {
+    for (auto &g : P->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
+    P->graphs.clear();
+}
+
+static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int stages) {
+    arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
+    arg_check(sched == SCHED_FUSED || sched == SCHED_SPLIT, "bad schedule");
+    arg_check(stages == 0 || stages == 2 || stages == 4, "BF16x3 stages must be 2 or 4 (0 keeps)");
+    if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+    const int kd = prec == MX_BF16X3 ? 1 : 0;
+    P->sched[dir][kd] = sched;
+    if (stages) P->x3_stages = stages;
+    drop_graphs(P);
+}
+
+// Time every candidate (schedule x BF16x3 pipeline depth) of each direction
+// built into the plan on scratch inputs and keep the fastest.
+static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st,
+                     float *best_ms) {
+    arg_check(nrhs >= 1 && reps >= 1, "bad nrhs or reps");
+    if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
+    const size_t elem = io == MX_IO_F64 ? sizeof(double) : sizeof(float);
+    const size_t bytes = (size_t)std::max<int64_t>(P->N, 1) * nrhs * elem;
+    void *W = nullptr, *Y = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int graph = P->use_graph, prof = P->profiling;
+    P->use_graph = 0;
+    P->profiling = 0;
+    auto cleanup = [&]() {
+        if (W) (void)hipFree(W);
+        if (Y) (void)hipFree(Y);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        P->use_graph = graph;
+        P->profiling = prof;
+    };
+    try {
+        HIPCHK(hipMalloc(&W, bytes));
+        HIPCHK(hipMalloc(&Y, bytes));
+        HIPCHK(hipMemsetAsync(W, 0, bytes, st));
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const int kd = prec == MX_BF16X3 ? 1 : 0;
+        const int ndir = P->adjoint ? 2 : 1;
+        const int stage_opts[2] = {2, 4};
+        const int nst = prec == MX_BF16X3 ? 2 : 1;
+        int best_stages = P->x3_stages;
+        float best_total = 0.f;
+        for (int si = 0; si < nst; ++si) {
+            if (prec == MX_BF16X3) P->x3_stages = stage_opts[si];
+            float total = 0.f;
+            int pick[2] = {P->sched[0][kd], P->sched[1][kd]};
+            for (int dir = 0; dir < ndir; ++dir) {
+                float best = -1.f;
+                for (int sc = 0; sc < NSCHED; ++sc) {
+                    P->sched[dir][kd] = sc;
+                    eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up (sizes work buffers)
+                    HIPCHK(hipEventRecord(e0, st));
+                    for (int r = 0; r < reps; ++r) eval(P, dir, prec, W, Y, nrhs, io, st);
+                    HIPCHK(hipEventRecord(e1, st));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    ms /= reps;
+                    if (best < 0.f || ms < best) {
+                        best = ms;
+                        pick[dir] = sc;
+                    }
+                }
+                P->sched[dir][kd] = pick[dir];
+                total += best;
+                if (best_ms) best_ms[dir] = best;
+            }
+            if (si == 0 || total < best_total) {
+                best_total = total;
+                best_stages = P->x3_stages;
+            }
+        }
+        if (prec == MX_BF16X3) P->x3_stages = best_stages;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    drop_graphs(P);
 }
 
 // Evaluation through a captured hipGraph: the launch sequence of one
@@ -1467,10 +1629,37 @@ mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable) {
     return (mx_status)guard([&]() {
         arg_check(plan != nullptr, "null plan");
         plan->use_graph = enable ? 1 : 0;
-        if (!enable) {
-            for (auto &g : plan->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
-            plan->graphs.clear();
-        }
+        if (!enable) drop_graphs(plan);
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
+                               int32_t x3_stages) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        set_schedule(plan, direction, prec, schedule, x3_stages);
+        return 0;
+    });
+}
+
+mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
+                               int32_t *x3_stages) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && schedule && x3_stages && (direction == 0 || direction == 1), "bad argument");
+        *schedule = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
+        *x3_stages = plan->x3_stages;
+        return 0;
+    });
+}
+
+mx_status mx_plan_autotune(mx_plan *plan, mx_precision prec, int32_t nrhs, mx_io_type io, int32_t reps,
+                           float *best_ms, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
+        if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 I/O");
+        autotune(plan, prec, nrhs, io, reps, (hipStream_t)stream, best_ms);
         return 0;
     });
 }
@@ -1503,11 +1692,12 @@ mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, do
         arg_check(plan && flops && total && nrhs >= 0, "bad argument");
         for (int i = 0; i < 4; ++i) flops[i] = 0.0;
         if (prec == MX_BF16X3) {
-            if (!plan->prog[DIR_FWD][1].built) throw MxError(MX_ERR_UNSUPPORTED, "no BF16x3 lists in the plan");
-            for (double f : plan->prog[DIR_FWD][1].flops) flops[MX_KIND_BF16X3] += f * nrhs;
+            const Program &pg = plan->prog[DIR_FWD][1][plan->sched[DIR_FWD][1]];
+            if (!pg.built) throw MxError(MX_ERR_UNSUPPORTED, "no BF16x3 lists in the plan");
+            for (double f : pg.flops) flops[MX_KIND_BF16X3] += f * nrhs;
         } else {
             int k = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
-            for (double f : plan->prog[DIR_FWD][0].flops) flops[k] += f * nrhs;
+            for (double f : plan->prog[DIR_FWD][0][plan->sched[DIR_FWD][0]].flops) flops[k] += f * nrhs;
         }
         *total = flops[0] + flops[1] + flops[2] + flops[3];
         return 0;
@@ -1518,7 +1708,8 @@ mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t n
                                int32_t *count) {
     return (mx_status)guard([&]() {
         arg_check(plan && count && cap >= 0 && (cap == 0 || flops), "bad argument");
-        const std::vector<double> &f = plan->prog[DIR_FWD][prec == MX_BF16X3 ? 1 : 0].flops;
+        const int kd = prec == MX_BF16X3 ? 1 : 0;
+        const std::vector<double> &f = plan->prog[DIR_FWD][kd][plan->sched[DIR_FWD][kd]].flops;
         *count = (int32_t)f.size();
         for (int32_t i = 0; i < (int32_t)f.size() && i < cap; ++i) flops[i] = f[i] * nrhs;
         return 0;

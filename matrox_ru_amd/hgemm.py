@@ -17,6 +17,8 @@ from . import _native
 FP64, FP32, BF16X3 = 0, 1, 2
 KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32"}
 PLAN_ADJOINT = 1 << 8
+SCHEDULES = {"fused": 0, "split": 1}
+DIRECTIONS = {"forward": 0, "adjoint": 1}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -69,6 +71,12 @@ def lib():
         L.mx_plan_set_blocks.restype = ctypes.c_int
         L.mx_plan_grad_flops.argtypes = [P, I32, ctypes.POINTER(ctypes.c_double)]
         L.mx_plan_grad_flops.restype = ctypes.c_int
+        L.mx_plan_set_schedule.argtypes = [P, I32, ctypes.c_int, I32, I32]
+        L.mx_plan_set_schedule.restype = ctypes.c_int
+        L.mx_plan_get_schedule.argtypes = [P, I32, ctypes.c_int, ctypes.POINTER(I32), ctypes.POINTER(I32)]
+        L.mx_plan_get_schedule.restype = ctypes.c_int
+        L.mx_plan_autotune.argtypes = [P, ctypes.c_int, I32, ctypes.c_int, I32, P, P]
+        L.mx_plan_autotune.restype = ctypes.c_int
         L.mx_plan_skeletons.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
         L.mx_plan_skeletons.restype = ctypes.c_int
         L.mx_plan_copy_skeletons.argtypes = [P, P, P, I32, P]
@@ -216,6 +224,26 @@ class Plan:
             assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n
             ps.append(t.data_ptr())
         _check(lib().mx_plan_set_blocks(self._h, ps[0], ps[1], ps[2], _stream_handle(stream)))
+
+    def set_schedule(self, direction, precision, schedule, x3_stages=0):
+        """Pick the coupling/downward schedule ("fused" or "split") of one
+        direction ("forward"/"adjoint") for a precision (mx_plan_set_schedule)."""
+        _check(lib().mx_plan_set_schedule(self._h, DIRECTIONS[direction], PRECISIONS[precision],
+                                          SCHEDULES[schedule], x3_stages))
+
+    def get_schedule(self, direction, precision):
+        sc, st = I32(), I32()
+        _check(lib().mx_plan_get_schedule(self._h, DIRECTIONS[direction], PRECISIONS[precision], ctypes.byref(sc),
+                                          ctypes.byref(st)))
+        names = {v: k for k, v in SCHEDULES.items()}
+        return names[sc.value], st.value
+
+    def autotune(self, precision, nrhs, io="f64", reps=3, stream=None):
+        """mx_plan_autotune; returns the chosen per-direction times in ms."""
+        ms = (ctypes.c_float * 2)()
+        _check(lib().mx_plan_autotune(self._h, PRECISIONS[precision], nrhs, IO_F64 if io == "f64" else IO_F32, reps,
+                                      ms, _stream_handle(stream)))
+        return [ms[0], ms[1]] if self.adjoint else [ms[0]]
 
     def grad_flops(self, nrhs):
         f = ctypes.c_double()

@@ -1,0 +1,72 @@
+"""Both coupling/downward schedules (fused per level, split: all couplings
+in one launch then per-level transfers accumulated with beta = 1), both
+directions and every precision, against the oracle; and the autotuner."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import make_problem, rel
+from matrox_ru_amd import hgemm, host
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "h2b_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.2, h=1.0),
+    "hss_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.0, h=1.0),
+    "h2b_ragged": dict(n=3001, d=5, leaf=100, rank=70, tol=1e-7, budget=0.3, h=1.2),
+}
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("schedule", ["fused", "split"])
+def test_schedules_match_oracle(gpu, case, schedule):
+    import torch
+    _say(f"[schedule] case={case} schedule={schedule}")
+    X, hm = make_problem(**CASES[case])
+    nrhs = 45
+    TW = host.normal((hm.N * nrhs,), seed=17)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    for prec, stages in (("fp64", 0), ("fp32", 0), ("bf16x3", 2), ("bf16x3", 4)):
+        for direction in ("forward", "adjoint"):
+            plan.set_schedule(direction, prec, schedule, stages)
+            assert plan.get_schedule(direction, prec)[0] == schedule
+        Y = torch.full_like(W, float("nan"))
+        G = torch.full_like(W, float("nan"))
+        plan.eval(W, Y, nrhs, prec)
+        plan.eval_adjoint(W, G, nrhs, prec)
+        tol = 1e-12 if prec == "fp64" else 1e-5
+        ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
+        _say(f"  {prec} stages={stages}: forward {ey:.2e} adjoint {eg:.2e}")
+        assert ey <= tol and eg <= tol, (prec, stages)
+
+
+def test_autotune_keeps_results(gpu):
+    import torch
+    _say("[autotune]")
+    X, hm = make_problem(**CASES["h2b_small"])
+    nrhs = 64
+    TW = host.normal((hm.N * nrhs,), seed=18)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    for prec in ("fp64", "bf16x3"):
+        ms = plan.autotune(prec, nrhs, "f64", reps=2)
+        assert len(ms) == 2 and all(m > 0 for m in ms)
+        for d in ("forward", "adjoint"):
+            sc, st = plan.get_schedule(d, prec)
+            assert sc in ("fused", "split") and st in (2, 4)
+        Y = torch.empty_like(W)
+        plan.eval(W, Y, nrhs, prec)
+        assert rel(Y.cpu().numpy(), Yo) <= (1e-12 if prec == "fp64" else 1e-5)
+    with pytest.raises(hgemm.MxError):
+        plan.autotune("fp32", nrhs)
