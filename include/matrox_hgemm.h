@@ -182,16 +182,22 @@ mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable);
  * u_c = sum_far B w + VT_p^T u_p; MX_SCHED_SPLIT runs the couplings of all
  * levels in one launch, then the parent transfers level by level.  Both give
  * the same result up to summation order.  Defaults: fused forward, split
- * adjoint.  x3_stages: BF16x3 pipeline depth, 2 or 4 (0 keeps the current). */
+ * adjoint.  x3_variant: BF16x3 kernel tile and pipeline depth (one per plan,
+ * 0 keeps the current): */
 enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1 };
 enum { MX_DIR_FORWARD = 0, MX_DIR_ADJOINT = 1 };
+enum {
+    MX_X3_128x128_S2 = 1, /* 4 waves, 2 stages, two workgroups per CU (default) */
+    MX_X3_128x128_S4 = 2, /* 4 waves, 4 stages */
+    MX_X3_128x256_S3 = 3  /* 8 waves, 3 stages */
+};
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
-                               int32_t x3_stages);
+                               int32_t x3_variant);
 mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
-                               int32_t *x3_stages);
+                               int32_t *x3_variant);
 
-/* Autotuner: times every schedule (and, for MX_BF16X3, both pipeline
- * depths) of each direction built into the plan with `reps` evaluations of
+/* Autotuner: times every schedule (and, for MX_BF16X3, every kernel
+ * variant) of each direction built into the plan with `reps` evaluations of
  * nrhs columns on scratch buffers it allocates and frees, and keeps the
  * fastest.  best_ms (optional, 2 entries) receives the chosen per-direction
  * times.  Synchronises `stream`; drops captured graphs. */

@@ -21,7 +21,7 @@
 //   evaluation; wskel/uskel panels are written, split, by the epilogue of the
 //   launch that produces them.
 //
-// Workgroup: 256 threads = 4 waves as 2 x 2, tile 128 x 128, each wave
+// Workgroup: 2 x 2 or 2 x 4 waves, tile 128 x 128 or 128 x 256, each wave
 // 64 x 64 = 4 x 4 fragments (48 MFMA per 32-deep chunk); fragments move
 // global -> LDS by direct LDS-DMA loads ahead of the MFMAs (see the kernel).
 // MFMA 16x16x32 bf16 operand layout (lane l): A(row l%16, k 8*(l/16)+e),
@@ -67,10 +67,8 @@ struct X3Args {
     void *Y;                  // output in the reference layout, FP32 or FP64
 };
 
-constexpr int X3_BM = 128, X3_BN = 128, X3_BK = 32;
+constexpr int X3_BM = 128, X3_BK = 32;   // tile rows of every variant, chunk depth
 constexpr int X3_MAXSEG = 256;      // segments per task the kernel holds in LDS
-constexpr int X3_CP = X3_BM + 1;    // padded column stride of the output staging
-constexpr int X3_THREADS = 256;
 
 __host__ __device__ inline int64_t x3_packed_elems(int M, int K) {
     return (int64_t)((M + 15) / 16) * ((K + 31) / 32) * 512;
@@ -95,7 +93,13 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
 void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_blocks, int nrhs, uint16_t *hi,
                        uint16_t *lo, hipStream_t st);
 
-// stages: 2 or 4 pipeline stages (see the kernel)
-void launch_x3(const X3Args &a, int stages, hipStream_t st);
+// Kernel variants (tile rows x columns, pipeline stages):
+//   128 x 128, 2 stages: 4 waves, 64 KB of stages, two workgroups per CU;
+//   128 x 128, 4 stages: 4 waves, 128 KB, one workgroup per CU;
+//   128 x 256, 3 stages: 8 waves, 144 KB, one workgroup per CU.
+enum X3Variant { X3_V128x128_S2 = 1, X3_V128x128_S4 = 2, X3_V128x256_S3 = 3 };
+constexpr int X3_NVARIANT = 3;
+int x3_variant_bn(int variant);   // output columns of a tile
+void launch_x3(const X3Args &a, int variant, hipStream_t st);
 
 }  // namespace mx

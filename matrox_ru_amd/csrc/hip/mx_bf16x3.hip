@@ -65,32 +65,29 @@ __device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Main loop: a pipeline of NST stages through LDS.  Per 32-deep chunk the
-// workgroup needs 32 fragments of 1 KB (8 A row groups and 8 B column groups,
-// hi and lo); each wave moves 8 of them with global_load_lds_dwordx4, the
-// gfx950 direct global->LDS load (one instruction per fragment, lane l
-// writing bytes 16l..16l+15, no VGPRs).  Loads run NST-1 chunks ahead of the
-// MFMAs.  Waves 0-1 load A row groups 0-3 / 4-7, waves 2-3 B column groups
-// 0-3 / 4-7.  Row groups past the block and columns past nrhs load a valid
-// neighbour (clamped) and are masked or discarded.  With NST = 2 (64 KB of
-// stages) two workgroups share a CU, so one's LDS reads overlap the other's
-// MFMAs; this measured faster than NST = 4 with one workgroup per CU.
+// Main loop: a pipeline of NST stages through LDS.  A workgroup of WM x WN
+// waves computes a (64 WM) x (64 WN) tile, each wave 64 x 64.  Per 32-deep
+// chunk it needs 8 (WM + WN) fragments of 1 KB (4 WM A row groups and 4 WN B
+// column groups, hi and lo), dealt evenly over the waves; each moves its
+// share with global_load_lds_dwordx4, the gfx950 direct global->LDS load (one
+// instruction per fragment, lane l writing bytes 16l..16l+15, no VGPRs).
+// Loads run NST-1 chunks ahead of the MFMAs.  Row groups past the block and
+// columns past nrhs load a valid neighbour (clamped) and are masked or
+// discarded.  A stage holds A fragments [row group][hi, lo] followed by B
+// fragments [column group][hi, lo], so each wave's 8 A and 8 B fragments are
+// two contiguous 8 KB runs.
 constexpr int X3_FRAG = 512;              // uint16 per fragment (1 KB)
-constexpr int X3_STAGE = 32 * X3_FRAG;    // uint16 per stage (32 KB)
 
-struct X3Cursor {
-    int seg, c;
+template <int WM, int WN>
+struct X3Shape {
+    static constexpr int NW = WM * WN;                 // waves
+    static constexpr int THREADS = 64 * NW;
+    static constexpr int BM = 64 * WM, BN = 64 * WN;
+    static constexpr int FA = 8 * WM, FB = 8 * WN;     // fragments per chunk
+    static constexpr int PER = (FA + FB) / NW;         // DMA loads per wave per chunk
+    static constexpr int STAGE = (FA + FB) * X3_FRAG;  // uint16 per stage
+    static_assert(PER * NW == FA + FB, "fragments must divide evenly over the waves");
 };
-
-// next 32-deep chunk of the task's segment sequence (empty segments skipped)
-__device__ __forceinline__ bool x3_next(const Seg *segs, int seg_end, X3Cursor &cu) {
-    ++cu.c;
-    while (cu.seg < seg_end && cu.c * 32 >= segs[cu.seg].k) {
-        ++cu.seg;
-        cu.c = 0;
-    }
-    return cu.seg < seg_end;
-}
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef const __attribute__((address_space(1))) void glb_void_t;
@@ -99,39 +96,70 @@ __device__ __forceinline__ void lds_dma16(const uint16_t *src, uint16_t *dst_wav
     __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 16, 0, 0);
 }
 
-__device__ __forceinline__ void x3_issue(const X3Args &a, const Seg &sg, int c, uint16_t *stage, int wv, int lane,
-                                         int mtile16, int nfm, int n0) {
-    if (wv < 2) {
-        const int nfk = (sg.k + 31) / 32;
-        const uint16_t *Ah = a.ahi[sg.a_buf] + sg.a_off + lane * 8;
-        const uint16_t *Al = a.alo[sg.a_buf] + sg.a_off + lane * 8;
+// Per-wave DMA source pointers.  Fragment t of the wave (f = wv*PER + t)
+// is an A fragment (f < FA: row group f/2, hi or lo) or a B fragment; its
+// source advances by a fixed stride per chunk (one packed fragment for A, 32
+// panel rows for B), so the per-chunk work is PER pointer increments and PER
+// DMA instructions.  The segment descriptor is read (from LDS) once per
+// segment, when its chunks start.
+template <int WM, int WN>
+struct X3Loader {
+    const uint16_t *p[X3Shape<WM, WN>::PER];
+    int seg, left;
+};
+
+template <int WM, int WN>
+__device__ __forceinline__ void x3_seg_begin(X3Loader<WM, WN> &L, const X3Args &a, const Seg &sg, int wv, int lane,
+                                             int mtile16, int nfm, int n0) {
+    typedef X3Shape<WM, WN> S;
+    const int nfk = (sg.k + 31) / 32;
+    const int64_t pb = sg.b_row * (int64_t)a.nrhs + 8 * (lane >> 4);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int rg = wv * 4 + t;
-            const int mi = min(mtile16 + rg, nfm - 1);
-            const int64_t off = ((int64_t)mi * nfk + c) * X3_FRAG;
-            lds_dma16(Ah + off, stage + (rg * 2 + 0) * X3_FRAG);
-            lds_dma16(Al + off, stage + (rg * 2 + 1) * X3_FRAG);
-        }
-    } else {
-        const int64_t pb = sg.b_row * (int64_t)a.nrhs + (int64_t)c * a.nrhs * 32 + 8 * (lane >> 4);
-        const uint16_t *Bh = a.phi[sg.b_buf] + pb;
-        const uint16_t *Bl = a.plo[sg.b_buf] + pb;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int cg = (wv - 2) * 4 + t;
-            const int col = min(n0 + cg * 16 + (lane & 15), a.nrhs - 1);
-            lds_dma16(Bh + (int64_t)col * 32, stage + (16 + cg * 2 + 0) * X3_FRAG);
-            lds_dma16(Bl + (int64_t)col * 32, stage + (16 + cg * 2 + 1) * X3_FRAG);
+    for (int t = 0; t < S::PER; ++t) {
+        const int f = wv * S::PER + t;   // wave-uniform
+        if (f < S::FA) {
+            const int mi = min(mtile16 + (f >> 1), nfm - 1);
+            L.p[t] = ((f & 1) ? a.alo[sg.a_buf] : a.ahi[sg.a_buf]) + sg.a_off + lane * 8 +
+                     (int64_t)mi * nfk * X3_FRAG;
+        } else {
+            const int g = f - S::FA;
+            const int col = min(n0 + (g >> 1) * 16 + (lane & 15), a.nrhs - 1);
+            L.p[t] = ((g & 1) ? a.plo[sg.b_buf] : a.phi[sg.b_buf]) + pb + (int64_t)col * 32;
         }
     }
+    L.left = nfk;
 }
 
-// wait until at most n of this wave's LDS-DMA loads are outstanding (n in 0..16 step 8)
-__device__ __forceinline__ void wait_dma(int n) {
-    if (n >= 16) __builtin_amdgcn_s_waitcnt(0x4F70);
-    else if (n >= 8) __builtin_amdgcn_s_waitcnt(0xF78);
-    else __builtin_amdgcn_s_waitcnt(0xF70);
+// DMA of the next chunk into a stage (segments with k == 0 are skipped)
+template <int WM, int WN>
+__device__ __forceinline__ void x3_issue(X3Loader<WM, WN> &L, const X3Args &a, const Seg *sseg, uint16_t *stage,
+                                         int wv, int lane, int mtile16, int nfm, int n0) {
+    typedef X3Shape<WM, WN> S;
+    while (L.left == 0) {
+        ++L.seg;
+        x3_seg_begin(L, a, sseg[L.seg], wv, lane, mtile16, nfm, n0);
+    }
+    const int64_t bstep = (int64_t)a.nrhs * 32;
+#pragma unroll
+    for (int t = 0; t < S::PER; ++t) {
+        const int f = wv * S::PER + t;
+        lds_dma16(L.p[t], stage + f * X3_FRAG);
+        L.p[t] += f < S::FA ? (int64_t)X3_FRAG : bstep;
+    }
+    --L.left;
+}
+
+// s_waitcnt immediate for vmcnt(v) with expcnt and lgkmcnt left alone
+constexpr int vmcnt_enc(int v) { return 0xF70 | (v & 15) | ((v >> 4) << 14); }
+
+// wait until at most j * PER of this wave's LDS-DMA loads are outstanding
+template <int PER>
+__device__ __forceinline__ void wait_dma(int j) {
+    static_assert(3 * PER < 64, "vmcnt is 6 bits");
+    if (j >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_enc(3 * PER));
+    else if (j == 2) __builtin_amdgcn_s_waitcnt(vmcnt_enc(2 * PER));
+    else if (j == 1) __builtin_amdgcn_s_waitcnt(vmcnt_enc(PER));
+    else __builtin_amdgcn_s_waitcnt(vmcnt_enc(0));
 }
 
 // workgroup barrier without the memory fences of __syncthreads (which would
@@ -178,11 +206,11 @@ __device__ __forceinline__ uint32_t lds_offset(const uint16_t *p) {
 }
 
 // Fragments of the current chunk from one stage, 48 MFMAs.
-__device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *stage, int lane, int wr, int wc,
-                                           int mtile16, int nfm) {
+__device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *stage, int fa, int lane, int wr,
+                                           int wc, int mtile16, int nfm) {
     const uint32_t base = lds_offset(stage) + lane * 16;
     u32x4_t A[8], B[8];
-    lds_read_frags(base + wr * 8192, base + 16384 + wc * 8192, A, B);
+    lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * 8192, A, B);
     bf16x8_t ah[4], al[4], bh[4], bl[4];
     const u32x4_t z = u32x4_t{0, 0, 0, 0};
 #pragma unroll
@@ -203,11 +231,13 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *s
         }
 }
 
-// NST-stage pipeline (NST = 2: 64 KB of stages, two workgroups per CU; NST
-// = 4: 128 KB, one workgroup per CU and three chunks of prefetch).
-template <int NST>
-__global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
-    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * X3_STAGE];
+// The pipeline depth NST and the wave grid are template parameters; the
+// autotuner picks among the instantiated variants (launch_x3).
+template <int WM, int WN, int NST>
+__global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
+    typedef X3Shape<WM, WN> S;
+    static_assert(S::NW * 64 * 64 * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
     // the task's segment list, read from LDS inside the pipeline (a vector
     // load there would wait for every outstanding LDS-DMA load)
     __shared__ Seg sseg[X3_MAXSEG];
@@ -219,14 +249,14 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
     const int2 tl = a.tiles[tile_id];
     const Task task = a.tasks[tl.x];
     const int M = task.m;
-    const int m0 = tl.y * X3_BM, n0 = nt * X3_BN;
+    const int m0 = tl.y * S::BM, n0 = nt * S::BN;
     const int lane = tid & 63, wv = tid >> 6;
-    const int wr = wv >> 1, wc = wv & 1;
+    const int wr = wv / WN, wc = wv % WN;
     const int nfm = (M + 15) / 16;
     const int mtile16 = m0 / 16;
 
     const int nseg = task.seg_end - task.seg_begin;   // <= X3_MAXSEG (checked by the plan)
-    for (int i = tid; i < nseg; i += X3_THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    for (int i = tid; i < nseg; i += S::THREADS) sseg[i] = a.segs[task.seg_begin + i];
     __syncthreads();
     int nch = 0;   // 32-deep chunks of the task
     for (int si = 0; si < nseg; ++si) nch += (sseg[si].k + 31) / 32;
@@ -237,12 +267,13 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
 
-    X3Cursor ld{0, -1};
+    X3Loader<WM, WN> ld;
+    ld.seg = -1;
+    ld.left = 0;
     int issued = 0;
     auto issue = [&]() {
         if (issued < nch) {
-            x3_next(sseg, nseg, ld);
-            x3_issue(a, sseg[ld.seg], ld.c, stg + (issued % NST) * X3_STAGE, wv, lane, mtile16, nfm, n0);
+            x3_issue<WM, WN>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, n0);
             ++issued;
         }
     };
@@ -251,10 +282,10 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
     for (int c = 0; c < nch; ++c) {
         // chunk c has landed (this wave's part, then everyone's), and every
         // wave is done with the stage chunk c-1 used, which chunk c+NST-1 refills
-        wait_dma(8 * (issued - c - 1));
+        wait_dma<S::PER>(issued - c - 1);
         wg_barrier();
         issue();
-        x3_compute(acc, stg + (c % NST) * X3_STAGE, lane, wr, wc, mtile16, nfm);
+        x3_compute(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
     }
     __syncthreads();
 
@@ -329,11 +360,23 @@ __global__ __launch_bounds__(X3_THREADS) void x3_kernel(X3Args a) {
     }
 }
 
-void launch_x3(const X3Args &a, int stages, hipStream_t st) {
+int x3_variant_bn(int variant) { return variant == X3_V128x256_S3 ? 256 : 128; }
+
+void launch_x3(const X3Args &a, int variant, hipStream_t st) {
     int64_t total = (int64_t)a.ntiles * a.n_ntiles;
     if (total <= 0) return;
-    if (stages == 4) hipLaunchKernelGGL(x3_kernel<4>, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
-    else hipLaunchKernelGGL(x3_kernel<2>, dim3((unsigned)total), dim3(X3_THREADS), 0, st, a);
+    const dim3 grid((unsigned)total);
+    switch (variant) {
+    case X3_V128x128_S4:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 4>), grid, dim3(256), 0, st, a);
+        break;
+    case X3_V128x256_S3:
+        hipLaunchKernelGGL((x3_kernel<2, 4, 3>), grid, dim3(512), 0, st, a);
+        break;
+    default:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);
+        break;
+    }
 }
 
 }  // namespace mx

@@ -101,7 +101,7 @@ struct mx_plan {
     bool adjoint = false;
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
-    int x3_stages = 2;                  // pipeline depth of the BF16x3 kernel (MX_X3_STAGES=2|4)
+    int x3_variant = mx::X3_V128x128_S2;   // BF16x3 kernel variant (tile, pipeline depth)
     mx::X3Rows *d_wrows = nullptr;
     uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
     size_t panel_bytes = 0;
@@ -658,7 +658,10 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
     P->VT_len = s.VT_len;
     P->precisions = precisions & 7u;
     P->adjoint = (precisions & MX_PLAN_ADJOINT) != 0;
-    if (const char *e = getenv("MX_X3_STAGES")) P->x3_stages = atoi(e) == 4 ? 4 : 2;
+    if (const char *e = getenv("MX_X3_VARIANT")) {
+        int v = atoi(e);
+        if (v >= 1 && v <= X3_NVARIANT) P->x3_variant = v;
+    }
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     const int ndir = P->adjoint ? 2 : 1;
@@ -939,7 +942,8 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
         const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
         X3Args xa{};
         xa.nrhs = nrhs;
-        xa.n_ntiles = (nrhs + X3_BN - 1) / X3_BN;
+        const int bn = x3_variant_bn(P->x3_variant);
+        xa.n_ntiles = (nrhs + bn - 1) / bn;
         xa.y_f64 = io == MX_IO_F64;
         xa.Y = Y;
         xa.tasks = pg1.tasks;
@@ -962,7 +966,7 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
             xa.tiles = pg1.tiles + pg1.launches[i].tile_offset;
             xa.ntiles = pg1.launches[i].ntiles;
             prof_mark(P, st, MX_KIND_BF16X3, true);
-            launch_x3(xa, P->x3_stages, st);
+            launch_x3(xa, P->x3_variant, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
     }
@@ -1087,23 +1091,38 @@ static void set_blocks(mx_plan *P, const double *D, const double *B, const doubl
     HIPCHK(hipGetLastError());
 }
 
+// Unit-variance pseudo-random fill (splitmix64 of the index, uniform on
+// [-sqrt(3), sqrt(3))): timing inputs with realistic bit patterns (MFMA
+// power, and so clocks, depend on the data).
+__global__ void fill_uniform(void *p, int64_t n, int f64) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const double u = ((double)(z >> 11) * 0x1.0p-53 - 0.5) * 3.4641016151377544;
+        if (f64) static_cast<double *>(p)[i] = u;
+        else static_cast<float *>(p)[i] = (float)u;
+    }
+}
+
 static void drop_graphs(mx_plan *P) {
     for (auto &g : P->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
     P->graphs.clear();
 }
 
-static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int stages) {
+static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int variant) {
     arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
     arg_check(sched == SCHED_FUSED || sched == SCHED_SPLIT, "bad schedule");
-    arg_check(stages == 0 || stages == 2 || stages == 4, "BF16x3 stages must be 2 or 4 (0 keeps)");
+    arg_check(variant >= 0 && variant <= X3_NVARIANT, "bad BF16x3 kernel variant (0 keeps)");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     const int kd = prec == MX_BF16X3 ? 1 : 0;
     P->sched[dir][kd] = sched;
-    if (stages) P->x3_stages = stages;
+    if (variant) P->x3_variant = variant;
     drop_graphs(P);
 }
 
-// Time every candidate (schedule x BF16x3 pipeline depth) of each direction
+// Time every candidate (schedule x BF16x3 kernel variant) of each direction
 // built into the plan on scratch inputs and keep the fastest.
 static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st,
                      float *best_ms) {
@@ -1127,17 +1146,20 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
     try {
         HIPCHK(hipMalloc(&W, bytes));
         HIPCHK(hipMalloc(&Y, bytes));
-        HIPCHK(hipMemsetAsync(W, 0, bytes, st));
+        const int64_t nel = std::max<int64_t>(P->N, 1) * nrhs;
+        fill_uniform<<<cvt_grid(nel), 256, 0, st>>>(W, nel, io == MX_IO_F64);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
         const int kd = prec == MX_BF16X3 ? 1 : 0;
         const int ndir = P->adjoint ? 2 : 1;
-        const int stage_opts[2] = {2, 4};
-        const int nst = prec == MX_BF16X3 ? 2 : 1;
-        int best_stages = P->x3_stages;
-        float best_total = 0.f;
-        for (int si = 0; si < nst; ++si) {
-            if (prec == MX_BF16X3) P->x3_stages = stage_opts[si];
+        const int nvar = prec == MX_BF16X3 ? X3_NVARIANT : 1;
+        int best_variant = P->x3_variant;
+        int best_pick[2] = {P->sched[0][kd], P->sched[1][kd]};
+        float best_total = 0.f, best_dir[2] = {0.f, 0.f};
+        for (int si = 0; si < nvar; ++si) {
+            if (prec == MX_BF16X3) P->x3_variant = si + 1;
+            float dir_ms[2] = {0.f, 0.f};
             float total = 0.f;
             int pick[2] = {P->sched[0][kd], P->sched[1][kd]};
             for (int dir = 0; dir < ndir; ++dir) {
@@ -1159,14 +1181,24 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
                 }
                 P->sched[dir][kd] = pick[dir];
                 total += best;
-                if (best_ms) best_ms[dir] = best;
+                dir_ms[dir] = best;
             }
             if (si == 0 || total < best_total) {
                 best_total = total;
-                best_stages = P->x3_stages;
+                best_variant = P->x3_variant;
+                best_pick[0] = pick[0];
+                best_pick[1] = pick[1];
+                best_dir[0] = dir_ms[0];
+                best_dir[1] = dir_ms[1];
             }
         }
-        if (prec == MX_BF16X3) P->x3_stages = best_stages;
+        if (prec == MX_BF16X3) P->x3_variant = best_variant;
+        P->sched[0][kd] = best_pick[0];
+        P->sched[1][kd] = best_pick[1];
+        if (best_ms) {
+            best_ms[0] = best_dir[0];
+            best_ms[1] = best_dir[1];
+        }
     } catch (...) {
         cleanup();
         throw;
@@ -1635,20 +1667,20 @@ mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable) {
 }
 
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
-                               int32_t x3_stages) {
+                               int32_t x3_variant) {
     return (mx_status)guard([&]() {
         arg_check(plan != nullptr, "null plan");
-        set_schedule(plan, direction, prec, schedule, x3_stages);
+        set_schedule(plan, direction, prec, schedule, x3_variant);
         return 0;
     });
 }
 
 mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
-                               int32_t *x3_stages) {
+                               int32_t *x3_variant) {
     return (mx_status)guard([&]() {
-        arg_check(plan && schedule && x3_stages && (direction == 0 || direction == 1), "bad argument");
+        arg_check(plan && schedule && x3_variant && (direction == 0 || direction == 1), "bad argument");
         *schedule = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
-        *x3_stages = plan->x3_stages;
+        *x3_variant = plan->x3_variant;
         return 0;
     });
 }

@@ -19,6 +19,7 @@ KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4
 PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
+X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -225,11 +226,12 @@ class Plan:
             ps.append(t.data_ptr())
         _check(lib().mx_plan_set_blocks(self._h, ps[0], ps[1], ps[2], _stream_handle(stream)))
 
-    def set_schedule(self, direction, precision, schedule, x3_stages=0):
+    def set_schedule(self, direction, precision, schedule, x3_variant=0):
         """Pick the coupling/downward schedule ("fused" or "split") of one
-        direction ("forward"/"adjoint") for a precision (mx_plan_set_schedule)."""
+        direction ("forward"/"adjoint") for a precision, and optionally the
+        BF16x3 kernel variant (X3_VARIANTS; mx_plan_set_schedule)."""
         _check(lib().mx_plan_set_schedule(self._h, DIRECTIONS[direction], PRECISIONS[precision],
-                                          SCHEDULES[schedule], x3_stages))
+                                          SCHEDULES[schedule], x3_variant))
 
     def get_schedule(self, direction, precision):
         sc, st = I32(), I32()
