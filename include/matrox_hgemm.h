@@ -189,7 +189,8 @@ enum { MX_DIR_FORWARD = 0, MX_DIR_ADJOINT = 1 };
 enum {
     MX_X3_128x128_S2 = 1, /* 4 waves, 2 stages, two workgroups per CU (default) */
     MX_X3_128x128_S4 = 2, /* 4 waves, 4 stages */
-    MX_X3_128x256_S3 = 3  /* 8 waves, 3 stages */
+    MX_X3_128x256_S3 = 3, /* 8 waves, 3 stages */
+    MX_X3_128x128_REG = 4 /* 4 waves, fragments staged through VGPRs */
 };
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
                                int32_t x3_variant);
@@ -245,6 +246,41 @@ mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t n
 
 /* Number of rows N and of skeleton rows sum(slen) of a plan. */
 mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);
+
+/* ---- many independent instances ------------------------------------------ */
+
+/* A batch evaluates `count` independent compressed matrices, instance i with
+ * nrhs[i] columns (no counterpart in the reference, which evaluates one
+ * matrix per process).  Instances with equal nrhs form a group: their trees
+ * are joined under rank-0 nodes into one tree whose plan covers a tree level
+ * of every member in each launch; groups run concurrently on streams forked
+ * from the caller's.  A group's W and Y are single device buffers of
+ * rows x nrhs elements in which instance i occupies the leaf-blocked rows
+ * [row_offset, row_offset + N_i) (element offset row_offset * nrhs).
+ * precisions as for mx_plan_create (without MX_PLAN_ADJOINT). */
+typedef struct mx_batch mx_batch;
+mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances, const int32_t *nrhs, int32_t count,
+                          uint32_t precisions);
+mx_status mx_batch_destroy(mx_batch *batch);
+mx_status mx_batch_groups(const mx_batch *batch, int32_t *ngroups);
+mx_status mx_batch_group_info(const mx_batch *batch, int32_t group, int32_t *nrhs, int64_t *rows);
+mx_status mx_batch_layout(const mx_batch *batch, int32_t instance, int32_t *group, int64_t *row_offset);
+/* The plan of a group (owned by the batch): schedules, profiling, flops. */
+mx_status mx_batch_plan(mx_batch *batch, int32_t group, mx_plan **plan);
+/* Y[g] = K~_g W[g] for every group g (arrays of ngroups device pointers). */
+mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+                        void *stream);
+/* Capture the whole batch (all groups and their fork/join) as one hipGraph
+ * per (precision, io, W, Y) and replay it. */
+mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable);
+/* mx_plan_autotune on every group with its own nrhs. */
+mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream);
+mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total);
+/* Per-instance dense check rows, as mx_plan_set_points / mx_dense_rows; W is
+ * the instance's FP64 block inside its group buffer, out is nrows x nrhs. */
+mx_status mx_batch_set_points(mx_batch *batch, int32_t instance, const double *X_leaf, int32_t d, double h);
+mx_status mx_batch_dense_rows(mx_batch *batch, int32_t instance, const int32_t *rows, int32_t nrows, const double *W,
+                              double *out, void *stream);
 
 const char *mx_last_error(void);
 

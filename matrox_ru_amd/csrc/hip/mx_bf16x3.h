@@ -96,9 +96,10 @@ void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_b
 // Kernel variants (tile rows x columns, pipeline stages):
 //   128 x 128, 2 stages: 4 waves, 64 KB of stages, two workgroups per CU;
 //   128 x 128, 4 stages: 4 waves, 128 KB, one workgroup per CU;
-//   128 x 256, 3 stages: 8 waves, 144 KB, one workgroup per CU.
-enum X3Variant { X3_V128x128_S2 = 1, X3_V128x128_S4 = 2, X3_V128x256_S3 = 3 };
-constexpr int X3_NVARIANT = 3;
+//   128 x 256, 3 stages: 8 waves, 144 KB, one workgroup per CU;
+//   128 x 128, fragments staged through VGPRs instead of LDS-DMA.
+enum X3Variant { X3_V128x128_S2 = 1, X3_V128x128_S4 = 2, X3_V128x256_S3 = 3, X3_V128x128_REG = 4 };
+constexpr int X3_NVARIANT = 4;
 int x3_variant_bn(int variant);   // output columns of a tile
 void launch_x3(const X3Args &a, int variant, hipStream_t st);
 

@@ -78,16 +78,21 @@ __device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
 // two contiguous 8 KB runs.
 constexpr int X3_FRAG = 512;              // uint16 per fragment (1 KB)
 
-template <int WM, int WN>
-struct X3Shape {
-    static constexpr int NW = WM * WN;                 // waves
+// Tile geometry: BM x BN output tile, NW waves.
+template <int BM_, int BN_, int NW_>
+struct X3Geo {
+    static constexpr int NW = NW_;                     // waves
     static constexpr int THREADS = 64 * NW;
-    static constexpr int BM = 64 * WM, BN = 64 * WN;
-    static constexpr int FA = 8 * WM, FB = 8 * WN;     // fragments per chunk
+    static constexpr int BM = BM_, BN = BN_;
+    static constexpr int FA = BM / 8, FB = BN / 8;     // fragments per chunk (16-row/column groups x hi, lo)
     static constexpr int PER = (FA + FB) / NW;         // DMA loads per wave per chunk
     static constexpr int STAGE = (FA + FB) * X3_FRAG;  // uint16 per stage
     static_assert(PER * NW == FA + FB, "fragments must divide evenly over the waves");
 };
+
+// WM x WN waves of 64 x 64
+template <int WM, int WN>
+struct X3Shape : X3Geo<64 * WM, 64 * WN, WM * WN> {};
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef const __attribute__((address_space(1))) void glb_void_t;
@@ -102,16 +107,15 @@ __device__ __forceinline__ void lds_dma16(const uint16_t *src, uint16_t *dst_wav
 // panel rows for B), so the per-chunk work is PER pointer increments and PER
 // DMA instructions.  The segment descriptor is read (from LDS) once per
 // segment, when its chunks start.
-template <int WM, int WN>
+template <typename S>
 struct X3Loader {
-    const uint16_t *p[X3Shape<WM, WN>::PER];
+    const uint16_t *p[S::PER];
     int seg, left;
 };
 
-template <int WM, int WN>
-__device__ __forceinline__ void x3_seg_begin(X3Loader<WM, WN> &L, const X3Args &a, const Seg &sg, int wv, int lane,
+template <typename S>
+__device__ __forceinline__ void x3_seg_begin(X3Loader<S> &L, const X3Args &a, const Seg &sg, int wv, int lane,
                                              int mtile16, int nfm, int n0) {
-    typedef X3Shape<WM, WN> S;
     const int nfk = (sg.k + 31) / 32;
     const int64_t pb = sg.b_row * (int64_t)a.nrhs + 8 * (lane >> 4);
 #pragma unroll
@@ -131,10 +135,9 @@ __device__ __forceinline__ void x3_seg_begin(X3Loader<WM, WN> &L, const X3Args &
 }
 
 // DMA of the next chunk into a stage (segments with k == 0 are skipped)
-template <int WM, int WN>
-__device__ __forceinline__ void x3_issue(X3Loader<WM, WN> &L, const X3Args &a, const Seg *sseg, uint16_t *stage,
-                                         int wv, int lane, int mtile16, int nfm, int n0) {
-    typedef X3Shape<WM, WN> S;
+template <typename S>
+__device__ __forceinline__ void x3_issue(X3Loader<S> &L, const X3Args &a, const Seg *sseg, uint16_t *stage, int wv,
+                                         int lane, int mtile16, int nfm, int n0) {
     while (L.left == 0) {
         ++L.seg;
         x3_seg_begin(L, a, sseg[L.seg], wv, lane, mtile16, nfm, n0);
@@ -231,6 +234,196 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *s
         }
 }
 
+// Epilogue of a wave's (16 NI) x 64 accumulator block (NI row groups x 4
+// column groups of 16 x 16; the wave's rows start at m0 + wr * 16 NI, its
+// columns at n0 + wc * 64).
+//  * Y: each 64 x 64 part is staged in the wave's 16 KB of LDS (rows
+//    XOR-swizzled by column so the 16 lanes of a write hit distinct banks)
+//    and stored as 64 consecutive rows per column (beta = 1 adds to Y).
+//  * Skeleton buffers: written as split panels straight from registers (rows
+//    up to the padded height; padding rows are 0); beta = 1 adds the panel's
+//    current value first.
+template <int NI>
+__device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][4], uint16_t *stg, const X3Args &a, const Task &task,
+                                            int lane, int wv, int wr, int wc, int m0, int n0) {
+    const int M = task.m;
+    if (task.c_buf == VEC_Y) {
+        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
+        const int64_t base = task.c_row * (int64_t)a.nrhs;
+#pragma unroll
+        for (int h = 0; h < NI / 4; ++h) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + r;
+                        Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[h * 4 + i][j][r];
+                    }
+            __builtin_amdgcn_wave_barrier();
+            const int rb = m0 + wr * 16 * NI + h * 64, cb = n0 + wc * 64;
+            for (int e = lane; e < 64 * 64; e += 64) {
+                const int col = e >> 6, row = e & 63;
+                const int gm = rb + row, gn = cb + col;
+                if (gm < M && gn < a.nrhs) {
+                    const float v = Cs[col * 64 + (row ^ ((col & 15) << 2))];
+                    const int64_t o = base + (int64_t)gn * M + gm;
+                    if (a.y_f64) {
+                        double *y = static_cast<double *>(a.Y) + o;
+                        *y = task.beta ? *y + (double)v : (double)v;
+                    } else {
+                        float *y = static_cast<float *>(a.Y) + o;
+                        *y = task.beta ? *y + v : v;
+                    }
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        const int Mp = (M + 31) / 32 * 32;
+        uint16_t *Ph = a.phi[task.c_buf] + task.c_row * (int64_t)a.nrhs;
+        uint16_t *Pl = a.plo[task.c_buf] + task.c_row * (int64_t)a.nrhs;
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const int r0 = m0 + wr * 16 * NI + i * 16 + 4 * (lane >> 4);
+            if (r0 >= Mp) continue;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+                if (col >= a.nrhs) continue;
+                const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
+                float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
+                if (task.beta) {
+                    const u32x2_t oh = *reinterpret_cast<const u32x2_t *>(Ph + o);
+                    const u32x2_t ol = *reinterpret_cast<const u32x2_t *>(Pl + o);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const uint32_t wh = oh[r >> 1], wl = ol[r >> 1];
+                        v[r] += bf16_f((uint16_t)(wh >> (16 * (r & 1)))) + bf16_f((uint16_t)(wl >> (16 * (r & 1))));
+                    }
+                }
+                uint16_t hv[4], lv[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    hv[r] = bf16_rne(v[r]);
+                    lv[r] = bf16_rne(v[r] - bf16_f(hv[r]));
+                }
+                *reinterpret_cast<u32x2_t *>(Ph + o) =
+                    u32x2_t{(uint32_t)hv[0] | ((uint32_t)hv[1] << 16), (uint32_t)hv[2] | ((uint32_t)hv[3] << 16)};
+                *reinterpret_cast<u32x2_t *>(Pl + o) =
+                    u32x2_t{(uint32_t)lv[0] | ((uint32_t)lv[1] << 16), (uint32_t)lv[2] | ((uint32_t)lv[3] << 16)};
+            }
+        }
+    }
+}
+
+// 16 A-or-B fragment pairs of one half: the 48 MFMAs of 4 row groups (A,
+// starting at row group i0) by 4 column groups (B).
+template <int NI>
+__device__ __forceinline__ void x3_mma4(x3_f4 (&acc)[NI][4], int i0, const u32x4_t (&A)[8], const u32x4_t (&B)[8],
+                                        int rg0, int nfm) {
+    bf16x8_t ah[4], al[4], bh[4], bl[4];
+    const u32x4_t z = u32x4_t{0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const bool ok = rg0 + i < nfm;   // row groups past the block multiply zeros
+        ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
+        al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
+        bh[i] = __builtin_bit_cast(bf16x8_t, B[2 * i]);
+        bl[i] = __builtin_bit_cast(bf16x8_t, B[2 * i + 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i0 + i][j] = mfma_bf16(al[i], bh[j], acc[i0 + i][j]);
+            acc[i0 + i][j] = mfma_bf16(ah[i], bl[j], acc[i0 + i][j]);
+            acc[i0 + i][j] = mfma_bf16(ah[i], bh[j], acc[i0 + i][j]);
+        }
+}
+
+// Register-staged variant of the 128 x 128 kernel: fragments travel
+// global -> VGPR (global_load_dwordx4, 16 B per lane per fragment) -> LDS
+// (ds_write_b128) instead of by LDS-DMA.  Two LDS stages; the loads of chunk
+// c+2 are issued after chunk c+1 is written to LDS, so they are in flight
+// during the MFMAs of chunk c+1.
+template <typename S>
+__device__ __forceinline__ void x3_load_regs(X3Loader<S> &L, const X3Args &a, const Seg *sseg, u32x4_t (&R)[S::PER],
+                                             int wv, int lane, int mtile16, int nfm, int n0) {
+    while (L.left == 0) {
+        ++L.seg;
+        x3_seg_begin(L, a, sseg[L.seg], wv, lane, mtile16, nfm, n0);
+    }
+    const int64_t bstep = (int64_t)a.nrhs * 32;
+#pragma unroll
+    for (int t = 0; t < S::PER; ++t) {
+        const int f = wv * S::PER + t;
+        R[t] = *reinterpret_cast<const u32x4_t *>(L.p[t]);
+        L.p[t] += f < S::FA ? (int64_t)X3_FRAG : bstep;
+    }
+    --L.left;
+}
+
+template <typename S>
+__device__ __forceinline__ void x3_store_regs(const u32x4_t (&R)[S::PER], uint16_t *stage, int wv, int lane) {
+#pragma unroll
+    for (int t = 0; t < S::PER; ++t)
+        *reinterpret_cast<u32x4_t *>(stage + (wv * S::PER + t) * X3_FRAG + lane * 8) = R[t];
+}
+
+__global__ __launch_bounds__(256) void x3r_kernel(X3Args a) {
+    typedef X3Shape<2, 2> S;
+    constexpr int NST = 2;
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
+    __shared__ Seg sseg[X3_MAXSEG];
+
+    const int tid = threadIdx.x;
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    const int M = task.m;
+    const int m0 = tl.y * S::BM, n0 = nt * S::BN;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int nfm = (M + 15) / 16;
+    const int mtile16 = m0 / 16;
+
+    const int nseg = task.seg_end - task.seg_begin;   // <= X3_MAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += S::THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+    int nch = 0;
+    for (int si = 0; si < nseg; ++si) nch += (sseg[si].k + 31) / 32;
+
+    x3_f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+
+    X3Loader<S> ld;
+    ld.seg = -1;
+    ld.left = 0;
+    u32x4_t R[S::PER];
+    if (nch > 0) {
+        x3_load_regs<S>(ld, a, sseg, R, wv, lane, mtile16, nfm, n0);
+        x3_store_regs<S>(R, stg, wv, lane);
+    }
+    if (nch > 1) x3_load_regs<S>(ld, a, sseg, R, wv, lane, mtile16, nfm, n0);
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        x3_compute(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+        if (c + 1 < nch) {
+            x3_store_regs<S>(R, stg + ((c + 1) % NST) * S::STAGE, wv, lane);
+            if (c + 2 < nch) x3_load_regs<S>(ld, a, sseg, R, wv, lane, mtile16, nfm, n0);
+        }
+        __syncthreads();
+    }
+    x3_epilogue<4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
+}
+
 // The pipeline depth NST and the wave grid are template parameters; the
 // autotuner picks among the instantiated variants (launch_x3).
 template <int WM, int WN, int NST>
@@ -267,13 +460,13 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
 
-    X3Loader<WM, WN> ld;
+    X3Loader<S> ld;
     ld.seg = -1;
     ld.left = 0;
     int issued = 0;
     auto issue = [&]() {
         if (issued < nch) {
-            x3_issue<WM, WN>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, n0);
+            x3_issue<S>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, n0);
             ++issued;
         }
     };
@@ -289,75 +482,7 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     }
     __syncthreads();
 
-    if (task.c_buf == VEC_Y) {
-        // each wave stages its 64 x 64 quarter (16 KB; rows XOR-swizzled by
-        // column so the 16 lanes of a write hit distinct banks) and stores 64
-        // consecutive rows per column
-        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + r;
-                    Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[i][j][r];
-                }
-        __builtin_amdgcn_wave_barrier();
-        const int64_t base = task.c_row * (int64_t)a.nrhs;
-        const int rb = m0 + wr * 64, cb = n0 + wc * 64;
-        for (int e = lane; e < 64 * 64; e += 64) {
-            const int col = e >> 6, row = e & 63;
-            const int gm = rb + row, gn = cb + col;
-            if (gm < M && gn < a.nrhs) {
-                const float v = Cs[col * 64 + (row ^ ((col & 15) << 2))];
-                const int64_t o = base + (int64_t)gn * M + gm;
-                if (a.y_f64) {
-                    double *y = static_cast<double *>(a.Y) + o;
-                    *y = task.beta ? *y + (double)v : (double)v;
-                } else {
-                    float *y = static_cast<float *>(a.Y) + o;
-                    *y = task.beta ? *y + v : v;
-                }
-            }
-        }
-    } else {
-        // split-panel output (rows up to the padded height; padding rows are 0)
-        const int Mp = (M + 31) / 32 * 32;
-        uint16_t *Ph = a.phi[task.c_buf] + task.c_row * (int64_t)a.nrhs;
-        uint16_t *Pl = a.plo[task.c_buf] + task.c_row * (int64_t)a.nrhs;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int r0 = m0 + wr * 64 + i * 16 + 4 * (lane >> 4);
-            if (r0 >= Mp) continue;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
-                if (col >= a.nrhs) continue;
-                const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
-                float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
-                if (task.beta) {   // accumulate onto the panel's current value
-                    const u32x2_t oh = *reinterpret_cast<const u32x2_t *>(Ph + o);
-                    const u32x2_t ol = *reinterpret_cast<const u32x2_t *>(Pl + o);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const uint32_t wh = oh[r >> 1], wl = ol[r >> 1];
-                        v[r] += bf16_f((uint16_t)(wh >> (16 * (r & 1)))) + bf16_f((uint16_t)(wl >> (16 * (r & 1))));
-                    }
-                }
-                uint16_t h[4], l[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    h[r] = bf16_rne(v[r]);
-                    l[r] = bf16_rne(v[r] - bf16_f(h[r]));
-                }
-                *reinterpret_cast<u32x2_t *>(Ph + o) =
-                    u32x2_t{(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
-                *reinterpret_cast<u32x2_t *>(Pl + o) =
-                    u32x2_t{(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
-            }
-        }
-    }
+    x3_epilogue<4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
 int x3_variant_bn(int variant) { return variant == X3_V128x256_S3 ? 256 : 128; }
@@ -372,6 +497,9 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V128x256_S3:
         hipLaunchKernelGGL((x3_kernel<2, 4, 3>), grid, dim3(512), 0, st, a);
+        break;
+    case X3_V128x128_REG:
+        hipLaunchKernelGGL(x3r_kernel, grid, dim3(256), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);

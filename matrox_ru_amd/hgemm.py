@@ -19,7 +19,7 @@ KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4
 PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
-X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3"}
+X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -78,6 +78,30 @@ def lib():
         L.mx_plan_get_schedule.restype = ctypes.c_int
         L.mx_plan_autotune.argtypes = [P, ctypes.c_int, I32, ctypes.c_int, I32, P, P]
         L.mx_plan_autotune.restype = ctypes.c_int
+        L.mx_batch_create.argtypes = [ctypes.POINTER(P), P, P, I32, ctypes.c_uint32]
+        L.mx_batch_create.restype = ctypes.c_int
+        L.mx_batch_destroy.argtypes = [P]
+        L.mx_batch_destroy.restype = ctypes.c_int
+        L.mx_batch_groups.argtypes = [P, ctypes.POINTER(I32)]
+        L.mx_batch_groups.restype = ctypes.c_int
+        L.mx_batch_group_info.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I64)]
+        L.mx_batch_group_info.restype = ctypes.c_int
+        L.mx_batch_layout.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I64)]
+        L.mx_batch_layout.restype = ctypes.c_int
+        L.mx_batch_plan.argtypes = [P, I32, ctypes.POINTER(P)]
+        L.mx_batch_plan.restype = ctypes.c_int
+        L.mx_batch_eval.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, P]
+        L.mx_batch_eval.restype = ctypes.c_int
+        L.mx_batch_set_graph.argtypes = [P, I32]
+        L.mx_batch_set_graph.restype = ctypes.c_int
+        L.mx_batch_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, I32, P]
+        L.mx_batch_autotune.restype = ctypes.c_int
+        L.mx_batch_flops.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        L.mx_batch_flops.restype = ctypes.c_int
+        L.mx_batch_set_points.argtypes = [P, I32, P, I32, ctypes.c_double]
+        L.mx_batch_set_points.restype = ctypes.c_int
+        L.mx_batch_dense_rows.argtypes = [P, I32, P, I32, P, P, P]
+        L.mx_batch_dense_rows.restype = ctypes.c_int
         L.mx_plan_skeletons.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P)]
         L.mx_plan_skeletons.restype = ctypes.c_int
         L.mx_plan_copy_skeletons.argtypes = [P, P, P, I32, P]
@@ -322,6 +346,137 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             lib().mx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _GroupPlan(Plan):
+    """A group plan owned by a Batch (not destroyed by Python)."""
+
+    def __init__(self, h, rows, nrhs):
+        self._h = h
+        self.rows = rows
+        self.nrhs = nrhs
+        self.adjoint = False
+
+    def close(self):
+        self._h = None
+
+
+class Batch:
+    """Many independent compressed matrices evaluated together (mx_batch).
+
+    Instances with equal nrhs share a group; a group's W and Y are one flat
+    CUDA tensor of rows_g * nrhs_g elements in which instance i occupies the
+    element range [off_i * nrhs_g, (off_i + N_i) * nrhs_g) in its own
+    leaf-blocked layout (``instance_view``)."""
+
+    def __init__(self, hms, nrhs, precisions=("fp32",)):
+        assert len(hms) == len(nrhs) and len(hms) >= 1
+        mask = 0
+        for p in precisions:
+            mask |= 1 << PRECISIONS[p]
+        structs, keep = [], []
+        for hm in hms:
+            s, k = structure_of(hm)
+            structs.append(s)
+            keep.append(k)
+        arr = (ctypes.POINTER(MxStructure) * len(hms))(*[ctypes.pointer(s) for s in structs])
+        nr = (I32 * len(hms))(*[int(n) for n in nrhs])
+        h = P()
+        _check(lib().mx_batch_create(ctypes.byref(h), ctypes.cast(arr, P), ctypes.cast(nr, P), len(hms), mask))
+        del keep, structs
+        self._h = h
+        self.precisions = tuple(precisions)
+        self.N = [int(hm.scalars["n"]) for hm in hms]
+        self.nrhs = [int(n) for n in nrhs]
+        g = I32()
+        _check(lib().mx_batch_groups(h, ctypes.byref(g)))
+        self.groups = []
+        for i in range(g.value):
+            nr_, rows = I32(), I64()
+            _check(lib().mx_batch_group_info(h, i, ctypes.byref(nr_), ctypes.byref(rows)))
+            self.groups.append((nr_.value, rows.value))
+        self.layout = []
+        for i in range(len(hms)):
+            gi, off = I32(), I64()
+            _check(lib().mx_batch_layout(h, i, ctypes.byref(gi), ctypes.byref(off)))
+            self.layout.append((gi.value, off.value))
+
+    def buffers(self, dtype, fill=None):
+        """One CUDA tensor per group (rows_g * nrhs_g elements)."""
+        import torch
+        out = []
+        for nr, rows in self.groups:
+            t = torch.empty(rows * nr, dtype=dtype, device="cuda")
+            if fill is not None:
+                t.fill_(fill)
+            out.append(t)
+        return out
+
+    def instance_view(self, bufs, i):
+        """Instance i's leaf-blocked N_i * nrhs_i slice of the group buffers."""
+        g, off = self.layout[i]
+        nr = self.groups[g][0]
+        return bufs[g][off * nr:(off + self.N[i]) * nr]
+
+    def eval(self, W, Y, precision="fp32", stream=None):
+        import torch
+        assert len(W) == len(self.groups) and len(Y) == len(self.groups)
+        dt = W[0].dtype
+        for w, y, (nr, rows) in zip(W, Y, self.groups):
+            assert w.is_cuda and y.is_cuda and w.dtype == dt and y.dtype == dt
+            assert w.numel() == rows * nr and y.numel() == rows * nr and w.is_contiguous() and y.is_contiguous()
+        io = IO_F64 if dt == torch.float64 else IO_F32
+        wp = (P * len(W))(*[w.data_ptr() for w in W])
+        yp = (P * len(Y))(*[y.data_ptr() for y in Y])
+        _check(lib().mx_batch_eval(self._h, PRECISIONS[precision], ctypes.cast(wp, P), ctypes.cast(yp, P), io,
+                                   _stream_handle(stream)))
+        return Y
+
+    def set_graph(self, enable=True):
+        _check(lib().mx_batch_set_graph(self._h, int(enable)))
+
+    def autotune(self, precision, io="f32", reps=3, stream=None):
+        _check(lib().mx_batch_autotune(self._h, PRECISIONS[precision], IO_F64 if io == "f64" else IO_F32, reps,
+                                       _stream_handle(stream)))
+
+    def flops(self, precision):
+        f = ctypes.c_double()
+        _check(lib().mx_batch_flops(self._h, PRECISIONS[precision], ctypes.byref(f)))
+        return f.value
+
+    def plan(self, g):
+        """The group's plan (owned by the batch): schedules, profiling."""
+        h = P()
+        _check(lib().mx_batch_plan(self._h, g, ctypes.byref(h)))
+        nr, rows = self.groups[g]
+        return _GroupPlan(h, rows, nr)
+
+    def set_points(self, i, X_leaf, h):
+        X_leaf = _np(X_leaf, np.float64)
+        assert X_leaf.shape[0] == self.N[i]
+        _check(lib().mx_batch_set_points(self._h, i, X_leaf.ctypes.data, X_leaf.shape[1], float(h)))
+
+    def dense_rows(self, i, rows, W_inst):
+        """Dense FP64 K_i(rows, :) W_i (W_i: the instance's FP64 slice)."""
+        import torch
+        rows = _np(rows, np.int32)
+        nr = self.nrhs[i]
+        assert W_inst.dtype == torch.float64 and W_inst.numel() == self.N[i] * nr
+        out = torch.empty(nr * len(rows), dtype=torch.float64, device=W_inst.device)
+        _check(lib().mx_batch_dense_rows(self._h, i, rows.ctypes.data, len(rows), W_inst.data_ptr(), out.data_ptr(),
+                                         None))
+        return out.view(nr, len(rows)).t()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mx_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

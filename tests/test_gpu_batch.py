@@ -1,0 +1,125 @@
+"""Batched evaluation of many independent instances (mx_batch): each
+instance's slice of the group outputs against the oracle on that instance
+alone; instances of different nrhs, HSS and H2-b structures, ragged sizes."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import make_problem, rel
+from matrox_ru_amd import hgemm, host
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+INSTANCES = [
+    # (builder arguments, nrhs)
+    (dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.2, h=1.0), 32),
+    (dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.0, h=1.0), 32),
+    (dict(n=3001, d=5, leaf=100, rank=70, tol=1e-7, budget=0.3, h=1.2), 8),
+    (dict(n=1024, d=5, leaf=64, rank=100000, tol=0.0, budget=0.25, h=1.3), 32),
+    (dict(n=1500, d=6, leaf=50, rank=30, tol=1e-6, budget=0.1, h=0.9, seed=3), 5),
+    (dict(n=700, d=3, leaf=40, rank=20, tol=1e-6, budget=0.0, h=1.1, seed=4), 8),
+]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def batch_case():
+    hms = [make_problem(**kw)[1] for kw, _ in INSTANCES]
+    nrhs = [n for _, n in INSTANCES]
+    TW = [host.normal((hm.N * n,), seed=900 + i) for i, (hm, n) in enumerate(zip(hms, nrhs))]
+    ref = [oracle.hgemm(hm, tw, n)[0] for hm, tw, n in zip(hms, TW, nrhs)]
+    return hms, nrhs, TW, ref
+
+
+def _fill(b, TW, dtype):
+    import torch
+    W = b.buffers(dtype, fill=float("nan"))
+    for i, tw in enumerate(TW):
+        b.instance_view(W, i).copy_(torch.from_numpy(tw).to(dtype))
+    return W
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_batch_instances_match_oracle(gpu, batch_case, prec):
+    import torch
+    _say(f"[batch] prec={prec}")
+    hms, nrhs, TW, ref = batch_case
+    b = hgemm.Batch(hms, nrhs, precisions=("fp64", "fp32", "bf16x3"))
+    assert sorted(nr for nr, _ in b.groups) == [5, 8, 32]
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    W = _fill(b, TW, dt)
+    Y = b.buffers(dt, fill=float("nan"))
+    b.eval(W, Y, prec)
+    torch.cuda.synchronize()
+    for i in range(len(hms)):
+        y = b.instance_view(Y, i).double().cpu().numpy()
+        ro = ref[i]
+        if prec != "fp64":   # against the FP64 result of the rounded input
+            ro = oracle.hgemm(hms[i], b.instance_view(W, i).double().cpu().numpy(), nrhs[i])[0]
+        e = rel(y, ro)
+        _say(f"  instance {i}: nrhs={nrhs[i]} N={hms[i].N} err={e:.2e}")
+        # split-BF16x3 keeps 16 significant bits per operand and re-splits
+        # every skeleton level: ~1e-5 on these small, far-field-heavy
+        # instances (the 65,536-point workload is checked at 1e-5)
+        assert e <= {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 2e-5}[prec], i
+
+
+def test_batch_graph_replays_and_autotune(gpu, batch_case):
+    import torch
+    _say("[batch graph]")
+    hms, nrhs, TW, ref = batch_case
+    b = hgemm.Batch(hms, nrhs, precisions=("fp32", "bf16x3"))
+    W = _fill(b, TW, torch.float32)
+    b.autotune("bf16x3", "f32", reps=2)
+    b.autotune("fp32", "f32", reps=2)
+    Y0 = b.buffers(torch.float32)
+    b.eval(W, Y0, "bf16x3")   # eager, with the tuned schedules
+    b.set_graph(True)
+    Y1 = b.buffers(torch.float32)
+    for _ in range(3):
+        b.eval(W, Y1, "bf16x3")
+        torch.cuda.synchronize()
+        for y0, y1 in zip(Y0, Y1):
+            assert torch.equal(y0, y1)
+    Y2 = b.buffers(torch.float32)
+    b.eval(W, Y2, "fp32")
+    torch.cuda.synchronize()
+    for i in range(len(hms)):
+        assert rel(b.instance_view(Y2, i).cpu().numpy(), b.instance_view(Y1, i).cpu().numpy()) <= 2e-5
+    f = b.flops("fp32")
+    assert f == pytest.approx(sum(oracle.flops(hm, n) for hm, n in zip(hms, nrhs)), rel=1e-12)
+
+
+def test_batch_dense_rows(gpu, batch_case):
+    import torch
+    _say("[batch dense rows]")
+    hms, nrhs, TW, ref = batch_case
+    b = hgemm.Batch(hms, nrhs, precisions=("fp64",))
+    W = _fill(b, TW, torch.float64)
+    for i, (kw, _) in enumerate(INSTANCES):
+        X, hm = make_problem(**kw)
+        b.set_points(i, X[hm.lids[:hm.N]], kw["h"])
+        rows = np.array([0, 3, hm.N - 1], dtype=np.int32)
+        got = b.dense_rows(i, rows, b.instance_view(W, i)).cpu().numpy()
+        Wn = hm.untransform(TW[i], nrhs[i])
+        exp = oracle.dense_rows(X, kw["h"], hm.lids[rows], Wn)
+        assert rel(got, exp) <= 1e-12, i
+
+
+def test_batch_errors(gpu, batch_case):
+    hms, nrhs, TW, ref = batch_case
+    with pytest.raises(hgemm.MxError):
+        hgemm.Batch(hms[:2], [4, 0])
+    b = hgemm.Batch(hms[:2], [4, 4], precisions=("fp32",))
+    import torch
+    W = b.buffers(torch.float32, fill=0.0)
+    Y = b.buffers(torch.float32)
+    with pytest.raises(hgemm.MxError) as e:
+        b.eval(W, Y, "bf16x3")
+    assert e.value.code == -4
