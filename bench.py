@@ -16,9 +16,13 @@ dW = K~^T dY and the backward pass with block gradients are timed beside it.
 The compressed matrix is built on the host before the timed region, as in
 the reference driver (matroxTest-V1/main.cpp:84-108).
 
-Other workloads (--workload): hss_n16384 (plain HSS FP64, leaf 128, nrhs 256)
-and adjoint_n32768 (FP32 forward and backward with block gradients, nrhs
-128, checked against FP64 central differences on 64 directions).
+Other workloads (--workload): hss_n16384 (plain HSS FP64, leaf 128, nrhs 256),
+adjoint_n32768 (FP32 forward and backward with block gradients, nrhs 128,
+checked against FP64 central differences on 64 directions) and batched64
+(64 independent H2-b matrices of N log-uniform in [2048, 32768] evaluated as
+one batch, FP32 storage) and scale_n524288 (N = 524,288, 2048 leaves, nrhs 512;
+near and coupling blocks generated on the device; BF16x3, FP32 and FP64
+plans one after another, each checked against dense FP64 rows).
 
 Prints one JSON line (N=1).
 """
@@ -50,6 +54,16 @@ WORKLOADS = {
 }
 
 
+BATCHED = dict(name="batched64_h2b_gmm16", count=64, n_lo=2048, n_hi=32768, dims=(8, 18, 28, 54), h_lo=0.8,
+               h_hi=2.0, leaves=(128, 256), rank=128, tol=1e-5, budget=0.03, nrhs=(32, 128, 512), ncomp=16,
+               sig_lo=0.5, sig_hi=2.0, seed=3, seed_points=1003, seed_w=2003, seed_rows=3003, check_rows=100)
+
+
+SCALE = dict(name="h2b_gmm64_n524288_d27", n=524288, d=27, ncomp=64, sig_lo=0.5, sig_hi=1.5, diag=False, h=1.2,
+             leaf=256, rank=256, tol=1e-5, budget=0.03, nrhs=512, seed_points=1005, seed_w=2005, seed_rows=3005,
+             check_rows=2000)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -59,7 +73,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="h2b_n65536", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="h2b_n65536", choices=sorted(WORKLOADS) + ["batched64", "scale_n524288"])
     ap.add_argument("--nrhs", type=int, default=0, help="override the workload's nrhs")
     ap.add_argument("--precision", default="", choices=["", "bf16x3", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -290,6 +304,176 @@ def fd_check(plan, hm, wl, nrhs, torch, np, host, ndir=64, eps=1e-6):
     return res
 
 
+def batched_instances(wl, host, np):
+    """The 64 instance descriptions (seeded; instance i offsets seeds by 100 i)."""
+    rng = np.random.default_rng(wl["seed"])
+    inst = []
+    for i in range(wl["count"]):
+        n = int(round(np.exp(rng.uniform(np.log(wl["n_lo"]), np.log(wl["n_hi"])))))
+        inst.append(dict(n=n, d=int(rng.choice(wl["dims"])), h=float(rng.uniform(wl["h_lo"], wl["h_hi"])),
+                         leaf=int(rng.choice(wl["leaves"])), nrhs=int(rng.choice(wl["nrhs"])),
+                         seed_points=wl["seed_points"] + 100 * i, seed_w=wl["seed_w"] + 100 * i,
+                         seed_rows=wl["seed_rows"] + 100 * i))
+    return inst
+
+
+def run_batched(args):
+    """Many-instance evaluation: every instance of one nrhs in one tree per
+    group, groups concurrent, the whole batch replayed from one hipGraph."""
+    import numpy as np
+    import torch
+    from matrox_ru_amd import hgemm, host
+    wl = dict(BATCHED)
+    inst = batched_instances(wl, host, np)
+    t0 = time.time()
+    hms, Xs = [], []
+    for it in inst:
+        X = host.gaussian_mixture(it["n"], it["d"], wl["ncomp"], sig_lo=wl["sig_lo"], sig_hi=wl["sig_hi"], diag=True,
+                                  seed=it["seed_points"])
+        hms.append(host.build_hmatrix(X, leaf_size=it["leaf"], max_rank=wl["rank"], tol=wl["tol"],
+                                      budget=wl["budget"], h=it["h"]))
+        Xs.append(X)
+    t_build = time.time() - t0
+    log(f"[bench] host build of {len(hms)} instances {t_build:.1f}s; total N {sum(hm.N for hm in hms)}; "
+        f"HSS-degenerate {sum(int(hm.is_hss) for hm in hms)}")
+    prec = args.precision or "bf16x3"
+    b = hgemm.Batch(hms, [it["nrhs"] for it in inst], precisions=("fp64", "fp32", "bf16x3"))
+    log(f"[bench] groups (nrhs, rows): {b.groups}")
+    W64 = b.buffers(torch.float64, fill=0.0)
+    for i, (hm, it) in enumerate(zip(hms, inst)):
+        b.instance_view(W64, i).copy_(torch.from_numpy(hm.transform(host.normal((hm.N, it["nrhs"]),
+                                                                                seed=it["seed_w"]))))
+    W32 = [w.float() for w in W64]
+    Y64 = b.buffers(torch.float64)
+    Y32 = b.buffers(torch.float32)
+    Ymx = b.buffers(torch.float32)
+    for p in ("fp32", "bf16x3"):
+        b.autotune(p, "f32", reps=2)
+    b.autotune("fp64", "f64", reps=1)
+    b.set_graph(not args.no_graph)
+    timer = Timer(torch, torch.cuda.current_stream())
+    ms64 = timer(lambda: b.eval(W64, Y64, "fp64"), max(3, args.steps // 4), 1)
+    ms32 = timer(lambda: b.eval(W32, Y32, "fp32"), args.steps, args.warmup)
+    msmx = timer(lambda: b.eval(W32, Ymx, "bf16x3"), args.steps, args.warmup)
+    fl = b.flops("fp64")
+    log(f"[bench] batch fp64 {ms64:.3f} ms, fp32 {ms32:.3f} ms, bf16x3 {msmx:.3f} ms; {fl / 1e12:.4f} TFLOP")
+    errs32, errsmx, dense = [], [], []
+    for i, (hm, it) in enumerate(zip(hms, inst)):
+        y64 = b.instance_view(Y64, i)
+        n64 = torch.linalg.vector_norm(y64)
+        errs32.append(float(torch.linalg.vector_norm(b.instance_view(Y32, i).double() - y64) / n64))
+        errsmx.append(float(torch.linalg.vector_norm(b.instance_view(Ymx, i).double() - y64) / n64))
+        if not args.skip_checks:
+            b.set_points(i, Xs[i][hm.lids[:hm.N]], it["h"])
+            rows = np.sort(np.random.default_rng(it["seed_rows"]).choice(hm.N, min(wl["check_rows"], hm.N),
+                                                                          replace=False)).astype(np.int32)
+            dr = b.dense_rows(i, rows, b.instance_view(W64, i))
+            r = hgemm.leaf_rows(hm.wrow, hm.Ddim, y64, rows, it["nrhs"])
+            dense.append(float(torch.linalg.vector_norm(r - dr) / torch.linalg.vector_norm(dr)))
+    per = [{"N": hm.N, "d": it["d"], "h": round(it["h"], 3), "leaf": it["leaf"], "nrhs": it["nrhs"],
+            "hss": bool(hm.is_hss), "rel_err_fp32_vs_fp64": errs32[i], "rel_err_bf16x3_vs_fp64": errsmx[i],
+            "rel_err_fp64_vs_dense": dense[i] if dense else None} for i, (hm, it) in enumerate(zip(hms, inst))]
+    ms = msmx if prec == "bf16x3" else ms32
+    out = {
+        "metric": f"batched HGEMM eval TFLOP/s+ms, {len(hms)} instances, {prec} vs FP64 plain",
+        "value": round(fl / (ms * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3-f32acc" if prec == "bf16x3" else "f32",
+        "data": "synthetic (splitmix64/Box-Muller Gaussian mixture points, N(0,1) W), FP32 storage",
+        "graph": not args.no_graph,
+        "config": {"workload": wl["name"], "instances": len(hms), "total_N": sum(hm.N for hm in hms),
+                   "groups": [{"nrhs": g[0], "rows": g[1]} for g in b.groups]},
+        "algorithmic_tflop_per_eval": round(fl / 1e12, 5),
+        "fp64_ms": round(ms64, 4), "fp32_ms": round(ms32, 4), "bf16x3_ms": round(msmx, 4),
+        "fp32_tflops": round(fl / (ms32 * 1e-3) / 1e12, 3), "bf16x3_tflops": round(fl / (msmx * 1e-3) / 1e12, 3),
+        "fp64_tflops": round(fl / (ms64 * 1e-3) / 1e12, 3),
+        "max_rel_err_fp32_vs_fp64": max(errs32), "max_rel_err_bf16x3_vs_fp64": max(errsmx),
+        "median_rel_err_bf16x3_vs_fp64": float(np.median(errsmx)),
+        "instances": per,
+        "host_build_s": round(t_build, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_scale(args):
+    """Precision at scale: the blocks (FP64 ~160 GB) are never stored on the
+    host; each precision's plan generates them on the device in its own
+    representation, one plan at a time."""
+    import numpy as np
+    import torch
+    from matrox_ru_amd import hgemm, host
+    wl = dict(SCALE)
+    nrhs = args.nrhs or wl["nrhs"]
+    t0 = time.time()
+    X = host.gaussian_mixture(wl["n"], wl["d"], wl["ncomp"], sig_lo=wl["sig_lo"], sig_hi=wl["sig_hi"],
+                              diag=wl["diag"], seed=wl["seed_points"])
+    hm = host.build_hmatrix(X, leaf_size=wl["leaf"], max_rank=wl["rank"], tol=wl["tol"], budget=wl["budget"],
+                            h=wl["h"], skip_blocks=1)
+    t_build = time.time() - t0
+    log(f"[bench] host build {t_build:.1f}s: nleaf={hm.nleaf} ncount={hm.ncount} fcount={hm.fcount} "
+        f"D {hm.scalars['D_len'] * 8 / 1e9:.1f} GB B {hm.scalars['B_len'] * 8 / 1e9:.1f} GB (FP64, not built)")
+    W = torch.from_numpy(hm.transform(host.normal((hm.N, nrhs), seed=wl["seed_w"]))).cuda()
+    rows = np.sort(np.random.default_rng(wl["seed_rows"]).choice(hm.N, wl["check_rows"], replace=False))
+    rows = rows.astype(np.int32)
+    timer = Timer(torch, torch.cuda.current_stream())
+    res, Y64, dense = {}, None, None
+    for prec in ("bf16x3", "fp32", "fp64"):
+        t1 = time.time()
+        plan = hgemm.Plan(hm, precisions=(prec,), generate=(X, wl["h"]))
+        t_plan = time.time() - t1
+        Wp = W if prec != "fp32" else W.float()
+        Y = torch.empty_like(Wp)
+        plan.set_graph(not args.no_graph)
+        steps = max(2, args.steps // (8 if prec == "fp64" else 2))
+        ms = timer(lambda: plan.eval(Wp, Y, nrhs, prec), steps, 1)
+        fl, _ = plan.flops(prec, nrhs)
+        r = {"ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 3), "plan_s": round(t_plan, 1)}
+        if prec == "fp64":
+            Y64 = Y
+            if not args.skip_checks:
+                plan.set_points(X[hm.lids[:hm.N]], wl["h"])
+                dense = plan.dense_rows(rows, W, nrhs)
+        res[prec] = (r, Y.double() if prec != "fp64" else Y)
+        log(f"[bench] {prec}: {r}")
+        plan.close()
+        del plan, Wp
+        torch.cuda.empty_cache()
+    fl64 = fl
+    out_prec = {}
+    for prec, (r, Y) in res.items():
+        if prec != "fp64":
+            r["rel_err_vs_fp64"] = float(torch.linalg.vector_norm(Y - Y64) / torch.linalg.vector_norm(Y64))
+        if dense is not None:
+            yr = hgemm.leaf_rows(hm.wrow, hm.Ddim, Y, rows, nrhs)
+            r["rel_err_vs_dense"] = float(torch.linalg.vector_norm(yr - dense) / torch.linalg.vector_norm(dense))
+        out_prec[prec] = r
+    log(f"[bench] scale: {out_prec}")
+    ms = out_prec["bf16x3"]["ms"]
+    out = {
+        "metric": f"HGEMM eval TFLOP/s+ms at N=524288 nrhs={nrhs}, bf16x3 vs FP32 vs FP64, generated blocks",
+        "value": round(fl64 / (ms * 1e-3) / 1e12, 3),
+        "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3-f32acc",
+        "data": "synthetic (splitmix64/Box-Muller Gaussian mixture points, N(0,1) W)",
+        "config": {"workload": wl["name"], "N": hm.N, "d": wl["d"], "nrhs": nrhs, "nleaf": int(hm.nleaf),
+                   "near_blocks": int(hm.ncount), "far_blocks": int(hm.fcount),
+                   "D_GB_fp64": round(hm.scalars["D_len"] * 8 / 1e9, 2),
+                   "B_GB_fp64": round(hm.scalars["B_len"] * 8 / 1e9, 2),
+                   "mean_rank": round(float(hm.slen[1:].mean()), 2), "check_rows": wl["check_rows"]},
+        "algorithmic_tflop_per_eval": round(fl64 / 1e12, 4),
+        "precisions": out_prec,
+        "host_build_s": round(t_build, 1),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -307,6 +491,10 @@ def main():
             b.build_all()
             break
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "batched64":
+        return run_batched(args)
+    if args.workload == "scale_n524288":
+        return run_scale(args)
     dev = torch.device("cuda:0")
     wl = dict(WORKLOADS[args.workload])
     nrhs = args.nrhs or wl["nrhs"]

@@ -113,6 +113,19 @@ typedef struct mx_plan mx_plan;
 /* Validate the structure, build the per-destination task lists and upload
  * blocks for each precision in `precisions` to the current HIP device. */
 mx_status mx_plan_create(mx_plan **plan, const mx_structure *s, uint32_t precisions);
+
+/* As mx_plan_create, but the near and coupling blocks are computed on the
+ * device from the points instead of read from s->D and s->B (which may be
+ * NULL; D_len, B_len, Dptr and Bptr still give the layout): D_k is the
+ * Gaussian kernel K(x_i, x_j) = exp(-|x_i - x_j|^2 / (2 h^2)) between the
+ * points of its two leaves, B_k between the skeleton points of its two nodes
+ * (the reference's Fsubmatrix, matroxTest-V1/compression.h:256-360).  X: host
+ * n x d row-major points in original order; lids: the N leaf-order point ids;
+ * skels / skelsoffset: skeleton point ids of node v at skels[skelsoffset[v]]
+ * (nskels entries).  For matrices whose FP64 blocks do not fit the host. */
+mx_status mx_plan_create_generated(mx_plan **plan, const mx_structure *s, uint32_t precisions, const double *X,
+                                   int64_t n, int32_t d, double h, const int32_t *lids, const int32_t *skels,
+                                   const int64_t *skelsoffset, int64_t nskels);
 mx_status mx_plan_destroy(mx_plan *plan);
 
 /* Y = K~ W on device buffers (leaf-blocked, N x nrhs), enqueued on `stream`

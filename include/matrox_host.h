@@ -48,6 +48,9 @@ typedef struct mxh_config {
     int32_t  nbins;          /* partitions per coarsened group (12)          */
     int32_t  nthreads;       /* OpenMP threads for the build, 0 = default    */
     uint64_t seed;           /* seed of the sampling random streams          */
+    int32_t  skip_blocks;    /* 1: lay out D and B (Dptr, Bptr, sizes in the
+                                scalars "D_len", "B_len") without computing
+                                them, for evaluators that generate them      */
 } mxh_config;
 
 /* Fill a config with the reference's defaults (m=256, rank 256, tol 1e-5,
@@ -68,8 +71,8 @@ mxh_handle *mxh_build_structure(const double *X, int32_t n, int32_t d, const mxh
 void mxh_free(mxh_handle *h);
 
 /* Scalar properties: "n", "d", "nleaf", "numnodes", "depth", "cdepth", "nl",
- * "fl", "ncount", "fcount", "is_hss", "N_total", "skel_rows".  Returns -1 if
- * unknown. */
+ * "fl", "ncount", "fcount", "is_hss", "N_total", "skel_rows", "D_len",
+ * "B_len", "VT_len".  Returns -1 if unknown. */
 int64_t mxh_get_scalar(const mxh_handle *h, const char *name);
 
 /* Named arrays (pointer stays valid until mxh_free).  int32 arrays:

@@ -632,6 +632,195 @@ static int cvt_grid(int64_t n) {
     return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 8192);
 }
 
+// ---- kernel blocks generated on the device --------------------------------
+//
+// D_k = K(points of leaf x, points of leaf y) and B_k = K(skeleton points of
+// x, skeleton points of y) computed where they are needed instead of being
+// uploaded, for matrices whose FP64 blocks do not fit the host (the
+// reference computes them on the host, matroxTest-V1/compression.h:256-360,
+// with the same norm expansion |a|^2 + |b|^2 - 2 a.b).
+
+struct GenInfo {
+    const double *X = nullptr;        // host, n x d row-major, original point order
+    int64_t n = 0;
+    int d = 0;
+    double h = 1.0;
+    const int32_t *lids = nullptr;    // host, N: leaf-order row -> point id
+    const int32_t *skels = nullptr;   // host: skeleton point ids, node v at skoff[v]
+    const int64_t *skoff = nullptr;   // host, numnodes
+    int64_t nskels = 0;
+};
+
+struct alignas(16) GenBlock {
+    int64_t out;    // element offset of the block (column-major na x nb)
+    int64_t a, b;   // offsets of its row and column point ids in the index array
+    int32_t na, nb;
+};
+
+__global__ void point_norms(const double *__restrict__ X, int64_t n, int d, double *__restrict__ nrm) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = 0; q < d; ++q) s += X[p * d + q] * X[p * d + q];
+        nrm[p] = s;
+    }
+}
+
+__device__ __forceinline__ double gauss_entry(const double *__restrict__ X, const double *__restrict__ nrm, int d,
+                                              double kscal, int64_t pa, int64_t pb) {
+    const double *xa = X + pa * d, *xb = X + pb * d;
+    double dot = 0.0;
+    for (int q = 0; q < d; ++q) dot += xa[q] * xb[q];
+    return exp(kscal * (-2.0 * dot + nrm[pa] + nrm[pb]));
+}
+
+// blocks[blockIdx.y] in its stored layout (column-major na x nb), FP64 or FP32
+template <typename T>
+__global__ void gen_blocks(const double *__restrict__ X, const double *__restrict__ nrm, int d, double kscal,
+                           const int32_t *__restrict__ idx, const GenBlock *__restrict__ blocks, T *__restrict__ out) {
+    const GenBlock g = blocks[blockIdx.y];
+    const int64_t ne = (int64_t)g.na * g.nb;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % g.na), j = (int)(e / g.na);
+        out[g.out + e] = (T)gauss_entry(X, nrm, d, kscal, idx[g.a + i], idx[g.b + j]);
+    }
+}
+
+// Packed split-BF16 operand xb[blockIdx.y] (fragment order as pack_x3_blocks)
+// computed from its generating block gb[blockIdx.y]: op(A)(i, k) is entry
+// (i, k) of the block, or (k, i) when transposed.
+__global__ void gen_pack_x3(const double *__restrict__ X, const double *__restrict__ nrm, int d, double kscal,
+                            const int32_t *__restrict__ idx, const X3Block *__restrict__ xb,
+                            const GenBlock *__restrict__ gb, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+    const X3Block bk = xb[blockIdx.y];
+    const GenBlock g = gb[blockIdx.y];
+    const int nfk = (bk.K + 31) / 32, nfm = (bk.M + 15) / 16;
+    const int64_t lanes = (int64_t)nfm * nfk * 64;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < lanes;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frag = t >> 6;
+        const int lane = (int)(t & 63);
+        const int mi = (int)(frag / nfk), ki = (int)(frag % nfk);
+        const int row = mi * 16 + (lane & 15);
+        const int kb = ki * 32 + 8 * (lane >> 4);
+        const int64_t dst = bk.dst_off + frag * 512 + (int64_t)lane * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = kb + e;
+            double x = 0.0;
+            if (row < bk.M && k < bk.K) {
+                const int r = bk.trans ? k : row, c = bk.trans ? row : k;
+                x = gauss_entry(X, nrm, d, kscal, idx[g.a + r], idx[g.b + c]);
+            }
+            const uint16_t h = bf16_rne((float)x);
+            hi[dst + e] = h;
+            lo[dst + e] = bf16_rne((float)(x - (double)bf16_f(h)));
+        }
+    }
+}
+
+// Points, norms and block lists of one matrix whose D and B are generated:
+// near blocks index the leaf-order point ids (lids), coupling blocks the
+// skeleton point ids.
+struct GenState {
+    double *Xd = nullptr, *nrm = nullptr;
+    int32_t *idx[2] = {nullptr, nullptr};   // 0: lids, 1: skeleton ids
+    int d = 0;
+    double kscal = 0.0;
+    std::vector<GenBlock> blk[2];           // 0: near (D), 1: coupling (B)
+    std::map<int64_t, int> at[2];           // block offset -> index in blk
+    GenState() = default;
+    GenState(const GenState &) = delete;
+    GenState &operator=(const GenState &) = delete;
+    ~GenState() {
+        for (void *p : {(void *)Xd, (void *)nrm, (void *)idx[0], (void *)idx[1]})
+            if (p) (void)hipFree(p);
+    }
+
+    void init(const mx_structure &s, const Topology &t, const GenInfo &gen) {
+        arg_check(gen.X && gen.n >= 1 && gen.d >= 1 && gen.h > 0 && gen.lids && gen.skels && gen.skoff,
+                  "bad block-generation arguments");
+        int64_t N = 0;
+        for (int l = 0; l < s.nleaf; ++l) N = std::max<int64_t>(N, (int64_t)s.wrow[l] + s.Ddim[l]);
+        for (int64_t r = 0; r < N; ++r) arg_check(gen.lids[r] >= 0 && gen.lids[r] < gen.n, "lids out of range");
+        for (int64_t i = 0; i < gen.nskels; ++i)
+            arg_check(gen.skels[i] >= 0 && gen.skels[i] < gen.n, "skeleton point id out of range");
+        for (int v = 0; v < s.numnodes; ++v)
+            arg_check(s.slen[v] == 0 || (gen.skoff[v] >= 0 && gen.skoff[v] + s.slen[v] <= gen.nskels),
+                      "skeleton offsets out of range");
+        d = gen.d;
+        kscal = -0.5 / (gen.h * gen.h);
+        Xd = upload(gen.X, (size_t)gen.n * gen.d);
+        HIPCHK(hipMalloc(&nrm, sizeof(double) * (size_t)gen.n));
+        point_norms<<<cvt_grid(gen.n), 256>>>(Xd, gen.n, gen.d, nrm);
+        HIPCHK(hipGetLastError());
+        idx[0] = upload(gen.lids, (size_t)N);
+        idx[1] = upload(gen.skels, (size_t)gen.nskels);
+        for (int64_t k = 0; k < s.ncount; ++k) {
+            int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
+            if (s.Ddim[x] == 0 || s.Ddim[y] == 0) continue;
+            at[0][(int64_t)s.Dptr[k]] = (int)blk[0].size();
+            blk[0].push_back(GenBlock{(int64_t)s.Dptr[k], s.wrow[x], s.wrow[y], s.Ddim[x], s.Ddim[y]});
+        }
+        for (int64_t k = 0; k < s.fcount; ++k) {
+            int x, y;
+            if (s.is_hss) {
+                x = (int)k + 1;
+                int p = t.parent[x];
+                y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+            } else {
+                x = s.fpairx[k];
+                y = s.fpairy[k];
+            }
+            if (s.slen[x] == 0 || s.slen[y] == 0) continue;
+            at[1][(int64_t)s.Bptr[k]] = (int)blk[1].size();
+            blk[1].push_back(GenBlock{(int64_t)s.Bptr[k], gen.skoff[x], gen.skoff[y], s.slen[x], s.slen[y]});
+        }
+    }
+
+    // w: 0 = D, 1 = B, in their stored layout
+    template <typename T>
+    void emit(int w, T *out) const {
+        const std::vector<GenBlock> &b = blk[w];
+        for (size_t b0 = 0; b0 < b.size(); b0 += 65535) {
+            const size_t nb = std::min<size_t>(65535, b.size() - b0);
+            GenBlock *db = upload(b.data() + b0, nb);
+            gen_blocks<T><<<dim3(64, (unsigned)nb), 256>>>(Xd, nrm, d, kscal, idx[w], db, out);
+            hipError_t e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+            (void)hipFree(db);
+            if (e != hipSuccess) throw MxError(MX_ERR_HIP, std::string("block generation: ") + hipGetErrorString(e));
+        }
+    }
+
+    // packed split-BF16 operands of buffer w for the packing list xb
+    void emit_x3(int w, const std::vector<X3Block> &xb, uint16_t *hi, uint16_t *lo) const {
+        std::vector<X3Block> todo;
+        std::vector<GenBlock> src;
+        for (const X3Block &x : xb) {
+            if (x.M == 0 || x.K == 0) continue;
+            auto it = at[w].find(x.src_off);
+            if (it == at[w].end()) throw MxError(MX_ERR_ARG, "a packed operand does not start at a generated block");
+            todo.push_back(x);
+            src.push_back(blk[w][it->second]);
+        }
+        for (size_t b0 = 0; b0 < todo.size(); b0 += 65535) {
+            const size_t nb = std::min<size_t>(65535, todo.size() - b0);
+            X3Block *dx = upload(todo.data() + b0, nb);
+            GenBlock *dg = nullptr;
+            hipError_t e = hipMalloc(&dg, nb * sizeof(GenBlock));
+            if (e == hipSuccess) e = hipMemcpy(dg, src.data() + b0, nb * sizeof(GenBlock), hipMemcpyHostToDevice);
+            if (e == hipSuccess) {
+                gen_pack_x3<<<dim3(64, (unsigned)nb), 256>>>(Xd, nrm, d, kscal, idx[w], dx, dg, hi, lo);
+                e = hipGetLastError();
+                if (e == hipSuccess) e = hipDeviceSynchronize();
+            }
+            (void)hipFree(dx);
+            if (dg) (void)hipFree(dg);
+            if (e != hipSuccess) throw MxError(MX_ERR_HIP, std::string("block generation: ") + hipGetErrorString(e));
+        }
+    }
+};
+
 // Pack the split-BF16 operands of D, B and VT from FP64 sources.
 static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     for (int w = 0; w < 3; ++w) {
@@ -644,14 +833,15 @@ static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     }
 }
 
-static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y) {
+static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y,
+                            const GenInfo *gen = nullptr) {
     arg_check(s.nleaf > 0 && s.numnodes >= 1, "empty structure");
     arg_check(s.numnodes == 2 * s.nleaf - 1, "numnodes must be 2*nleaf-1 for a binary tree");
     arg_check(s.lchildren && s.rchildren && s.lm && s.Ddim && s.slen && s.VTptr && s.wrow && s.skelrow,
               "null structure array");
     arg_check(s.ncount >= 0 && s.fcount >= 0, "negative pair count");
-    arg_check(s.ncount == 0 || (s.Dptr && s.D), "near blocks missing");
-    arg_check(s.fcount == 0 || (s.Bptr && s.B), "far blocks missing");
+    arg_check(s.ncount == 0 || (s.Dptr && (s.D || gen)), "near blocks missing");
+    arg_check(s.fcount == 0 || (s.Bptr && (s.B || gen)), "far blocks missing");
     arg_check(s.is_hss || s.ncount == 0 || (s.npairx && s.npairy), "near pair arrays missing");
     arg_check(s.is_hss || s.fcount == 0 || (s.fpairx && s.fpairy), "far pair arrays missing");
     arg_check((precisions & 7u) != 0 && (precisions & ~(7u | MX_PLAN_ADJOINT)) == 0, "bad precision mask");
@@ -770,37 +960,61 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         P->g_ntiles = (int)G.tiles.size();
         P->g_flops = G.flops;
     }
-    // stored blocks
-    double *D64 = upload(s.D, (size_t)s.D_len), *B64 = upload(s.B, (size_t)s.B_len);
-    double *VT64 = upload(s.VT, (size_t)s.VT_len);
+    // stored blocks: uploaded, or generated from the points straight into
+    // each precision's representation (the FP64 blocks of a large matrix
+    // need not exist anywhere for an FP32 or BF16x3 plan)
+    std::unique_ptr<GenState> G;
+    if (gen) {
+        G.reset(new GenState());
+        G->init(s, t, *gen);
+    }
+    const bool need64 = !gen || (precisions & MX_PLAN_FP64);
+    if (need64) {
+        P->D64 = upload(gen ? nullptr : s.D, (size_t)s.D_len);
+        P->B64 = upload(gen ? nullptr : s.B, (size_t)s.B_len);
+        if (gen) {
+            G->emit<double>(0, P->D64);
+            G->emit<double>(1, P->B64);
+        }
+    }
+    P->VT64 = upload(s.VT, (size_t)s.VT_len);
     if (precisions & MX_PLAN_FP32) {
         P->VT32 = upload<float>(nullptr, (size_t)s.VT_len);
-        cvt_f64_f32<<<cvt_grid(s.VT_len), 256>>>(VT64, P->VT32, s.VT_len);
+        cvt_f64_f32<<<cvt_grid(s.VT_len), 256>>>(P->VT64, P->VT32, s.VT_len);
         P->D32 = upload<float>(nullptr, (size_t)s.D_len);
         P->B32 = upload<float>(nullptr, (size_t)s.B_len);
-        cvt_f64_f32<<<cvt_grid(s.D_len), 256>>>(D64, P->D32, s.D_len);
-        cvt_f64_f32<<<cvt_grid(s.B_len), 256>>>(B64, P->B32, s.B_len);
+        if (gen) {
+            G->emit<float>(0, P->D32);
+            G->emit<float>(1, P->B32);
+        } else {
+            cvt_f64_f32<<<cvt_grid(s.D_len), 256>>>(P->D64, P->D32, s.D_len);
+            cvt_f64_f32<<<cvt_grid(s.B_len), 256>>>(P->B64, P->B32, s.B_len);
+        }
     }
     if (precisions & MX_PLAN_BF16X3) {
-        double *src[3] = {D64, B64, VT64};
         for (int w = 0; w < 3; ++w) {
             P->xhi[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
             P->xlo[w] = upload<uint16_t>(nullptr, (size_t)P->x3_elems[w]);
             P->d_xblk[w] = upload(xblk[w].data(), xblk[w].size());
             P->n_xblk[w] = (int64_t)xblk[w].size();
         }
-        pack_all(P.get(), src, nullptr);
+        if (gen) {
+            const double *src[3] = {nullptr, nullptr, P->VT64};
+            pack_all(P.get(), src, nullptr);
+            G->emit_x3(0, xblk[0], P->xhi[0], P->xlo[0]);
+            G->emit_x3(1, xblk[1], P->xhi[1], P->xlo[1]);
+        } else {
+            const double *src[3] = {P->D64, P->B64, P->VT64};
+            pack_all(P.get(), src, nullptr);
+        }
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
-    if (precisions & MX_PLAN_FP64) {
-        P->D64 = D64;
-        P->B64 = B64;
-        P->VT64 = VT64;
-    } else {
-        HIPCHK(hipFree(D64));
-        HIPCHK(hipFree(B64));
-        HIPCHK(hipFree(VT64));
+    if (!(precisions & MX_PLAN_FP64)) {   // FP64 blocks were only a source
+        for (double **p : {&P->D64, &P->B64, &P->VT64}) {
+            if (*p) HIPCHK(hipFree(*p));
+            *p = nullptr;
+        }
     }
     return P.release();
 }
@@ -1903,6 +2117,26 @@ mx_status mx_plan_create(mx_plan **plan, const mx_structure *s, uint32_t precisi
         arg_check(plan && s, "null argument");
         *plan = nullptr;
         *plan = create_plan(*s, precisions, 0, 0);
+        return 0;
+    });
+}
+
+mx_status mx_plan_create_generated(mx_plan **plan, const mx_structure *s, uint32_t precisions, const double *X,
+                                   int64_t n, int32_t d, double h, const int32_t *lids, const int32_t *skels,
+                                   const int64_t *skelsoffset, int64_t nskels) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && s, "null argument");
+        *plan = nullptr;
+        GenInfo g;
+        g.X = X;
+        g.n = n;
+        g.d = d;
+        g.h = h;
+        g.lids = lids;
+        g.skels = skels;
+        g.skoff = skelsoffset;
+        g.nskels = nskels;
+        *plan = create_plan(*s, precisions, 0, 0, &g);
         return 0;
     });
 }

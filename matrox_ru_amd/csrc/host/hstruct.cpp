@@ -863,6 +863,11 @@ static void emit_cds(Handle &H) {
             uint64_t b = near ? H.pts[f + py[k]].size() : H.skels[py[k]].size();
             off += a * b;
         }
+        H.scal[near ? "D_len" : "B_len"] = (int64_t)off;
+        if (H.cfg.skip_blocks) {
+            buf.clear();
+            return;
+        }
         buf.assign(off, 0.0);
 #pragma omp parallel for schedule(dynamic)
         for (long long k = 0; k < (long long)cnt; ++k) {
@@ -1239,6 +1244,7 @@ void mxh_config_default(mxh_config *c) {
     c->nbins = 12;
     c->nthreads = 0;
     c->seed = 0x4D617452ULL;
+    c->skip_blocks = 0;
 }
 
 static mxh_handle *build_c(const double *X, int32_t n, int32_t d, const mxh_config *cfg, bool full) {

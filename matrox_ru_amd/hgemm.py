@@ -60,6 +60,9 @@ def lib():
         L.HGEMM_HSS.restype = I32
         L.mx_plan_create.argtypes = [ctypes.POINTER(P), ctypes.POINTER(MxStructure), ctypes.c_uint32]
         L.mx_plan_create.restype = ctypes.c_int
+        L.mx_plan_create_generated.argtypes = [ctypes.POINTER(P), ctypes.POINTER(MxStructure), ctypes.c_uint32, P,
+                                               I64, I32, ctypes.c_double, P, P, P, I64]
+        L.mx_plan_create_generated.restype = ctypes.c_int
         L.mx_plan_destroy.argtypes = [P]
         L.mx_plan_destroy.restype = ctypes.c_int
         L.mx_eval.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
@@ -156,10 +159,16 @@ def structure_of(hm):
         setattr(s, f, ptr(f, np.int32))
     s.Dptr = ptr("Dptr", np.uint64)
     s.Bptr = ptr("Bptr", np.uint64)
-    s.D = ptr("D", np.float64)
-    s.B = ptr("B", np.float64)
     s.VT = ptr("VT", np.float64)
-    s.D_len, s.B_len, s.VT_len = len(a["D"]), len(a["B"]), len(a["VT"])
+    s.VT_len = len(a["VT"])
+    for f in ("D", "B"):   # blocks laid out but not computed (skip_blocks) stay NULL
+        n = len(a[f])
+        if n == 0 and hm.scalars.get(f + "_len", -1) > 0:
+            setattr(s, f, None)
+            setattr(s, f + "_len", int(hm.scalars[f + "_len"]))
+        else:
+            setattr(s, f, ptr(f, np.float64))
+            setattr(s, f + "_len", n)
     return s, keep
 
 
@@ -175,13 +184,27 @@ def _stream_handle(stream):
 class Plan:
     """Device-resident compressed matrix (mx_plan)."""
 
-    def __init__(self, hm, precisions=("fp64",), adjoint=False):
+    def __init__(self, hm, precisions=("fp64",), adjoint=False, generate=None):
+        """generate=(X, h): compute D and B on the device from the points X
+        (n x d, original order) with bandwidth h (mx_plan_create_generated)
+        instead of uploading hm's D and B, which may then be empty."""
         mask = PLAN_ADJOINT if adjoint else 0
         for p in precisions:
             mask |= 1 << PRECISIONS[p]
         s, keep = structure_of(hm)
         h = P()
-        _check(lib().mx_plan_create(ctypes.byref(h), ctypes.byref(s), mask))
+        if generate is None:
+            _check(lib().mx_plan_create(ctypes.byref(h), ctypes.byref(s), mask))
+        else:
+            X, bw = generate
+            X = _np(X, np.float64)
+            lids = _np(hm.arrays["lids"][:int(hm.scalars["n"])], np.int32)
+            sk = _np(hm.arrays["skels"], np.int32)
+            so = _np(hm.arrays["skelsoffset"], np.int64)
+            _check(lib().mx_plan_create_generated(ctypes.byref(h), ctypes.byref(s), mask, X.ctypes.data, X.shape[0],
+                                                  X.shape[1], float(bw), lids.ctypes.data, sk.ctypes.data,
+                                                  so.ctypes.data, len(sk)))
+            del X, lids, sk, so
         del keep
         self._h = h
         self.N = int(hm.scalars["n"])
@@ -191,7 +214,7 @@ class Plan:
         self.skel_rows = sk.value
         self.precisions = tuple(precisions)
         self.adjoint = bool(adjoint)
-        self.block_lens = (len(hm.arrays["D"]), len(hm.arrays["B"]), len(hm.arrays["VT"]))
+        self.block_lens = (s.D_len, s.B_len, s.VT_len)
         self._wrow = np.asarray(hm.arrays["wrow"], dtype=np.int64)
         self._ddim = np.asarray(hm.arrays["Ddim"], dtype=np.int64)
 
@@ -303,14 +326,7 @@ class Plan:
 
     def rows_of(self, Y, rows, nrhs):
         """Rows (leaf-order ids) of a leaf-blocked N x nrhs device result, as (nrows, nrhs)."""
-        import torch
-        rows = np.asarray(rows, dtype=np.int64)
-        wrow = self._wrow
-        dim = self._ddim
-        leaf = np.searchsorted(wrow, rows, side="right") - 1
-        base = wrow[leaf] * nrhs + (rows - wrow[leaf])
-        idx = base[:, None] + np.arange(nrhs)[None, :] * dim[leaf][:, None]
-        return Y[torch.from_numpy(idx).to(Y.device)]
+        return leaf_rows(self._wrow, self._ddim, Y, rows, nrhs)
 
     def set_graph(self, enable=True):
         """Replay evaluations from captured hipGraphs (mx_plan_set_graph)."""
@@ -353,6 +369,19 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+
+def leaf_rows(wrow, Ddim, Y, rows, nrhs):
+    """Rows (leaf-order ids) of a leaf-blocked N x nrhs tensor Y (leaf l's
+    block starts at element wrow[l]*nrhs, leading dimension Ddim[l])."""
+    import torch
+    wrow = np.asarray(wrow, dtype=np.int64)
+    dim = np.asarray(Ddim, dtype=np.int64)
+    rows = np.asarray(rows, dtype=np.int64)
+    leaf = np.searchsorted(wrow, rows, side="right") - 1
+    base = wrow[leaf] * nrhs + (rows - wrow[leaf])
+    idx = base[:, None] + np.arange(nrhs)[None, :] * dim[leaf][:, None]
+    return Y[torch.from_numpy(idx).to(Y.device)]
 
 
 class _GroupPlan(Plan):

@@ -21,7 +21,7 @@ _I32 = ["VTptr", "lchildren", "rchildren", "levelset", "idx_level", "idx", "Ddim
 _U64 = ["Dptr", "Bptr"]
 _F64 = ["D", "B", "VT", "X"]
 _SCALARS = ["n", "d", "nleaf", "numnodes", "depth", "cdepth", "nl", "fl", "ncount",
-            "fcount", "is_hss", "N_total", "skel_rows"]
+            "fcount", "is_hss", "N_total", "skel_rows", "D_len", "B_len"]
 
 
 class MxhConfig(ctypes.Structure):
@@ -31,7 +31,7 @@ class MxhConfig(ctypes.Structure):
                 ("near_block", ctypes.c_int32), ("far_block", ctypes.c_int32),
                 ("agg_levels", ctypes.c_int32), ("bottom_levels", ctypes.c_int32),
                 ("nbins", ctypes.c_int32), ("nthreads", ctypes.c_int32),
-                ("seed", ctypes.c_uint64)]
+                ("seed", ctypes.c_uint64), ("skip_blocks", ctypes.c_int32)]
 
 
 _configured = False

@@ -210,3 +210,19 @@ def test_transposed_structure_is_the_transpose(budget):
     Kx, _, _ = oracle.hgemm(hm, x, nrhs)
     Kty, _, _ = oracle.hgemm(T, y, nrhs)
     assert abs(float(Kx @ y) - float(x @ Kty)) <= 1e-13 * np.linalg.norm(Kx) * np.linalg.norm(y)
+
+
+@pytest.mark.parametrize("budget", [0.0, 0.25])
+def test_skip_blocks_keeps_the_layout(budget):
+    """skip_blocks lays out D and B (Dptr, Bptr, D_len, B_len) exactly as a
+    full build, without computing them; everything else is identical."""
+    X = host.gaussian_mixture(1800, 5, 4, seed=11)
+    kw = dict(leaf_size=60, max_rank=30, tol=1e-6, budget=budget, h=1.2)
+    full = host.build_hmatrix(X, **kw)
+    lay = host.build_hmatrix(X, skip_blocks=1, **kw)
+    assert len(lay.D) == 0 and len(lay.B) == 0
+    assert lay.scalars["D_len"] == len(full.D) and lay.scalars["B_len"] == len(full.B)
+    for k, v in full.arrays.items():
+        if k in ("D", "B"):
+            continue
+        assert np.array_equal(v, lay.arrays[k]), k
