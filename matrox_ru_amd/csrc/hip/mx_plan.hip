@@ -185,6 +185,7 @@ struct TaskLists {
     std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
     std::vector<int> task_kind;   // per task
     std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
+    int dummy = -1;               // an empty task (no rows, no segments) padding launches
 };
 
 static void compute_launch_flops(TaskLists &L) {
@@ -297,50 +298,61 @@ static int add_tiles(TaskLists &L, int task, int m) {
 }
 
 // Order of the tiles of every launch.  xcd_remap hands XCD x the x-th
-// contiguous eighth of a launch's work items.  When the eighths of the
-// launch in structural order (tree order: neighbouring leaves share W blocks
-// and skeleton blocks, which then stay in one XCD's L2) carry similar work,
-// each eighth is only sorted longest first.  Otherwise (long lists at a few
-// nodes, e.g. the adjoint's column-grouped couplings at the top of the tree)
-// all tiles are sorted longest first and dealt round-robin into the eight
-// groups, so every XCD gets a similar mix and starts with its longest tiles.
+// contiguous eighth of a launch's work items (the column tiles of a tile are
+// adjacent).  The tiles, in structural order (tree order: neighbouring
+// leaves and nodes share W and skeleton blocks, which then stay in one XCD's
+// L2), are cut into eight contiguous groups of about equal work; each group
+// is sorted longest first (so long lists, such as the adjoint's
+// column-grouped couplings at the top of the tree, start early) and padded
+// with empty tiles to the size of the largest, so that group x is exactly
+// the eighth xcd_remap gives XCD x.
 static void order_tiles(TaskLists &L, int bk) {
+    if (L.dummy < 0) {
+        Task tk{};
+        tk.c_buf = VEC_USKEL;
+        L.dummy = (int)L.tasks.size();
+        L.tasks.push_back(tk);
+        L.task_kind.push_back(L.task_kind.empty() ? 0 : L.task_kind.back());
+    }
+    std::vector<int2> out;
+    std::vector<Launch> lns;
     for (const Launch &ln : L.launches) {
         const int T = ln.ntiles;
-        if (T <= 1) continue;
         std::vector<std::pair<int64_t, int2>> v;
         v.reserve(T);
+        int64_t total = 0;
         for (int i = 0; i < T; ++i) {
             int2 tl = L.tiles[ln.tile_offset + i];
             const Task &tk = L.tasks[tl.x];
             int64_t cost = 2;   // prologue and epilogue
             for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
             v.push_back({cost, tl});
+            total += cost;
         }
-        auto longer = [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
-            return a.first > b.first;
-        };
         const int ng = 8;
-        int64_t gmax = 0, gsum = 0;
-        for (int g = 0; g < ng; ++g) {
-            int64_t c = 0;
-            for (int i = (int)((int64_t)g * T / ng); i < (int)((int64_t)(g + 1) * T / ng); ++i) c += v[i].first;
-            gmax = std::max(gmax, c);
-            gsum += c;
+        std::vector<int> cut(ng + 1, T);
+        cut[0] = 0;
+        int64_t acc = 0;
+        int g = 1;
+        for (int i = 0; i < T && g < ng; ++i) {
+            acc += v[i].first;
+            while (g < ng && acc * ng >= total * g) cut[g++] = i + 1;
         }
-        int o = ln.tile_offset;
-        if (gmax * ng <= gsum * 11 / 10) {
-            for (int g = 0; g < ng; ++g) {
-                auto b0 = v.begin() + (int64_t)g * T / ng, b1 = v.begin() + (int64_t)(g + 1) * T / ng;
-                std::stable_sort(b0, b1, longer);
-            }
-            for (int i = 0; i < T; ++i) L.tiles[o + i] = v[i].second;
-        } else {
-            std::stable_sort(v.begin(), v.end(), longer);
-            for (int g = 0; g < ng; ++g)
-                for (int i = g; i < T; i += ng) L.tiles[o++] = v[i].second;
+        int gmax = 0;
+        for (int q = 0; q < ng; ++q) gmax = std::max(gmax, cut[q + 1] - cut[q]);
+        Launch nl{(int)out.size(), gmax * ng};
+        for (int q = 0; q < ng; ++q) {
+            std::stable_sort(v.begin() + cut[q], v.begin() + cut[q + 1],
+                             [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
+                                 return a.first > b.first;
+                             });
+            for (int i = cut[q]; i < cut[q + 1]; ++i) out.push_back(v[i].second);
+            for (int i = cut[q + 1] - cut[q]; i < gmax; ++i) out.push_back(make_int2(L.dummy, 0));
         }
+        lns.push_back(nl);
     }
+    L.tiles.swap(out);
+    L.launches.swap(lns);
 }
 
 // Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
@@ -1356,7 +1368,9 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
 }
 
 // Time every candidate (schedule x BF16x3 kernel variant) of each direction
-// built into the plan on scratch inputs and keep the fastest.
+// built into the plan on scratch inputs and keep the fastest.  A candidate
+// replaces the current choice only when it is more than 2% faster, so that
+// timing noise does not flip between near-equal choices.
 static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st,
                      float *best_ms) {
     arg_check(nrhs >= 1 && reps >= 1, "bad nrhs or reps");
@@ -1387,17 +1401,21 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
         const int kd = prec == MX_BF16X3 ? 1 : 0;
         const int ndir = P->adjoint ? 2 : 1;
         const int nvar = prec == MX_BF16X3 ? X3_NVARIANT : 1;
-        int best_variant = P->x3_variant;
-        int best_pick[2] = {P->sched[0][kd], P->sched[1][kd]};
-        float best_total = 0.f, best_dir[2] = {0.f, 0.f};
+        const int cur_variant = P->x3_variant;
+        const int cur_sched[2] = {P->sched[0][kd], P->sched[1][kd]};
+        int best_variant = cur_variant;
+        int best_pick[2] = {cur_sched[0], cur_sched[1]};
+        float best_total = -1.f, best_dir[2] = {0.f, 0.f};
+        // the current variant first, so that it is the reference
         for (int si = 0; si < nvar; ++si) {
-            if (prec == MX_BF16X3) P->x3_variant = si + 1;
+            if (prec == MX_BF16X3) P->x3_variant = si == 0 ? cur_variant : (si <= cur_variant - 1 ? si : si + 1);
             float dir_ms[2] = {0.f, 0.f};
             float total = 0.f;
-            int pick[2] = {P->sched[0][kd], P->sched[1][kd]};
+            int pick[2] = {cur_sched[0], cur_sched[1]};
             for (int dir = 0; dir < ndir; ++dir) {
                 float best = -1.f;
-                for (int sc = 0; sc < NSCHED; ++sc) {
+                for (int si2 = 0; si2 < NSCHED; ++si2) {
+                    const int sc = si2 == 0 ? cur_sched[dir] : 1 - cur_sched[dir];   // current first
                     P->sched[dir][kd] = sc;
                     eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up (sizes work buffers)
                     HIPCHK(hipEventRecord(e0, st));
@@ -1407,7 +1425,7 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
                     float ms = 0.f;
                     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
                     ms /= reps;
-                    if (best < 0.f || ms < best) {
+                    if (best < 0.f || ms < 0.98f * best) {
                         best = ms;
                         pick[dir] = sc;
                     }
@@ -1416,7 +1434,7 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
                 total += best;
                 dir_ms[dir] = best;
             }
-            if (si == 0 || total < best_total) {
+            if (best_total < 0.f || total < 0.98f * best_total) {
                 best_total = total;
                 best_variant = P->x3_variant;
                 best_pick[0] = pick[0];
