@@ -355,6 +355,60 @@ static void order_tiles(TaskLists &L, int bk) {
     L.launches.swap(lns);
 }
 
+// A task with more than maxseg segments keeps its first maxseg; the rest go
+// to continuation tasks (beta = 1, same output block and row tiles), the
+// k-th continuation in a launch of its own inserted right after the task's
+// launch, so the parts accumulate in order without write conflicts.  Used
+// for the BF16x3 kernel, which stages a task's segment list in LDS.
+static void split_long_tasks(TaskLists &L, int maxseg) {
+    std::vector<int2> tiles;
+    std::vector<Launch> launches;
+    std::vector<int> kinds;
+    std::vector<int> seen(L.tasks.size(), -1);
+    for (size_t li = 0; li < L.launches.size(); ++li) {
+        const Launch ln = L.launches[li];
+        // continuation tasks of this launch, by depth
+        std::vector<std::vector<int>> cont;
+        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) {
+            const int task = L.tiles[t].x;
+            if (seen[task] == (int)li) continue;
+            seen[task] = (int)li;
+            const Task base = L.tasks[task];
+            const int nseg = base.seg_end - base.seg_begin;
+            if (nseg <= maxseg) continue;
+            L.tasks[task].seg_end = base.seg_begin + maxseg;
+            for (int k = 1; k * maxseg < nseg; ++k) {
+                Task c = base;
+                c.seg_begin = base.seg_begin + k * maxseg;
+                c.seg_end = std::min(base.seg_end, c.seg_begin + maxseg);
+                c.beta = 1;
+                if ((int)cont.size() < k) cont.resize(k);
+                cont[k - 1].push_back((int)L.tasks.size());
+                L.tasks.push_back(c);
+                L.task_kind.push_back(L.task_kind[task]);
+            }
+        }
+        Launch nl{(int)tiles.size(), ln.ntiles};
+        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) tiles.push_back(L.tiles[t]);
+        launches.push_back(nl);
+        kinds.push_back(L.kinds[li]);
+        for (auto &group : cont) {
+            Launch cl{(int)tiles.size(), 0};
+            for (int ct : group) {
+                const int bm = L.task_kind[ct] == 1 ? X3_BM : SG_BM;
+                for (int i = 0; i < (L.tasks[ct].m + bm - 1) / bm; ++i) tiles.push_back(make_int2(ct, i));
+            }
+            cl.ntiles = (int)tiles.size() - cl.tile_offset;
+            launches.push_back(cl);
+            kinds.push_back(L.kinds[li]);
+        }
+        seen.resize(L.tasks.size(), -1);
+    }
+    L.tiles.swap(tiles);
+    L.launches.swap(launches);
+    L.kinds.swap(kinds);
+}
+
 // Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
 // in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
 // beta_u / beta_y select the accumulation rule of the coupling and leaf
@@ -924,10 +978,10 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         for (int dsc = 0; dsc < ndir * NSCHED; ++dsc) {
             const int dir = dsc / NSCHED, sc = dsc % NSCHED;
             TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, sc, P->wprow.data(), P->sprow.data());
+            split_long_tasks(X, X3_MAXSEG);
             for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
                 if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
-                    throw MxError(MX_ERR_UNSUPPORTED, "a block has more than " + std::to_string(X3_MAXSEG) +
-                                                          " interactions; the BF16x3 kernel cannot hold its list");
+                    throw MxError(MX_ERR_ARG, "internal: a BF16x3 task longer than its segment window");
                 for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
                     Seg &sg = X.segs[si];
                     const int w = sg.a_buf, m = X.tasks[ti].m;

@@ -70,3 +70,28 @@ def test_autotune_keeps_results(gpu):
         assert rel(Y.cpu().numpy(), Yo) <= (1e-12 if prec == "fp64" else 1e-5)
     with pytest.raises(hgemm.MxError):
         plan.autotune("fp32", nrhs)
+
+
+def test_long_interaction_lists_are_split(gpu):
+    """Blocks with more interactions than the BF16x3 kernel's segment window
+    (up to 459 near blocks per row leaf, 512 per column leaf here) run as a
+    task plus continuation launches; forward and adjoint still match."""
+    import torch
+    _say("[long lists]")
+    X, hm = make_problem(n=2600, d=3, ncomp=4, leaf=10, rank=6, tol=1e-6, budget=0.9, h=0.8, seed=5)
+    nx = np.bincount(hm.npairx, minlength=hm.nleaf)
+    assert nx.max() > 256
+    nrhs = 20
+    TW = host.normal((hm.N * nrhs,), seed=31)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    for prec, tol in (("fp64", 1e-12), ("bf16x3", 1e-5)):
+        Y = torch.full_like(W, float("nan"))
+        G = torch.full_like(W, float("nan"))
+        plan.eval(W, Y, nrhs, prec)
+        plan.eval_adjoint(W, G, nrhs, prec)
+        ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
+        _say(f"  {prec}: forward {ey:.2e} adjoint {eg:.2e}")
+        assert ey <= tol and eg <= tol, prec
