@@ -203,7 +203,9 @@ enum {
     MX_X3_128x128_S2 = 1, /* 4 waves, 2 stages, two workgroups per CU (default) */
     MX_X3_128x128_S4 = 2, /* 4 waves, 4 stages */
     MX_X3_128x256_S3 = 3, /* 8 waves, 3 stages */
-    MX_X3_128x128_REG = 4 /* 4 waves, fragments staged through VGPRs */
+    MX_X3_128x128_REG = 4, /* 4 waves, fragments staged through VGPRs */
+    MX_X3_128x64_S2 = 5,   /* 4 waves of 64 x 32, for small nrhs */
+    MX_X3_128x32_S2 = 6    /* 4 waves of 64 x 16, for small nrhs */
 };
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
                                int32_t x3_variant);

@@ -21,9 +21,10 @@
 //   evaluation; wskel/uskel panels are written, split, by the epilogue of the
 //   launch that produces them.
 //
-// Workgroup: 2 x 2 or 2 x 4 waves, tile 128 x 128 or 128 x 256, each wave
-// 64 x 64 = 4 x 4 fragments (48 MFMA per 32-deep chunk); fragments move
-// global -> LDS by direct LDS-DMA loads ahead of the MFMAs (see the kernel).
+// Workgroup: 2 x 2 or 2 x 4 waves, tiles of 128 rows by 32 to 256 columns,
+// each wave 64 x 64 (4 x 4 fragments, 48 MFMA per 32-deep chunk) or, for
+// small nrhs, 64 x 32 / 64 x 16; fragments move global -> LDS by direct
+// LDS-DMA loads ahead of the MFMAs (see the kernel).
 // MFMA 16x16x32 bf16 operand layout (lane l): A(row l%16, k 8*(l/16)+e),
 // B(k 8*(l/16)+e, col l%16), e = 0..7; result register r holds
 // C(4*(l/16)+r, l%16).
@@ -97,9 +98,17 @@ void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_b
 //   128 x 128, 2 stages: 4 waves, 64 KB of stages, two workgroups per CU;
 //   128 x 128, 4 stages: 4 waves, 128 KB, one workgroup per CU;
 //   128 x 256, 3 stages: 8 waves, 144 KB, one workgroup per CU;
-//   128 x 128, fragments staged through VGPRs instead of LDS-DMA.
-enum X3Variant { X3_V128x128_S2 = 1, X3_V128x128_S4 = 2, X3_V128x256_S3 = 3, X3_V128x128_REG = 4 };
-constexpr int X3_NVARIANT = 4;
+//   128 x 128, fragments staged through VGPRs instead of LDS-DMA;
+//   128 x 64 and 128 x 32, 2 stages (narrow waves, for small nrhs).
+enum X3Variant {
+    X3_V128x128_S2 = 1,
+    X3_V128x128_S4 = 2,
+    X3_V128x256_S3 = 3,
+    X3_V128x128_REG = 4,
+    X3_V128x64_S2 = 5,
+    X3_V128x32_S2 = 6
+};
+constexpr int X3_NVARIANT = 6;
 int x3_variant_bn(int variant);   // output columns of a tile
 void launch_x3(const X3Args &a, int variant, hipStream_t st);
 

@@ -90,9 +90,9 @@ struct X3Geo {
     static_assert(PER * NW == FA + FB, "fragments must divide evenly over the waves");
 };
 
-// WM x WN waves of 64 x 64
-template <int WM, int WN>
-struct X3Shape : X3Geo<64 * WM, 64 * WN, WM * WN> {};
+// WM x WN waves of 64 x (16 NJ)
+template <int WM, int WN, int NJ = 4>
+struct X3Shape : X3Geo<64 * WM, 16 * NJ * WN, WM * WN> {};
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef const __attribute__((address_space(1))) void glb_void_t;
@@ -204,66 +204,114 @@ __device__ __forceinline__ void lds_read_frags(uint32_t abase, uint32_t bbase, u
         : "memory");
 }
 
+// 8 A and 4 B fragments (waves of 64 x 32)
+__device__ __forceinline__ void lds_read_frags(uint32_t abase, uint32_t bbase, u32x4_t (&A)[8], u32x4_t (&B)[4]) {
+    asm volatile(
+        "ds_read_b128 %0, %12 offset:0\n\t"
+        "ds_read_b128 %1, %12 offset:1024\n\t"
+        "ds_read_b128 %2, %12 offset:2048\n\t"
+        "ds_read_b128 %3, %12 offset:3072\n\t"
+        "ds_read_b128 %4, %12 offset:4096\n\t"
+        "ds_read_b128 %5, %12 offset:5120\n\t"
+        "ds_read_b128 %6, %12 offset:6144\n\t"
+        "ds_read_b128 %7, %12 offset:7168\n\t"
+        "ds_read_b128 %8, %13 offset:0\n\t"
+        "ds_read_b128 %9, %13 offset:1024\n\t"
+        "ds_read_b128 %10, %13 offset:2048\n\t"
+        "ds_read_b128 %11, %13 offset:3072\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(A[6]), "=&v"(A[7]),
+          "=&v"(B[0]), "=&v"(B[1]), "=&v"(B[2]), "=&v"(B[3])
+        : "v"(abase), "v"(bbase)
+        : "memory");
+}
+
+// 8 A and 2 B fragments (waves of 64 x 16)
+__device__ __forceinline__ void lds_read_frags(uint32_t abase, uint32_t bbase, u32x4_t (&A)[8], u32x4_t (&B)[2]) {
+    asm volatile(
+        "ds_read_b128 %0, %10 offset:0\n\t"
+        "ds_read_b128 %1, %10 offset:1024\n\t"
+        "ds_read_b128 %2, %10 offset:2048\n\t"
+        "ds_read_b128 %3, %10 offset:3072\n\t"
+        "ds_read_b128 %4, %10 offset:4096\n\t"
+        "ds_read_b128 %5, %10 offset:5120\n\t"
+        "ds_read_b128 %6, %10 offset:6144\n\t"
+        "ds_read_b128 %7, %10 offset:7168\n\t"
+        "ds_read_b128 %8, %11 offset:0\n\t"
+        "ds_read_b128 %9, %11 offset:1024\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(A[4]), "=&v"(A[5]), "=&v"(A[6]), "=&v"(A[7]),
+          "=&v"(B[0]), "=&v"(B[1])
+        : "v"(abase), "v"(bbase)
+        : "memory");
+}
+
 __device__ __forceinline__ uint32_t lds_offset(const uint16_t *p) {
     return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint16_t *)p;
 }
 
-// Fragments of the current chunk from one stage, 48 MFMAs.
-__device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][4], const uint16_t *stage, int fa, int lane, int wr,
+// Fragments of the current chunk from one stage: 4 A row groups by NJ B
+// column groups, 12 NJ MFMAs.
+template <int NJ>
+__device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][NJ], const uint16_t *stage, int fa, int lane, int wr,
                                            int wc, int mtile16, int nfm) {
     const uint32_t base = lds_offset(stage) + lane * 16;
-    u32x4_t A[8], B[8];
-    lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * 8192, A, B);
-    bf16x8_t ah[4], al[4], bh[4], bl[4];
+    u32x4_t A[8], B[2 * NJ];
+    lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * (2 * NJ) * 1024, A, B);
+    bf16x8_t ah[4], al[4], bh[NJ], bl[NJ];
     const u32x4_t z = u32x4_t{0, 0, 0, 0};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const bool ok = mtile16 + wr * 4 + i < nfm;
         ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
         al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
-        bh[i] = __builtin_bit_cast(bf16x8_t, B[2 * i]);
-        bl[i] = __builtin_bit_cast(bf16x8_t, B[2 * i + 1]);
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+        bh[j] = __builtin_bit_cast(bf16x8_t, B[2 * j]);
+        bl[j] = __builtin_bit_cast(bf16x8_t, B[2 * j + 1]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NJ; ++j) {
             acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
             acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
             acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
         }
 }
 
-// Epilogue of a wave's (16 NI) x 64 accumulator block (NI row groups x 4
-// column groups of 16 x 16; the wave's rows start at m0 + wr * 16 NI, its
-// columns at n0 + wc * 64).
-//  * Y: each 64 x 64 part is staged in the wave's 16 KB of LDS (rows
+// Epilogue of a wave's (16 NI) x (16 NJ) accumulator block (NI row groups x
+// NJ column groups of 16 x 16; the wave's rows start at m0 + wr * 16 NI, its
+// columns at n0 + wc * 16 NJ).
+//  * Y: each 64 x (16 NJ) part is staged in the wave's LDS (rows
 //    XOR-swizzled by column so the 16 lanes of a write hit distinct banks)
 //    and stored as 64 consecutive rows per column (beta = 1 adds to Y).
 //  * Skeleton buffers: written as split panels straight from registers (rows
 //    up to the padded height; padding rows are 0); beta = 1 adds the panel's
 //    current value first.
-template <int NI>
-__device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][4], uint16_t *stg, const X3Args &a, const Task &task,
+template <int NI, int NJ>
+__device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg, const X3Args &a, const Task &task,
                                             int lane, int wv, int wr, int wc, int m0, int n0) {
+    constexpr int WC = 16 * NJ;   // columns of the wave
     const int M = task.m;
     if (task.c_buf == VEC_Y) {
-        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
+        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * WC;
         const int64_t base = task.c_row * (int64_t)a.nrhs;
 #pragma unroll
         for (int h = 0; h < NI / 4; ++h) {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < NJ; ++j)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + r;
                         Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[h * 4 + i][j][r];
                     }
             __builtin_amdgcn_wave_barrier();
-            const int rb = m0 + wr * 16 * NI + h * 64, cb = n0 + wc * 64;
-            for (int e = lane; e < 64 * 64; e += 64) {
+            const int rb = m0 + wr * 16 * NI + h * 64, cb = n0 + wc * WC;
+            for (int e = lane; e < 64 * WC; e += 64) {
                 const int col = e >> 6, row = e & 63;
                 const int gm = rb + row, gn = cb + col;
                 if (gm < M && gn < a.nrhs) {
@@ -289,8 +337,8 @@ __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][4], uint16_t *stg, 
             const int r0 = m0 + wr * 16 * NI + i * 16 + 4 * (lane >> 4);
             if (r0 >= Mp) continue;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int col = n0 + wc * 64 + j * 16 + (lane & 15);
+            for (int j = 0; j < NJ; ++j) {
+                const int col = n0 + wc * WC + j * 16 + (lane & 15);
                 if (col >= a.nrhs) continue;
                 const int64_t o = ((int64_t)(r0 >> 5) * a.nrhs + col) * 32 + (r0 & 31);
                 float v[4] = {acc[i][j][0], acc[i][j][1], acc[i][j][2], acc[i][j][3]};
@@ -316,31 +364,6 @@ __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][4], uint16_t *stg, 
             }
         }
     }
-}
-
-// 16 A-or-B fragment pairs of one half: the 48 MFMAs of 4 row groups (A,
-// starting at row group i0) by 4 column groups (B).
-template <int NI>
-__device__ __forceinline__ void x3_mma4(x3_f4 (&acc)[NI][4], int i0, const u32x4_t (&A)[8], const u32x4_t (&B)[8],
-                                        int rg0, int nfm) {
-    bf16x8_t ah[4], al[4], bh[4], bl[4];
-    const u32x4_t z = u32x4_t{0, 0, 0, 0};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const bool ok = rg0 + i < nfm;   // row groups past the block multiply zeros
-        ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
-        al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
-        bh[i] = __builtin_bit_cast(bf16x8_t, B[2 * i]);
-        bl[i] = __builtin_bit_cast(bf16x8_t, B[2 * i + 1]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc[i0 + i][j] = mfma_bf16(al[i], bh[j], acc[i0 + i][j]);
-            acc[i0 + i][j] = mfma_bf16(ah[i], bl[j], acc[i0 + i][j]);
-            acc[i0 + i][j] = mfma_bf16(ah[i], bh[j], acc[i0 + i][j]);
-        }
 }
 
 // Register-staged variant of the 128 x 128 kernel: fragments travel
@@ -414,22 +437,22 @@ __global__ __launch_bounds__(256) void x3r_kernel(X3Args a) {
     if (nch > 1) x3_load_regs<S>(ld, a, sseg, R, wv, lane, mtile16, nfm, n0);
     __syncthreads();
     for (int c = 0; c < nch; ++c) {
-        x3_compute(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+        x3_compute<4>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
         if (c + 1 < nch) {
             x3_store_regs<S>(R, stg + ((c + 1) % NST) * S::STAGE, wv, lane);
             if (c + 2 < nch) x3_load_regs<S>(ld, a, sseg, R, wv, lane, mtile16, nfm, n0);
         }
         __syncthreads();
     }
-    x3_epilogue<4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
+    x3_epilogue<4, 4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
 // The pipeline depth NST and the wave grid are template parameters; the
 // autotuner picks among the instantiated variants (launch_x3).
-template <int WM, int WN, int NST>
+template <int WM, int WN, int NST, int NJ = 4>
 __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
-    typedef X3Shape<WM, WN> S;
-    static_assert(S::NW * 64 * 64 * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
+    typedef X3Shape<WM, WN, NJ> S;
+    static_assert(S::NW * 64 * 16 * NJ * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
     __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
     // the task's segment list, read from LDS inside the pipeline (a vector
     // load there would wait for every outstanding LDS-DMA load)
@@ -454,11 +477,11 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     int nch = 0;   // 32-deep chunks of the task
     for (int si = 0; si < nseg; ++si) nch += (sseg[si].k + 31) / 32;
 
-    x3_f4 acc[4][4];
+    x3_f4 acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < NJ; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
 
     X3Loader<S> ld;
     ld.seg = -1;
@@ -478,14 +501,20 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
         wait_dma<S::PER>(issued - c - 1);
         wg_barrier();
         issue();
-        x3_compute(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+        x3_compute<NJ>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
     }
     __syncthreads();
-
-    x3_epilogue<4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
+    x3_epilogue<4, NJ>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
-int x3_variant_bn(int variant) { return variant == X3_V128x256_S3 ? 256 : 128; }
+int x3_variant_bn(int variant) {
+    switch (variant) {
+    case X3_V128x256_S3: return 256;
+    case X3_V128x64_S2: return 64;
+    case X3_V128x32_S2: return 32;
+    default: return 128;
+    }
+}
 
 void launch_x3(const X3Args &a, int variant, hipStream_t st) {
     int64_t total = (int64_t)a.ntiles * a.n_ntiles;
@@ -500,6 +529,12 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V128x128_REG:
         hipLaunchKernelGGL(x3r_kernel, grid, dim3(256), 0, st, a);
+        break;
+    case X3_V128x64_S2:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 2, 2>), grid, dim3(256), 0, st, a);
+        break;
+    case X3_V128x32_S2:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 2, 1>), grid, dim3(256), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);

@@ -19,7 +19,7 @@ KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4
 PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
-X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg"}
+X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
