@@ -527,6 +527,9 @@ def main():
         ms = plan.autotune(p, nrhs, "f64", reps=2)
         tuning[p] = {d: plan.get_schedule(d, p) for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
         tuning[p]["ms"] = [round(m, 4) for m in ms]
+        if p == "bf16x3":
+            tuning[p]["launch_variants"] = {d: plan.launch_variants(d)
+                                            for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
     log(f"[bench] autotuned: {tuning}")
     plan.set_graph(not args.no_graph)
     ms64 = timer(lambda: plan.eval(Wd, Y64, nrhs, precision="fp64"), max(3, args.steps // 4), 1)

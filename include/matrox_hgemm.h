@@ -205,15 +205,23 @@ enum {
     MX_X3_128x256_S3 = 3, /* 8 waves, 3 stages */
     MX_X3_128x128_REG = 4, /* 4 waves, fragments staged through VGPRs */
     MX_X3_128x64_S2 = 5,   /* 4 waves of 64 x 32, for small nrhs */
-    MX_X3_128x32_S2 = 6    /* 4 waves of 64 x 16, for small nrhs */
+    MX_X3_128x32_S2 = 6,   /* 4 waves of 64 x 16, for small nrhs */
+    MX_X3_128x128_W8S3 = 7, /* 8 waves of 64 x 32, 3 stages */
+    MX_X3_128x128_W8S4 = 8  /* 8 waves of 64 x 32, 4 stages */
 };
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
                                int32_t x3_variant);
 mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
                                int32_t *x3_variant);
 
+/* The BF16x3 kernel variant of each launch of one direction (in launch
+ * order, under the direction's current schedule). */
+mx_status mx_plan_launch_variants(const mx_plan *plan, int32_t direction, int32_t *variants, int32_t cap,
+                                  int32_t *count);
+
 /* Autotuner: times every schedule (and, for MX_BF16X3, every kernel
- * variant) of each direction built into the plan with `reps` evaluations of
+ * variant, first for the whole evaluation, then launch by launch) of each
+ * direction built into the plan with `reps` evaluations of
  * nrhs columns on scratch buffers it allocates and frees, and keeps the
  * fastest.  best_ms (optional, 2 entries) receives the chosen per-direction
  * times.  Synchronises `stream`; drops captured graphs. */

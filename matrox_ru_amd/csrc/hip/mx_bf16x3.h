@@ -99,16 +99,20 @@ void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_b
 //   128 x 128, 4 stages: 4 waves, 128 KB, one workgroup per CU;
 //   128 x 256, 3 stages: 8 waves, 144 KB, one workgroup per CU;
 //   128 x 128, fragments staged through VGPRs instead of LDS-DMA;
-//   128 x 64 and 128 x 32, 2 stages (narrow waves, for small nrhs).
+//   128 x 64 and 128 x 32, 2 stages (narrow waves, for small nrhs);
+//   128 x 128 with 8 waves of 64 x 32 and 3 or 4 stages (one workgroup per
+//   CU, two waves per SIMD, two or three chunks of prefetch).
 enum X3Variant {
     X3_V128x128_S2 = 1,
     X3_V128x128_S4 = 2,
     X3_V128x256_S3 = 3,
     X3_V128x128_REG = 4,
     X3_V128x64_S2 = 5,
-    X3_V128x32_S2 = 6
+    X3_V128x32_S2 = 6,
+    X3_V128x128_W8S3 = 7,
+    X3_V128x128_W8S4 = 8
 };
-constexpr int X3_NVARIANT = 6;
+constexpr int X3_NVARIANT = 8;
 int x3_variant_bn(int variant);   // output columns of a tile
 void launch_x3(const X3Args &a, int variant, hipStream_t st);
 

@@ -536,6 +536,12 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
     case X3_V128x32_S2:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2, 1>), grid, dim3(256), 0, st, a);
         break;
+    case X3_V128x128_W8S3:
+        hipLaunchKernelGGL((x3_kernel<2, 4, 3, 2>), grid, dim3(512), 0, st, a);
+        break;
+    case X3_V128x128_W8S4:
+        hipLaunchKernelGGL((x3_kernel<2, 4, 4, 2>), grid, dim3(512), 0, st, a);
+        break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);
         break;

@@ -102,6 +102,9 @@ struct mx_plan {
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
     int x3_variant = mx::X3_V128x128_S2;   // BF16x3 kernel variant (tile, pipeline depth)
+    // per-launch BF16x3 variants [direction][schedule] chosen by the
+    // autotuner (empty: x3_variant for every launch)
+    std::vector<int> lvar[2][mx::NSCHED];
     mx::X3Rows *d_wrows = nullptr;
     uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
     size_t panel_bytes = 0;
@@ -1241,8 +1244,7 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
         const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
         X3Args xa{};
         xa.nrhs = nrhs;
-        const int bn = x3_variant_bn(P->x3_variant);
-        xa.n_ntiles = (nrhs + bn - 1) / bn;
+        const std::vector<int> &lv = P->lvar[dir][P->sched[dir][1]];
         xa.y_f64 = io == MX_IO_F64;
         xa.Y = Y;
         xa.tasks = pg1.tasks;
@@ -1262,10 +1264,13 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
         launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
         prof_mark(P, st, MX_KIND_CONVERT, false);
         for (size_t i = 0; i < pg1.launches.size(); ++i) {
+            const int v = lv.size() == pg1.launches.size() ? lv[i] : P->x3_variant;
+            const int bn = x3_variant_bn(v);
+            xa.n_ntiles = (nrhs + bn - 1) / bn;
             xa.tiles = pg1.tiles + pg1.launches[i].tile_offset;
             xa.ntiles = pg1.launches[i].ntiles;
             prof_mark(P, st, MX_KIND_BF16X3, true);
-            launch_x3(xa, P->x3_variant, st);
+            launch_x3(xa, v, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
     }
@@ -1417,7 +1422,11 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     const int kd = prec == MX_BF16X3 ? 1 : 0;
     P->sched[dir][kd] = sched;
-    if (variant) P->x3_variant = variant;
+    if (variant) {
+        P->x3_variant = variant;
+        for (auto &d : P->lvar)
+            for (auto &v : d) v.clear();
+    }
     drop_graphs(P);
 }
 
@@ -1500,6 +1509,41 @@ static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int
         if (prec == MX_BF16X3) P->x3_variant = best_variant;
         P->sched[0][kd] = best_pick[0];
         P->sched[1][kd] = best_pick[1];
+        if (prec == MX_BF16X3) {
+            // per launch: the deep top-of-tree launches and the wide leaf
+            // launches favour different variants
+            P->profiling = 1;
+            for (int dir = 0; dir < ndir; ++dir) {
+                const int sc = P->sched[dir][1];
+                const size_t nl = P->prog[dir][1][sc].launches.size();
+                std::vector<int> &lv = P->lvar[dir][sc];
+                std::vector<int> pick(nl, P->x3_variant);
+                std::vector<float> best(nl, 0.f);
+                for (int vi = 0; vi < X3_NVARIANT; ++vi) {
+                    const int v = vi == 0 ? P->x3_variant : (vi <= P->x3_variant - 1 ? vi : vi + 1);
+                    lv.assign(nl, v);
+                    eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
+                    std::vector<float> t(nl, 0.f);
+                    for (int r = 0; r < reps; ++r) {
+                        eval(P, dir, prec, W, Y, nrhs, io, st);
+                        HIPCHK(hipEventSynchronize(P->ev[2 * P->ev_used - 1]));
+                        // event pairs: the W split, then one per BF16x3 launch
+                        for (size_t i = 0; i < nl && (int)i + 1 < P->ev_used; ++i) {
+                            float ms = 0.f;
+                            HIPCHK(hipEventElapsedTime(&ms, P->ev[2 * (i + 1)], P->ev[2 * (i + 1) + 1]));
+                            t[i] += ms;
+                        }
+                    }
+                    for (size_t i = 0; i < nl; ++i)
+                        if (vi == 0 || t[i] < 0.97f * best[i]) {
+                            best[i] = t[i];
+                            pick[i] = v;
+                        }
+                }
+                lv = pick;
+            }
+            P->profiling = 0;
+        }
         if (best_ms) {
             best_ms[0] = best_dir[0];
             best_ms[1] = best_dir[1];
@@ -2341,6 +2385,21 @@ mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precis
         arg_check(plan && schedule && x3_variant && (direction == 0 || direction == 1), "bad argument");
         *schedule = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
         *x3_variant = plan->x3_variant;
+        return 0;
+    });
+}
+
+mx_status mx_plan_launch_variants(const mx_plan *plan, int32_t direction, int32_t *variants, int32_t cap,
+                                  int32_t *count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && count && (direction == 0 || direction == 1) && cap >= 0 && (cap == 0 || variants),
+                  "bad argument");
+        const int sc = plan->sched[direction][1];
+        const std::vector<Launch> &ln = plan->prog[direction][1][sc].launches;
+        const std::vector<int> &lv = plan->lvar[direction][sc];
+        *count = (int32_t)ln.size();
+        for (int32_t i = 0; i < (int32_t)ln.size() && i < cap; ++i)
+            variants[i] = lv.size() == ln.size() ? lv[i] : plan->x3_variant;
         return 0;
     });
 }

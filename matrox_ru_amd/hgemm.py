@@ -19,7 +19,8 @@ KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4
 PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
-X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2"}
+X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
+               7: "128x128_w8s3", 8: "128x128_w8s4"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
@@ -81,6 +82,8 @@ def lib():
         L.mx_plan_get_schedule.restype = ctypes.c_int
         L.mx_plan_autotune.argtypes = [P, ctypes.c_int, I32, ctypes.c_int, I32, P, P]
         L.mx_plan_autotune.restype = ctypes.c_int
+        L.mx_plan_launch_variants.argtypes = [P, I32, P, I32, ctypes.POINTER(I32)]
+        L.mx_plan_launch_variants.restype = ctypes.c_int
         L.mx_batch_create.argtypes = [ctypes.POINTER(P), P, P, I32, ctypes.c_uint32]
         L.mx_batch_create.restype = ctypes.c_int
         L.mx_batch_destroy.argtypes = [P]
@@ -293,6 +296,14 @@ class Plan:
         _check(lib().mx_plan_autotune(self._h, PRECISIONS[precision], nrhs, IO_F64 if io == "f64" else IO_F32, reps,
                                       ms, _stream_handle(stream)))
         return [ms[0], ms[1]] if self.adjoint else [ms[0]]
+
+    def launch_variants(self, direction="forward"):
+        """BF16x3 kernel variant of each launch (mx_plan_launch_variants)."""
+        cap = 4096
+        v = (I32 * cap)()
+        cnt = I32()
+        _check(lib().mx_plan_launch_variants(self._h, DIRECTIONS[direction], v, cap, ctypes.byref(cnt)))
+        return [v[i] for i in range(min(cnt.value, cap))]
 
     def grad_flops(self, nrhs):
         f = ctypes.c_double()

@@ -37,7 +37,7 @@ def test_schedules_match_oracle(gpu, case, schedule):
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
     for prec, stages in (("fp64", 0), ("fp32", 0), ("bf16x3", 1), ("bf16x3", 2), ("bf16x3", 3), ("bf16x3", 4),
-                          ("bf16x3", 5), ("bf16x3", 6)):
+                          ("bf16x3", 5), ("bf16x3", 6), ("bf16x3", 7), ("bf16x3", 8)):
         for direction in ("forward", "adjoint"):
             plan.set_schedule(direction, prec, schedule, stages)
             assert plan.get_schedule(direction, prec)[0] == schedule
@@ -65,7 +65,10 @@ def test_autotune_keeps_results(gpu):
         assert len(ms) == 2 and all(m > 0 for m in ms)
         for d in ("forward", "adjoint"):
             sc, st = plan.get_schedule(d, prec)
-            assert sc in ("fused", "split") and st in (1, 2, 3, 4, 5, 6)
+            assert sc in ("fused", "split") and st in range(1, 9)
+        if prec == "bf16x3":
+            lv = plan.launch_variants("forward")
+            assert len(lv) >= 3 and all(1 <= v <= 8 for v in lv)
         Y = torch.empty_like(W)
         plan.eval(W, Y, nrhs, prec)
         assert rel(Y.cpu().numpy(), Yo) <= (1e-12 if prec == "fp64" else 1e-5)
