@@ -219,7 +219,7 @@ def adjoint_timings(plan, timer, Wd, nrhs, prec, ms_fwd_mixed, ms_fwd_64, steps,
                 "adjoint_fp64_ms": round(ms_adj64, 4), "adjoint_fp64_ratio_vs_forward": round(ms_adj64 / ms_fwd_64, 3)})
     # backward with block gradients: forward skeletons from a forward evaluation
     grad_flops = plan.grad_flops(nrhs)
-    for gp in ("fp32", "fp64"):
+    for gp in ("bf16x3", "fp32", "fp64"):
         if gp not in plan.precisions:
             continue
         plan.eval(Wd, Y, nrhs, gp)

@@ -153,10 +153,11 @@ mx_status mx_eval_adjoint(mx_plan *plan, mx_precision prec, const void *dY, void
  * where w^, u^ are the skeletons of the forward evaluation of W and
  * y^ = V^T-upward pass of dY, g^ = dL/dw^ those of the adjoint evaluation.
  * wskel / uskel: the forward skeletons as mx_plan_copy_skeletons writes them
- * (double for MX_FP64, float for MX_FP32); NULL takes the plan's own, valid
- * when the plan's last forward evaluation was of this W in this precision.
- * prec is MX_FP64 (FP64 I/O) or MX_FP32 (either I/O type).  Needs
- * MX_PLAN_ADJOINT. */
+ * (double for MX_FP64, float for MX_FP32 and MX_BF16X3); NULL takes the
+ * plan's own, valid when the plan's last forward evaluation was of this W in
+ * this precision.  prec is MX_FP64 (FP64 I/O), MX_FP32 or MX_BF16X3 (either
+ * I/O type; with MX_BF16X3 the evaluations are split-BF16x3 and the block
+ * products FP32).  Needs MX_PLAN_ADJOINT. */
 mx_status mx_backward(mx_plan *plan, mx_precision prec, const void *W, const void *dY, void *dW, int32_t nrhs,
                       mx_io_type io, const void *wskel, const void *uskel, void *dD, void *dB, void *dVT,
                       void *stream);
@@ -180,7 +181,9 @@ mx_status mx_plan_grad_flops(const mx_plan *plan, int32_t nrhs, double *flops);
 mx_status mx_plan_skeletons(mx_plan *plan, const void **wskel, const void **uskel);
 
 /* Copy the skeleton buffers of the last evaluation (skel_rows x nrhs
- * elements each, layout as above) into caller device buffers on `stream`. */
+ * elements each, layout as above) into caller device buffers on `stream`.
+ * After an MX_BF16X3 evaluation the copies are FP32 (hi + lo of the split
+ * panels). */
 mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst, int32_t nrhs, void *stream);
 
 /* Replay evaluations from captured hipGraphs: with enable != 0, the first

@@ -11,7 +11,7 @@ matroxTest-V1/hcds.h:12-34 (D/B/VT, Dptr/Bptr/VTptr).
 """
 import torch
 
-_SKEL_DTYPE = {"fp64": torch.float64, "fp32": torch.float32}
+_SKEL_DTYPE = {"fp64": torch.float64, "fp32": torch.float32, "bf16x3": torch.float32}
 
 
 class HGEMMFunction(torch.autograd.Function):
@@ -26,8 +26,6 @@ class HGEMMFunction(torch.autograd.Function):
         need_blocks = tuple(bool(ctx.needs_input_grad[i]) for i in (1, 2, 3))
         ws = us = None
         if any(need_blocks):
-            if precision not in _SKEL_DTYPE:
-                raise RuntimeError("block gradients need an fp64 or fp32 forward evaluation")
             ws, us = plan.skeletons(nrhs, _SKEL_DTYPE[precision])
         ctx.plan, ctx.nrhs, ctx.precision, ctx.need_blocks = plan, nrhs, precision, need_blocks
         ctx.blocks = blocks

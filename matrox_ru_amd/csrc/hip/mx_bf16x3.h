@@ -94,6 +94,11 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
 void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nrows_blocks, int nrhs, uint16_t *hi,
                        uint16_t *lo, hipStream_t st);
 
+// Skeleton panels back to FP32 in the reference layout (block b: R rows at
+// panel row prow, written at src_row*nrhs, leading dimension R).
+void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Rows *blocks, int nblocks, int nrhs,
+                             float *out, hipStream_t st);
+
 // Kernel variants (tile rows x columns, pipeline stages):
 //   128 x 128, 2 stages: 4 waves, 64 KB of stages, two workgroups per CU;
 //   128 x 128, 4 stages: 4 waves, 128 KB, one workgroup per CU;

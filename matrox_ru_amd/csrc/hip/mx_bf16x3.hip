@@ -1,6 +1,8 @@
 // Split-BF16 x3 kernels (layouts and arithmetic described in mx_bf16x3.h).
 #include "mx_bf16x3.h"
 
+#include <algorithm>
+
 namespace mx {
 
 __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__restrict__ blocks,
@@ -55,6 +57,28 @@ void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nblocks
         hipLaunchKernelGGL(x3_split_w_kernel<double>, grid, dim3(256), 0, st, (const double *)W, rows, nrhs, hi, lo);
     else
         hipLaunchKernelGGL(x3_split_w_kernel<float>, grid, dim3(256), 0, st, (const float *)W, rows, nrhs, hi, lo);
+}
+
+// Split panels back to FP32 blocks in the reference skeleton layout: block b
+// (R rows at panel row prow) -> out[src_row*nrhs + col*R + k] = hi + lo.
+__global__ void x3_panels_to_f32_kernel(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
+                                        const X3Rows *__restrict__ blocks, int nrhs, float *__restrict__ out) {
+    const X3Rows b = blocks[blockIdx.y];
+    const int64_t n = (int64_t)b.R * nrhs;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int col = (int)(t / b.R), k = (int)(t % b.R);
+        const int64_t o = b.prow * nrhs + ((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31);
+        out[b.src_row * nrhs + (int64_t)col * b.R + k] = bf16_f(hi[o]) + bf16_f(lo[o]);
+    }
+}
+
+void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Rows *blocks, int nblocks, int nrhs,
+                             float *out, hipStream_t st) {
+    for (int b0 = 0; b0 < nblocks; b0 += 65535) {
+        const int nb = std::min(65535, nblocks - b0);
+        hipLaunchKernelGGL(x3_panels_to_f32_kernel, dim3(64, (unsigned)nb), dim3(256), 0, st, hi, lo, blocks + b0,
+                           nrhs, out);
+    }
 }
 
 __device__ __forceinline__ bf16x8_t ld16(const uint16_t *p) {

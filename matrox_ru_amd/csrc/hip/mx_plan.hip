@@ -106,6 +106,9 @@ struct mx_plan {
     // autotuner (empty: x3_variant for every launch)
     std::vector<int> lvar[2][mx::NSCHED];
     mx::X3Rows *d_wrows = nullptr;
+    mx::X3Rows *d_srows = nullptr;      // skeleton blocks (slen > 0) for panel export
+    int n_srows = 0;
+    int panels_dir = -1, panels_nrhs = 0;   // the evaluation the skeleton panels hold
     uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
     size_t panel_bytes = 0;
 
@@ -168,7 +171,7 @@ struct mx_plan {
                 for (auto &ps : pk) ps.release();
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
-                        d_wrows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
+                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
                         wskel_adj, uskel_adj, io_g};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -973,6 +976,11 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         std::vector<X3Rows> wr(s.nleaf);
         for (int l = 0; l < s.nleaf; ++l) wr[l] = X3Rows{s.wrow[l], P->wprow[l], s.Ddim[l], 0};
         P->d_wrows = upload(wr.data(), wr.size());
+        std::vector<X3Rows> sr;
+        for (int v = 0; v < s.numnodes; ++v)
+            if (s.slen[v] > 0) sr.push_back(X3Rows{s.skelrow[v], P->sprow[v], s.slen[v], 0});
+        P->d_srows = upload(sr.data(), sr.size());
+        P->n_srows = (int)sr.size();
         // every A operand is read from packed split-BF16 fragments: assign each
         // distinct (buffer, offset, orientation, shape) its packed offset,
         // shared by the forward and adjoint lists
@@ -1273,9 +1281,23 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
             launch_x3(xa, v, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
+        P->panels_dir = dir;
+        P->panels_nrhs = nrhs;
     }
     HIPCHK(hipGetLastError());
     if (dir == DIR_FWD) P->last_prec = prec;
+}
+
+// FP32 copies of the skeleton panels of the last BF16x3 evaluation (of
+// direction dir, nrhs columns) in the reference skeleton layout.
+static void export_panels(mx_plan *P, int dir, int nrhs, float *wdst, float *udst, hipStream_t st) {
+    if (P->panels_dir != dir || P->panels_nrhs != nrhs)
+        throw MxError(MX_ERR_STATE, "the skeleton panels do not hold that evaluation");
+    const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
+    uint16_t *pw = P->panels;
+    launch_x3_panels_to_f32(pw + 2 * wp, pw + 2 * wp + sp, P->d_srows, P->n_srows, nrhs, wdst, st);
+    launch_x3_panels_to_f32(pw + 2 * wp + 2 * sp, pw + 2 * wp + 3 * sp, P->d_srows, P->n_srows, nrhs, udst, st);
+    HIPCHK(hipGetLastError());
 }
 
 // Backward pass of Y = K~ W for the upstream gradient dY: dW = K~^T dY by the
@@ -1285,16 +1307,22 @@ static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y,
 static void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
                      const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
     arg_check(W && dY && dW, "null W, dY or dW");
-    arg_check(prec == MX_FP64 || prec == MX_FP32, "block gradients are computed in FP64 or FP32");
+    arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
     arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
     if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 backward needs FP64 I/O");
     if (!P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+    if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const bool grads = dD || dB || dVT;
+    const bool x3 = prec == MX_BF16X3;   // BF16x3 evaluations, FP32 gradient products
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
     if (grads && (!wskel || !uskel)) {
         if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
-        if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes)
+        if (x3) {   // the forward skeletons are split panels: FP32 copies before the adjoint reuses the panels
+            ensure_work(P, nrhs, sizeof(float), false);
+            export_panels(P, DIR_FWD, nrhs, (float *)P->wskel, (float *)P->uskel, st);
+        } else if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes) {
             throw MxError(MX_ERR_STATE, "the last forward evaluation had fewer columns");
+        }
         wskel = P->wskel;
         uskel = P->uskel;
     }
@@ -1310,6 +1338,14 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
     if (!grads) return;
     const int64_t nel = P->N * (int64_t)nrhs;
     const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : MX_KIND_GRAD_F32;
+    if (x3) {   // adjoint skeletons to FP32; W and dY to FP32 when the I/O is FP64
+        ensure_adj_work(P, nrhs, sizeof(float));
+        export_panels(P, DIR_ADJ, nrhs, (float *)P->wskel_adj, (float *)P->uskel_adj, st);
+        if (io == MX_IO_F64) {
+            ensure_work(P, nrhs, sizeof(float), true);
+            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)dY, (float *)P->io_w, nel);
+        }
+    }
     if (prec == MX_FP64) {
         GradArgs<double, double> g{};
         g.tasks = P->g_tasks;
@@ -1331,7 +1367,7 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
         prof_mark(P, st, kind, false);
     } else {
         const float *Wf = (const float *)W, *dYf = (const float *)dY;
-        if (io == MX_IO_F64) {   // the adjoint left dY (FP32) in io_w; W goes to io_g
+        if (io == MX_IO_F64) {   // dY (FP32) is in io_w (left by the FP32 adjoint); W goes to io_g
             ensure_io_g(P, nrhs);
             prof_mark(P, st, MX_KIND_CONVERT, true);
             cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)P->io_g, nel);
@@ -2329,8 +2365,10 @@ mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst
     return (mx_status)guard([&]() {
         arg_check(plan && wskel_dst && uskel_dst && nrhs >= 1, "bad argument");
         if (plan->last_prec < 0) throw MxError(MX_ERR_STATE, "no evaluation has run on this plan");
-        if (plan->last_prec == MX_BF16X3)
-            throw MxError(MX_ERR_UNSUPPORTED, "split-BF16x3 evaluations keep skeletons as split panels");
+        if (plan->last_prec == MX_BF16X3) {   // FP32 copies of the split panels
+            export_panels(plan, DIR_FWD, nrhs, (float *)wskel_dst, (float *)uskel_dst, (hipStream_t)stream);
+            return 0;
+        }
         size_t elem = plan->last_prec == MX_FP64 ? sizeof(double) : sizeof(float);
         size_t bytes = (size_t)plan->skel_rows * nrhs * elem;
         arg_check(bytes <= plan->skel_bytes, "nrhs exceeds the last evaluation's buffers");

@@ -281,13 +281,47 @@ def test_backward_errors(gpu):
         plain.backward(W, W, nrhs, "fp64")
     assert e.value.code == -4
     plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
-    with pytest.raises(hgemm.MxError) as e:
-        plan.backward(W, W, nrhs, "bf16x3")
-    assert e.value.code == -1
     with pytest.raises(hgemm.MxError) as e:   # no forward evaluation to take skeletons from
+        plan.backward(W, W, nrhs, "bf16x3")
+    assert e.value.code == -3
+    with pytest.raises(hgemm.MxError) as e:
         plan.backward(W, W, nrhs, "fp64")
     assert e.value.code == -3
+    with pytest.raises(hgemm.MxError) as e:   # not built into the plan
+        plan.backward(W, W, nrhs, "fp32")
+    assert e.value.code == -4
     D_host = np.ascontiguousarray(hm.D)
     import ctypes
     rc = hgemm.lib().mx_plan_set_blocks(plan._h, D_host.ctypes.data, None, None, None)
     assert rc == -1
+
+
+@pytest.mark.parametrize("io", ["f64", "f32"])
+def test_bf16x3_backward_close_to_fp64(gpu, io):
+    """Split-BF16x3 forward and adjoint with FP32 block products: gradients
+    within 5e-5 of the FP64 ones (the skeletons carry ~1e-5 relative error
+    from 16-bit split operands), and the FP32 skeleton export of a BF16x3
+    evaluation against the FP64 skeletons."""
+    import torch
+    _say(f"[grad bf16x3] io={io}")
+    X, hm = make_problem(**CASES["h2b_ragged"])
+    nrhs = 48
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
+    dt = torch.float64 if io == "f64" else torch.float32
+    W = _dev(host.normal((hm.N * nrhs,), seed=161)).to(dt)
+    dY = _dev(host.normal((hm.N * nrhs,), seed=162)).to(dt)
+    Y64 = torch.empty_like(W.double())
+    plan.eval(W.double(), Y64, nrhs, "fp64")
+    ws64, us64 = plan.skeletons(nrhs, torch.float64)
+    ref = plan.backward(W.double(), dY.double(), nrhs, "fp64")
+    Y = torch.empty_like(W)
+    plan.eval(W, Y, nrhs, "bf16x3")
+    ws, us = plan.skeletons(nrhs, torch.float32)
+    assert rel(ws.double().cpu().numpy(), ws64.cpu().numpy()) <= 2e-5
+    assert rel(us.double().cpu().numpy(), us64.cpu().numpy()) <= 2e-5
+    got = plan.backward(W, dY, nrhs, "bf16x3")
+    for a, b, nm in zip(got, ref, ["dW", "dD", "dB", "dVT"]):
+        assert a.dtype == dt
+        e = rel(a.double().cpu().numpy(), b.cpu().numpy())
+        _say(f"  {nm}: {e:.2e}")
+        assert e <= 5e-5, nm
