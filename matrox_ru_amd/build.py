@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
 HOST_SRC = [os.path.join(CSRC, "host", "hstruct.cpp")]
-HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("mx_plan.hip", "mx_bf16x3.hip", "mx_grad.hip")]
+HIP_SRC = [os.path.join(CSRC, "hip", f) for f in ("mx_plan.hip", "mx_lists.hip", "mx_generate.hip", "mx_batch.hip", "mx_bf16x3.hip", "mx_grad.hip")]
 
 # Packed FP32 instructions are disabled in all device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

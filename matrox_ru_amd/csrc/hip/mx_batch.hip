@@ -1,0 +1,462 @@
+// Many independent instances evaluated together (mx_batch, C-ABI in
+// include/matrox_hgemm.h).
+#include "mx_internal.h"
+
+namespace mx {
+
+// ---- many independent instances (mx_batch) -------------------------------
+//
+// Instances are grouped by nrhs.  The instances of a group become one forest
+// joined under rank-0 nodes into a single binary tree: a dummy node has no
+// skeleton, so it takes part in no product, and every launch of the group's
+// plan covers the same tree level (or phase) of all its instances at once.
+// HSS instances are rewritten with explicit pair lists (near (l, l) at
+// Dptr[l], coupling (v, sibling) at Bptr[v-1]).  The groups run concurrently
+// on their own streams, forked from and joined back to the caller's stream,
+// and the whole batch can be captured as one hipGraph.
+
+struct BatchInst {
+    int group;
+    int64_t row_off;                 // first row in the group's W / Y
+    int64_t N;
+    int nleaf;
+    std::vector<int32_t> wrow, Ddim; // instance-local leaf layout
+    double *X_leaf = nullptr;        // points for dense check rows
+    int pts_d = 0;
+    double pts_h = 0.0;
+};
+
+// Combined structure of a group; owns its arrays.
+struct GroupStructure {
+    std::vector<int32_t> lc, rc, lm, Ddim, slen, npx, npy, fpx, fpy, VTptr, wrow, skelrow;
+    std::vector<uint64_t> Dptr, Bptr;
+    std::vector<double> D, B, VT;
+    int64_t N = 0;
+    mx_structure view() {
+        mx_structure s{};
+        s.nleaf = (int32_t)Ddim.size();
+        s.numnodes = (int32_t)lc.size();
+        s.is_hss = 0;
+        s.ncount = (int64_t)npx.size();
+        s.fcount = (int64_t)fpx.size();
+        s.lchildren = lc.data();
+        s.rchildren = rc.data();
+        s.lm = lm.data();
+        s.Ddim = Ddim.data();
+        s.slen = slen.data();
+        s.npairx = npx.data();
+        s.npairy = npy.data();
+        s.fpairx = fpx.data();
+        s.fpairy = fpy.data();
+        s.Dptr = Dptr.data();
+        s.Bptr = Bptr.data();
+        s.VTptr = VTptr.data();
+        s.wrow = wrow.data();
+        s.skelrow = skelrow.data();
+        s.D = D.data();
+        s.B = B.data();
+        s.VT = VT.data();
+        s.D_len = (int64_t)D.size();
+        s.B_len = (int64_t)B.size();
+        s.VT_len = (int64_t)VT.size();
+        return s;
+    }
+};
+
+static int64_t struct_rows(const mx_structure &s) {
+    int64_t n = 0;
+    for (int l = 0; l < s.nleaf; ++l) n = std::max<int64_t>(n, (int64_t)s.wrow[l] + s.Ddim[l]);
+    return n;
+}
+
+static void combine(const std::vector<const mx_structure *> &in, GroupStructure &G) {
+    const int k = (int)in.size();
+    int64_t nn = k - 1;
+    for (auto *s : in) nn += s->numnodes;
+    G.lc.assign(nn, -1);
+    G.rc.assign(nn, -1);
+    G.lm.assign(nn, -1);
+    G.slen.assign(nn, 0);
+    G.skelrow.assign(nn, 0);
+    G.VTptr.assign(nn, 0);
+    std::vector<int64_t> nbase(k), lbase(k), rbase(k), sbase(k), vbase(k), dbase(k), bbase(k);
+    int64_t nb = k - 1, lb = 0, rb = 0, sb = 0, vb = 0, db = 0, bb = 0;
+    for (int i = 0; i < k; ++i) {
+        const mx_structure &s = *in[i];
+        nbase[i] = nb;
+        lbase[i] = lb;
+        rbase[i] = rb;
+        sbase[i] = sb;
+        vbase[i] = vb;
+        dbase[i] = db;
+        bbase[i] = bb;
+        nb += s.numnodes;
+        lb += s.nleaf;
+        rb += struct_rows(s);
+        int64_t sk = 0;
+        for (int v = 0; v < s.numnodes; ++v) sk = std::max<int64_t>(sk, (int64_t)s.skelrow[v] + s.slen[v]);
+        sb += sk;
+        vb += s.VT_len;
+        db += s.D_len;
+        bb += s.B_len;
+    }
+    arg_check(rb < INT32_MAX && sb < INT32_MAX && vb < INT32_MAX, "batch group too large for 32-bit offsets");
+    G.N = rb;
+    // dummy nodes 0..k-2 in heap order; child ids >= k-1 are instance roots
+    for (int j = 0; j < k - 1; ++j) {
+        int c[2] = {2 * j + 1, 2 * j + 2};
+        int64_t id[2];
+        for (int t = 0; t < 2; ++t) id[t] = c[t] < k - 1 ? c[t] : nbase[c[t] - (k - 1)];
+        G.lc[j] = (int32_t)id[0];
+        G.rc[j] = (int32_t)id[1];
+    }
+    G.Ddim.reserve(lb);
+    G.wrow.reserve(lb);
+    G.D.reserve(db);
+    G.B.reserve(bb);
+    G.VT.reserve(vb);
+    for (int i = 0; i < k; ++i) {
+        const mx_structure &s = *in[i];
+        const int64_t o = nbase[i];
+        std::vector<int> parent(s.numnodes, -1);
+        for (int v = 0; v < s.numnodes; ++v) {
+            if (s.lchildren[v] >= 0) {
+                parent[s.lchildren[v]] = v;
+                parent[s.rchildren[v]] = v;
+                G.lc[o + v] = (int32_t)(s.lchildren[v] + o);
+                G.rc[o + v] = (int32_t)(s.rchildren[v] + o);
+            } else {
+                G.lm[o + v] = (int32_t)(s.lm[v] + lbase[i]);
+            }
+            G.slen[o + v] = s.slen[v];
+            G.skelrow[o + v] = (int32_t)(s.skelrow[v] + sbase[i]);
+            G.VTptr[o + v] = (int32_t)(s.VTptr[v] + vbase[i]);
+        }
+        for (int l = 0; l < s.nleaf; ++l) {
+            G.Ddim.push_back(s.Ddim[l]);
+            G.wrow.push_back((int32_t)(s.wrow[l] + rbase[i]));
+        }
+        for (int64_t q = 0; q < s.ncount; ++q) {
+            G.npx.push_back((int32_t)((s.is_hss ? q : s.npairx[q]) + lbase[i]));
+            G.npy.push_back((int32_t)((s.is_hss ? q : s.npairy[q]) + lbase[i]));
+            G.Dptr.push_back(s.Dptr[q] + (uint64_t)dbase[i]);
+        }
+        for (int64_t q = 0; q < s.fcount; ++q) {
+            int x, y;
+            if (s.is_hss) {
+                x = (int)q + 1;
+                arg_check(x < s.numnodes && parent[x] >= 0, "HSS coupling list longer than the tree");
+                int p = parent[x];
+                y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+            } else {
+                x = s.fpairx[q];
+                y = s.fpairy[q];
+            }
+            G.fpx.push_back((int32_t)(x + o));
+            G.fpy.push_back((int32_t)(y + o));
+            G.Bptr.push_back(s.Bptr[q] + (uint64_t)bbase[i]);
+        }
+        if (s.D_len) G.D.insert(G.D.end(), s.D, s.D + s.D_len);
+        if (s.B_len) G.B.insert(G.B.end(), s.B, s.B + s.B_len);
+        if (s.VT_len) G.VT.insert(G.VT.end(), s.VT, s.VT + s.VT_len);
+    }
+}
+
+}  // namespace mx
+
+struct mx_batch {
+    std::vector<mx_plan *> plans;          // one per group
+    std::vector<int> group_nrhs;
+    std::vector<int64_t> group_rows;
+    std::vector<hipStream_t> streams;      // one per group (when there are several)
+    std::vector<hipEvent_t> events;        // fork + one join event per group
+    std::vector<mx::BatchInst> inst;
+    int use_graph = 0;
+    struct GraphEntry {
+        int prec, io;
+        std::vector<const void *> W;
+        std::vector<void *> Y;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    std::vector<GraphEntry> graphs;
+    uint64_t clock = 0;
+    ~mx_batch() {
+        for (auto *p : plans) delete p;
+        for (auto st : streams) (void)hipStreamDestroy(st);
+        for (auto e : events) (void)hipEventDestroy(e);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+        for (auto &i : inst)
+            if (i.X_leaf) (void)hipFree(i.X_leaf);
+    }
+};
+
+namespace mx {
+
+static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs, int count, uint32_t precisions) {
+    arg_check(in && nrhs && count >= 1, "bad batch arguments");
+    std::unique_ptr<mx_batch> B(new mx_batch());
+    std::map<int, std::vector<int>> by_nrhs;
+    for (int i = 0; i < count; ++i) {
+        arg_check(in[i] != nullptr && nrhs[i] >= 1, "instance " + std::to_string(i) + ": null structure or bad nrhs");
+        by_nrhs[nrhs[i]].push_back(i);
+    }
+    B->inst.resize(count);
+    for (auto &kv : by_nrhs) {
+        const int g = (int)B->plans.size();
+        std::vector<const mx_structure *> members;
+        int64_t row = 0;
+        for (int i : kv.second) {
+            const mx_structure &s = *in[i];
+            arg_check(s.nleaf > 0 && s.numnodes == 2 * s.nleaf - 1 && s.wrow && s.Ddim,
+                      "instance " + std::to_string(i) + ": bad tree");
+            BatchInst &bi = B->inst[i];
+            bi.group = g;
+            bi.row_off = row;
+            bi.N = struct_rows(s);
+            bi.nleaf = s.nleaf;
+            bi.wrow.assign(s.wrow, s.wrow + s.nleaf);
+            bi.Ddim.assign(s.Ddim, s.Ddim + s.nleaf);
+            row += bi.N;
+            members.push_back(&s);
+        }
+        GroupStructure G;
+        combine(members, G);
+        mx_structure v = G.view();
+        B->plans.push_back(create_plan(v, precisions, 0, 0));
+        B->group_nrhs.push_back(kv.first);
+        B->group_rows.push_back(G.N);
+    }
+    const int ng = (int)B->plans.size();
+    if (ng > 1) {
+        B->streams.resize(ng, nullptr);
+        for (auto &st : B->streams) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        B->events.resize(ng + 1, nullptr);
+        for (auto &e : B->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    return B.release();
+}
+
+// all groups, forked from and joined back to st
+static void batch_eval(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+                       hipStream_t st) {
+    const int ng = (int)B->plans.size();
+    for (int g = 0; g < ng; ++g) arg_check(W[g] && Y[g], "null W or Y of group " + std::to_string(g));
+    if (ng == 1) {
+        eval(B->plans[0], DIR_FWD, prec, W[0], Y[0], B->group_nrhs[0], io, st);
+        return;
+    }
+    HIPCHK(hipEventRecord(B->events[0], st));
+    for (int g = 0; g < ng; ++g) {
+        HIPCHK(hipStreamWaitEvent(B->streams[g], B->events[0], 0));
+        eval(B->plans[g], DIR_FWD, prec, W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
+        HIPCHK(hipEventRecord(B->events[g + 1], B->streams[g]));
+    }
+    for (int g = 0; g < ng; ++g) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
+}
+
+static void batch_size_work(mx_batch *B, mx_precision prec, mx_io_type io) {
+    for (size_t g = 0; g < B->plans.size(); ++g) {
+        mx_plan *P = B->plans[g];
+        if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
+        if (prec == MX_BF16X3) ensure_panels(P, B->group_nrhs[g]);
+        else ensure_work(P, B->group_nrhs[g], prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
+    }
+}
+
+static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+                             hipStream_t st) {
+    const int ng = (int)B->plans.size();
+    for (auto &g : B->graphs) {
+        bool same = g.prec == prec && g.io == io;
+        for (int i = 0; same && i < ng; ++i) same = g.W[i] == W[i] && g.Y[i] == Y[i];
+        if (same) {
+            g.used = ++B->clock;
+            HIPCHK(hipGraphLaunch(g.exec, st));
+            return;
+        }
+    }
+    batch_size_work(B, prec, io);
+    for (auto *P : B->plans) P->profiling = 0;
+    hipStream_t cs = st;
+    bool own = false;
+    if (cs == nullptr) {
+        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        own = true;
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    try {
+        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        try {
+            batch_eval(B, prec, W, Y, io, cs);
+        } catch (...) {
+            hipGraph_t g2 = nullptr;
+            (void)hipStreamEndCapture(cs, &g2);
+            if (g2) (void)hipGraphDestroy(g2);
+            throw;
+        }
+        HIPCHK(hipStreamEndCapture(cs, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        graph = nullptr;
+    } catch (...) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own) (void)hipStreamDestroy(cs);
+        throw;
+    }
+    if (own) HIPCHK(hipStreamDestroy(cs));
+    if (B->graphs.size() >= 4) {
+        auto lru = std::min_element(B->graphs.begin(), B->graphs.end(),
+                                    [](const mx_batch::GraphEntry &a, const mx_batch::GraphEntry &b) {
+                                        return a.used < b.used;
+                                    });
+        HIPCHK(hipGraphExecDestroy(lru->exec));
+        B->graphs.erase(lru);
+    }
+    mx_batch::GraphEntry e;
+    e.prec = prec;
+    e.io = io;
+    e.W.assign(W, W + ng);
+    e.Y.assign(Y, Y + ng);
+    e.exec = exec;
+    e.used = ++B->clock;
+    B->graphs.push_back(e);
+    HIPCHK(hipGraphLaunch(exec, st));
+}
+
+static void batch_drop_graphs(mx_batch *B) {
+    for (auto &g : B->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
+    B->graphs.clear();
+}
+
+}  // namespace mx
+
+using namespace mx;
+
+extern "C" {
+
+mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances, const int32_t *nrhs, int32_t count,
+                          uint32_t precisions) {
+    return (mx_status)guard([&]() {
+        arg_check(batch != nullptr, "null argument");
+        *batch = nullptr;
+        arg_check((precisions & MX_PLAN_ADJOINT) == 0, "batches are forward only");
+        *batch = create_batch(instances, nrhs, count, precisions);
+        return 0;
+    });
+}
+
+mx_status mx_batch_destroy(mx_batch *batch) {
+    delete batch;
+    return MX_OK;
+}
+
+mx_status mx_batch_groups(const mx_batch *batch, int32_t *ngroups) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && ngroups, "null argument");
+        *ngroups = (int32_t)batch->plans.size();
+        return 0;
+    });
+}
+
+mx_status mx_batch_group_info(const mx_batch *batch, int32_t group, int32_t *nrhs, int64_t *rows) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && nrhs && rows && group >= 0 && group < (int32_t)batch->plans.size(), "bad argument");
+        *nrhs = batch->group_nrhs[group];
+        *rows = batch->group_rows[group];
+        return 0;
+    });
+}
+
+mx_status mx_batch_layout(const mx_batch *batch, int32_t instance, int32_t *group, int64_t *row_offset) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && group && row_offset && instance >= 0 && instance < (int32_t)batch->inst.size(),
+                  "bad argument");
+        *group = batch->inst[instance].group;
+        *row_offset = batch->inst[instance].row_off;
+        return 0;
+    });
+}
+
+mx_status mx_batch_plan(mx_batch *batch, int32_t group, mx_plan **plan) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && plan && group >= 0 && group < (int32_t)batch->plans.size(), "bad argument");
+        *plan = batch->plans[group];
+        return 0;
+    });
+}
+
+mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+                        void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && W && Y, "null argument");
+        arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
+        bool prof = false;
+        for (auto *P : batch->plans) prof = prof || P->profiling;
+        if (batch->use_graph && !prof) batch_eval_graph(batch, prec, W, Y, io, (hipStream_t)stream);
+        else batch_eval(batch, prec, W, Y, io, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable) {
+    return (mx_status)guard([&]() {
+        arg_check(batch != nullptr, "null batch");
+        batch->use_graph = enable ? 1 : 0;
+        if (!enable) batch_drop_graphs(batch);
+        return 0;
+    });
+}
+
+mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(batch != nullptr, "null batch");
+        if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 I/O");
+        for (size_t g = 0; g < batch->plans.size(); ++g)
+            autotune(batch->plans[g], prec, batch->group_nrhs[g], io, reps, (hipStream_t)stream, nullptr);
+        batch_drop_graphs(batch);
+        return 0;
+    });
+}
+
+mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && total, "null argument");
+        *total = 0.0;
+        for (size_t g = 0; g < batch->plans.size(); ++g) {
+            const mx_plan *P = batch->plans[g];
+            const int kd = prec == MX_BF16X3 ? 1 : 0;
+            for (double f : P->prog[DIR_FWD][kd][P->sched[DIR_FWD][kd]].flops) *total += f * batch->group_nrhs[g];
+        }
+        return 0;
+    });
+}
+
+mx_status mx_batch_set_points(mx_batch *batch, int32_t instance, const double *X_leaf, int32_t d, double h) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && X_leaf && d >= 1 && h > 0 && instance >= 0 && instance < (int32_t)batch->inst.size(),
+                  "bad argument");
+        BatchInst &bi = batch->inst[instance];
+        if (bi.X_leaf) HIPCHK(hipFree(bi.X_leaf));
+        bi.X_leaf = nullptr;
+        bi.X_leaf = upload(X_leaf, (size_t)bi.N * d);
+        bi.pts_d = d;
+        bi.pts_h = h;
+        return 0;
+    });
+}
+
+mx_status mx_batch_dense_rows(mx_batch *batch, int32_t instance, const int32_t *rows, int32_t nrows, const double *W,
+                              double *out, void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && instance >= 0 && instance < (int32_t)batch->inst.size(), "bad argument");
+        BatchInst &bi = batch->inst[instance];
+        if (!bi.X_leaf) throw MxError(MX_ERR_STATE, "mx_batch_set_points has not been called for this instance");
+        dense_rows_core(bi.X_leaf, bi.N, bi.pts_d, bi.pts_h, bi.nleaf, bi.wrow.data(), bi.Ddim.data(), rows, nrows,
+                        W, batch->group_nrhs[bi.group], out, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+}  // extern "C"

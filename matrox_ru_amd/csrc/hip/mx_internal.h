@@ -1,0 +1,293 @@
+// Internal declarations shared by the evaluator's translation units:
+//   mx_lists.hip     structure checks and per-destination task lists
+//   mx_generate.hip  kernel blocks computed on the device from the points
+//   mx_plan.hip      plans, evaluation, backward, autotuner, C-ABI
+//   mx_batch.hip     many independent instances
+#pragma once
+#include "../../../include/matrox_hgemm.h"
+#include "mx_bf16x3.h"
+#include "mx_grad.h"
+#include "mx_seg_gemm.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace mx {
+
+extern thread_local std::string g_err;
+
+struct MxError : std::runtime_error {
+    mx_status code;
+    MxError(mx_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            throw MxError(MX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
+    } while (0)
+
+inline void arg_check(bool ok, const std::string &msg) {
+    if (!ok) throw MxError(MX_ERR_ARG, msg);
+}
+
+struct Launch {
+    int tile_offset;  // into the plan's tile array
+    int ntiles;
+};
+
+// The task lists of one evaluation direction (forward Y = K~ W or adjoint
+// dW = K~^T dY) for one kernel kind (0: segmented FP64/FP32 GEMM, 1: BF16x3).
+struct Program {
+    Task *tasks = nullptr;
+    Seg *segs = nullptr;
+    int2 *tiles = nullptr;
+    std::vector<Launch> launches;
+    std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
+    bool built = false;
+    void release() {
+        if (tasks) (void)hipFree(tasks);
+        if (segs) (void)hipFree(segs);
+        if (tiles) (void)hipFree(tiles);
+        tasks = nullptr;
+        segs = nullptr;
+        tiles = nullptr;
+    }
+};
+
+enum { DIR_FWD = 0, DIR_ADJ = 1 };
+
+// Schedules of the coupling and downward phases:
+//  SCHED_FUSED: one launch per level, u_c = sum_far B w + VT_p[:, part]^T u_p;
+//  SCHED_SPLIT: one launch for the couplings of every level (they only need
+//      the upward results), then one launch per level adding the transfer
+//      from the parent (beta = 1).  Long coupling lists at the top of the
+//      tree (the adjoint's column-grouped lists) then run beside everything
+//      else instead of in a nearly empty launch.
+enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, NSCHED = 2 };
+
+}  // namespace mx
+
+
+struct mx_plan {
+    int device = 0;
+    int nleaf = 0, numnodes = 0, is_hss = 0;
+    int64_t N = 0, skel_rows = 0;
+    std::vector<int32_t> wrow, Ddim, skelrow, slen;
+
+    // task lists: prog[direction][kind][schedule]; each is UP levels, the
+    // coupling and downward launches, NEAR+LEAF (kind 1: 128-row tiles and
+    // packed split A operands)
+    mx::Program prog[2][2][mx::NSCHED];
+    int sched[2][2] = {{mx::SCHED_FUSED, mx::SCHED_FUSED}, {mx::SCHED_SPLIT, mx::SCHED_SPLIT}};
+    bool adjoint = false;
+    std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
+    int64_t wp_rows = 0, sp_rows = 0;
+    int x3_variant = mx::X3_V128x128_S2;   // BF16x3 kernel variant (tile, pipeline depth)
+    // per-launch BF16x3 variants [direction][schedule] chosen by the
+    // autotuner (empty: x3_variant for every launch)
+    std::vector<int> lvar[2][mx::NSCHED];
+    mx::X3Rows *d_wrows = nullptr;
+    mx::X3Rows *d_srows = nullptr;      // skeleton blocks (slen > 0) for panel export
+    int n_srows = 0;
+    int panels_dir = -1, panels_nrhs = 0;   // the evaluation the skeleton panels hold
+    uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
+    size_t panel_bytes = 0;
+
+    // stored blocks per precision
+    double *D64 = nullptr, *B64 = nullptr, *VT64 = nullptr;
+    float *D32 = nullptr, *B32 = nullptr, *VT32 = nullptr;
+    uint16_t *xhi[3] = {nullptr, nullptr, nullptr};   // packed split D, B, VT operands
+    uint16_t *xlo[3] = {nullptr, nullptr, nullptr};
+    int64_t x3_elems[3] = {0, 0, 0};
+    int64_t D_len = 0, B_len = 0, VT_len = 0;
+    uint32_t precisions = 0;
+
+    // gradient products of the stored blocks (plans built with MX_PLAN_ADJOINT)
+    mx::GTask *g_tasks = nullptr;
+    mx::GSeg *g_segs = nullptr;
+    int4 *g_tiles = nullptr;
+    int g_ntiles = 0;
+    double g_flops = 0.0;               // algorithmic flops per nrhs column
+    // packing lists of the split-BF16 operands, kept to repack new blocks
+    mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
+    int64_t n_xblk[3] = {0, 0, 0};
+
+    // work buffers: skeletons of the forward (w^, u^) and of the adjoint
+    // (y^ = dL/du^, g^ = dL/dw^) evaluations
+    void *wskel = nullptr, *uskel = nullptr;
+    void *wskel_adj = nullptr, *uskel_adj = nullptr;
+    size_t skel_bytes = 0, skel_adj_bytes = 0;
+    void *io_w = nullptr, *io_y = nullptr, *io_g = nullptr;   // FP32 staging when io is FP64
+    size_t io_bytes = 0, io_g_bytes = 0;
+    int last_prec = -1;
+
+
+    // points for the dense reference rows
+    double *X_leaf = nullptr;
+    int pts_d = 0;
+    double pts_h = 0.0;
+
+    // captured evaluations (hipGraph) keyed by their arguments
+    struct GraphEntry {
+        int dir, prec, io, nrhs;
+        const void *W;
+        void *Y;
+        hipGraphExec_t exec;
+        uint64_t used;
+    };
+    int use_graph = 0;
+    std::vector<GraphEntry> graphs;
+    uint64_t graph_clock = 0;
+
+    // launch timing
+    int profiling = 0;
+    int keep_prof = 0;                // set while one call records several evaluations
+    std::vector<hipEvent_t> ev;       // 2 per launch
+    std::vector<int> ev_kind;
+    int ev_used = 0;
+
+    ~mx_plan() {
+        for (auto &pd : prog)
+            for (auto &pk : pd)
+                for (auto &ps : pk) ps.release();
+        void *ptrs[] = {D64, B64, VT64, D32, B32,
+                        VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
+                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
+                        wskel_adj, uskel_adj, io_g};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    }
+};
+
+namespace mx {
+
+struct TaskLists {
+    std::vector<Task> tasks;
+    std::vector<Seg> segs;
+    std::vector<int2> tiles;
+    std::vector<Launch> launches;
+    std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
+    std::vector<int> task_kind;   // per task
+    std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
+    int dummy = -1;               // an empty task (no rows, no segments) padding launches
+};
+
+struct Topology {
+    std::vector<int> parent, depth;
+    std::vector<std::vector<int>> levels;  // node ids by depth (root excluded)
+    std::vector<int> node_of_leaf;
+};
+
+struct GradLists {
+    std::vector<GTask> tasks;
+    std::vector<GSeg> segs;
+    std::vector<int4> tiles;
+    double flops = 0.0;   // per nrhs column
+};
+
+// ---- task lists (mx_lists.hip) ----------------------------------------------
+
+void compute_launch_flops(TaskLists &L);
+Topology topology(const mx_structure &s);
+// Bounds of every offset in the structure (throws MX_ERR_ARG).
+void validate(const mx_structure &s, const Topology &t);
+void split_long_tasks(TaskLists &L, int maxseg);
+TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir, int sched,
+                      const int64_t *wrow_in, const int64_t *skel_in);
+GradLists build_grad_lists(const mx_structure &s, const Topology &t);
+
+// ---- device buffers ----------------------------------------------------------
+
+template <typename T>
+T *upload(const T *h, size_t n) {
+    T *d = nullptr;
+    HIPCHK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
+    if (h && n > 0) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    return d;
+}
+
+inline int cvt_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 8192);
+}
+
+// ---- generated blocks (mx_generate.hip) ---------------------------------------
+
+struct GenInfo {
+    const double *X = nullptr;        // host, n x d row-major, original point order
+    int64_t n = 0;
+    int d = 0;
+    double h = 1.0;
+    const int32_t *lids = nullptr;    // host, N: leaf-order row -> point id
+    const int32_t *skels = nullptr;   // host: skeleton point ids, node v at skoff[v]
+    const int64_t *skoff = nullptr;   // host, numnodes
+    int64_t nskels = 0;
+};
+
+struct alignas(16) GenBlock {
+    int64_t out;    // element offset of the block (column-major na x nb)
+    int64_t a, b;   // offsets of its row and column point ids in the index array
+    int32_t na, nb;
+};
+
+// Points, norms and block lists of one matrix whose D and B are generated:
+// near blocks index the leaf-order point ids (lids), coupling blocks the
+// skeleton point ids.
+struct GenState {
+    double *Xd = nullptr, *nrm = nullptr;
+    int32_t *idx[2] = {nullptr, nullptr};   // 0: lids, 1: skeleton ids
+    int d = 0;
+    double kscal = 0.0;
+    std::vector<GenBlock> blk[2];           // 0: near (D), 1: coupling (B)
+    std::map<int64_t, int> at[2];           // block offset -> index in blk
+    GenState() = default;
+    GenState(const GenState &) = delete;
+    GenState &operator=(const GenState &) = delete;
+    ~GenState();
+    void init(const mx_structure &s, const Topology &t, const GenInfo &gen);
+    // w: 0 = D, 1 = B, in their stored layout
+    void emit(int w, double *out) const;
+    void emit(int w, float *out) const;
+    // packed split-BF16 operands of buffer w for the packing list xb
+    void emit_x3(int w, const std::vector<X3Block> &xb, uint16_t *hi, uint16_t *lo) const;
+};
+
+// ---- plan core (mx_plan.hip) -----------------------------------------------------
+
+mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y,
+                     const GenInfo *gen = nullptr);
+void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io, hipStream_t st);
+void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);
+void ensure_panels(mx_plan *P, int nrhs);
+void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st, float *best_ms);
+void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,
+                     const int32_t *Ddim, const int32_t *rows, int nrows, const double *W, int nrhs, double *out,
+                     hipStream_t st);
+
+// C-ABI error channel: run f, turn exceptions into status codes and keep the
+// message for mx_last_error.
+template <typename F>
+int32_t guard(F &&f) {
+    try {
+        return (int32_t)f();
+    } catch (const MxError &e) {
+        g_err = e.what();
+        return (int32_t)e.code;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return (int32_t)MX_ERR_ARG;
+    }
+}
+
+}  // namespace mx

@@ -1,0 +1,475 @@
+// Structure checks and the per-destination task lists of an evaluation
+// (forward and adjoint, two coupling schedules, two kernel kinds) and of the
+// block gradients.
+#include "mx_internal.h"
+
+namespace mx {
+
+void compute_launch_flops(TaskLists &L) {
+    L.flops.assign(L.launches.size(), 0.0);
+    std::vector<int> seen(L.tasks.size(), -1);
+    for (size_t li = 0; li < L.launches.size(); ++li) {
+        for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
+            int task = L.tiles[t].x;
+            if (seen[task] == (int)li) continue;   // count each task once per launch
+            seen[task] = (int)li;
+            const Task &tk = L.tasks[task];
+            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) L.flops[li] += 2.0 * tk.m * (double)L.segs[sg].k;
+        }
+    }
+}
+
+Topology topology(const mx_structure &s) {
+    Topology t;
+    const int nn = s.numnodes;
+    t.parent.assign(nn, -1);
+    t.depth.assign(nn, -1);
+    t.node_of_leaf.assign(s.nleaf, -1);
+    for (int v = 0; v < nn; ++v) {
+        int l = s.lchildren[v], r = s.rchildren[v];
+        arg_check((l == -1) == (r == -1), "node " + std::to_string(v) + " has exactly one child");
+        if (l != -1) {
+            arg_check(l > 0 && l < nn && r > 0 && r < nn && l != r, "child index out of range");
+            arg_check(t.parent[l] == -1 && t.parent[r] == -1, "node with two parents");
+            t.parent[l] = v;
+            t.parent[r] = v;
+        } else {
+            int lf = s.lm[v];
+            arg_check(lf >= 0 && lf < s.nleaf, "leaf node " + std::to_string(v) + " has no leaf index");
+            arg_check(t.node_of_leaf[lf] == -1, "leaf index used twice");
+            t.node_of_leaf[lf] = v;
+        }
+    }
+    arg_check(t.parent[0] == -1, "node 0 must be the root");
+    // depth by BFS from the root
+    std::vector<int> q{0};
+    t.depth[0] = 0;
+    for (size_t i = 0; i < q.size(); ++i) {
+        int v = q[i];
+        if (s.lchildren[v] != -1) {
+            for (int c : {s.lchildren[v], s.rchildren[v]}) {
+                t.depth[c] = t.depth[v] + 1;
+                q.push_back(c);
+            }
+        }
+    }
+    arg_check((int)q.size() == nn, "tree does not reach every node from the root");
+    int maxd = 0;
+    for (int v = 0; v < nn; ++v) maxd = std::max(maxd, t.depth[v]);
+    t.levels.assign(maxd + 1, {});
+    for (int v = 1; v < nn; ++v) t.levels[t.depth[v]].push_back(v);
+    for (int l = 0; l < s.nleaf; ++l) arg_check(t.node_of_leaf[l] != -1, "leaf index without node");
+    return t;
+}
+
+static int cols_of(const mx_structure &s, int v) {
+    if (s.lchildren[v] == -1) return s.Ddim[s.lm[v]];
+    return s.slen[s.lchildren[v]] + s.slen[s.rchildren[v]];
+}
+
+void validate(const mx_structure &s, const Topology &t) {
+    const int nn = s.numnodes;
+    for (int l = 0; l < s.nleaf; ++l) arg_check(s.Ddim[l] >= 0, "negative leaf size");
+    for (int v = 0; v < nn; ++v) arg_check(s.slen[v] >= 0, "negative skeleton length");
+    for (int v = 1; v < nn; ++v) {
+        int64_t need = (int64_t)s.VTptr[v] + (int64_t)s.slen[v] * cols_of(s, v);
+        arg_check(s.slen[v] == 0 || (s.VTptr[v] >= 0 && need <= s.VT_len),
+                  "VT block of node " + std::to_string(v) + " exceeds VT");
+    }
+    for (int64_t k = 0; k < s.ncount; ++k) {
+        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
+        arg_check(x >= 0 && x < s.nleaf && y >= 0 && y < s.nleaf, "near pair leaf out of range");
+        int64_t need = (int64_t)s.Dptr[k] + (int64_t)s.Ddim[x] * s.Ddim[y];
+        arg_check(need <= s.D_len, "near block " + std::to_string(k) + " exceeds D");
+    }
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x, y;
+        if (s.is_hss) {
+            x = (int)k + 1;
+            arg_check(x < nn, "HSS far count exceeds numnodes-1");
+            int p = t.parent[x];
+            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+        } else {
+            x = s.fpairx[k];
+            y = s.fpairy[k];
+        }
+        arg_check(x > 0 && x < nn && y > 0 && y < nn, "far pair node out of range");
+        int64_t need = (int64_t)s.Bptr[k] + (int64_t)s.slen[x] * s.slen[y];
+        arg_check(need <= s.B_len, "far block " + std::to_string(k) + " exceeds B");
+    }
+    for (int l = 0; l < s.nleaf; ++l) arg_check(s.wrow[l] >= 0, "negative leaf row offset");
+    for (int v = 0; v < nn; ++v) arg_check(s.skelrow[v] >= 0, "negative skeleton row offset");
+}
+
+static int add_tiles(TaskLists &L, int task, int m) {
+    const int bm = L.task_kind[task] == 1 ? X3_BM : SG_BM;
+    int n = (m + bm - 1) / bm;
+    for (int i = 0; i < n; ++i) L.tiles.push_back(make_int2(task, i));
+    return n;
+}
+
+// Order of the tiles of every launch.  xcd_remap hands XCD x the x-th
+// contiguous eighth of a launch's work items (the column tiles of a tile are
+// adjacent).  The tiles, in structural order (tree order: neighbouring
+// leaves and nodes share W and skeleton blocks, which then stay in one XCD's
+// L2), are cut into eight contiguous groups of about equal work; each group
+// is sorted longest first (so long lists, such as the adjoint's
+// column-grouped couplings at the top of the tree, start early) and padded
+// with empty tiles to the size of the largest, so that group x is exactly
+// the eighth xcd_remap gives XCD x.
+static void order_tiles(TaskLists &L, int bk) {
+    if (L.dummy < 0) {
+        Task tk{};
+        tk.c_buf = VEC_USKEL;
+        L.dummy = (int)L.tasks.size();
+        L.tasks.push_back(tk);
+        L.task_kind.push_back(L.task_kind.empty() ? 0 : L.task_kind.back());
+    }
+    std::vector<int2> out;
+    std::vector<Launch> lns;
+    for (const Launch &ln : L.launches) {
+        const int T = ln.ntiles;
+        std::vector<std::pair<int64_t, int2>> v;
+        v.reserve(T);
+        int64_t total = 0;
+        for (int i = 0; i < T; ++i) {
+            int2 tl = L.tiles[ln.tile_offset + i];
+            const Task &tk = L.tasks[tl.x];
+            int64_t cost = 2;   // prologue and epilogue
+            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
+            v.push_back({cost, tl});
+            total += cost;
+        }
+        const int ng = 8;
+        std::vector<int> cut(ng + 1, T);
+        cut[0] = 0;
+        int64_t acc = 0;
+        int g = 1;
+        for (int i = 0; i < T && g < ng; ++i) {
+            acc += v[i].first;
+            while (g < ng && acc * ng >= total * g) cut[g++] = i + 1;
+        }
+        int gmax = 0;
+        for (int q = 0; q < ng; ++q) gmax = std::max(gmax, cut[q + 1] - cut[q]);
+        Launch nl{(int)out.size(), gmax * ng};
+        for (int q = 0; q < ng; ++q) {
+            std::stable_sort(v.begin() + cut[q], v.begin() + cut[q + 1],
+                             [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
+                                 return a.first > b.first;
+                             });
+            for (int i = cut[q]; i < cut[q + 1]; ++i) out.push_back(v[i].second);
+            for (int i = cut[q + 1] - cut[q]; i < gmax; ++i) out.push_back(make_int2(L.dummy, 0));
+        }
+        lns.push_back(nl);
+    }
+    L.tiles.swap(out);
+    L.launches.swap(lns);
+}
+
+// A task with more than maxseg segments keeps its first maxseg; the rest go
+// to continuation tasks (beta = 1, same output block and row tiles), the
+// k-th continuation in a launch of its own inserted right after the task's
+// launch, so the parts accumulate in order without write conflicts.  Used
+// for the BF16x3 kernel, which stages a task's segment list in LDS.
+void split_long_tasks(TaskLists &L, int maxseg) {
+    std::vector<int2> tiles;
+    std::vector<Launch> launches;
+    std::vector<int> kinds;
+    std::vector<int> seen(L.tasks.size(), -1);
+    for (size_t li = 0; li < L.launches.size(); ++li) {
+        const Launch ln = L.launches[li];
+        // continuation tasks of this launch, by depth
+        std::vector<std::vector<int>> cont;
+        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) {
+            const int task = L.tiles[t].x;
+            if (seen[task] == (int)li) continue;
+            seen[task] = (int)li;
+            const Task base = L.tasks[task];
+            const int nseg = base.seg_end - base.seg_begin;
+            if (nseg <= maxseg) continue;
+            L.tasks[task].seg_end = base.seg_begin + maxseg;
+            for (int k = 1; k * maxseg < nseg; ++k) {
+                Task c = base;
+                c.seg_begin = base.seg_begin + k * maxseg;
+                c.seg_end = std::min(base.seg_end, c.seg_begin + maxseg);
+                c.beta = 1;
+                if ((int)cont.size() < k) cont.resize(k);
+                cont[k - 1].push_back((int)L.tasks.size());
+                L.tasks.push_back(c);
+                L.task_kind.push_back(L.task_kind[task]);
+            }
+        }
+        Launch nl{(int)tiles.size(), ln.ntiles};
+        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) tiles.push_back(L.tiles[t]);
+        launches.push_back(nl);
+        kinds.push_back(L.kinds[li]);
+        for (auto &group : cont) {
+            Launch cl{(int)tiles.size(), 0};
+            for (int ct : group) {
+                const int bm = L.task_kind[ct] == 1 ? X3_BM : SG_BM;
+                for (int i = 0; i < (L.tasks[ct].m + bm - 1) / bm; ++i) tiles.push_back(make_int2(ct, i));
+            }
+            cl.ntiles = (int)tiles.size() - cl.tile_offset;
+            launches.push_back(cl);
+            kinds.push_back(L.kinds[li]);
+        }
+        seen.resize(L.tasks.size(), -1);
+    }
+    L.tiles.swap(tiles);
+    L.launches.swap(launches);
+    L.kinds.swap(kinds);
+}
+
+// Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
+// in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
+// beta_u / beta_y select the accumulation rule of the coupling and leaf
+// tasks (0 for mx_eval, the reference's rules for the drop-in entry points).
+// wrow_in / skel_in give the row offsets of W blocks and skeleton blocks in
+// the buffers the kernel reads (reference layout for kind 0, split panels for
+// kind 1); Y always uses the reference layout.
+TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir,
+                      int sched, const int64_t *wrow_in, const int64_t *skel_in) {
+    TaskLists L;
+    const int nn = s.numnodes;
+    // far pair k couples row node fx(k) with column node fy(k) through B_k
+    auto fx = [&](int64_t k) { return s.is_hss ? (int)k + 1 : s.fpairx[k]; };
+    auto fy = [&](int64_t k) {
+        if (!s.is_hss) return s.fpairy[k];
+        int x = (int)k + 1, p = t.parent[x];
+        return s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+    };
+    auto nx = [&](int64_t k) { return s.is_hss ? (int)k : s.npairx[k]; };
+    auto ny = [&](int64_t k) { return s.is_hss ? (int)k : s.npairy[k]; };
+    // pairs grouped by destination: the row node (forward) or the column
+    // node (adjoint, where the block acts transposed)
+    std::vector<std::vector<int64_t>> far_of(nn), near_of(s.nleaf);
+    for (int64_t k = 0; k < s.fcount; ++k) far_of[dir == DIR_FWD ? fx(k) : fy(k)].push_back(k);
+    for (int64_t k = 0; k < s.ncount; ++k) near_of[dir == DIR_FWD ? nx(k) : ny(k)].push_back(k);
+
+    auto begin_launch = [&]() {
+        L.launches.push_back(Launch{(int)L.tiles.size(), 0});
+        L.kinds.push_back(kind);
+    };
+    auto end_launch = [&]() {
+        L.launches.back().ntiles = (int)L.tiles.size() - L.launches.back().tile_offset;
+        if (L.launches.back().ntiles == 0) {
+            L.launches.pop_back();
+            L.kinds.pop_back();
+        }
+    };
+    auto new_task = [&](int64_t c_row, int m, int c_buf, int beta) {
+        Task tk{};
+        tk.c_row = c_row;
+        tk.m = m;
+        tk.c_buf = c_buf;
+        tk.seg_begin = (int)L.segs.size();
+        tk.seg_end = tk.seg_begin;
+        tk.beta = beta;
+        L.tasks.push_back(tk);
+        L.task_kind.push_back(kind);
+        return (int)L.tasks.size() - 1;
+    };
+    auto add_seg = [&](int task, int64_t a_off, int a_buf, int a_trans, int lda, int k, int b_buf, int64_t b_row,
+                       int b_ld) {
+        if (k <= 0) return;
+        Seg sg{};
+        sg.a_off = a_off;
+        sg.a_buf = a_buf;
+        sg.a_trans = a_trans;
+        sg.lda = lda;
+        sg.k = k;
+        sg.b_buf = b_buf;
+        sg.b_row = b_row;
+        sg.b_ld = b_ld;
+        L.segs.push_back(sg);
+        L.tasks[task].seg_end = (int)L.segs.size();
+    };
+    const int maxd = (int)t.levels.size() - 1;
+    // UP: deepest level first
+    for (int d = maxd; d >= 1; --d) {
+        begin_launch();
+        for (int v : t.levels[d]) {
+            int sv = s.slen[v];
+            if (sv == 0) continue;
+            int tk = new_task(skel_in[v], sv, VEC_WSKEL, 0);
+            if (s.lchildren[v] == -1) {
+                int lf = s.lm[v];
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.Ddim[lf], VEC_W, wrow_in[lf], s.Ddim[lf]);
+            } else {
+                int l = s.lchildren[v], r = s.rchildren[v];
+                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.slen[l], VEC_WSKEL, skel_in[l], s.slen[l]);
+                add_seg(tk, (int64_t)s.VTptr[v] + (int64_t)sv * s.slen[l], MAT_VT, 0, sv, s.slen[r], VEC_WSKEL,
+                        skel_in[r], s.slen[r]);
+            }
+            add_tiles(L, tk, sv);
+        }
+        end_launch();
+    }
+    auto add_far = [&](int tk, int c) {
+        const int sc = s.slen[c];
+        for (int64_t k : far_of[c]) {
+            if (dir == DIR_FWD) {
+                int y = fy(k);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
+            } else {   // B_k^T, B_k stored s_x x s_y
+                int x = fx(k);
+                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 1, s.slen[x], s.slen[x], VEC_WSKEL, skel_in[x], s.slen[x]);
+            }
+        }
+    };
+    auto add_down = [&](int tk, int c) {   // VT_p[:, part(c)]^T u_p; false when there is none
+        int p = t.parent[c];
+        if (p <= 0 || s.slen[p] == 0) return false;
+        int sp = s.slen[p];
+        int64_t off = s.VTptr[p];
+        if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
+        add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
+        return true;
+    };
+    if (sched == SCHED_FUSED) {
+        // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
+        for (int d = 1; d <= maxd; ++d) {
+            begin_launch();
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
+                add_far(tk, c);
+                add_down(tk, c);
+                add_tiles(L, tk, sc);
+            }
+            end_launch();
+        }
+    } else {
+        // FAR for every level at once, then DOWN level by level (beta = 1)
+        begin_launch();
+        for (int d = 1; d <= maxd; ++d)
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
+                add_far(tk, c);
+                add_tiles(L, tk, sc);
+            }
+        end_launch();
+        for (int d = 2; d <= maxd; ++d) {
+            begin_launch();
+            for (int c : t.levels[d]) {
+                int sc = s.slen[c];
+                if (sc == 0) continue;
+                int tk = new_task(skel_in[c], sc, VEC_USKEL, 1);
+                if (add_down(tk, c)) add_tiles(L, tk, sc);
+                else L.tasks.pop_back();
+            }
+            end_launch();
+        }
+    }
+    // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x
+    begin_launch();
+    for (int lf = 0; lf < s.nleaf; ++lf) {
+        if (s.Ddim[lf] == 0) continue;
+        int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
+        for (int64_t k : near_of[lf]) {
+            if (dir == DIR_FWD) {
+                int y = ny(k);
+                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
+            } else {   // D_k^T, D_k stored Dim_x x Dim_y
+                int x = nx(k);
+                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 1, s.Ddim[x], s.Ddim[x], VEC_W, wrow_in[x], s.Ddim[x]);
+            }
+        }
+        int v = t.node_of_leaf[lf];
+        int sv = s.slen[v];
+        if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, skel_in[v], sv);
+        add_tiles(L, tk, s.Ddim[lf]);
+    }
+    end_launch();
+    order_tiles(L, kind == 1 ? X3_BK : SG_BK);
+    return L;
+}
+
+// Gradient products of every stored block (see mx_grad.h): one task per
+// near block, far block, leaf VT and half of an internal VT, tiled 64 x 64.
+
+GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
+    GradLists G;
+    auto task = [&](int64_t c_off, int m, int n, int ldc, int c_buf) {
+        GTask k{};
+        k.c_off = c_off;
+        k.m = m;
+        k.n = n;
+        k.ldc = ldc;
+        k.c_buf = c_buf;
+        k.seg_begin = k.seg_end = (int)G.segs.size();
+        G.tasks.push_back(k);
+        return (int)G.tasks.size() - 1;
+    };
+    auto seg = [&](int tk, int x_buf, int64_t x_row, int x_ld, int z_buf, int64_t z_row, int z_ld) {
+        GSeg g{};
+        g.x_buf = x_buf;
+        g.x_row = x_row;
+        g.x_ld = x_ld;
+        g.z_buf = z_buf;
+        g.z_row = z_row;
+        g.z_ld = z_ld;
+        G.segs.push_back(g);
+        G.tasks[tk].seg_end = (int)G.segs.size();
+        G.flops += 2.0 * x_ld * z_ld;
+    };
+    auto tiles = [&](int tk) {
+        const GTask &k = G.tasks[tk];
+        for (int i = 0; i < (k.m + 63) / 64; ++i)
+            for (int j = 0; j < (k.n + 63) / 64; ++j) G.tiles.push_back(make_int4(tk, i, j, 0));
+    };
+    for (int64_t k = 0; k < s.ncount; ++k) {
+        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
+        int mx_ = s.Ddim[x], ny_ = s.Ddim[y];
+        if (mx_ == 0 || ny_ == 0) continue;
+        int tk = task((int64_t)s.Dptr[k], mx_, ny_, mx_, MAT_D);
+        seg(tk, GB_DY, s.wrow[x], mx_, GB_W, s.wrow[y], ny_);
+        tiles(tk);
+    }
+    for (int64_t k = 0; k < s.fcount; ++k) {
+        int x, y;
+        if (s.is_hss) {
+            x = (int)k + 1;
+            int p = t.parent[x];
+            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
+        } else {
+            x = s.fpairx[k];
+            y = s.fpairy[k];
+        }
+        int sx = s.slen[x], sy = s.slen[y];
+        if (sx == 0 || sy == 0) continue;
+        int tk = task((int64_t)s.Bptr[k], sx, sy, sx, MAT_B);
+        seg(tk, GB_YHAT, s.skelrow[x], sx, GB_WHAT, s.skelrow[y], sy);
+        tiles(tk);
+    }
+    for (int v = 1; v < s.numnodes; ++v) {
+        int sv = s.slen[v];
+        if (sv == 0) continue;
+        if (s.lchildren[v] == -1) {
+            int lf = s.lm[v], dm = s.Ddim[lf];
+            if (dm == 0) continue;
+            int tk = task(s.VTptr[v], sv, dm, sv, MAT_VT);
+            seg(tk, GB_GHAT, s.skelrow[v], sv, GB_W, s.wrow[lf], dm);
+            seg(tk, GB_UHAT, s.skelrow[v], sv, GB_DY, s.wrow[lf], dm);
+            tiles(tk);
+        } else {
+            int64_t off = s.VTptr[v];
+            for (int c : {s.lchildren[v], s.rchildren[v]}) {
+                int sc = s.slen[c];
+                if (sc > 0) {
+                    int tk = task(off, sv, sc, sv, MAT_VT);
+                    seg(tk, GB_GHAT, s.skelrow[v], sv, GB_WHAT, s.skelrow[c], sc);
+                    seg(tk, GB_UHAT, s.skelrow[v], sv, GB_YHAT, s.skelrow[c], sc);
+                    tiles(tk);
+                }
+                off += (int64_t)sv * sc;
+            }
+        }
+    }
+    return G;
+}
+
+}  // namespace mx

@@ -1,685 +1,32 @@
-// Plans: structure validation, per-destination task lists, device uploads and
-// the launch sequence of one evaluation.  C-ABI in include/matrox_hgemm.h.
+// Plans: device uploads, the launch sequence of one evaluation (forward and
+// adjoint), the backward pass, the autotuner and the C-ABI of
+// include/matrox_hgemm.h (batches are in mx_batch.hip, task lists in
+// mx_lists.hip, device-generated blocks in mx_generate.hip).
 //
 // Evaluation order (Y = K~ W, semantics of symGen/HGEMM_gen.h:117-238):
 //   UP        one launch per tree level, leaves first:
 //             w_v = VT_v W_leaf  |  w_v = VT_v[:, :s_l] w_l + VT_v[:, s_l:] w_r
-//   FAR+DOWN  one launch per level, root's children first; each task owns one
-//             node c and sums its far couplings and its parent's transfer:
+//   couplings fused schedule: one launch per level, root's children first;
+//             each task owns one node c and sums its far couplings and its
+//             parent's transfer:
 //             u_c = beta*u_c + sum_{far (c,y)} B_cy w_y + VT_p[:, part(c)]^T u_p
+//             split schedule: one launch of every far coupling of every level,
+//             then one transfer launch per level with beta = 1
 //   NEAR+LEAF one launch; each task owns one leaf block of Y:
 //             Y_x = beta*Y_x + sum_{near (x,y)} D_xy W_y + VT_x^T u_x
 // The reference runs NEAR first, then UP, FAR and DOWN with beta = 1 updates
 // (HGEMM_gen.h:125-236); fusing the transfers into the coupling and near
 // tasks gives the same sums with each output block written once per launch.
-#include "../../../include/matrox_hgemm.h"
-#include "mx_bf16x3.h"
-#include "mx_grad.h"
-#include "mx_seg_gemm.h"
+#include "mx_internal.h"
 
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <map>
 #include <tuple>
-#include <memory>
-#include <stdexcept>
-#include <string>
-#include <vector>
 
 namespace mx {
 
-static thread_local std::string g_err;
-
-struct MxError : std::runtime_error {
-    mx_status code;
-    MxError(mx_status c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-#define HIPCHK(x)                                                                                  \
-    do {                                                                                           \
-        hipError_t e_ = (x);                                                                       \
-        if (e_ != hipSuccess)                                                                      \
-            throw MxError(MX_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));             \
-    } while (0)
-
-static void arg_check(bool ok, const std::string &msg) {
-    if (!ok) throw MxError(MX_ERR_ARG, msg);
-}
-
-struct Launch {
-    int tile_offset;  // into the plan's tile array
-    int ntiles;
-};
-
-// The task lists of one evaluation direction (forward Y = K~ W or adjoint
-// dW = K~^T dY) for one kernel kind (0: segmented FP64/FP32 GEMM, 1: BF16x3).
-struct Program {
-    Task *tasks = nullptr;
-    Seg *segs = nullptr;
-    int2 *tiles = nullptr;
-    std::vector<Launch> launches;
-    std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
-    bool built = false;
-    void release() {
-        if (tasks) (void)hipFree(tasks);
-        if (segs) (void)hipFree(segs);
-        if (tiles) (void)hipFree(tiles);
-        tasks = nullptr;
-        segs = nullptr;
-        tiles = nullptr;
-    }
-};
-
-enum { DIR_FWD = 0, DIR_ADJ = 1 };
-
-// Schedules of the coupling and downward phases:
-//  SCHED_FUSED: one launch per level, u_c = sum_far B w + VT_p[:, part]^T u_p;
-//  SCHED_SPLIT: one launch for the couplings of every level (they only need
-//      the upward results), then one launch per level adding the transfer
-//      from the parent (beta = 1).  Long coupling lists at the top of the
-//      tree (the adjoint's column-grouped lists) then run beside everything
-//      else instead of in a nearly empty launch.
-enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, NSCHED = 2 };
-
-}  // namespace mx
-
-struct mx_plan {
-    int device = 0;
-    int nleaf = 0, numnodes = 0, is_hss = 0;
-    int64_t N = 0, skel_rows = 0;
-    std::vector<int32_t> wrow, Ddim, skelrow, slen;
-
-    // task lists: prog[direction][kind][schedule]; each is UP levels, the
-    // coupling and downward launches, NEAR+LEAF (kind 1: 128-row tiles and
-    // packed split A operands)
-    mx::Program prog[2][2][mx::NSCHED];
-    int sched[2][2] = {{mx::SCHED_FUSED, mx::SCHED_FUSED}, {mx::SCHED_SPLIT, mx::SCHED_SPLIT}};
-    bool adjoint = false;
-    std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
-    int64_t wp_rows = 0, sp_rows = 0;
-    int x3_variant = mx::X3_V128x128_S2;   // BF16x3 kernel variant (tile, pipeline depth)
-    // per-launch BF16x3 variants [direction][schedule] chosen by the
-    // autotuner (empty: x3_variant for every launch)
-    std::vector<int> lvar[2][mx::NSCHED];
-    mx::X3Rows *d_wrows = nullptr;
-    mx::X3Rows *d_srows = nullptr;      // skeleton blocks (slen > 0) for panel export
-    int n_srows = 0;
-    int panels_dir = -1, panels_nrhs = 0;   // the evaluation the skeleton panels hold
-    uint16_t *panels = nullptr;         // W, wskel, uskel split panels (hi, lo)
-    size_t panel_bytes = 0;
-
-    // stored blocks per precision
-    double *D64 = nullptr, *B64 = nullptr, *VT64 = nullptr;
-    float *D32 = nullptr, *B32 = nullptr, *VT32 = nullptr;
-    uint16_t *xhi[3] = {nullptr, nullptr, nullptr};   // packed split D, B, VT operands
-    uint16_t *xlo[3] = {nullptr, nullptr, nullptr};
-    int64_t x3_elems[3] = {0, 0, 0};
-    int64_t D_len = 0, B_len = 0, VT_len = 0;
-    uint32_t precisions = 0;
-
-    // gradient products of the stored blocks (plans built with MX_PLAN_ADJOINT)
-    mx::GTask *g_tasks = nullptr;
-    mx::GSeg *g_segs = nullptr;
-    int4 *g_tiles = nullptr;
-    int g_ntiles = 0;
-    double g_flops = 0.0;               // algorithmic flops per nrhs column
-    // packing lists of the split-BF16 operands, kept to repack new blocks
-    mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
-    int64_t n_xblk[3] = {0, 0, 0};
-
-    // work buffers: skeletons of the forward (w^, u^) and of the adjoint
-    // (y^ = dL/du^, g^ = dL/dw^) evaluations
-    void *wskel = nullptr, *uskel = nullptr;
-    void *wskel_adj = nullptr, *uskel_adj = nullptr;
-    size_t skel_bytes = 0, skel_adj_bytes = 0;
-    void *io_w = nullptr, *io_y = nullptr, *io_g = nullptr;   // FP32 staging when io is FP64
-    size_t io_bytes = 0, io_g_bytes = 0;
-    int last_prec = -1;
-
-
-    // points for the dense reference rows
-    double *X_leaf = nullptr;
-    int pts_d = 0;
-    double pts_h = 0.0;
-
-    // captured evaluations (hipGraph) keyed by their arguments
-    struct GraphEntry {
-        int dir, prec, io, nrhs;
-        const void *W;
-        void *Y;
-        hipGraphExec_t exec;
-        uint64_t used;
-    };
-    int use_graph = 0;
-    std::vector<GraphEntry> graphs;
-    uint64_t graph_clock = 0;
-
-    // launch timing
-    int profiling = 0;
-    int keep_prof = 0;                // set while one call records several evaluations
-    std::vector<hipEvent_t> ev;       // 2 per launch
-    std::vector<int> ev_kind;
-    int ev_used = 0;
-
-    ~mx_plan() {
-        for (auto &pd : prog)
-            for (auto &pk : pd)
-                for (auto &ps : pk) ps.release();
-        void *ptrs[] = {D64, B64, VT64, D32, B32,
-                        VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
-                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
-                        wskel_adj, uskel_adj, io_g};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
-    }
-};
-
-namespace mx {
-
-// Host description of the task lists of one evaluation.
-struct TaskLists {
-    std::vector<Task> tasks;
-    std::vector<Seg> segs;
-    std::vector<int2> tiles;
-    std::vector<Launch> launches;
-    std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
-    std::vector<int> task_kind;   // per task
-    std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
-    int dummy = -1;               // an empty task (no rows, no segments) padding launches
-};
-
-static void compute_launch_flops(TaskLists &L) {
-    L.flops.assign(L.launches.size(), 0.0);
-    std::vector<int> seen(L.tasks.size(), -1);
-    for (size_t li = 0; li < L.launches.size(); ++li) {
-        for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
-            int task = L.tiles[t].x;
-            if (seen[task] == (int)li) continue;   // count each task once per launch
-            seen[task] = (int)li;
-            const Task &tk = L.tasks[task];
-            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) L.flops[li] += 2.0 * tk.m * (double)L.segs[sg].k;
-        }
-    }
-}
-
-struct Topology {
-    std::vector<int> parent, depth;
-    std::vector<std::vector<int>> levels;  // node ids by depth (root excluded)
-    std::vector<int> node_of_leaf;
-};
-
-static Topology topology(const mx_structure &s) {
-    Topology t;
-    const int nn = s.numnodes;
-    t.parent.assign(nn, -1);
-    t.depth.assign(nn, -1);
-    t.node_of_leaf.assign(s.nleaf, -1);
-    for (int v = 0; v < nn; ++v) {
-        int l = s.lchildren[v], r = s.rchildren[v];
-        arg_check((l == -1) == (r == -1), "node " + std::to_string(v) + " has exactly one child");
-        if (l != -1) {
-            arg_check(l > 0 && l < nn && r > 0 && r < nn && l != r, "child index out of range");
-            arg_check(t.parent[l] == -1 && t.parent[r] == -1, "node with two parents");
-            t.parent[l] = v;
-            t.parent[r] = v;
-        } else {
-            int lf = s.lm[v];
-            arg_check(lf >= 0 && lf < s.nleaf, "leaf node " + std::to_string(v) + " has no leaf index");
-            arg_check(t.node_of_leaf[lf] == -1, "leaf index used twice");
-            t.node_of_leaf[lf] = v;
-        }
-    }
-    arg_check(t.parent[0] == -1, "node 0 must be the root");
-    // depth by BFS from the root
-    std::vector<int> q{0};
-    t.depth[0] = 0;
-    for (size_t i = 0; i < q.size(); ++i) {
-        int v = q[i];
-        if (s.lchildren[v] != -1) {
-            for (int c : {s.lchildren[v], s.rchildren[v]}) {
-                t.depth[c] = t.depth[v] + 1;
-                q.push_back(c);
-            }
-        }
-    }
-    arg_check((int)q.size() == nn, "tree does not reach every node from the root");
-    int maxd = 0;
-    for (int v = 0; v < nn; ++v) maxd = std::max(maxd, t.depth[v]);
-    t.levels.assign(maxd + 1, {});
-    for (int v = 1; v < nn; ++v) t.levels[t.depth[v]].push_back(v);
-    for (int l = 0; l < s.nleaf; ++l) arg_check(t.node_of_leaf[l] != -1, "leaf index without node");
-    return t;
-}
-
-static int cols_of(const mx_structure &s, int v) {
-    if (s.lchildren[v] == -1) return s.Ddim[s.lm[v]];
-    return s.slen[s.lchildren[v]] + s.slen[s.rchildren[v]];
-}
-
-static void validate(const mx_structure &s, const Topology &t) {
-    const int nn = s.numnodes;
-    for (int l = 0; l < s.nleaf; ++l) arg_check(s.Ddim[l] >= 0, "negative leaf size");
-    for (int v = 0; v < nn; ++v) arg_check(s.slen[v] >= 0, "negative skeleton length");
-    for (int v = 1; v < nn; ++v) {
-        int64_t need = (int64_t)s.VTptr[v] + (int64_t)s.slen[v] * cols_of(s, v);
-        arg_check(s.slen[v] == 0 || (s.VTptr[v] >= 0 && need <= s.VT_len),
-                  "VT block of node " + std::to_string(v) + " exceeds VT");
-    }
-    for (int64_t k = 0; k < s.ncount; ++k) {
-        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
-        arg_check(x >= 0 && x < s.nleaf && y >= 0 && y < s.nleaf, "near pair leaf out of range");
-        int64_t need = (int64_t)s.Dptr[k] + (int64_t)s.Ddim[x] * s.Ddim[y];
-        arg_check(need <= s.D_len, "near block " + std::to_string(k) + " exceeds D");
-    }
-    for (int64_t k = 0; k < s.fcount; ++k) {
-        int x, y;
-        if (s.is_hss) {
-            x = (int)k + 1;
-            arg_check(x < nn, "HSS far count exceeds numnodes-1");
-            int p = t.parent[x];
-            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
-        } else {
-            x = s.fpairx[k];
-            y = s.fpairy[k];
-        }
-        arg_check(x > 0 && x < nn && y > 0 && y < nn, "far pair node out of range");
-        int64_t need = (int64_t)s.Bptr[k] + (int64_t)s.slen[x] * s.slen[y];
-        arg_check(need <= s.B_len, "far block " + std::to_string(k) + " exceeds B");
-    }
-    for (int l = 0; l < s.nleaf; ++l) arg_check(s.wrow[l] >= 0, "negative leaf row offset");
-    for (int v = 0; v < nn; ++v) arg_check(s.skelrow[v] >= 0, "negative skeleton row offset");
-}
-
-static int add_tiles(TaskLists &L, int task, int m) {
-    const int bm = L.task_kind[task] == 1 ? X3_BM : SG_BM;
-    int n = (m + bm - 1) / bm;
-    for (int i = 0; i < n; ++i) L.tiles.push_back(make_int2(task, i));
-    return n;
-}
-
-// Order of the tiles of every launch.  xcd_remap hands XCD x the x-th
-// contiguous eighth of a launch's work items (the column tiles of a tile are
-// adjacent).  The tiles, in structural order (tree order: neighbouring
-// leaves and nodes share W and skeleton blocks, which then stay in one XCD's
-// L2), are cut into eight contiguous groups of about equal work; each group
-// is sorted longest first (so long lists, such as the adjoint's
-// column-grouped couplings at the top of the tree, start early) and padded
-// with empty tiles to the size of the largest, so that group x is exactly
-// the eighth xcd_remap gives XCD x.
-static void order_tiles(TaskLists &L, int bk) {
-    if (L.dummy < 0) {
-        Task tk{};
-        tk.c_buf = VEC_USKEL;
-        L.dummy = (int)L.tasks.size();
-        L.tasks.push_back(tk);
-        L.task_kind.push_back(L.task_kind.empty() ? 0 : L.task_kind.back());
-    }
-    std::vector<int2> out;
-    std::vector<Launch> lns;
-    for (const Launch &ln : L.launches) {
-        const int T = ln.ntiles;
-        std::vector<std::pair<int64_t, int2>> v;
-        v.reserve(T);
-        int64_t total = 0;
-        for (int i = 0; i < T; ++i) {
-            int2 tl = L.tiles[ln.tile_offset + i];
-            const Task &tk = L.tasks[tl.x];
-            int64_t cost = 2;   // prologue and epilogue
-            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
-            v.push_back({cost, tl});
-            total += cost;
-        }
-        const int ng = 8;
-        std::vector<int> cut(ng + 1, T);
-        cut[0] = 0;
-        int64_t acc = 0;
-        int g = 1;
-        for (int i = 0; i < T && g < ng; ++i) {
-            acc += v[i].first;
-            while (g < ng && acc * ng >= total * g) cut[g++] = i + 1;
-        }
-        int gmax = 0;
-        for (int q = 0; q < ng; ++q) gmax = std::max(gmax, cut[q + 1] - cut[q]);
-        Launch nl{(int)out.size(), gmax * ng};
-        for (int q = 0; q < ng; ++q) {
-            std::stable_sort(v.begin() + cut[q], v.begin() + cut[q + 1],
-                             [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
-                                 return a.first > b.first;
-                             });
-            for (int i = cut[q]; i < cut[q + 1]; ++i) out.push_back(v[i].second);
-            for (int i = cut[q + 1] - cut[q]; i < gmax; ++i) out.push_back(make_int2(L.dummy, 0));
-        }
-        lns.push_back(nl);
-    }
-    L.tiles.swap(out);
-    L.launches.swap(lns);
-}
-
-// A task with more than maxseg segments keeps its first maxseg; the rest go
-// to continuation tasks (beta = 1, same output block and row tiles), the
-// k-th continuation in a launch of its own inserted right after the task's
-// launch, so the parts accumulate in order without write conflicts.  Used
-// for the BF16x3 kernel, which stages a task's segment list in LDS.
-static void split_long_tasks(TaskLists &L, int maxseg) {
-    std::vector<int2> tiles;
-    std::vector<Launch> launches;
-    std::vector<int> kinds;
-    std::vector<int> seen(L.tasks.size(), -1);
-    for (size_t li = 0; li < L.launches.size(); ++li) {
-        const Launch ln = L.launches[li];
-        // continuation tasks of this launch, by depth
-        std::vector<std::vector<int>> cont;
-        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) {
-            const int task = L.tiles[t].x;
-            if (seen[task] == (int)li) continue;
-            seen[task] = (int)li;
-            const Task base = L.tasks[task];
-            const int nseg = base.seg_end - base.seg_begin;
-            if (nseg <= maxseg) continue;
-            L.tasks[task].seg_end = base.seg_begin + maxseg;
-            for (int k = 1; k * maxseg < nseg; ++k) {
-                Task c = base;
-                c.seg_begin = base.seg_begin + k * maxseg;
-                c.seg_end = std::min(base.seg_end, c.seg_begin + maxseg);
-                c.beta = 1;
-                if ((int)cont.size() < k) cont.resize(k);
-                cont[k - 1].push_back((int)L.tasks.size());
-                L.tasks.push_back(c);
-                L.task_kind.push_back(L.task_kind[task]);
-            }
-        }
-        Launch nl{(int)tiles.size(), ln.ntiles};
-        for (int t = ln.tile_offset; t < ln.tile_offset + ln.ntiles; ++t) tiles.push_back(L.tiles[t]);
-        launches.push_back(nl);
-        kinds.push_back(L.kinds[li]);
-        for (auto &group : cont) {
-            Launch cl{(int)tiles.size(), 0};
-            for (int ct : group) {
-                const int bm = L.task_kind[ct] == 1 ? X3_BM : SG_BM;
-                for (int i = 0; i < (L.tasks[ct].m + bm - 1) / bm; ++i) tiles.push_back(make_int2(ct, i));
-            }
-            cl.ntiles = (int)tiles.size() - cl.tile_offset;
-            launches.push_back(cl);
-            kinds.push_back(L.kinds[li]);
-        }
-        seen.resize(L.tasks.size(), -1);
-    }
-    L.tiles.swap(tiles);
-    L.launches.swap(launches);
-    L.kinds.swap(kinds);
-}
-
-// Build the forward task lists for one kernel kind (0: segmented MFMA GEMM
-// in FP64/FP32 with 64-row tiles, 1: split-BF16x3 with 128-row tiles).
-// beta_u / beta_y select the accumulation rule of the coupling and leaf
-// tasks (0 for mx_eval, the reference's rules for the drop-in entry points).
-// wrow_in / skel_in give the row offsets of W blocks and skeleton blocks in
-// the buffers the kernel reads (reference layout for kind 0, split panels for
-// kind 1); Y always uses the reference layout.
-static TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir,
-                             int sched, const int64_t *wrow_in, const int64_t *skel_in) {
-    TaskLists L;
-    const int nn = s.numnodes;
-    // far pair k couples row node fx(k) with column node fy(k) through B_k
-    auto fx = [&](int64_t k) { return s.is_hss ? (int)k + 1 : s.fpairx[k]; };
-    auto fy = [&](int64_t k) {
-        if (!s.is_hss) return s.fpairy[k];
-        int x = (int)k + 1, p = t.parent[x];
-        return s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
-    };
-    auto nx = [&](int64_t k) { return s.is_hss ? (int)k : s.npairx[k]; };
-    auto ny = [&](int64_t k) { return s.is_hss ? (int)k : s.npairy[k]; };
-    // pairs grouped by destination: the row node (forward) or the column
-    // node (adjoint, where the block acts transposed)
-    std::vector<std::vector<int64_t>> far_of(nn), near_of(s.nleaf);
-    for (int64_t k = 0; k < s.fcount; ++k) far_of[dir == DIR_FWD ? fx(k) : fy(k)].push_back(k);
-    for (int64_t k = 0; k < s.ncount; ++k) near_of[dir == DIR_FWD ? nx(k) : ny(k)].push_back(k);
-
-    auto begin_launch = [&]() {
-        L.launches.push_back(Launch{(int)L.tiles.size(), 0});
-        L.kinds.push_back(kind);
-    };
-    auto end_launch = [&]() {
-        L.launches.back().ntiles = (int)L.tiles.size() - L.launches.back().tile_offset;
-        if (L.launches.back().ntiles == 0) {
-            L.launches.pop_back();
-            L.kinds.pop_back();
-        }
-    };
-    auto new_task = [&](int64_t c_row, int m, int c_buf, int beta) {
-        Task tk{};
-        tk.c_row = c_row;
-        tk.m = m;
-        tk.c_buf = c_buf;
-        tk.seg_begin = (int)L.segs.size();
-        tk.seg_end = tk.seg_begin;
-        tk.beta = beta;
-        L.tasks.push_back(tk);
-        L.task_kind.push_back(kind);
-        return (int)L.tasks.size() - 1;
-    };
-    auto add_seg = [&](int task, int64_t a_off, int a_buf, int a_trans, int lda, int k, int b_buf, int64_t b_row,
-                       int b_ld) {
-        if (k <= 0) return;
-        Seg sg{};
-        sg.a_off = a_off;
-        sg.a_buf = a_buf;
-        sg.a_trans = a_trans;
-        sg.lda = lda;
-        sg.k = k;
-        sg.b_buf = b_buf;
-        sg.b_row = b_row;
-        sg.b_ld = b_ld;
-        L.segs.push_back(sg);
-        L.tasks[task].seg_end = (int)L.segs.size();
-    };
-    const int maxd = (int)t.levels.size() - 1;
-    // UP: deepest level first
-    for (int d = maxd; d >= 1; --d) {
-        begin_launch();
-        for (int v : t.levels[d]) {
-            int sv = s.slen[v];
-            if (sv == 0) continue;
-            int tk = new_task(skel_in[v], sv, VEC_WSKEL, 0);
-            if (s.lchildren[v] == -1) {
-                int lf = s.lm[v];
-                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.Ddim[lf], VEC_W, wrow_in[lf], s.Ddim[lf]);
-            } else {
-                int l = s.lchildren[v], r = s.rchildren[v];
-                add_seg(tk, s.VTptr[v], MAT_VT, 0, sv, s.slen[l], VEC_WSKEL, skel_in[l], s.slen[l]);
-                add_seg(tk, (int64_t)s.VTptr[v] + (int64_t)sv * s.slen[l], MAT_VT, 0, sv, s.slen[r], VEC_WSKEL,
-                        skel_in[r], s.slen[r]);
-            }
-            add_tiles(L, tk, sv);
-        }
-        end_launch();
-    }
-    auto add_far = [&](int tk, int c) {
-        const int sc = s.slen[c];
-        for (int64_t k : far_of[c]) {
-            if (dir == DIR_FWD) {
-                int y = fy(k);
-                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 0, sc, s.slen[y], VEC_WSKEL, skel_in[y], s.slen[y]);
-            } else {   // B_k^T, B_k stored s_x x s_y
-                int x = fx(k);
-                add_seg(tk, (int64_t)s.Bptr[k], MAT_B, 1, s.slen[x], s.slen[x], VEC_WSKEL, skel_in[x], s.slen[x]);
-            }
-        }
-    };
-    auto add_down = [&](int tk, int c) {   // VT_p[:, part(c)]^T u_p; false when there is none
-        int p = t.parent[c];
-        if (p <= 0 || s.slen[p] == 0) return false;
-        int sp = s.slen[p];
-        int64_t off = s.VTptr[p];
-        if (s.rchildren[p] == c) off += (int64_t)sp * s.slen[s.lchildren[p]];
-        add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
-        return true;
-    };
-    if (sched == SCHED_FUSED) {
-        // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
-        for (int d = 1; d <= maxd; ++d) {
-            begin_launch();
-            for (int c : t.levels[d]) {
-                int sc = s.slen[c];
-                if (sc == 0) continue;
-                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
-                add_far(tk, c);
-                add_down(tk, c);
-                add_tiles(L, tk, sc);
-            }
-            end_launch();
-        }
-    } else {
-        // FAR for every level at once, then DOWN level by level (beta = 1)
-        begin_launch();
-        for (int d = 1; d <= maxd; ++d)
-            for (int c : t.levels[d]) {
-                int sc = s.slen[c];
-                if (sc == 0) continue;
-                int tk = new_task(skel_in[c], sc, VEC_USKEL, beta_u);
-                add_far(tk, c);
-                add_tiles(L, tk, sc);
-            }
-        end_launch();
-        for (int d = 2; d <= maxd; ++d) {
-            begin_launch();
-            for (int c : t.levels[d]) {
-                int sc = s.slen[c];
-                if (sc == 0) continue;
-                int tk = new_task(skel_in[c], sc, VEC_USKEL, 1);
-                if (add_down(tk, c)) add_tiles(L, tk, sc);
-                else L.tasks.pop_back();
-            }
-            end_launch();
-        }
-    }
-    // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x
-    begin_launch();
-    for (int lf = 0; lf < s.nleaf; ++lf) {
-        if (s.Ddim[lf] == 0) continue;
-        int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
-        for (int64_t k : near_of[lf]) {
-            if (dir == DIR_FWD) {
-                int y = ny(k);
-                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
-            } else {   // D_k^T, D_k stored Dim_x x Dim_y
-                int x = nx(k);
-                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 1, s.Ddim[x], s.Ddim[x], VEC_W, wrow_in[x], s.Ddim[x]);
-            }
-        }
-        int v = t.node_of_leaf[lf];
-        int sv = s.slen[v];
-        if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, skel_in[v], sv);
-        add_tiles(L, tk, s.Ddim[lf]);
-    }
-    end_launch();
-    order_tiles(L, kind == 1 ? X3_BK : SG_BK);
-    return L;
-}
-
-// Gradient products of every stored block (see mx_grad.h): one task per
-// near block, far block, leaf VT and half of an internal VT, tiled 64 x 64.
-struct GradLists {
-    std::vector<GTask> tasks;
-    std::vector<GSeg> segs;
-    std::vector<int4> tiles;
-    double flops = 0.0;   // per nrhs column
-};
-
-static GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
-    GradLists G;
-    auto task = [&](int64_t c_off, int m, int n, int ldc, int c_buf) {
-        GTask k{};
-        k.c_off = c_off;
-        k.m = m;
-        k.n = n;
-        k.ldc = ldc;
-        k.c_buf = c_buf;
-        k.seg_begin = k.seg_end = (int)G.segs.size();
-        G.tasks.push_back(k);
-        return (int)G.tasks.size() - 1;
-    };
-    auto seg = [&](int tk, int x_buf, int64_t x_row, int x_ld, int z_buf, int64_t z_row, int z_ld) {
-        GSeg g{};
-        g.x_buf = x_buf;
-        g.x_row = x_row;
-        g.x_ld = x_ld;
-        g.z_buf = z_buf;
-        g.z_row = z_row;
-        g.z_ld = z_ld;
-        G.segs.push_back(g);
-        G.tasks[tk].seg_end = (int)G.segs.size();
-        G.flops += 2.0 * x_ld * z_ld;
-    };
-    auto tiles = [&](int tk) {
-        const GTask &k = G.tasks[tk];
-        for (int i = 0; i < (k.m + 63) / 64; ++i)
-            for (int j = 0; j < (k.n + 63) / 64; ++j) G.tiles.push_back(make_int4(tk, i, j, 0));
-    };
-    for (int64_t k = 0; k < s.ncount; ++k) {
-        int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
-        int mx_ = s.Ddim[x], ny_ = s.Ddim[y];
-        if (mx_ == 0 || ny_ == 0) continue;
-        int tk = task((int64_t)s.Dptr[k], mx_, ny_, mx_, MAT_D);
-        seg(tk, GB_DY, s.wrow[x], mx_, GB_W, s.wrow[y], ny_);
-        tiles(tk);
-    }
-    for (int64_t k = 0; k < s.fcount; ++k) {
-        int x, y;
-        if (s.is_hss) {
-            x = (int)k + 1;
-            int p = t.parent[x];
-            y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
-        } else {
-            x = s.fpairx[k];
-            y = s.fpairy[k];
-        }
-        int sx = s.slen[x], sy = s.slen[y];
-        if (sx == 0 || sy == 0) continue;
-        int tk = task((int64_t)s.Bptr[k], sx, sy, sx, MAT_B);
-        seg(tk, GB_YHAT, s.skelrow[x], sx, GB_WHAT, s.skelrow[y], sy);
-        tiles(tk);
-    }
-    for (int v = 1; v < s.numnodes; ++v) {
-        int sv = s.slen[v];
-        if (sv == 0) continue;
-        if (s.lchildren[v] == -1) {
-            int lf = s.lm[v], dm = s.Ddim[lf];
-            if (dm == 0) continue;
-            int tk = task(s.VTptr[v], sv, dm, sv, MAT_VT);
-            seg(tk, GB_GHAT, s.skelrow[v], sv, GB_W, s.wrow[lf], dm);
-            seg(tk, GB_UHAT, s.skelrow[v], sv, GB_DY, s.wrow[lf], dm);
-            tiles(tk);
-        } else {
-            int64_t off = s.VTptr[v];
-            for (int c : {s.lchildren[v], s.rchildren[v]}) {
-                int sc = s.slen[c];
-                if (sc > 0) {
-                    int tk = task(off, sv, sc, sv, MAT_VT);
-                    seg(tk, GB_GHAT, s.skelrow[v], sv, GB_WHAT, s.skelrow[c], sc);
-                    seg(tk, GB_UHAT, s.skelrow[v], sv, GB_YHAT, s.skelrow[c], sc);
-                    tiles(tk);
-                }
-                off += (int64_t)sv * sc;
-            }
-        }
-    }
-    return G;
-}
-
-template <typename T>
-static T *upload(const T *h, size_t n) {
-    T *d = nullptr;
-    HIPCHK(hipMalloc(&d, std::max<size_t>(n, 1) * sizeof(T)));
-    if (h && n > 0) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
-    return d;
-}
+thread_local std::string g_err;
 
 static void upload_lists(const TaskLists &L, Task **t, Seg **s, int2 **tl) {
     *t = upload(L.tasks.data(), L.tasks.size());
@@ -699,200 +46,6 @@ __global__ void cvt_f32_f64(const float *__restrict__ in, double *__restrict__ o
     for (; i < n; i += stride) out[i] = (double)in[i];
 }
 
-static int cvt_grid(int64_t n) {
-    int64_t g = (n + 255) / 256;
-    return (int)std::min<int64_t>(std::max<int64_t>(g, 1), 8192);
-}
-
-// ---- kernel blocks generated on the device --------------------------------
-//
-// D_k = K(points of leaf x, points of leaf y) and B_k = K(skeleton points of
-// x, skeleton points of y) computed where they are needed instead of being
-// uploaded, for matrices whose FP64 blocks do not fit the host (the
-// reference computes them on the host, matroxTest-V1/compression.h:256-360,
-// with the same norm expansion |a|^2 + |b|^2 - 2 a.b).
-
-struct GenInfo {
-    const double *X = nullptr;        // host, n x d row-major, original point order
-    int64_t n = 0;
-    int d = 0;
-    double h = 1.0;
-    const int32_t *lids = nullptr;    // host, N: leaf-order row -> point id
-    const int32_t *skels = nullptr;   // host: skeleton point ids, node v at skoff[v]
-    const int64_t *skoff = nullptr;   // host, numnodes
-    int64_t nskels = 0;
-};
-
-struct alignas(16) GenBlock {
-    int64_t out;    // element offset of the block (column-major na x nb)
-    int64_t a, b;   // offsets of its row and column point ids in the index array
-    int32_t na, nb;
-};
-
-__global__ void point_norms(const double *__restrict__ X, int64_t n, int d, double *__restrict__ nrm) {
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int q = 0; q < d; ++q) s += X[p * d + q] * X[p * d + q];
-        nrm[p] = s;
-    }
-}
-
-__device__ __forceinline__ double gauss_entry(const double *__restrict__ X, const double *__restrict__ nrm, int d,
-                                              double kscal, int64_t pa, int64_t pb) {
-    const double *xa = X + pa * d, *xb = X + pb * d;
-    double dot = 0.0;
-    for (int q = 0; q < d; ++q) dot += xa[q] * xb[q];
-    return exp(kscal * (-2.0 * dot + nrm[pa] + nrm[pb]));
-}
-
-// blocks[blockIdx.y] in its stored layout (column-major na x nb), FP64 or FP32
-template <typename T>
-__global__ void gen_blocks(const double *__restrict__ X, const double *__restrict__ nrm, int d, double kscal,
-                           const int32_t *__restrict__ idx, const GenBlock *__restrict__ blocks, T *__restrict__ out) {
-    const GenBlock g = blocks[blockIdx.y];
-    const int64_t ne = (int64_t)g.na * g.nb;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(e % g.na), j = (int)(e / g.na);
-        out[g.out + e] = (T)gauss_entry(X, nrm, d, kscal, idx[g.a + i], idx[g.b + j]);
-    }
-}
-
-// Packed split-BF16 operand xb[blockIdx.y] (fragment order as pack_x3_blocks)
-// computed from its generating block gb[blockIdx.y]: op(A)(i, k) is entry
-// (i, k) of the block, or (k, i) when transposed.
-__global__ void gen_pack_x3(const double *__restrict__ X, const double *__restrict__ nrm, int d, double kscal,
-                            const int32_t *__restrict__ idx, const X3Block *__restrict__ xb,
-                            const GenBlock *__restrict__ gb, uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
-    const X3Block bk = xb[blockIdx.y];
-    const GenBlock g = gb[blockIdx.y];
-    const int nfk = (bk.K + 31) / 32, nfm = (bk.M + 15) / 16;
-    const int64_t lanes = (int64_t)nfm * nfk * 64;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < lanes;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t frag = t >> 6;
-        const int lane = (int)(t & 63);
-        const int mi = (int)(frag / nfk), ki = (int)(frag % nfk);
-        const int row = mi * 16 + (lane & 15);
-        const int kb = ki * 32 + 8 * (lane >> 4);
-        const int64_t dst = bk.dst_off + frag * 512 + (int64_t)lane * 8;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int k = kb + e;
-            double x = 0.0;
-            if (row < bk.M && k < bk.K) {
-                const int r = bk.trans ? k : row, c = bk.trans ? row : k;
-                x = gauss_entry(X, nrm, d, kscal, idx[g.a + r], idx[g.b + c]);
-            }
-            const uint16_t h = bf16_rne((float)x);
-            hi[dst + e] = h;
-            lo[dst + e] = bf16_rne((float)(x - (double)bf16_f(h)));
-        }
-    }
-}
-
-// Points, norms and block lists of one matrix whose D and B are generated:
-// near blocks index the leaf-order point ids (lids), coupling blocks the
-// skeleton point ids.
-struct GenState {
-    double *Xd = nullptr, *nrm = nullptr;
-    int32_t *idx[2] = {nullptr, nullptr};   // 0: lids, 1: skeleton ids
-    int d = 0;
-    double kscal = 0.0;
-    std::vector<GenBlock> blk[2];           // 0: near (D), 1: coupling (B)
-    std::map<int64_t, int> at[2];           // block offset -> index in blk
-    GenState() = default;
-    GenState(const GenState &) = delete;
-    GenState &operator=(const GenState &) = delete;
-    ~GenState() {
-        for (void *p : {(void *)Xd, (void *)nrm, (void *)idx[0], (void *)idx[1]})
-            if (p) (void)hipFree(p);
-    }
-
-    void init(const mx_structure &s, const Topology &t, const GenInfo &gen) {
-        arg_check(gen.X && gen.n >= 1 && gen.d >= 1 && gen.h > 0 && gen.lids && gen.skels && gen.skoff,
-                  "bad block-generation arguments");
-        int64_t N = 0;
-        for (int l = 0; l < s.nleaf; ++l) N = std::max<int64_t>(N, (int64_t)s.wrow[l] + s.Ddim[l]);
-        for (int64_t r = 0; r < N; ++r) arg_check(gen.lids[r] >= 0 && gen.lids[r] < gen.n, "lids out of range");
-        for (int64_t i = 0; i < gen.nskels; ++i)
-            arg_check(gen.skels[i] >= 0 && gen.skels[i] < gen.n, "skeleton point id out of range");
-        for (int v = 0; v < s.numnodes; ++v)
-            arg_check(s.slen[v] == 0 || (gen.skoff[v] >= 0 && gen.skoff[v] + s.slen[v] <= gen.nskels),
-                      "skeleton offsets out of range");
-        d = gen.d;
-        kscal = -0.5 / (gen.h * gen.h);
-        Xd = upload(gen.X, (size_t)gen.n * gen.d);
-        HIPCHK(hipMalloc(&nrm, sizeof(double) * (size_t)gen.n));
-        point_norms<<<cvt_grid(gen.n), 256>>>(Xd, gen.n, gen.d, nrm);
-        HIPCHK(hipGetLastError());
-        idx[0] = upload(gen.lids, (size_t)N);
-        idx[1] = upload(gen.skels, (size_t)gen.nskels);
-        for (int64_t k = 0; k < s.ncount; ++k) {
-            int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
-            if (s.Ddim[x] == 0 || s.Ddim[y] == 0) continue;
-            at[0][(int64_t)s.Dptr[k]] = (int)blk[0].size();
-            blk[0].push_back(GenBlock{(int64_t)s.Dptr[k], s.wrow[x], s.wrow[y], s.Ddim[x], s.Ddim[y]});
-        }
-        for (int64_t k = 0; k < s.fcount; ++k) {
-            int x, y;
-            if (s.is_hss) {
-                x = (int)k + 1;
-                int p = t.parent[x];
-                y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
-            } else {
-                x = s.fpairx[k];
-                y = s.fpairy[k];
-            }
-            if (s.slen[x] == 0 || s.slen[y] == 0) continue;
-            at[1][(int64_t)s.Bptr[k]] = (int)blk[1].size();
-            blk[1].push_back(GenBlock{(int64_t)s.Bptr[k], gen.skoff[x], gen.skoff[y], s.slen[x], s.slen[y]});
-        }
-    }
-
-    // w: 0 = D, 1 = B, in their stored layout
-    template <typename T>
-    void emit(int w, T *out) const {
-        const std::vector<GenBlock> &b = blk[w];
-        for (size_t b0 = 0; b0 < b.size(); b0 += 65535) {
-            const size_t nb = std::min<size_t>(65535, b.size() - b0);
-            GenBlock *db = upload(b.data() + b0, nb);
-            gen_blocks<T><<<dim3(64, (unsigned)nb), 256>>>(Xd, nrm, d, kscal, idx[w], db, out);
-            hipError_t e = hipGetLastError();
-            if (e == hipSuccess) e = hipDeviceSynchronize();
-            (void)hipFree(db);
-            if (e != hipSuccess) throw MxError(MX_ERR_HIP, std::string("block generation: ") + hipGetErrorString(e));
-        }
-    }
-
-    // packed split-BF16 operands of buffer w for the packing list xb
-    void emit_x3(int w, const std::vector<X3Block> &xb, uint16_t *hi, uint16_t *lo) const {
-        std::vector<X3Block> todo;
-        std::vector<GenBlock> src;
-        for (const X3Block &x : xb) {
-            if (x.M == 0 || x.K == 0) continue;
-            auto it = at[w].find(x.src_off);
-            if (it == at[w].end()) throw MxError(MX_ERR_ARG, "a packed operand does not start at a generated block");
-            todo.push_back(x);
-            src.push_back(blk[w][it->second]);
-        }
-        for (size_t b0 = 0; b0 < todo.size(); b0 += 65535) {
-            const size_t nb = std::min<size_t>(65535, todo.size() - b0);
-            X3Block *dx = upload(todo.data() + b0, nb);
-            GenBlock *dg = nullptr;
-            hipError_t e = hipMalloc(&dg, nb * sizeof(GenBlock));
-            if (e == hipSuccess) e = hipMemcpy(dg, src.data() + b0, nb * sizeof(GenBlock), hipMemcpyHostToDevice);
-            if (e == hipSuccess) {
-                gen_pack_x3<<<dim3(64, (unsigned)nb), 256>>>(Xd, nrm, d, kscal, idx[w], dx, dg, hi, lo);
-                e = hipGetLastError();
-                if (e == hipSuccess) e = hipDeviceSynchronize();
-            }
-            (void)hipFree(dx);
-            if (dg) (void)hipFree(dg);
-            if (e != hipSuccess) throw MxError(MX_ERR_HIP, std::string("block generation: ") + hipGetErrorString(e));
-        }
-    }
-};
-
 // Pack the split-BF16 operands of D, B and VT from FP64 sources.
 static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     for (int w = 0; w < 3; ++w) {
@@ -905,8 +58,7 @@ static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     }
 }
 
-static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y,
-                            const GenInfo *gen = nullptr) {
+mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y, const GenInfo *gen) {
     arg_check(s.nleaf > 0 && s.numnodes >= 1, "empty structure");
     arg_check(s.numnodes == 2 * s.nleaf - 1, "numnodes must be 2*nleaf-1 for a binary tree");
     arg_check(s.lchildren && s.rchildren && s.lm && s.Ddim && s.slen && s.VTptr && s.wrow && s.skelrow,
@@ -1050,8 +202,8 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         P->D64 = upload(gen ? nullptr : s.D, (size_t)s.D_len);
         P->B64 = upload(gen ? nullptr : s.B, (size_t)s.B_len);
         if (gen) {
-            G->emit<double>(0, P->D64);
-            G->emit<double>(1, P->B64);
+            G->emit(0, P->D64);
+            G->emit(1, P->B64);
         }
     }
     P->VT64 = upload(s.VT, (size_t)s.VT_len);
@@ -1061,8 +213,8 @@ static mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta
         P->D32 = upload<float>(nullptr, (size_t)s.D_len);
         P->B32 = upload<float>(nullptr, (size_t)s.B_len);
         if (gen) {
-            G->emit<float>(0, P->D32);
-            G->emit<float>(1, P->B32);
+            G->emit(0, P->D32);
+            G->emit(1, P->B32);
         } else {
             cvt_f64_f32<<<cvt_grid(s.D_len), 256>>>(P->D64, P->D32, s.D_len);
             cvt_f64_f32<<<cvt_grid(s.B_len), 256>>>(P->B64, P->B32, s.B_len);
@@ -1118,7 +270,7 @@ static void ensure_io_g(mx_plan *P, int nrhs) {
     }
 }
 
-static void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
+void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
     size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
     if (sb > P->skel_bytes) {
         if (P->wskel) HIPCHK(hipFree(P->wskel));
@@ -1141,7 +293,7 @@ static void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
     }
 }
 
-static void ensure_panels(mx_plan *P, int nrhs) {
+void ensure_panels(mx_plan *P, int nrhs) {
     size_t bytes = ((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t);
     if (bytes > P->panel_bytes) {
         if (P->panels) HIPCHK(hipFree(P->panels));
@@ -1180,8 +332,8 @@ static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
     }
 }
 
-static void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
-                 hipStream_t st) {
+void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+          hipStream_t st) {
     arg_check(W && Y, "null W or Y");
     arg_check(nrhs >= 1, "nrhs must be >= 1");
     arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
@@ -1470,8 +622,8 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
 // built into the plan on scratch inputs and keep the fastest.  A candidate
 // replaces the current choice only when it is more than 2% faster, so that
 // timing noise does not flip between near-equal choices.
-static void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st,
-                     float *best_ms) {
+void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st,
+              float *best_ms) {
     arg_check(nrhs >= 1 && reps >= 1, "bad nrhs or reps");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const size_t elem = io == MX_IO_F64 ? sizeof(double) : sizeof(float);
@@ -1679,9 +831,9 @@ __global__ void dense_kernel_rows(const double *__restrict__ X, int64_t N, int d
 
 // K(rows, :) W for points X_leaf (device, leaf order) of a leaf-blocked W:
 // the kernel rows, then one segmented GEMM over the leaf blocks.
-static void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,
-                            const int32_t *Ddim, const int32_t *rows, int nrows, const double *W, int nrhs,
-                            double *out, hipStream_t st) {
+void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,
+                     const int32_t *Ddim, const int32_t *rows, int nrows, const double *W, int nrhs,
+                     double *out, hipStream_t st) {
     arg_check(rows && W && out && nrows >= 1 && nrhs >= 1, "bad argument");
     for (int r = 0; r < nrows; ++r) arg_check(rows[r] >= 0 && rows[r] < N, "row index out of range");
     TaskLists L;
@@ -1872,345 +1024,6 @@ static int32_t compat_eval(bool hss, double *D, double *B, double *VT, uint64_t 
     HIPCHK(hipFree(dW));
     HIPCHK(hipFree(dY));
     return 0;
-}
-
-template <typename F>
-static int32_t guard(F &&f) {
-    try {
-        return (int32_t)f();
-    } catch (const MxError &e) {
-        g_err = e.what();
-        return (int32_t)e.code;
-    } catch (const std::exception &e) {
-        g_err = e.what();
-        return (int32_t)MX_ERR_ARG;
-    }
-}
-
-// ---- many independent instances (mx_batch) -------------------------------
-//
-// Instances are grouped by nrhs.  The instances of a group become one forest
-// joined under rank-0 nodes into a single binary tree: a dummy node has no
-// skeleton, so it takes part in no product, and every launch of the group's
-// plan covers the same tree level (or phase) of all its instances at once.
-// HSS instances are rewritten with explicit pair lists (near (l, l) at
-// Dptr[l], coupling (v, sibling) at Bptr[v-1]).  The groups run concurrently
-// on their own streams, forked from and joined back to the caller's stream,
-// and the whole batch can be captured as one hipGraph.
-
-struct BatchInst {
-    int group;
-    int64_t row_off;                 // first row in the group's W / Y
-    int64_t N;
-    int nleaf;
-    std::vector<int32_t> wrow, Ddim; // instance-local leaf layout
-    double *X_leaf = nullptr;        // points for dense check rows
-    int pts_d = 0;
-    double pts_h = 0.0;
-};
-
-// Combined structure of a group; owns its arrays.
-struct GroupStructure {
-    std::vector<int32_t> lc, rc, lm, Ddim, slen, npx, npy, fpx, fpy, VTptr, wrow, skelrow;
-    std::vector<uint64_t> Dptr, Bptr;
-    std::vector<double> D, B, VT;
-    int64_t N = 0;
-    mx_structure view() {
-        mx_structure s{};
-        s.nleaf = (int32_t)Ddim.size();
-        s.numnodes = (int32_t)lc.size();
-        s.is_hss = 0;
-        s.ncount = (int64_t)npx.size();
-        s.fcount = (int64_t)fpx.size();
-        s.lchildren = lc.data();
-        s.rchildren = rc.data();
-        s.lm = lm.data();
-        s.Ddim = Ddim.data();
-        s.slen = slen.data();
-        s.npairx = npx.data();
-        s.npairy = npy.data();
-        s.fpairx = fpx.data();
-        s.fpairy = fpy.data();
-        s.Dptr = Dptr.data();
-        s.Bptr = Bptr.data();
-        s.VTptr = VTptr.data();
-        s.wrow = wrow.data();
-        s.skelrow = skelrow.data();
-        s.D = D.data();
-        s.B = B.data();
-        s.VT = VT.data();
-        s.D_len = (int64_t)D.size();
-        s.B_len = (int64_t)B.size();
-        s.VT_len = (int64_t)VT.size();
-        return s;
-    }
-};
-
-static int64_t struct_rows(const mx_structure &s) {
-    int64_t n = 0;
-    for (int l = 0; l < s.nleaf; ++l) n = std::max<int64_t>(n, (int64_t)s.wrow[l] + s.Ddim[l]);
-    return n;
-}
-
-static void combine(const std::vector<const mx_structure *> &in, GroupStructure &G) {
-    const int k = (int)in.size();
-    int64_t nn = k - 1;
-    for (auto *s : in) nn += s->numnodes;
-    G.lc.assign(nn, -1);
-    G.rc.assign(nn, -1);
-    G.lm.assign(nn, -1);
-    G.slen.assign(nn, 0);
-    G.skelrow.assign(nn, 0);
-    G.VTptr.assign(nn, 0);
-    std::vector<int64_t> nbase(k), lbase(k), rbase(k), sbase(k), vbase(k), dbase(k), bbase(k);
-    int64_t nb = k - 1, lb = 0, rb = 0, sb = 0, vb = 0, db = 0, bb = 0;
-    for (int i = 0; i < k; ++i) {
-        const mx_structure &s = *in[i];
-        nbase[i] = nb;
-        lbase[i] = lb;
-        rbase[i] = rb;
-        sbase[i] = sb;
-        vbase[i] = vb;
-        dbase[i] = db;
-        bbase[i] = bb;
-        nb += s.numnodes;
-        lb += s.nleaf;
-        rb += struct_rows(s);
-        int64_t sk = 0;
-        for (int v = 0; v < s.numnodes; ++v) sk = std::max<int64_t>(sk, (int64_t)s.skelrow[v] + s.slen[v]);
-        sb += sk;
-        vb += s.VT_len;
-        db += s.D_len;
-        bb += s.B_len;
-    }
-    arg_check(rb < INT32_MAX && sb < INT32_MAX && vb < INT32_MAX, "batch group too large for 32-bit offsets");
-    G.N = rb;
-    // dummy nodes 0..k-2 in heap order; child ids >= k-1 are instance roots
-    for (int j = 0; j < k - 1; ++j) {
-        int c[2] = {2 * j + 1, 2 * j + 2};
-        int64_t id[2];
-        for (int t = 0; t < 2; ++t) id[t] = c[t] < k - 1 ? c[t] : nbase[c[t] - (k - 1)];
-        G.lc[j] = (int32_t)id[0];
-        G.rc[j] = (int32_t)id[1];
-    }
-    G.Ddim.reserve(lb);
-    G.wrow.reserve(lb);
-    G.D.reserve(db);
-    G.B.reserve(bb);
-    G.VT.reserve(vb);
-    for (int i = 0; i < k; ++i) {
-        const mx_structure &s = *in[i];
-        const int64_t o = nbase[i];
-        std::vector<int> parent(s.numnodes, -1);
-        for (int v = 0; v < s.numnodes; ++v) {
-            if (s.lchildren[v] >= 0) {
-                parent[s.lchildren[v]] = v;
-                parent[s.rchildren[v]] = v;
-                G.lc[o + v] = (int32_t)(s.lchildren[v] + o);
-                G.rc[o + v] = (int32_t)(s.rchildren[v] + o);
-            } else {
-                G.lm[o + v] = (int32_t)(s.lm[v] + lbase[i]);
-            }
-            G.slen[o + v] = s.slen[v];
-            G.skelrow[o + v] = (int32_t)(s.skelrow[v] + sbase[i]);
-            G.VTptr[o + v] = (int32_t)(s.VTptr[v] + vbase[i]);
-        }
-        for (int l = 0; l < s.nleaf; ++l) {
-            G.Ddim.push_back(s.Ddim[l]);
-            G.wrow.push_back((int32_t)(s.wrow[l] + rbase[i]));
-        }
-        for (int64_t q = 0; q < s.ncount; ++q) {
-            G.npx.push_back((int32_t)((s.is_hss ? q : s.npairx[q]) + lbase[i]));
-            G.npy.push_back((int32_t)((s.is_hss ? q : s.npairy[q]) + lbase[i]));
-            G.Dptr.push_back(s.Dptr[q] + (uint64_t)dbase[i]);
-        }
-        for (int64_t q = 0; q < s.fcount; ++q) {
-            int x, y;
-            if (s.is_hss) {
-                x = (int)q + 1;
-                arg_check(x < s.numnodes && parent[x] >= 0, "HSS coupling list longer than the tree");
-                int p = parent[x];
-                y = s.lchildren[p] == x ? s.rchildren[p] : s.lchildren[p];
-            } else {
-                x = s.fpairx[q];
-                y = s.fpairy[q];
-            }
-            G.fpx.push_back((int32_t)(x + o));
-            G.fpy.push_back((int32_t)(y + o));
-            G.Bptr.push_back(s.Bptr[q] + (uint64_t)bbase[i]);
-        }
-        if (s.D_len) G.D.insert(G.D.end(), s.D, s.D + s.D_len);
-        if (s.B_len) G.B.insert(G.B.end(), s.B, s.B + s.B_len);
-        if (s.VT_len) G.VT.insert(G.VT.end(), s.VT, s.VT + s.VT_len);
-    }
-}
-
-}  // namespace mx
-
-struct mx_batch {
-    std::vector<mx_plan *> plans;          // one per group
-    std::vector<int> group_nrhs;
-    std::vector<int64_t> group_rows;
-    std::vector<hipStream_t> streams;      // one per group (when there are several)
-    std::vector<hipEvent_t> events;        // fork + one join event per group
-    std::vector<mx::BatchInst> inst;
-    int use_graph = 0;
-    struct GraphEntry {
-        int prec, io;
-        std::vector<const void *> W;
-        std::vector<void *> Y;
-        hipGraphExec_t exec;
-        uint64_t used;
-    };
-    std::vector<GraphEntry> graphs;
-    uint64_t clock = 0;
-    ~mx_batch() {
-        for (auto *p : plans) delete p;
-        for (auto st : streams) (void)hipStreamDestroy(st);
-        for (auto e : events) (void)hipEventDestroy(e);
-        for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
-        for (auto &i : inst)
-            if (i.X_leaf) (void)hipFree(i.X_leaf);
-    }
-};
-
-namespace mx {
-
-static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs, int count, uint32_t precisions) {
-    arg_check(in && nrhs && count >= 1, "bad batch arguments");
-    std::unique_ptr<mx_batch> B(new mx_batch());
-    std::map<int, std::vector<int>> by_nrhs;
-    for (int i = 0; i < count; ++i) {
-        arg_check(in[i] != nullptr && nrhs[i] >= 1, "instance " + std::to_string(i) + ": null structure or bad nrhs");
-        by_nrhs[nrhs[i]].push_back(i);
-    }
-    B->inst.resize(count);
-    for (auto &kv : by_nrhs) {
-        const int g = (int)B->plans.size();
-        std::vector<const mx_structure *> members;
-        int64_t row = 0;
-        for (int i : kv.second) {
-            const mx_structure &s = *in[i];
-            arg_check(s.nleaf > 0 && s.numnodes == 2 * s.nleaf - 1 && s.wrow && s.Ddim,
-                      "instance " + std::to_string(i) + ": bad tree");
-            BatchInst &bi = B->inst[i];
-            bi.group = g;
-            bi.row_off = row;
-            bi.N = struct_rows(s);
-            bi.nleaf = s.nleaf;
-            bi.wrow.assign(s.wrow, s.wrow + s.nleaf);
-            bi.Ddim.assign(s.Ddim, s.Ddim + s.nleaf);
-            row += bi.N;
-            members.push_back(&s);
-        }
-        GroupStructure G;
-        combine(members, G);
-        mx_structure v = G.view();
-        B->plans.push_back(create_plan(v, precisions, 0, 0));
-        B->group_nrhs.push_back(kv.first);
-        B->group_rows.push_back(G.N);
-    }
-    const int ng = (int)B->plans.size();
-    if (ng > 1) {
-        B->streams.resize(ng, nullptr);
-        for (auto &st : B->streams) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        B->events.resize(ng + 1, nullptr);
-        for (auto &e : B->events) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    }
-    return B.release();
-}
-
-// all groups, forked from and joined back to st
-static void batch_eval(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
-                       hipStream_t st) {
-    const int ng = (int)B->plans.size();
-    for (int g = 0; g < ng; ++g) arg_check(W[g] && Y[g], "null W or Y of group " + std::to_string(g));
-    if (ng == 1) {
-        eval(B->plans[0], DIR_FWD, prec, W[0], Y[0], B->group_nrhs[0], io, st);
-        return;
-    }
-    HIPCHK(hipEventRecord(B->events[0], st));
-    for (int g = 0; g < ng; ++g) {
-        HIPCHK(hipStreamWaitEvent(B->streams[g], B->events[0], 0));
-        eval(B->plans[g], DIR_FWD, prec, W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
-        HIPCHK(hipEventRecord(B->events[g + 1], B->streams[g]));
-    }
-    for (int g = 0; g < ng; ++g) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
-}
-
-static void batch_size_work(mx_batch *B, mx_precision prec, mx_io_type io) {
-    for (size_t g = 0; g < B->plans.size(); ++g) {
-        mx_plan *P = B->plans[g];
-        if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
-        if (prec == MX_BF16X3) ensure_panels(P, B->group_nrhs[g]);
-        else ensure_work(P, B->group_nrhs[g], prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
-    }
-}
-
-static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
-                             hipStream_t st) {
-    const int ng = (int)B->plans.size();
-    for (auto &g : B->graphs) {
-        bool same = g.prec == prec && g.io == io;
-        for (int i = 0; same && i < ng; ++i) same = g.W[i] == W[i] && g.Y[i] == Y[i];
-        if (same) {
-            g.used = ++B->clock;
-            HIPCHK(hipGraphLaunch(g.exec, st));
-            return;
-        }
-    }
-    batch_size_work(B, prec, io);
-    for (auto *P : B->plans) P->profiling = 0;
-    hipStream_t cs = st;
-    bool own = false;
-    if (cs == nullptr) {
-        HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-        own = true;
-    }
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    try {
-        HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-        try {
-            batch_eval(B, prec, W, Y, io, cs);
-        } catch (...) {
-            hipGraph_t g2 = nullptr;
-            (void)hipStreamEndCapture(cs, &g2);
-            if (g2) (void)hipGraphDestroy(g2);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(cs, &graph));
-        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-        HIPCHK(hipGraphDestroy(graph));
-        graph = nullptr;
-    } catch (...) {
-        if (graph) (void)hipGraphDestroy(graph);
-        if (own) (void)hipStreamDestroy(cs);
-        throw;
-    }
-    if (own) HIPCHK(hipStreamDestroy(cs));
-    if (B->graphs.size() >= 4) {
-        auto lru = std::min_element(B->graphs.begin(), B->graphs.end(),
-                                    [](const mx_batch::GraphEntry &a, const mx_batch::GraphEntry &b) {
-                                        return a.used < b.used;
-                                    });
-        HIPCHK(hipGraphExecDestroy(lru->exec));
-        B->graphs.erase(lru);
-    }
-    mx_batch::GraphEntry e;
-    e.prec = prec;
-    e.io = io;
-    e.W.assign(W, W + ng);
-    e.Y.assign(Y, Y + ng);
-    e.exec = exec;
-    e.used = ++B->clock;
-    B->graphs.push_back(e);
-    HIPCHK(hipGraphLaunch(exec, st));
-}
-
-static void batch_drop_graphs(mx_batch *B) {
-    for (auto &g : B->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
-    B->graphs.clear();
 }
 
 }  // namespace mx
@@ -2510,129 +1323,6 @@ mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows) {
         arg_check(plan && N && skel_rows, "null argument");
         *N = plan->N;
         *skel_rows = plan->skel_rows;
-        return 0;
-    });
-}
-
-mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances, const int32_t *nrhs, int32_t count,
-                          uint32_t precisions) {
-    return (mx_status)guard([&]() {
-        arg_check(batch != nullptr, "null argument");
-        *batch = nullptr;
-        arg_check((precisions & MX_PLAN_ADJOINT) == 0, "batches are forward only");
-        *batch = create_batch(instances, nrhs, count, precisions);
-        return 0;
-    });
-}
-
-mx_status mx_batch_destroy(mx_batch *batch) {
-    delete batch;
-    return MX_OK;
-}
-
-mx_status mx_batch_groups(const mx_batch *batch, int32_t *ngroups) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && ngroups, "null argument");
-        *ngroups = (int32_t)batch->plans.size();
-        return 0;
-    });
-}
-
-mx_status mx_batch_group_info(const mx_batch *batch, int32_t group, int32_t *nrhs, int64_t *rows) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && nrhs && rows && group >= 0 && group < (int32_t)batch->plans.size(), "bad argument");
-        *nrhs = batch->group_nrhs[group];
-        *rows = batch->group_rows[group];
-        return 0;
-    });
-}
-
-mx_status mx_batch_layout(const mx_batch *batch, int32_t instance, int32_t *group, int64_t *row_offset) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && group && row_offset && instance >= 0 && instance < (int32_t)batch->inst.size(),
-                  "bad argument");
-        *group = batch->inst[instance].group;
-        *row_offset = batch->inst[instance].row_off;
-        return 0;
-    });
-}
-
-mx_status mx_batch_plan(mx_batch *batch, int32_t group, mx_plan **plan) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && plan && group >= 0 && group < (int32_t)batch->plans.size(), "bad argument");
-        *plan = batch->plans[group];
-        return 0;
-    });
-}
-
-mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
-                        void *stream) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && W && Y, "null argument");
-        arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
-        bool prof = false;
-        for (auto *P : batch->plans) prof = prof || P->profiling;
-        if (batch->use_graph && !prof) batch_eval_graph(batch, prec, W, Y, io, (hipStream_t)stream);
-        else batch_eval(batch, prec, W, Y, io, (hipStream_t)stream);
-        return 0;
-    });
-}
-
-mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable) {
-    return (mx_status)guard([&]() {
-        arg_check(batch != nullptr, "null batch");
-        batch->use_graph = enable ? 1 : 0;
-        if (!enable) batch_drop_graphs(batch);
-        return 0;
-    });
-}
-
-mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream) {
-    return (mx_status)guard([&]() {
-        arg_check(batch != nullptr, "null batch");
-        if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 I/O");
-        for (size_t g = 0; g < batch->plans.size(); ++g)
-            autotune(batch->plans[g], prec, batch->group_nrhs[g], io, reps, (hipStream_t)stream, nullptr);
-        batch_drop_graphs(batch);
-        return 0;
-    });
-}
-
-mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && total, "null argument");
-        *total = 0.0;
-        for (size_t g = 0; g < batch->plans.size(); ++g) {
-            const mx_plan *P = batch->plans[g];
-            const int kd = prec == MX_BF16X3 ? 1 : 0;
-            for (double f : P->prog[DIR_FWD][kd][P->sched[DIR_FWD][kd]].flops) *total += f * batch->group_nrhs[g];
-        }
-        return 0;
-    });
-}
-
-mx_status mx_batch_set_points(mx_batch *batch, int32_t instance, const double *X_leaf, int32_t d, double h) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && X_leaf && d >= 1 && h > 0 && instance >= 0 && instance < (int32_t)batch->inst.size(),
-                  "bad argument");
-        BatchInst &bi = batch->inst[instance];
-        if (bi.X_leaf) HIPCHK(hipFree(bi.X_leaf));
-        bi.X_leaf = nullptr;
-        bi.X_leaf = upload(X_leaf, (size_t)bi.N * d);
-        bi.pts_d = d;
-        bi.pts_h = h;
-        return 0;
-    });
-}
-
-mx_status mx_batch_dense_rows(mx_batch *batch, int32_t instance, const int32_t *rows, int32_t nrows, const double *W,
-                              double *out, void *stream) {
-    return (mx_status)guard([&]() {
-        arg_check(batch && instance >= 0 && instance < (int32_t)batch->inst.size(), "bad argument");
-        BatchInst &bi = batch->inst[instance];
-        if (!bi.X_leaf) throw MxError(MX_ERR_STATE, "mx_batch_set_points has not been called for this instance");
-        dense_rows_core(bi.X_leaf, bi.N, bi.pts_d, bi.pts_h, bi.nleaf, bi.wrow.data(), bi.Ddim.data(), rows, nrows,
-                        W, batch->group_nrhs[bi.group], out, (hipStream_t)stream);
         return 0;
     });
 }
