@@ -83,6 +83,9 @@ def parse():
     ap.add_argument("--skip-checks", action="store_true", help="skip the dense-row error checks")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and backward timings")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
+    ap.add_argument("--launch-variants", default="",
+                    help="comma-separated BF16x3 kernel variant per forward launch under the fused schedule, "
+                         "replacing the autotuner's choice (to profile a fixed configuration)")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the drop-in HGEMM entry point on host arrays (PCIe-inclusive)")
     return ap.parse_args()
@@ -243,9 +246,10 @@ def adjoint_timings(plan, timer, Wd, nrhs, prec, ms_fwd_mixed, ms_fwd_64, steps,
 
 
 def fd_check(plan, hm, wl, nrhs, torch, np, host, ndir=64, eps=1e-6):
-    """Gradients of L = 1/2|K~W - T|^2 (W, T FP32; FP32 backward) against FP64
-    central differences on ndir random directions (a quarter each in W, D, B,
-    VT), and dW against the oracle on the transposed structure."""
+    """Gradients of L = 1/2|K~W - T|^2 (W, T FP32; FP32, FP64 and, when the
+    plan has it, split-BF16x3 backward) against FP64 central differences on
+    ndir random directions (a quarter each in W, D, B, VT), and dW against the
+    oracle on the transposed structure."""
     from matrox_ru_amd.autograd import hgemm_apply
     from oracle import oracle
     dev = torch.device("cuda:0")
@@ -256,8 +260,9 @@ def fd_check(plan, hm, wl, nrhs, torch, np, host, ndir=64, eps=1e-6):
     VT0 = torch.from_numpy(np.ascontiguousarray(hm.VT)).to(dev)
     res = {}
     grads = {}
-    for gp in ("fp32", "fp64"):
-        Wg = (W32 if gp == "fp32" else W32.double()).clone().requires_grad_()
+    gps = [gp for gp in ("fp32", "fp64", "bf16x3") if gp in plan.precisions]
+    for gp in gps:
+        Wg = (W32.double() if gp == "fp64" else W32).clone().requires_grad_()
         D, B, VT = (t.clone().requires_grad_() for t in (D0, B0, VT0))
         Y = hgemm_apply(plan, Wg, D, B, VT, nrhs=nrhs, precision=gp)
         L = 0.5 * ((Y - T32.to(Y.dtype)) ** 2).sum()
@@ -270,7 +275,7 @@ def fd_check(plan, hm, wl, nrhs, torch, np, host, ndir=64, eps=1e-6):
                                                         / np.linalg.norm(Wo))
     W64, T64 = W32.double(), T32.double()
     lens = [hm.N * nrhs, len(hm.D), len(hm.B), len(hm.VT)]
-    an = {"fp32": [], "fp64": []}
+    an = {gp: [] for gp in gps}
     fd = []
     group = []
     for i in range(ndir):
@@ -508,6 +513,8 @@ def main():
     TW = hm.transform(W)
     del W
     precs = ["fp64", prec] + (["fp32"] if prec != "fp32" and not args.no_adjoint else [])
+    if adjoint_wl and "bf16x3" not in precs:
+        precs.append("bf16x3")   # mixed-precision backward in the finite-difference check
     plan = hgemm.Plan(hm, precisions=tuple(precs), adjoint=not args.no_adjoint or adjoint_wl)
     perm = hm.lids[: hm.N]
     plan.set_points(X[perm], wl["h"])
@@ -530,6 +537,12 @@ def main():
         if p == "bf16x3":
             tuning[p]["launch_variants"] = {d: plan.launch_variants(d)
                                             for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
+    if args.launch_variants and "bf16x3" in plan.precisions:
+        plan.set_schedule("forward", "bf16x3", "fused")
+        plan.set_launch_variants([int(v) for v in args.launch_variants.split(",")], "forward")
+        tuning["bf16x3"]["forward"] = plan.get_schedule("forward", "bf16x3")
+        tuning["bf16x3"]["launch_variants"]["forward"] = plan.launch_variants("forward")
+        tuning["bf16x3"]["pinned"] = True
     log(f"[bench] autotuned: {tuning}")
     plan.set_graph(not args.no_graph)
     ms64 = timer(lambda: plan.eval(Wd, Y64, nrhs, precision="fp64"), max(3, args.steps // 4), 1)

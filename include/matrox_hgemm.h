@@ -222,6 +222,12 @@ mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precis
 mx_status mx_plan_launch_variants(const mx_plan *plan, int32_t direction, int32_t *variants, int32_t cap,
                                   int32_t *count);
 
+/* Fix the BF16x3 kernel variant of each launch of one direction under its
+ * current schedule (count = number of launches, as mx_plan_launch_variants
+ * reports; count = 0 returns to the plan-wide variant).  No counterpart in the
+ * reference; lets a profiled run replay the autotuner's choices. */
+mx_status mx_plan_set_launch_variants(mx_plan *plan, int32_t direction, const int32_t *variants, int32_t count);
+
 /* Autotuner: times every schedule (and, for MX_BF16X3, every kernel
  * variant, first for the whole evaluation, then launch by launch) of each
  * direction built into the plan with `reps` evaluations of

@@ -1255,6 +1255,28 @@ mx_status mx_plan_launch_variants(const mx_plan *plan, int32_t direction, int32_
     });
 }
 
+mx_status mx_plan_set_launch_variants(mx_plan *plan, int32_t direction, const int32_t *variants, int32_t count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && (direction == 0 || direction == 1), "bad argument");
+        if (direction == DIR_ADJ && !plan->adjoint)
+            throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+        const int sc = plan->sched[direction][1];
+        std::vector<int> &lv = plan->lvar[direction][sc];
+        if (count == 0) {
+            lv.clear();
+        } else {
+            arg_check(variants != nullptr, "null variants");
+            arg_check(count == (int32_t)plan->prog[direction][1][sc].launches.size(),
+                      "count must equal the number of launches of the current schedule");
+            for (int32_t i = 0; i < count; ++i)
+                arg_check(variants[i] >= 1 && variants[i] <= X3_NVARIANT, "bad BF16x3 kernel variant");
+            lv.assign(variants, variants + count);
+        }
+        drop_graphs(plan);
+        return 0;
+    });
+}
+
 mx_status mx_plan_autotune(mx_plan *plan, mx_precision prec, int32_t nrhs, mx_io_type io, int32_t reps,
                            float *best_ms, void *stream) {
     return (mx_status)guard([&]() {

@@ -84,6 +84,8 @@ def lib():
         L.mx_plan_autotune.restype = ctypes.c_int
         L.mx_plan_launch_variants.argtypes = [P, I32, P, I32, ctypes.POINTER(I32)]
         L.mx_plan_launch_variants.restype = ctypes.c_int
+        L.mx_plan_set_launch_variants.argtypes = [P, I32, P, I32]
+        L.mx_plan_set_launch_variants.restype = ctypes.c_int
         L.mx_batch_create.argtypes = [ctypes.POINTER(P), P, P, I32, ctypes.c_uint32]
         L.mx_batch_create.restype = ctypes.c_int
         L.mx_batch_destroy.argtypes = [P]
@@ -304,6 +306,12 @@ class Plan:
         cnt = I32()
         _check(lib().mx_plan_launch_variants(self._h, DIRECTIONS[direction], v, cap, ctypes.byref(cnt)))
         return [v[i] for i in range(min(cnt.value, cap))]
+
+    def set_launch_variants(self, variants, direction="forward"):
+        """Fix the BF16x3 kernel variant of each launch (mx_plan_set_launch_variants);
+        an empty list returns to the plan-wide variant."""
+        v = (I32 * max(len(variants), 1))(*variants)
+        _check(lib().mx_plan_set_launch_variants(self._h, DIRECTIONS[direction], v, len(variants)))
 
     def grad_flops(self, nrhs):
         f = ctypes.c_double()

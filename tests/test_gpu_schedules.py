@@ -76,6 +76,43 @@ def test_autotune_keeps_results(gpu):
         plan.autotune("fp32", nrhs)
 
 
+def test_pinned_launch_variants(gpu):
+    """mx_plan_set_launch_variants: every launch on its own variant gives the
+    same result as the plan-wide default, in both directions."""
+    import torch
+    _say("[pinned launch variants]")
+    X, hm = make_problem(**CASES["h2b_small"])
+    nrhs = 64
+    TW = host.normal((hm.N * nrhs,), seed=19)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("bf16x3",), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    Y0, Y1 = torch.empty_like(W), torch.empty_like(W)
+    for d in ("forward", "adjoint"):
+        run = (lambda Y: plan.eval(W, Y, nrhs, "bf16x3")) if d == "forward" else \
+              (lambda Y: plan.eval_adjoint(W, Y, nrhs, "bf16x3"))
+        run(Y0)
+        n = len(plan.launch_variants(d))
+        pinned = [1 + (i * 3) % 8 for i in range(n)]
+        plan.set_launch_variants(pinned, d)
+        assert plan.launch_variants(d) == pinned
+        run(Y1)
+        torch.cuda.synchronize()
+        e = rel(Y1.cpu().numpy(), Y0.cpu().numpy())
+        _say(f"  {d}: {n} launches, pinned vs default {e:.2e}")
+        assert e <= 1e-6
+        with pytest.raises(hgemm.MxError):
+            plan.set_launch_variants(pinned[:-1], d)
+        with pytest.raises(hgemm.MxError):
+            plan.set_launch_variants([9] * n, d)
+        plan.set_launch_variants([], d)
+        assert plan.launch_variants(d) == [plan.get_schedule(d, "bf16x3")[1]] * n
+    Yf = torch.empty_like(W)
+    plan.eval(W, Yf, nrhs, "bf16x3")
+    torch.cuda.synchronize()
+    assert rel(Yf.cpu().numpy(), Yo) <= 1e-5
+
+
 def test_long_interaction_lists_are_split(gpu):
     """Blocks with more interactions than the BF16x3 kernel's segment window
     (up to 459 near blocks per row leaf, 512 per column leaf here) run as a
