@@ -156,8 +156,9 @@ mx_status mx_eval_adjoint(mx_plan *plan, mx_precision prec, const void *dY, void
  * (double for MX_FP64, float for MX_FP32 and MX_BF16X3); NULL takes the
  * plan's own, valid when the plan's last forward evaluation was of this W in
  * this precision.  prec is MX_FP64 (FP64 I/O), MX_FP32 or MX_BF16X3 (either
- * I/O type; with MX_BF16X3 the evaluations are split-BF16x3 and the block
- * products FP32).  Needs MX_PLAN_ADJOINT. */
+ * I/O type; with MX_BF16X3 the evaluations and the block products are
+ * split-BF16x3, the products on FP32 copies of their operands).  Needs
+ * MX_PLAN_ADJOINT. */
 mx_status mx_backward(mx_plan *plan, mx_precision prec, const void *W, const void *dY, void *dW, int32_t nrhs,
                       mx_io_type io, const void *wskel, const void *uskel, void *dD, void *dB, void *dVT,
                       void *stream);
@@ -261,7 +262,8 @@ enum {
     MX_KIND_BF16X3 = 2,
     MX_KIND_CONVERT = 3,
     MX_KIND_GRAD_F64 = 4,
-    MX_KIND_GRAD_F32 = 5
+    MX_KIND_GRAD_F32 = 5,
+    MX_KIND_GRAD_X3 = 6
 };
 mx_status mx_plan_set_profiling(mx_plan *plan, int32_t enable);
 mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t cap, int32_t *count);

@@ -45,7 +45,7 @@ template <typename T, typename O>
 struct GradArgs {
     const GTask *tasks;
     const GSeg *segs;
-    const int4 *tiles;        // (task, row tile, column tile, unused)
+    const int4 *tiles;        // (task, row tile, column tile, unused), 64 x 64 or GRAD_X3_TILE tiles
     int ntiles, nrhs;
     const T *src[GB_COUNT];
     O *out[3];                // dD, dB, dVT
@@ -54,5 +54,11 @@ struct GradArgs {
 // One launch over every tile of the gradient task list.
 template <typename T, typename O>
 void launch_grad(const GradArgs<T, O> &a, hipStream_t st);
+
+// Split-BF16x3 products of FP32 operands with FP32 accumulation: the same
+// task list, cut into 128 x 128 tiles (GRAD_X3_TILE).
+constexpr int GRAD_X3_TILE = 128;
+template <typename O>
+void launch_grad_x3(const GradArgs<float, O> &a, hipStream_t st);
 
 }  // namespace mx

@@ -114,8 +114,10 @@ struct mx_plan {
     // gradient products of the stored blocks (plans built with MX_PLAN_ADJOINT)
     mx::GTask *g_tasks = nullptr;
     mx::GSeg *g_segs = nullptr;
-    int4 *g_tiles = nullptr;
+    int4 *g_tiles = nullptr;            // 64 x 64 tiles
     int g_ntiles = 0;
+    int4 *g_tiles_x3 = nullptr;         // GRAD_X3_TILE tiles
+    int g_ntiles_x3 = 0;
     double g_flops = 0.0;               // algorithmic flops per nrhs column
     // packing lists of the split-BF16 operands, kept to repack new blocks
     mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
@@ -161,7 +163,7 @@ struct mx_plan {
                 for (auto &ps : pk) ps.release();
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
-                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, d_xblk[0], d_xblk[1], d_xblk[2],
+                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, d_xblk[0], d_xblk[1], d_xblk[2],
                         wskel_adj, uskel_adj, io_g};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -192,7 +194,8 @@ struct Topology {
 struct GradLists {
     std::vector<GTask> tasks;
     std::vector<GSeg> segs;
-    std::vector<int4> tiles;
+    std::vector<int4> tiles;      // 64 x 64 tiles (FP64 / FP32 products)
+    std::vector<int4> tiles_x3;   // GRAD_X3_TILE tiles (split-BF16x3 products)
     double flops = 0.0;   // per nrhs column
 };
 

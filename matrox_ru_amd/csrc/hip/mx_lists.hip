@@ -420,6 +420,9 @@ GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
         const GTask &k = G.tasks[tk];
         for (int i = 0; i < (k.m + 63) / 64; ++i)
             for (int j = 0; j < (k.n + 63) / 64; ++j) G.tiles.push_back(make_int4(tk, i, j, 0));
+        constexpr int T = GRAD_X3_TILE;
+        for (int i = 0; i < (k.m + T - 1) / T; ++i)
+            for (int j = 0; j < (k.n + T - 1) / T; ++j) G.tiles_x3.push_back(make_int4(tk, i, j, 0));
     };
     for (int64_t k = 0; k < s.ncount; ++k) {
         int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];

@@ -187,6 +187,8 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         P->g_tiles = upload(G.tiles.data(), G.tiles.size());
         if (G.tiles.size() > (size_t)INT32_MAX) throw MxError(MX_ERR_ARG, "gradient launch too large");
         P->g_ntiles = (int)G.tiles.size();
+        P->g_tiles_x3 = upload(G.tiles_x3.data(), G.tiles_x3.size());
+        P->g_ntiles_x3 = (int)G.tiles_x3.size();
         P->g_flops = G.flops;
     }
     // stored blocks: uploaded, or generated from the points straight into
@@ -455,7 +457,8 @@ static void export_panels(mx_plan *P, int dir, int nrhs, float *wdst, float *uds
 // Backward pass of Y = K~ W for the upstream gradient dY: dW = K~^T dY by the
 // adjoint evaluation, then (when requested) the block gradients from the
 // forward skeletons (w^, u^: caller copies, or the plan's own buffers) and the
-// adjoint ones (y^, g^).  FP64 or FP32 compute; outputs in the I/O type.
+// adjoint ones (y^, g^).  FP64, FP32 or split-BF16x3 compute (the latter on
+// FP32 copies of the operands); outputs in the I/O type.
 static void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
                      const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
     arg_check(W && dY && dW, "null W, dY or dW");
@@ -465,7 +468,7 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
     if (!P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const bool grads = dD || dB || dVT;
-    const bool x3 = prec == MX_BF16X3;   // BF16x3 evaluations, FP32 gradient products
+    const bool x3 = prec == MX_BF16X3;   // BF16x3 evaluations and gradient products (FP32 operands)
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
     if (grads && (!wskel || !uskel)) {
         if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
@@ -489,7 +492,7 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
     P->keep_prof = 0;
     if (!grads) return;
     const int64_t nel = P->N * (int64_t)nrhs;
-    const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : MX_KIND_GRAD_F32;
+    const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : (prec == MX_FP32 ? MX_KIND_GRAD_F32 : MX_KIND_GRAD_X3);
     if (x3) {   // adjoint skeletons to FP32; W and dY to FP32 when the I/O is FP64
         ensure_adj_work(P, nrhs, sizeof(float));
         export_panels(P, DIR_ADJ, nrhs, (float *)P->wskel_adj, (float *)P->uskel_adj, st);
@@ -530,8 +533,8 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
         auto fill = [&](auto &g) {
             g.tasks = P->g_tasks;
             g.segs = P->g_segs;
-            g.tiles = P->g_tiles;
-            g.ntiles = P->g_ntiles;
+            g.tiles = x3 ? P->g_tiles_x3 : P->g_tiles;
+            g.ntiles = x3 ? P->g_ntiles_x3 : P->g_ntiles;
             g.nrhs = nrhs;
             g.src[GB_W] = Wf;
             g.src[GB_DY] = dYf;
@@ -547,14 +550,16 @@ static void backward(mx_plan *P, mx_precision prec, const void *W, const void *d
             g.out[MAT_D] = (double *)dD;
             g.out[MAT_B] = (double *)dB;
             g.out[MAT_VT] = (double *)dVT;
-            launch_grad(g, st);
+            if (x3) launch_grad_x3(g, st);
+            else launch_grad(g, st);
         } else {
             GradArgs<float, float> g{};
             fill(g);
             g.out[MAT_D] = (float *)dD;
             g.out[MAT_B] = (float *)dB;
             g.out[MAT_VT] = (float *)dVT;
-            launch_grad(g, st);
+            if (x3) launch_grad_x3(g, st);
+            else launch_grad(g, st);
         }
         prof_mark(P, st, kind, false);
     }

@@ -27,6 +27,8 @@ CASES = {
     "h2b_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.2, h=1.0),
     "hss_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.0, h=1.0),
     "h2b_ragged": dict(n=3001, d=5, leaf=100, rank=70, tol=1e-7, budget=0.3, h=1.2),
+    # blocks above 128 rows: several ragged 128 x 128 gradient tiles per block
+    "h2b_wide": dict(n=3200, d=6, leaf=200, rank=150, tol=1e-9, budget=0.3, h=1.0),
 }
 
 
@@ -296,16 +298,17 @@ def test_backward_errors(gpu):
     assert rc == -1
 
 
-@pytest.mark.parametrize("io", ["f64", "f32"])
-def test_bf16x3_backward_close_to_fp64(gpu, io):
-    """Split-BF16x3 forward and adjoint with FP32 block products: gradients
-    within 5e-5 of the FP64 ones (the skeletons carry ~1e-5 relative error
-    from 16-bit split operands), and the FP32 skeleton export of a BF16x3
-    evaluation against the FP64 skeletons."""
+@pytest.mark.parametrize("case,nrhs,io", [("h2b_ragged", 48, "f64"), ("h2b_ragged", 48, "f32"),
+                                          ("h2b_wide", 160, "f64"), ("h2b_wide", 77, "f32")])
+def test_bf16x3_backward_close_to_fp64(gpu, case, nrhs, io):
+    """Split-BF16x3 forward, adjoint and block products: gradients within
+    5e-5 of the FP64 ones (operands split to 16 significant bits carry ~1e-5
+    relative error), and the FP32 skeleton export of a BF16x3 evaluation
+    against the FP64 skeletons.  nrhs not a multiple of the 32-column chunk
+    and blocks of more than 128 rows exercise the ragged tile edges."""
     import torch
-    _say(f"[grad bf16x3] io={io}")
-    X, hm = make_problem(**CASES["h2b_ragged"])
-    nrhs = 48
+    _say(f"[grad bf16x3] {case} nrhs={nrhs} io={io}")
+    X, hm = make_problem(**CASES[case])
     plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
     dt = torch.float64 if io == "f64" else torch.float32
     W = _dev(host.normal((hm.N * nrhs,), seed=161)).to(dt)
