@@ -291,7 +291,8 @@ mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);
  * from the caller's.  A group's W and Y are single device buffers of
  * rows x nrhs elements in which instance i occupies the leaf-blocked rows
  * [row_offset, row_offset + N_i) (element offset row_offset * nrhs).
- * precisions as for mx_plan_create (without MX_PLAN_ADJOINT). */
+ * precisions as for mx_plan_create; with MX_PLAN_ADJOINT every group can
+ * also be evaluated transposed (mx_batch_eval_adjoint). */
 typedef struct mx_batch mx_batch;
 mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances, const int32_t *nrhs, int32_t count,
                           uint32_t precisions);
@@ -304,8 +305,13 @@ mx_status mx_batch_plan(mx_batch *batch, int32_t group, mx_plan **plan);
 /* Y[g] = K~_g W[g] for every group g (arrays of ngroups device pointers). */
 mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
                         void *stream);
+/* dW[g] = K~_g^T dY[g] for every group (instance by instance, the adjoint of
+ * mx_batch_eval in the same layout); needs MX_PLAN_ADJOINT.  No counterpart
+ * in the reference. */
+mx_status mx_batch_eval_adjoint(mx_batch *batch, mx_precision prec, const void *const *dY, void *const *dW,
+                                mx_io_type io, void *stream);
 /* Capture the whole batch (all groups and their fork/join) as one hipGraph
- * per (precision, io, W, Y) and replay it. */
+ * per (direction, precision, io, W, Y) and replay it. */
 mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable);
 /* mx_plan_autotune on every group with its own nrhs. */
 mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream);

@@ -173,7 +173,7 @@ struct mx_batch {
     std::vector<mx::BatchInst> inst;
     int use_graph = 0;
     struct GraphEntry {
-        int prec, io;
+        int dir, prec, io;
         std::vector<const void *> W;
         std::vector<void *> Y;
         hipGraphExec_t exec;
@@ -237,38 +237,46 @@ static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs
     return B.release();
 }
 
-// all groups, forked from and joined back to st
-static void batch_eval(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+// all groups, forked from and joined back to st; dir DIR_FWD (Y = K~ W) or
+// DIR_ADJ (Y = K~^T W)
+static void batch_eval(mx_batch *B, int dir, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
                        hipStream_t st) {
     const int ng = (int)B->plans.size();
     for (int g = 0; g < ng; ++g) arg_check(W[g] && Y[g], "null W or Y of group " + std::to_string(g));
     if (ng == 1) {
-        eval(B->plans[0], DIR_FWD, prec, W[0], Y[0], B->group_nrhs[0], io, st);
+        eval(B->plans[0], dir, prec, W[0], Y[0], B->group_nrhs[0], io, st);
         return;
     }
     HIPCHK(hipEventRecord(B->events[0], st));
     for (int g = 0; g < ng; ++g) {
         HIPCHK(hipStreamWaitEvent(B->streams[g], B->events[0], 0));
-        eval(B->plans[g], DIR_FWD, prec, W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
+        eval(B->plans[g], dir, prec, W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
         HIPCHK(hipEventRecord(B->events[g + 1], B->streams[g]));
     }
     for (int g = 0; g < ng; ++g) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
 }
 
-static void batch_size_work(mx_batch *B, mx_precision prec, mx_io_type io) {
+// Allocate every work buffer an evaluation needs before it is captured.
+static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type io) {
     for (size_t g = 0; g < B->plans.size(); ++g) {
         mx_plan *P = B->plans[g];
         if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
-        if (prec == MX_BF16X3) ensure_panels(P, B->group_nrhs[g]);
-        else ensure_work(P, B->group_nrhs[g], prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
+        if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
+        const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
+        if (prec == MX_BF16X3) {
+            ensure_panels(P, B->group_nrhs[g]);
+        } else {
+            ensure_work(P, B->group_nrhs[g], elem, io == MX_IO_F64);
+            if (dir == DIR_ADJ) ensure_adj_work(P, B->group_nrhs[g], elem);
+        }
     }
 }
 
-static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
-                             hipStream_t st) {
+static void batch_eval_graph(mx_batch *B, int dir, mx_precision prec, const void *const *W, void *const *Y,
+                             mx_io_type io, hipStream_t st) {
     const int ng = (int)B->plans.size();
     for (auto &g : B->graphs) {
-        bool same = g.prec == prec && g.io == io;
+        bool same = g.dir == dir && g.prec == prec && g.io == io;
         for (int i = 0; same && i < ng; ++i) same = g.W[i] == W[i] && g.Y[i] == Y[i];
         if (same) {
             g.used = ++B->clock;
@@ -276,7 +284,7 @@ static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *
             return;
         }
     }
-    batch_size_work(B, prec, io);
+    batch_size_work(B, dir, prec, io);
     for (auto *P : B->plans) P->profiling = 0;
     hipStream_t cs = st;
     bool own = false;
@@ -289,7 +297,7 @@ static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *
     try {
         HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
         try {
-            batch_eval(B, prec, W, Y, io, cs);
+            batch_eval(B, dir, prec, W, Y, io, cs);
         } catch (...) {
             hipGraph_t g2 = nullptr;
             (void)hipStreamEndCapture(cs, &g2);
@@ -315,6 +323,7 @@ static void batch_eval_graph(mx_batch *B, mx_precision prec, const void *const *
         B->graphs.erase(lru);
     }
     mx_batch::GraphEntry e;
+    e.dir = dir;
     e.prec = prec;
     e.io = io;
     e.W.assign(W, W + ng);
@@ -341,7 +350,6 @@ mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances
     return (mx_status)guard([&]() {
         arg_check(batch != nullptr, "null argument");
         *batch = nullptr;
-        arg_check((precisions & MX_PLAN_ADJOINT) == 0, "batches are forward only");
         *batch = create_batch(instances, nrhs, count, precisions);
         return 0;
     });
@@ -387,17 +395,29 @@ mx_status mx_batch_plan(mx_batch *batch, int32_t group, mx_plan **plan) {
     });
 }
 
-mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
-                        void *stream) {
-    return (mx_status)guard([&]() {
+static int32_t batch_run(mx_batch *batch, int dir, mx_precision prec, const void *const *W, void *const *Y,
+                         mx_io_type io, void *stream) {
+    return guard([&]() {
         arg_check(batch && W && Y, "null argument");
         arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
+        if (dir == DIR_ADJ && !batch->plans[0]->adjoint)
+            throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
         bool prof = false;
         for (auto *P : batch->plans) prof = prof || P->profiling;
-        if (batch->use_graph && !prof) batch_eval_graph(batch, prec, W, Y, io, (hipStream_t)stream);
-        else batch_eval(batch, prec, W, Y, io, (hipStream_t)stream);
+        if (batch->use_graph && !prof) batch_eval_graph(batch, dir, prec, W, Y, io, (hipStream_t)stream);
+        else batch_eval(batch, dir, prec, W, Y, io, (hipStream_t)stream);
         return 0;
     });
+}
+
+mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
+                        void *stream) {
+    return (mx_status)batch_run(batch, DIR_FWD, prec, W, Y, io, stream);
+}
+
+mx_status mx_batch_eval_adjoint(mx_batch *batch, mx_precision prec, const void *const *dY, void *const *dW,
+                                mx_io_type io, void *stream) {
+    return (mx_status)batch_run(batch, DIR_ADJ, prec, dY, dW, io, stream);
 }
 
 mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable) {

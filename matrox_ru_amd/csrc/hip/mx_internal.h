@@ -272,6 +272,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                      const GenInfo *gen = nullptr);
 void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io, hipStream_t st);
 void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);
+void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);
 void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st, float *best_ms);
 void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,

@@ -250,7 +250,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     return P.release();
 }
 
-static void ensure_adj_work(mx_plan *P, int nrhs, size_t elem) {
+void ensure_adj_work(mx_plan *P, int nrhs, size_t elem) {
     size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
     if (sb > P->skel_adj_bytes) {
         if (P->wskel_adj) HIPCHK(hipFree(P->wskel_adj));

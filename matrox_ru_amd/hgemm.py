@@ -101,6 +101,8 @@ def lib():
         L.mx_batch_plan.restype = ctypes.c_int
         L.mx_batch_eval.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, P]
         L.mx_batch_eval.restype = ctypes.c_int
+        L.mx_batch_eval_adjoint.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, P]
+        L.mx_batch_eval_adjoint.restype = ctypes.c_int
         L.mx_batch_set_graph.argtypes = [P, I32]
         L.mx_batch_set_graph.restype = ctypes.c_int
         L.mx_batch_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, I32, P]
@@ -425,9 +427,9 @@ class Batch:
     element range [off_i * nrhs_g, (off_i + N_i) * nrhs_g) in its own
     leaf-blocked layout (``instance_view``)."""
 
-    def __init__(self, hms, nrhs, precisions=("fp32",)):
+    def __init__(self, hms, nrhs, precisions=("fp32",), adjoint=False):
         assert len(hms) == len(nrhs) and len(hms) >= 1
-        mask = 0
+        mask = PLAN_ADJOINT if adjoint else 0
         for p in precisions:
             mask |= 1 << PRECISIONS[p]
         structs, keep = [], []
@@ -442,6 +444,7 @@ class Batch:
         del keep, structs
         self._h = h
         self.precisions = tuple(precisions)
+        self.adjoint = bool(adjoint)
         self.N = [int(hm.scalars["n"]) for hm in hms]
         self.nrhs = [int(n) for n in nrhs]
         g = I32()
@@ -474,7 +477,7 @@ class Batch:
         nr = self.groups[g][0]
         return bufs[g][off * nr:(off + self.N[i]) * nr]
 
-    def eval(self, W, Y, precision="fp32", stream=None):
+    def _run(self, fn, W, Y, precision, stream):
         import torch
         assert len(W) == len(self.groups) and len(Y) == len(self.groups)
         dt = W[0].dtype
@@ -484,9 +487,17 @@ class Batch:
         io = IO_F64 if dt == torch.float64 else IO_F32
         wp = (P * len(W))(*[w.data_ptr() for w in W])
         yp = (P * len(Y))(*[y.data_ptr() for y in Y])
-        _check(lib().mx_batch_eval(self._h, PRECISIONS[precision], ctypes.cast(wp, P), ctypes.cast(yp, P), io,
-                                   _stream_handle(stream)))
+        _check(fn(self._h, PRECISIONS[precision], ctypes.cast(wp, P), ctypes.cast(yp, P), io, _stream_handle(stream)))
         return Y
+
+    def eval(self, W, Y, precision="fp32", stream=None):
+        """Y[g] = K~ W[g], instance by instance (mx_batch_eval)."""
+        return self._run(lib().mx_batch_eval, W, Y, precision, stream)
+
+    def eval_adjoint(self, dY, dW, precision="fp32", stream=None):
+        """dW[g] = K~^T dY[g], instance by instance (mx_batch_eval_adjoint;
+        the batch must be built with adjoint=True)."""
+        return self._run(lib().mx_batch_eval_adjoint, dY, dW, precision, stream)
 
     def set_graph(self, enable=True):
         _check(lib().mx_batch_set_graph(self._h, int(enable)))
