@@ -1,6 +1,8 @@
 // Split-BF16 x3 kernels (layouts and arithmetic described in mx_bf16x3.h).
 #include "mx_bf16x3.h"
 
+#include <type_traits>
+
 #include <algorithm>
 
 namespace mx {
@@ -31,53 +33,117 @@ __global__ void pack_x3_blocks(const double *__restrict__ src, const X3Block *__
     }
 }
 
-// One thread per padded element of a leaf block: consecutive threads read
-// consecutive rows of one column (coalesced) and write consecutive k of one
-// 32-row chunk of the panel; rows past R are written as zeros.
+// Eight columns per workgroup, 32 threads per column, each thread two
+// consecutive rows: the loads read 64 consecutive rows of a column
+// (coalesced), the stores write 64-byte runs of the panel (a 32-row chunk of
+// one column), and no index needs a division.  hi = bf16(x) and
+// lo = bf16(x - hi), rounded to nearest even (v_cvt_pk_bf16_f32; FP64 input
+// goes through FP32 first); rows past R are written as zeros.
+typedef __bf16 x3_bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float x3_f2_t __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void split2(double a, double b, uint32_t &h, uint32_t &l) {
+    const x3_bf16x2_t xh = __builtin_convertvector((x3_f2_t{(float)a, (float)b}), x3_bf16x2_t);
+    const x3_f2_t hf = __builtin_convertvector(xh, x3_f2_t);
+    const x3_f2_t r = {(float)(a - (double)hf.x), (float)(b - (double)hf.y)};
+    h = __builtin_bit_cast(uint32_t, xh);
+    l = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, x3_bf16x2_t));
+}
+__device__ __forceinline__ void split2(float a, float b, uint32_t &h, uint32_t &l) {
+    const x3_f2_t x = {a, b};
+    const x3_bf16x2_t xh = __builtin_convertvector(x, x3_bf16x2_t);
+    h = __builtin_bit_cast(uint32_t, xh);
+    l = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - __builtin_convertvector(xh, x3_f2_t), x3_bf16x2_t));
+}
+
+constexpr int X3_SPLIT_COLS = 8;   // columns per workgroup of the panel kernels
+
 template <typename TW>
-__global__ void x3_split_w_kernel(const TW *__restrict__ W, const X3Rows *__restrict__ blocks, int nrhs,
-                                  uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+__global__ __launch_bounds__(256) void x3_split_w_kernel(const TW *__restrict__ W, const X3Rows *__restrict__ blocks,
+                                                         int nrhs, uint16_t *__restrict__ hi,
+                                                         uint16_t *__restrict__ lo) {
     const X3Rows b = blocks[blockIdx.y];
-    const int Rp = (b.R + 31) / 32 * 32;
-    const int64_t n = (int64_t)Rp * nrhs;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        const int col = (int)(t / Rp), k = (int)(t % Rp);
-        const double x = k < b.R ? (double)W[b.src_row * nrhs + (int64_t)col * b.R + k] : 0.0;
-        const uint16_t h = bf16_rne((float)x);
-        const int64_t o = b.prow * nrhs + ((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31);
-        hi[o] = h;
-        lo[o] = bf16_rne((float)(x - (double)bf16_f(h)));
+    const int R = b.R, Rp = (R + 31) & ~31;
+    const int lane = threadIdx.x & 31, cw = threadIdx.x >> 5;
+    const TW *src = W + b.src_row * nrhs;
+    // panel rows start at multiples of 32, so every (even) offset below is 4-byte aligned
+    uint32_t *ph = reinterpret_cast<uint32_t *>(hi + b.prow * nrhs);
+    uint32_t *pl = reinterpret_cast<uint32_t *>(lo + b.prow * nrhs);
+    typedef typename std::conditional<sizeof(TW) == 8, uint64_t, uint32_t>::type bits_t;
+    const bits_t *src_bits = reinterpret_cast<const bits_t *>(src);
+    for (int col = blockIdx.x * X3_SPLIT_COLS + cw; col < nrhs; col += gridDim.x * X3_SPLIT_COLS) {
+        const bits_t *s = src_bits + (int64_t)col * R;
+        // four row pairs per lane per pass, all loads issued before any is
+        // used; rows past R load row R-1 and are masked to zero (no branches)
+        for (int kb = 0; kb < Rp; kb += 256) {
+            bits_t v[4][2];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) v[q][e] = s[min(kb + 64 * q + 2 * lane + e, R - 1)];
+            // keep the eight loads together (not sunk into the guarded stores below)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) asm volatile("" : "+v"(v[q][e]));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int k = kb + 64 * q + 2 * lane;
+                if (k >= Rp) break;
+                const bits_t m0 = k < R ? ~bits_t(0) : bits_t(0), m1 = k + 1 < R ? ~bits_t(0) : bits_t(0);
+                uint32_t h, l;
+                split2(__builtin_bit_cast(TW, bits_t(v[q][0] & m0)), __builtin_bit_cast(TW, bits_t(v[q][1] & m1)), h, l);
+                const int64_t o = (((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31)) >> 1;
+                ph[o] = h;
+                pl[o] = l;
+            }
+        }
     }
 }
 
 void launch_x3_split_w(const void *W, int w_f64, const X3Rows *rows, int nblocks, int nrhs, uint16_t *hi,
                        uint16_t *lo, hipStream_t st) {
-    dim3 grid(64, (unsigned)nblocks);
-    if (w_f64)
-        hipLaunchKernelGGL(x3_split_w_kernel<double>, grid, dim3(256), 0, st, (const double *)W, rows, nrhs, hi, lo);
-    else
-        hipLaunchKernelGGL(x3_split_w_kernel<float>, grid, dim3(256), 0, st, (const float *)W, rows, nrhs, hi, lo);
+    const int gx = std::min(128, (nrhs + X3_SPLIT_COLS - 1) / X3_SPLIT_COLS);
+    for (int b0 = 0; b0 < nblocks; b0 += 65535) {
+        const dim3 grid((unsigned)gx, (unsigned)std::min(65535, nblocks - b0));
+        if (w_f64)
+            hipLaunchKernelGGL(x3_split_w_kernel<double>, grid, dim3(256), 0, st, (const double *)W, rows + b0, nrhs,
+                               hi, lo);
+        else
+            hipLaunchKernelGGL(x3_split_w_kernel<float>, grid, dim3(256), 0, st, (const float *)W, rows + b0, nrhs,
+                               hi, lo);
+    }
 }
 
 // Split panels back to FP32 blocks in the reference skeleton layout: block b
 // (R rows at panel row prow) -> out[src_row*nrhs + col*R + k] = hi + lo.
-__global__ void x3_panels_to_f32_kernel(const uint16_t *__restrict__ hi, const uint16_t *__restrict__ lo,
-                                        const X3Rows *__restrict__ blocks, int nrhs, float *__restrict__ out) {
+// Same thread layout as the split kernel.
+__global__ __launch_bounds__(256) void x3_panels_to_f32_kernel(const uint16_t *__restrict__ hi,
+                                                               const uint16_t *__restrict__ lo,
+                                                               const X3Rows *__restrict__ blocks, int nrhs,
+                                                               float *__restrict__ out) {
     const X3Rows b = blocks[blockIdx.y];
-    const int64_t n = (int64_t)b.R * nrhs;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        const int col = (int)(t / b.R), k = (int)(t % b.R);
-        const int64_t o = b.prow * nrhs + ((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31);
-        out[b.src_row * nrhs + (int64_t)col * b.R + k] = bf16_f(hi[o]) + bf16_f(lo[o]);
+    const int R = b.R;
+    const int lane = threadIdx.x & 31, cw = threadIdx.x >> 5;
+    const uint16_t *ph = hi + b.prow * nrhs;
+    const uint16_t *pl = lo + b.prow * nrhs;
+    float *dst = out + b.src_row * nrhs;
+    for (int col = blockIdx.x * X3_SPLIT_COLS + cw; col < nrhs; col += gridDim.x * X3_SPLIT_COLS) {
+        float *d = dst + (int64_t)col * R;
+        for (int k = lane; k < R; k += 32) {
+            const int64_t o = ((int64_t)(k >> 5) * nrhs + col) * 32 + (k & 31);
+            d[k] = bf16_f(ph[o]) + bf16_f(pl[o]);
+        }
     }
 }
 
 void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Rows *blocks, int nblocks, int nrhs,
                              float *out, hipStream_t st) {
+    const int gx = std::min(128, (nrhs + X3_SPLIT_COLS - 1) / X3_SPLIT_COLS);
     for (int b0 = 0; b0 < nblocks; b0 += 65535) {
         const int nb = std::min(65535, nblocks - b0);
-        hipLaunchKernelGGL(x3_panels_to_f32_kernel, dim3(64, (unsigned)nb), dim3(256), 0, st, hi, lo, blocks + b0,
-                           nrhs, out);
+        hipLaunchKernelGGL(x3_panels_to_f32_kernel, dim3((unsigned)gx, (unsigned)nb), dim3(256), 0, st, hi, lo,
+                           blocks + b0, nrhs, out);
     }
 }
 
