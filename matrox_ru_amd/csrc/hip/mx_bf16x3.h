@@ -106,7 +106,11 @@ void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Row
 //   128 x 128, fragments staged through VGPRs instead of LDS-DMA;
 //   128 x 64 and 128 x 32, 2 stages (narrow waves, for small nrhs);
 //   128 x 128 with 8 waves of 64 x 32 and 3 or 4 stages (one workgroup per
-//   CU, two waves per SIMD, two or three chunks of prefetch).
+//   CU, two waves per SIMD, two or three chunks of prefetch);
+//   256 x 128, 3 stages: 8 waves (4 x 2) of 64 x 64, 144 KB, one workgroup
+//   per CU; both 128-row halves of a 256-row block in one workgroup, so the
+//   B operand (W, skeleton panels) is read once per 256 rows (its launches
+//   use the 256-row tile lists of the plan).
 enum X3Variant {
     X3_V128x128_S2 = 1,
     X3_V128x128_S4 = 2,
@@ -115,10 +119,12 @@ enum X3Variant {
     X3_V128x64_S2 = 5,
     X3_V128x32_S2 = 6,
     X3_V128x128_W8S3 = 7,
-    X3_V128x128_W8S4 = 8
+    X3_V128x128_W8S4 = 8,
+    X3_V256x128_W8S3 = 9
 };
-constexpr int X3_NVARIANT = 8;
+constexpr int X3_NVARIANT = 9;
 int x3_variant_bn(int variant);   // output columns of a tile
+int x3_variant_bm(int variant);   // output rows of a tile (X3_BM or 2 X3_BM)
 void launch_x3(const X3Args &a, int variant, hipStream_t st);
 
 }  // namespace mx

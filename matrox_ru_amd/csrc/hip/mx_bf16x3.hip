@@ -597,6 +597,8 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     x3_epilogue<4, NJ>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
+int x3_variant_bm(int variant) { return variant == X3_V256x128_W8S3 ? 2 * X3_BM : X3_BM; }
+
 int x3_variant_bn(int variant) {
     switch (variant) {
     case X3_V128x256_S3: return 256;
@@ -631,6 +633,9 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V128x128_W8S4:
         hipLaunchKernelGGL((x3_kernel<2, 4, 4, 2>), grid, dim3(512), 0, st, a);
+        break;
+    case X3_V256x128_W8S3:
+        hipLaunchKernelGGL((x3_kernel<4, 2, 3, 4>), grid, dim3(512), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);

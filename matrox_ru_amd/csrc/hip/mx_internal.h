@@ -51,15 +51,20 @@ struct Program {
     Seg *segs = nullptr;
     int2 *tiles = nullptr;
     std::vector<Launch> launches;
+    // BF16x3 programs: the same launches cut into tiles of 2 X3_BM rows
+    int2 *tiles2 = nullptr;
+    std::vector<Launch> launches2;
     std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
     bool built = false;
     void release() {
         if (tasks) (void)hipFree(tasks);
         if (segs) (void)hipFree(segs);
         if (tiles) (void)hipFree(tiles);
+        if (tiles2) (void)hipFree(tiles2);
         tasks = nullptr;
         segs = nullptr;
         tiles = nullptr;
+        tiles2 = nullptr;
     }
 };
 
@@ -183,6 +188,8 @@ struct TaskLists {
     std::vector<int> task_kind;   // per task
     std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
     int dummy = -1;               // an empty task (no rows, no segments) padding launches
+    std::vector<int2> tiles2;     // tiles of twice the rows (pair_tiles)
+    std::vector<Launch> launches2;
 };
 
 struct Topology {
@@ -206,6 +213,9 @@ Topology topology(const mx_structure &s);
 // Bounds of every offset in the structure (throws MX_ERR_ARG).
 void validate(const mx_structure &s, const Topology &t);
 void split_long_tasks(TaskLists &L, int maxseg);
+// tiles2 / launches2: every launch's tiles merged in row pairs, in the same
+// order and XCD groups
+void pair_tiles(TaskLists &L);
 TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir, int sched,
                       const int64_t *wrow_in, const int64_t *skel_in);
 GradLists build_grad_lists(const mx_structure &s, const Topology &t);

@@ -3,6 +3,8 @@
 // block gradients.
 #include "mx_internal.h"
 
+#include <set>
+
 namespace mx {
 
 void compute_launch_flops(TaskLists &L) {
@@ -386,6 +388,36 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
     end_launch();
     order_tiles(L, kind == 1 ? X3_BK : SG_BK);
     return L;
+}
+
+// Row tile r of a task maps to row pair r / 2; each pair appears once per
+// launch, where its first tile was.  A launch that order_tiles cut into
+// eight padded XCD groups keeps that cut (each group re-padded to the
+// longest).
+void pair_tiles(TaskLists &L) {
+    L.tiles2.clear();
+    L.launches2.clear();
+    for (const Launch &ln : L.launches) {
+        const int ng = (ln.ntiles % 8 == 0) ? 8 : 1;
+        const int per = ln.ntiles / ng;
+        std::vector<std::vector<int2>> grp(ng);
+        std::set<std::pair<int, int>> seen;
+        for (int q = 0; q < ng; ++q) {
+            for (int i = 0; i < per; ++i) {
+                const int2 t = L.tiles[ln.tile_offset + q * per + i];
+                if (t.x == L.dummy) continue;
+                if (seen.insert({t.x, t.y / 2}).second) grp[q].push_back(make_int2(t.x, t.y / 2));
+            }
+        }
+        size_t gmax = 0;
+        for (auto &g : grp) gmax = std::max(gmax, g.size());
+        Launch nl{(int)L.tiles2.size(), (int)(gmax * ng)};
+        for (auto &g : grp) {
+            for (const int2 &p : g) L.tiles2.push_back(p);
+            for (size_t i = g.size(); i < gmax; ++i) L.tiles2.push_back(make_int2(L.dummy, 0));
+        }
+        L.launches2.push_back(nl);
+    }
 }
 
 // Gradient products of every stored block (see mx_grad.h): one task per

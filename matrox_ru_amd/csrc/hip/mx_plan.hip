@@ -172,9 +172,12 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                 }
             }
             compute_launch_flops(X);
+            pair_tiles(X);
             Program &pg = P->prog[dir][1][sc];
             upload_lists(X, &pg.tasks, &pg.segs, &pg.tiles);
             pg.launches = X.launches;
+            pg.tiles2 = upload(X.tiles2.data(), X.tiles2.size());
+            pg.launches2 = X.launches2;
             pg.flops = X.flops;
             pg.built = true;
         }
@@ -428,9 +431,11 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         for (size_t i = 0; i < pg1.launches.size(); ++i) {
             const int v = lv.size() == pg1.launches.size() ? lv[i] : P->x3_variant;
             const int bn = x3_variant_bn(v);
+            const bool pairs = x3_variant_bm(v) == 2 * X3_BM;
+            const Launch &ln = pairs ? pg1.launches2[i] : pg1.launches[i];
             xa.n_ntiles = (nrhs + bn - 1) / bn;
-            xa.tiles = pg1.tiles + pg1.launches[i].tile_offset;
-            xa.ntiles = pg1.launches[i].ntiles;
+            xa.tiles = (pairs ? pg1.tiles2 : pg1.tiles) + ln.tile_offset;
+            xa.ntiles = ln.ntiles;
             prof_mark(P, st, MX_KIND_BF16X3, true);
             launch_x3(xa, v, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);

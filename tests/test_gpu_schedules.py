@@ -16,6 +16,9 @@ CASES = {
     "h2b_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.2, h=1.0),
     "hss_small": dict(n=2048, d=8, leaf=64, rank=48, tol=1e-6, budget=0.0, h=1.0),
     "h2b_ragged": dict(n=3001, d=5, leaf=100, rank=70, tol=1e-7, budget=0.3, h=1.2),
+    # leaves of 300 points and ranks up to 150: three 128-row tiles per near
+    # block, so the 256-row tiles of variant 9 cover a full and a half pair
+    "h2b_tall": dict(n=4800, d=6, leaf=300, rank=150, tol=1e-9, budget=0.3, h=1.0),
 }
 
 
@@ -36,8 +39,7 @@ def test_schedules_match_oracle(gpu, case, schedule):
     Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
-    for prec, stages in (("fp64", 0), ("fp32", 0), ("bf16x3", 1), ("bf16x3", 2), ("bf16x3", 3), ("bf16x3", 4),
-                          ("bf16x3", 5), ("bf16x3", 6), ("bf16x3", 7), ("bf16x3", 8)):
+    for prec, stages in [("fp64", 0), ("fp32", 0)] + [("bf16x3", v) for v in sorted(hgemm.X3_VARIANTS)]:
         for direction in ("forward", "adjoint"):
             plan.set_schedule(direction, prec, schedule, stages)
             assert plan.get_schedule(direction, prec)[0] == schedule
@@ -65,7 +67,7 @@ def test_autotune_keeps_results(gpu):
         assert len(ms) == 2 and all(m > 0 for m in ms)
         for d in ("forward", "adjoint"):
             sc, st = plan.get_schedule(d, prec)
-            assert sc in ("fused", "split") and st in range(1, 9)
+            assert sc in ("fused", "split") and st in hgemm.X3_VARIANTS
         if prec == "bf16x3":
             lv = plan.launch_variants("forward")
             assert len(lv) >= 3 and all(1 <= v <= 8 for v in lv)
@@ -104,7 +106,7 @@ def test_pinned_launch_variants(gpu):
         with pytest.raises(hgemm.MxError):
             plan.set_launch_variants(pinned[:-1], d)
         with pytest.raises(hgemm.MxError):
-            plan.set_launch_variants([9] * n, d)
+            plan.set_launch_variants([max(hgemm.X3_VARIANTS) + 1] * n, d)
         plan.set_launch_variants([], d)
         assert plan.launch_variants(d) == [plan.get_schedule(d, "bf16x3")[1]] * n
     Yf = torch.empty_like(W)
