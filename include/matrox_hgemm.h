@@ -197,11 +197,13 @@ mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable);
 /* Schedules of the coupling and downward phases (per direction and kernel
  * kind): MX_SCHED_FUSED runs one launch per tree level computing
  * u_c = sum_far B w + VT_p^T u_p; MX_SCHED_SPLIT runs the couplings of all
- * levels in one launch, then the parent transfers level by level.  Both give
- * the same result up to summation order.  Defaults: fused forward, split
- * adjoint.  x3_variant: BF16x3 kernel tile and pipeline depth (one per plan,
- * 0 keeps the current): */
-enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1 };
+ * levels in one launch, then the parent transfers level by level.  The
+ * _BALANCED forms also move the leading near blocks of leaves with long near
+ * lists (the adjoint's column lists) out of the final NEAR+LEAF launch into
+ * the upper-level UP launches.  All give the same result up to summation
+ * order.  Defaults: fused forward, split adjoint.  x3_variant: BF16x3 kernel
+ * tile and pipeline depth (one per plan, 0 keeps the current): */
+enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1, MX_SCHED_FUSED_BALANCED = 2, MX_SCHED_SPLIT_BALANCED = 3 };
 enum { MX_DIR_FORWARD = 0, MX_DIR_ADJOINT = 1 };
 enum {
     MX_X3_128x128_S2 = 1, /* 4 waves, 2 stages, two workgroups per CU (default) */

@@ -77,7 +77,8 @@ enum { DIR_FWD = 0, DIR_ADJ = 1 };
 //      from the parent (beta = 1).  Long coupling lists at the top of the
 //      tree (the adjoint's column-grouped lists) then run beside everything
 //      else instead of in a nearly empty launch.
-enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, NSCHED = 2 };
+// coupling/downward schedule (bit 0: split) and NEAR+LEAF balancing (bit 1)
+enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, SCHED_FUSED_BAL = 2, SCHED_SPLIT_BAL = 3, NSCHED = 4 };
 
 }  // namespace mx
 

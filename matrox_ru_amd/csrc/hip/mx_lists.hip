@@ -287,6 +287,86 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         L.tasks[task].seg_end = (int)L.segs.size();
     };
     const int maxd = (int)t.levels.size() - 1;
+    // near pair k as a segment of the task of destination leaf lf
+    auto near_src = [&](int64_t k) { return dir == DIR_FWD ? ny(k) : nx(k); };
+    auto add_near = [&](int tk, int lf, int64_t k) {
+        if (dir == DIR_FWD) {
+            int y = ny(k);
+            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
+        } else {   // D_k^T, D_k stored Dim_x x Dim_y
+            int x = nx(k);
+            add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 1, s.Ddim[x], s.Ddim[x], VEC_W, wrow_in[x], s.Ddim[x]);
+        }
+    };
+
+    // Balance of the NEAR+LEAF launch (schedules with bit 1 set; the
+    // autotuner decides whether it pays).  Its length is that of its longest
+    // task; a leaf whose near list is much longer than the mean (the adjoint
+    // groups near blocks by column, and one leaf can be near to many others)
+    // would set it alone.  The first near segments of such a leaf move, as
+    // partial tasks writing Y, into the UP launches of the upper levels,
+    // which hold few tasks of bounded length: a piece is no longer than the
+    // longest UP task of its launch and a launch takes at most 64 extra row
+    // tiles, so neither grows much.  The first write of a leaf's Y keeps
+    // beta_y; the later pieces and the NEAR+LEAF task accumulate (beta = 1).
+    const int bk = kind == 1 ? X3_BK : SG_BK;
+    const int bm = kind == 1 ? X3_BM : SG_BM;
+    auto chunks = [&](int k) { return (k + bk - 1) / bk; };
+    std::vector<size_t> nhoisted(s.nleaf, 0);                                  // leading near segments moved
+    std::vector<std::vector<std::pair<int, std::pair<size_t, size_t>>>> pieces(maxd + 1);   // (leaf, [i0, i1))
+    if (maxd >= 2 && (sched & 2)) {
+        std::vector<int64_t> cost(s.nleaf, 0);
+        int64_t total = 0, active = 0;
+        for (int lf = 0; lf < s.nleaf; ++lf) {
+            if (s.Ddim[lf] == 0) continue;
+            for (int64_t k : near_of[lf]) cost[lf] += chunks(s.Ddim[near_src(k)]);
+            const int v = t.node_of_leaf[lf];
+            if (v > 0) cost[lf] += chunks(s.slen[v]);
+            total += cost[lf];
+            ++active;
+        }
+        const int64_t cap = active ? (5 * total + 4 * active - 1) / (4 * active) : 0;   // 1.25 x the mean
+        std::vector<int> room(maxd + 1, 0), budget(maxd + 1, 0);
+        for (int d = maxd - 1; d >= 1; --d) {
+            int rows = 0;
+            for (int v : t.levels[d]) {
+                if (s.slen[v] == 0) continue;
+                const int c = s.lchildren[v] == -1
+                                  ? chunks(s.Ddim[s.lm[v]])
+                                  : chunks(s.slen[s.lchildren[v]]) + chunks(s.slen[s.rchildren[v]]);
+                room[d] = std::max(room[d], c);
+                rows += (s.slen[v] + bm - 1) / bm;
+            }
+            budget[d] = std::max(0, 64 - rows);
+        }
+        std::vector<int> heavy;
+        for (int lf = 0; lf < s.nleaf; ++lf)
+            if (s.Ddim[lf] > 0 && cost[lf] > cap) heavy.push_back(lf);
+        std::stable_sort(heavy.begin(), heavy.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+        for (int lf : heavy) {
+            const int rt = (s.Ddim[lf] + bm - 1) / bm;
+            size_t next = 0;
+            for (int d = maxd - 1; d >= 1 && cost[lf] > cap; --d) {   // launch order
+                if (budget[d] < rt) continue;
+                size_t end = next;
+                int c = 0;
+                while (end < near_of[lf].size() && cost[lf] > cap) {
+                    const int sc = chunks(s.Ddim[near_src(near_of[lf][end])]);
+                    if (c + sc > room[d]) break;
+                    c += sc;
+                    cost[lf] -= sc;
+                    ++end;
+                }
+                if (end == next) continue;
+                pieces[d].push_back({lf, {next, end}});
+                next = end;
+                budget[d] -= rt;
+            }
+            nhoisted[lf] = next;
+        }
+    }
+    std::vector<char> y_written(s.nleaf, 0);
+
     // UP: deepest level first
     for (int d = maxd; d >= 1; --d) {
         begin_launch();
@@ -304,6 +384,13 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
                         skel_in[r], s.slen[r]);
             }
             add_tiles(L, tk, sv);
+        }
+        for (const auto &pc : pieces[d]) {   // near segments moved out of NEAR+LEAF
+            const int lf = pc.first;
+            int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, y_written[lf] ? 1 : beta_y);
+            for (size_t i = pc.second.first; i < pc.second.second; ++i) add_near(tk, lf, near_of[lf][i]);
+            y_written[lf] = 1;
+            add_tiles(L, tk, s.Ddim[lf]);
         }
         end_launch();
     }
@@ -328,7 +415,7 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         add_seg(tk, off, MAT_VT, 1, sp, sp, VEC_USKEL, skel_in[p], sp);
         return true;
     };
-    if (sched == SCHED_FUSED) {
+    if ((sched & 1) == 0) {
         // FAR + DOWN, root's children first: u_c = sum_far B w_y + VT_p[:, part(c)]^T u_p
         for (int d = 1; d <= maxd; ++d) {
             begin_launch();
@@ -366,20 +453,13 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
             end_launch();
         }
     }
-    // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x
+    // NEAR + LEAF: Y_x = sum_near D W_y + VT_x^T u_x (less the near segments
+    // moved into the UP launches)
     begin_launch();
     for (int lf = 0; lf < s.nleaf; ++lf) {
         if (s.Ddim[lf] == 0) continue;
-        int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, beta_y);
-        for (int64_t k : near_of[lf]) {
-            if (dir == DIR_FWD) {
-                int y = ny(k);
-                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 0, s.Ddim[lf], s.Ddim[y], VEC_W, wrow_in[y], s.Ddim[y]);
-            } else {   // D_k^T, D_k stored Dim_x x Dim_y
-                int x = nx(k);
-                add_seg(tk, (int64_t)s.Dptr[k], MAT_D, 1, s.Ddim[x], s.Ddim[x], VEC_W, wrow_in[x], s.Ddim[x]);
-            }
-        }
+        int tk = new_task(s.wrow[lf], s.Ddim[lf], VEC_Y, y_written[lf] ? 1 : beta_y);
+        for (size_t i = nhoisted[lf]; i < near_of[lf].size(); ++i) add_near(tk, lf, near_of[lf][i]);
         int v = t.node_of_leaf[lf];
         int sv = s.slen[v];
         if (v > 0 && sv > 0) add_seg(tk, s.VTptr[v], MAT_VT, 1, sv, sv, VEC_USKEL, skel_in[v], sv);

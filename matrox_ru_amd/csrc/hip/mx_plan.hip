@@ -615,7 +615,7 @@ static void drop_graphs(mx_plan *P) {
 
 static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int variant) {
     arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
-    arg_check(sched == SCHED_FUSED || sched == SCHED_SPLIT, "bad schedule");
+    arg_check(sched >= 0 && sched < NSCHED, "bad schedule");
     arg_check(variant >= 0 && variant <= X3_NVARIANT, "bad BF16x3 kernel variant (0 keeps)");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     const int kd = prec == MX_BF16X3 ? 1 : 0;
@@ -676,7 +676,8 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
             for (int dir = 0; dir < ndir; ++dir) {
                 float best = -1.f;
                 for (int si2 = 0; si2 < NSCHED; ++si2) {
-                    const int sc = si2 == 0 ? cur_sched[dir] : 1 - cur_sched[dir];   // current first
+                    // the current schedule first
+                    const int sc = si2 == 0 ? cur_sched[dir] : (si2 <= cur_sched[dir] ? si2 - 1 : si2);
                     P->sched[dir][kd] = sc;
                     eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up (sizes work buffers)
                     HIPCHK(hipEventRecord(e0, st));

@@ -28,7 +28,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
-@pytest.mark.parametrize("schedule", ["fused", "split"])
+@pytest.mark.parametrize("schedule", sorted(hgemm.SCHEDULES))
 def test_schedules_match_oracle(gpu, case, schedule):
     import torch
     _say(f"[schedule] case={case} schedule={schedule}")
@@ -67,7 +67,7 @@ def test_autotune_keeps_results(gpu):
         assert len(ms) == 2 and all(m > 0 for m in ms)
         for d in ("forward", "adjoint"):
             sc, st = plan.get_schedule(d, prec)
-            assert sc in ("fused", "split") and st in hgemm.X3_VARIANTS
+            assert sc in hgemm.SCHEDULES and st in hgemm.X3_VARIANTS
         if prec == "bf16x3":
             lv = plan.launch_variants("forward")
             assert len(lv) >= 3 and all(1 <= v <= 8 for v in lv)
