@@ -364,6 +364,25 @@ def run_batched(args):
     msmx = timer(lambda: b.eval(W32, Ymx, "bf16x3"), args.steps, args.warmup)
     fl = b.flops("fp64")
     log(f"[bench] batch fp64 {ms64:.3f} ms, fp32 {ms32:.3f} ms, bf16x3 {msmx:.3f} ms; {fl / 1e12:.4f} TFLOP")
+    # kernel roofline per group, each group's plan run alone (HIP events per
+    # launch); small nrhs is HBM-bound on the packed blocks: every 4-byte
+    # split element serves 2 nrhs flops, so the ceiling is min(BF16 peak / 3,
+    # nrhs / 2 flop/B x 8 TB/s)
+    roof_groups = []
+    x3_fl, x3_ms = 0.0, 0.0
+    for g, (nr, rows) in enumerate(b.groups):
+        gp = b.plan(g)
+        _, parts_g = gp.flops("bf16x3", nr)
+        Yg = torch.empty_like(W32[g])
+        roof, kinds = launch_profile(gp, lambda: gp.eval(W32[g], Yg, nr, "bf16x3"), "bf16x3", nr, parts_g, nprof=2)
+        ceiling = min(MI355X_PEAK["bf16"] / 3, nr / 2 * MI355X_PEAK["hbm_gbs"] / 1e3)
+        roof.update({"group_nrhs": nr, "group_rows": rows, "ceiling_x3": round(ceiling, 1),
+                     "bound": "hbm" if ceiling < MI355X_PEAK["bf16"] / 3 else "mfma",
+                     "frac_of_ceiling": round(roof["achieved"] / ceiling, 4)})
+        roof_groups.append(roof)
+        x3_fl += roof["flops_per_launch"] * roof["launches_per_eval"]
+        x3_ms += roof["avg_launch_ms"] * roof["launches_per_eval"]
+        del Yg
     errs32, errsmx, dense = [], [], []
     for i, (hm, it) in enumerate(zip(hms, inst)):
         y64 = b.instance_view(Y64, i)
@@ -403,6 +422,11 @@ def run_batched(args):
         "fp64_tflops": round(fl / (ms64 * 1e-3) / 1e12, 3),
         "max_rel_err_fp32_vs_fp64": max(errs32), "max_rel_err_bf16x3_vs_fp64": max(errsmx),
         "median_rel_err_bf16x3_vs_fp64": float(np.median(errsmx)),
+        "roofline": {"bound": "mfma", "kernel": "bf16x3", "unit": "TFLOP/s", "peak": MI355X_PEAK["bf16"],
+                     "achieved": round(x3_fl / (x3_ms * 1e-3) / 1e12, 3),
+                     "frac": round(x3_fl / (x3_ms * 1e-3) / 1e12 / MI355X_PEAK["bf16"], 5), "traffic": None,
+                     "note": "groups run one at a time; per-group ceilings in roofline_groups"},
+        "roofline_groups": roof_groups,
         "instances": per,
         "host_build_s": round(t_build, 1),
     }
@@ -437,11 +461,16 @@ def run_scale(args):
         t_plan = time.time() - t1
         Wp = W if prec != "fp32" else W.float()
         Y = torch.empty_like(Wp)
+        t1 = time.time()
+        plan.autotune(prec, nrhs, "f32" if prec == "fp32" else "f64", reps=1)
+        t_tune = time.time() - t1
         plan.set_graph(not args.no_graph)
         steps = max(2, args.steps // (8 if prec == "fp64" else 2))
         ms = timer(lambda: plan.eval(Wp, Y, nrhs, prec), steps, 1)
-        fl, _ = plan.flops(prec, nrhs)
-        r = {"ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 3), "plan_s": round(t_plan, 1)}
+        fl, parts = plan.flops(prec, nrhs)
+        roof, _ = launch_profile(plan, lambda: plan.eval(Wp, Y, nrhs, prec), prec, nrhs, parts, nprof=1)
+        r = {"ms": round(ms, 3), "tflops": round(fl / (ms * 1e-3) / 1e12, 3), "plan_s": round(t_plan, 1),
+             "autotune_s": round(t_tune, 1), "schedule": plan.get_schedule("forward", prec), "roofline": roof}
         if prec == "fp64":
             Y64 = Y
             if not args.skip_checks:
@@ -476,6 +505,7 @@ def run_scale(args):
                    "mean_rank": round(float(hm.slen[1:].mean()), 2), "check_rows": wl["check_rows"]},
         "algorithmic_tflop_per_eval": round(fl64 / 1e12, 4),
         "precisions": out_prec,
+        "roofline": out_prec["bf16x3"]["roofline"],
         "host_build_s": round(t_build, 1),
     }
     print(json.dumps(out), flush=True)
