@@ -310,13 +310,24 @@ void ensure_panels(mx_plan *P, int nrhs) {
 
 template <typename T>
 static void run_seg_launch(const Launch &ln, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs,
-                           const Bufs<T> &b, hipStream_t st) {
+                           const Bufs<T> &b, int depth, hipStream_t st) {
     int nnt = (nrhs + SG_BN - 1) / SG_BN;
     int64_t total = (int64_t)ln.ntiles * nnt;
     if (total <= 0) return;
     if (total > INT32_MAX) throw MxError(MX_ERR_ARG, "launch too large");
-    hipLaunchKernelGGL(seg_gemm_kernel<T>, dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
-                       tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+    if (depth == 2)
+        hipLaunchKernelGGL((seg_gemm_kernel<T, 2>), dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
+                           tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+    else
+        hipLaunchKernelGGL((seg_gemm_kernel<T, 1>), dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
+                           tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+}
+
+// prefetch depth of launch i of the FP64/FP32 program in use (1 unless the
+// autotuner chose otherwise)
+static int seg_depth(const mx_plan *P, int dir, int kd, size_t i) {
+    const std::vector<int> &d = P->ldepth[dir][kd][P->sched[dir][0]];
+    return i < d.size() ? d[i] : 1;
 }
 
 static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
@@ -368,7 +379,8 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         b.vec[VEC_Y] = (double *)Y;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F64, true);
-            run_seg_launch<double>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, st);
+            run_seg_launch<double>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, seg_depth(P, dir, 0, i),
+                                   st);
             prof_mark(P, st, MX_KIND_SEG_F64, false);
         }
     } else if (prec == MX_FP32) {
@@ -396,7 +408,8 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         b.vec[VEC_Y] = Yf;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F32, true);
-            run_seg_launch<float>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, st);
+            run_seg_launch<float>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, seg_depth(P, dir, 1, i),
+                                  st);
             prof_mark(P, st, MX_KIND_SEG_F32, false);
         }
         if (io == MX_IO_F64) {
@@ -708,6 +721,43 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         if (prec == MX_BF16X3) P->x3_variant = best_variant;
         P->sched[0][kd] = best_pick[0];
         P->sched[1][kd] = best_pick[1];
+        if (prec != MX_BF16X3) {
+            // per launch: prefetch depth 2 for latency-bound launches, depth
+            // 1 where more workgroups per CU pay
+            const int seg_kind = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
+            const int pk = prec == MX_FP64 ? 0 : 1;
+            P->profiling = 1;
+            for (int dir = 0; dir < ndir; ++dir) {
+                const int sc = P->sched[dir][0];
+                const size_t nl = P->prog[dir][0][sc].launches.size();
+                std::vector<int> &ld = P->ldepth[dir][pk][sc];
+                std::vector<int> pick(nl, 1);
+                std::vector<float> best(nl, 0.f);
+                for (int depth = 1; depth <= 2; ++depth) {
+                    ld.assign(nl, depth);
+                    eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
+                    std::vector<float> t(nl, 0.f);
+                    for (int r = 0; r < reps; ++r) {
+                        eval(P, dir, prec, W, Y, nrhs, io, st);
+                        HIPCHK(hipEventSynchronize(P->ev[2 * P->ev_used - 1]));
+                        size_t li = 0;
+                        for (int e = 0; e < P->ev_used && li < nl; ++e) {
+                            if (P->ev_kind[e] != seg_kind) continue;
+                            float ms = 0.f;
+                            HIPCHK(hipEventElapsedTime(&ms, P->ev[2 * e], P->ev[2 * e + 1]));
+                            t[li++] += ms;
+                        }
+                    }
+                    for (size_t i = 0; i < nl; ++i)
+                        if (depth == 1 || t[i] < 0.97f * best[i]) {
+                            best[i] = t[i];
+                            pick[i] = depth;
+                        }
+                }
+                ld = pick;
+            }
+            P->profiling = 0;
+        }
         if (prec == MX_BF16X3) {
             // per launch: the deep top-of-tree launches and the wide leaf
             // launches favour different variants
@@ -886,7 +936,7 @@ void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf
         b.vec[VEC_W] = const_cast<double *>(W);
         b.vec[VEC_Y] = out;
         Launch ln{0, (int)L.tiles.size()};
-        run_seg_launch<double>(ln, dt, ds, dtl, nrhs, b, st);
+        run_seg_launch<double>(ln, dt, ds, dtl, nrhs, b, 1, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
     } catch (...) {

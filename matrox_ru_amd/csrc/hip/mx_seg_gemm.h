@@ -6,8 +6,10 @@
 // (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32).  The K loop walks the
 // task's segments in 16-deep chunks, staging the A chunk (64 x 16, padded,
 // m fastest) and the B chunk (16 x 64, k fastest, padded) in LDS with a
-// double buffer: the global loads of chunk i+1 are in flight while chunk i
-// is multiplied.  Ragged edges (block sizes that are not multiples of the
+// double buffer; the global loads of chunk i+1 (DEPTH 1) or of chunks i+1
+// and i+2 (DEPTH 2, two register stages) are in flight while chunk i is
+// multiplied.  DEPTH 2 serves launches whose chains of chunks are
+// latency-bound (few, deep tasks); DEPTH 1 keeps more workgroups per CU.  Ragged edges (block sizes that are not multiples of the
 // tile) load zeros.  The finished tile goes through LDS so that the global
 // stores (and, for beta = 1, loads) of C are 64 consecutive rows per column.
 //
@@ -53,15 +55,16 @@ struct SegCursor {
     int k0;    // first k of the chunk within the segment
 };
 
-// Per-thread register staging of one chunk: 4 A values and 4 B values.
+// Depth-1 staging: 4 A values and 4 B values per thread, loaded only where
+// they lie inside the block (zeros elsewhere).
 template <typename T>
-struct Stage {
+struct Stage1 {
     T a[4];
     T b[4];
 };
 
 template <typename T>
-__device__ __forceinline__ void load_chunk(Stage<T> &st, const Seg &sg, int k0, int m0, int M, int n0,
+__device__ __forceinline__ void load_chunk1(Stage1<T> &st, const Seg &sg, int k0, int m0, int M, int n0,
                                            int nrhs, const Bufs<T> &bufs, int tid) {
     const T *A = bufs.mat[sg.a_buf] + sg.a_off;
     const int K = sg.k;
@@ -93,7 +96,7 @@ __device__ __forceinline__ void load_chunk(Stage<T> &st, const Seg &sg, int k0, 
 }
 
 template <typename T>
-__device__ __forceinline__ void store_chunk(const Stage<T> &st, int a_trans, T *As, T *Bs, int tid) {
+__device__ __forceinline__ void store_chunk1(const Stage1<T> &st, int a_trans, T *As, T *Bs, int tid) {
     if (!a_trans) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -111,6 +114,78 @@ __device__ __forceinline__ void store_chunk(const Stage<T> &st, int a_trans, T *
     for (int r = 0; r < 4; ++r) {
         int e = r * SG_THREADS + tid;
         Bs[(e / SG_BK) * SG_BP + (e % SG_BK)] = st.b[r];
+    }
+}
+
+// Per-thread register staging of one chunk: 4 A values and 4 B values, as
+// loaded, plus a bit per value that says whether it lies inside the block
+// (out-of-range elements are loaded from a clamped, valid address and zeroed
+// when the chunk is stored to LDS, so the loads carry no branches and the
+// compiler can wait for one chunk's loads while the next chunk's are still in
+// flight).
+template <typename T>
+struct Stage {
+    T a[4];
+    T b[4];
+    uint32_t in;   // bits 0-3: a[r] in range, bits 4-7: b[r]
+};
+
+template <typename T>
+__device__ __forceinline__ void load_chunk(Stage<T> &st, const Seg &sg, int k0, int m0, int M, int n0,
+                                           int nrhs, const Bufs<T> &bufs, int tid) {
+    const T *A = bufs.mat[sg.a_buf] + sg.a_off;
+    const int K = sg.k;
+    uint32_t in = 0;
+    if (!sg.a_trans) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            int kk = e / SG_BM, mm = e % SG_BM;
+            int gm = m0 + mm, gk = k0 + kk;
+            in |= (gm < M && gk < K ? 1u : 0u) << r;
+            st.a[r] = A[(int64_t)min(gk, K - 1) * sg.lda + min(gm, M - 1)];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            int mm = e / SG_BK, kk = e % SG_BK;
+            int gm = m0 + mm, gk = k0 + kk;
+            in |= (gm < M && gk < K ? 1u : 0u) << r;
+            st.a[r] = A[(int64_t)min(gm, M - 1) * sg.lda + min(gk, K - 1)];
+        }
+    }
+    const T *Bv = bufs.vec[sg.b_buf] + sg.b_row * (int64_t)nrhs;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int e = r * SG_THREADS + tid;
+        int nn = e / SG_BK, kk = e % SG_BK;
+        int gn = n0 + nn, gk = k0 + kk;
+        in |= (gn < nrhs && gk < K ? 1u : 0u) << (4 + r);
+        st.b[r] = Bv[(int64_t)min(gn, nrhs - 1) * sg.b_ld + min(gk, K - 1)];
+    }
+    st.in = in;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_chunk(const Stage<T> &st, int a_trans, T *As, T *Bs, int tid) {
+    if (!a_trans) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            As[(e / SG_BM) * SG_AP + (e % SG_BM)] = (st.in >> r) & 1u ? st.a[r] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            int e = r * SG_THREADS + tid;
+            As[(e % SG_BK) * SG_AP + (e / SG_BK)] = (st.in >> r) & 1u ? st.a[r] : T(0);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        int e = r * SG_THREADS + tid;
+        Bs[(e / SG_BK) * SG_BP + (e % SG_BK)] = (st.in >> (4 + r)) & 1u ? st.b[r] : T(0);
     }
 }
 
@@ -132,7 +207,10 @@ __device__ __forceinline__ bool first_chunk(const Seg *segs, int seg_begin, int 
     return seg < seg_end;
 }
 
-template <typename T>
+// DEPTH: chunks whose global loads are in flight while a chunk is multiplied
+// (1: one register stage, more workgroups per CU; 2: two register stages,
+// for latency-bound launches).  The plan picks it per launch.
+template <typename T, int DEPTH>
 __global__ __launch_bounds__(SG_THREADS) void seg_gemm_kernel(const Task *__restrict__ tasks,
                                                               const Seg *__restrict__ segs,
                                                               const int2 *__restrict__ tiles, int ntiles,
@@ -163,49 +241,121 @@ __global__ __launch_bounds__(SG_THREADS) void seg_gemm_kernel(const Task *__rest
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
+    // Chunk c is multiplied from LDS stage c % 2 while the loads of chunks
+    // c+1 and c+2 are in flight in the two register stages st0 and st1; the
+    // loop is unrolled by two so that each stage is a fixed set of registers.
+    // Every step issues the same loads whether or not its chunk exists (a
+    // missing chunk re-reads the task's first chunk and is never stored), so
+    // the wait before storing chunk c+1 leaves chunk c+2's loads in flight
+    // on every path.
     int seg, k0;
     bool have = first_chunk(segs, task.seg_begin, task.seg_end, seg, k0);
-    Stage<T> st;
-    int cur = 0;
-    if (have) {
-        Seg sg = segs[seg];
-        load_chunk(st, sg, k0, m0, M, n0, nrhs, bufs, tid);
-        store_chunk(st, sg.a_trans, smem, smem + A_ELEMS, tid);
-    }
-    __syncthreads();
-    while (have) {
-        int nseg = seg, nk0 = k0;
-        bool nhave = next_chunk(segs, task.seg_end, nseg, nk0);
-        Seg nsg;
-        if (nhave) {
-            nsg = segs[nseg];
-            load_chunk(st, nsg, nk0, m0, M, n0, nrhs, bufs, tid);
-        }
-        const T *As = smem + cur * STAGE_ELEMS;
-        const T *Bs = As + A_ELEMS;
-#pragma unroll
-        for (int kk = 0; kk < SG_BK; kk += 4) {
-            T a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[(kk + lk) * SG_AP + wr * 32 + i * 16 + lr];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[(wc * 32 + j * 16 + lr) * SG_BP + kk + lk];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
-        }
-        if (nhave) {
-            T *Asn = smem + (cur ^ 1) * STAGE_ELEMS;
-            store_chunk(st, nsg.a_trans, Asn, Asn + A_ELEMS, tid);
+    if (have && DEPTH == 1) {
+        Stage1<T> st;
+        int cur = 0;
+        {
+            Seg sg = segs[seg];
+            load_chunk1(st, sg, k0, m0, M, n0, nrhs, bufs, tid);
+            store_chunk1(st, sg.a_trans, smem, smem + A_ELEMS, tid);
         }
         __syncthreads();
-        cur ^= 1;
-        seg = nseg;
-        k0 = nk0;
-        have = nhave;
+        while (have) {
+            int nseg = seg, nk0 = k0;
+            bool nhave = next_chunk(segs, task.seg_end, nseg, nk0);
+            Seg nsg;
+            if (nhave) {
+                nsg = segs[nseg];
+                load_chunk1(st, nsg, nk0, m0, M, n0, nrhs, bufs, tid);
+            }
+            const T *As = smem + cur * STAGE_ELEMS;
+            const T *Bs = As + A_ELEMS;
+#pragma unroll
+            for (int kk = 0; kk < SG_BK; kk += 4) {
+                T a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a[i] = As[(kk + lk) * SG_AP + wr * 32 + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) b[j] = Bs[(wc * 32 + j * 16 + lr) * SG_BP + kk + lk];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
+            }
+            if (nhave) {
+                T *Asn = smem + (cur ^ 1) * STAGE_ELEMS;
+                store_chunk1(st, nsg.a_trans, Asn, Asn + A_ELEMS, tid);
+            }
+            __syncthreads();
+            cur ^= 1;
+            seg = nseg;
+            k0 = nk0;
+            have = nhave;
+        }
+    } else if (have) {
+        const int sf = seg, kf = k0;   // a chunk that always exists
+        Stage<T> st0, st1;
+        // cursors of the chunks c+1 and c+2 (valid when h1 / h2), their a_trans
+        int s1 = seg, k1 = k0, s2 = 0, k2 = 0;
+        bool h1, h2 = false;
+        int t1 = 0, t2 = 0;
+        auto load_or_dummy = [&](Stage<T> &st, bool h, int sc, int kc) {
+            const Seg sg = segs[h ? sc : sf];
+            load_chunk(st, sg, h ? kc : kf, m0, M, n0, nrhs, bufs, tid);
+            return sg.a_trans;
+        };
+        {
+            const Seg sg = segs[seg];
+            load_chunk(st0, sg, k0, m0, M, n0, nrhs, bufs, tid);
+            store_chunk(st0, sg.a_trans, smem, smem + A_ELEMS, tid);
+        }
+        h1 = next_chunk(segs, task.seg_end, s1, k1);
+        t1 = load_or_dummy(st0, h1, s1, k1);
+        s2 = s1;
+        k2 = k1;
+        h2 = h1 && next_chunk(segs, task.seg_end, s2, k2);
+        t2 = load_or_dummy(st1, h2, s2, k2);
+        __syncthreads();
+        // one chunk: multiply LDS stage cur, move chunk c+1 (register stage
+        // sp) to the other LDS stage, load chunk c+3 into sp
+        auto step = [&](Stage<T> &sp, int cur) {
+            const T *As = smem + cur * STAGE_ELEMS;
+            const T *Bs = As + A_ELEMS;
+#pragma unroll
+            for (int kk = 0; kk < SG_BK; kk += 4) {
+                T a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) a[i] = As[(kk + lk) * SG_AP + wr * 32 + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) b[j] = Bs[(wc * 32 + j * 16 + lr) * SG_BP + kk + lk];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
+            }
+            if (h1) {
+                T *Asn = smem + (cur ^ 1) * STAGE_ELEMS;
+                store_chunk(sp, t1, Asn, Asn + A_ELEMS, tid);
+            }
+            int s3 = s2, k3 = k2;
+            const bool h3 = h2 && next_chunk(segs, task.seg_end, s3, k3);
+            const int t3 = load_or_dummy(sp, h3, s3, k3);
+            __syncthreads();
+            have = h1;
+            h1 = h2;
+            s1 = s2;
+            k1 = k2;
+            t1 = t2;
+            h2 = h3;
+            s2 = s3;
+            k2 = k3;
+            t2 = t3;
+        };
+        while (have) {
+            step(st0, 0);
+            if (!have) break;
+            step(st1, 1);
+        }
     }
-
     // C tile -> LDS (column-major, padded) -> coalesced global update
     T *Cs = smem;
 #pragma unroll
