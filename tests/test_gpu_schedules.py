@@ -115,6 +115,38 @@ def test_pinned_launch_variants(gpu):
     assert rel(Yf.cpu().numpy(), Yo) <= 1e-5
 
 
+def test_balanced_schedule_moves_near_work(gpu):
+    """The balanced schedules move the leading near blocks of leaves with long
+    near lists into the UP launches: on the transposed structure (whose row
+    lists are the original's column lists, up to twice the mean here) the
+    final NEAR+LEAF launch loses flops to earlier launches, the total stays,
+    and the result still matches the oracle."""
+    import torch
+    _say("[balanced schedule]")
+    X, hm = make_problem(**CASES["h2b_small"])
+    hmT = hm.transposed()
+    nrhs = 40
+    TW = host.normal((hmT.N * nrhs,), seed=23)
+    Yo, _, _ = oracle.hgemm(hmT, TW, nrhs)
+    plan = hgemm.Plan(hmT, precisions=("fp64", "fp32", "bf16x3"))
+    W = torch.from_numpy(TW).cuda()
+    for prec in ("fp64", "fp32", "bf16x3"):
+        flops = {}
+        for sc in ("split", "split_balanced", "fused", "fused_balanced"):
+            plan.set_schedule("forward", prec, sc)
+            flops[sc] = plan.launch_flops(prec, nrhs)
+            Y = torch.full_like(W, float("nan"))
+            plan.eval(W, Y, nrhs, prec)
+            torch.cuda.synchronize()
+            assert rel(Y.cpu().numpy(), Yo) <= (1e-12 if prec == "fp64" else 1e-5), (prec, sc)
+        for base in ("split", "fused"):
+            f0, f1 = flops[base], flops[base + "_balanced"]
+            _say(f"  {prec} {base}: NEAR+LEAF {f0[-1] / 1e6:.1f} -> {f1[-1] / 1e6:.1f} MFLOP")
+            assert len(f0) == len(f1)
+            assert sum(f1) == pytest.approx(sum(f0), rel=1e-12)
+            assert f1[-1] < f0[-1]
+
+
 def test_long_interaction_lists_are_split(gpu):
     """Blocks with more interactions than the BF16x3 kernel's segment window
     (up to 459 near blocks per row leaf, 512 per column leaf here) run as a
