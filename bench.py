@@ -514,8 +514,9 @@ def run_scale(args):
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this evaluator runs on one GPU; multi-GPU runs (--gpus > 1) are not part of this "
-              "build (the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py: this evaluator runs on one GPU; runs on more than one GPU (--gpus > 1) are not "
+              "supported (the tree would shard by subtrees, with one exchange of skeleton blocks before the "
+              "coupling phase; not built).", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

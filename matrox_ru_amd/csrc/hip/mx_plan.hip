@@ -91,10 +91,6 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     P->VT_len = s.VT_len;
     P->precisions = precisions & 7u;
     P->adjoint = (precisions & MX_PLAN_ADJOINT) != 0;
-    if (const char *e = getenv("MX_X3_VARIANT")) {
-        int v = atoi(e);
-        if (v >= 1 && v <= X3_NVARIANT) P->x3_variant = v;
-    }
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     const int ndir = P->adjoint ? 2 : 1;
