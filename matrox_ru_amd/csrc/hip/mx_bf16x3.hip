@@ -378,8 +378,9 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][NJ], const uint16_t *
 //    XOR-swizzled by column so the 16 lanes of a write hit distinct banks)
 //    and stored as 64 consecutive rows per column (beta = 1 adds to Y).
 //  * Skeleton buffers: written as split panels straight from registers (rows
-//    up to the padded height; padding rows are 0); beta = 1 adds the panel's
-//    current value first.
+//    up to the padded height; padding rows are 0), split with the hardware
+//    round-to-nearest-even conversion; beta = 1 adds the panel's current
+//    value first.
 template <int NI, int NJ>
 __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg, const X3Args &a, const Task &task,
                                             int lane, int wv, int wr, int wc, int m0, int n0) {
@@ -441,16 +442,11 @@ __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg,
                         v[r] += bf16_f((uint16_t)(wh >> (16 * (r & 1)))) + bf16_f((uint16_t)(wl >> (16 * (r & 1))));
                     }
                 }
-                uint16_t hv[4], lv[4];
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    hv[r] = bf16_rne(v[r]);
-                    lv[r] = bf16_rne(v[r] - bf16_f(hv[r]));
-                }
-                *reinterpret_cast<u32x2_t *>(Ph + o) =
-                    u32x2_t{(uint32_t)hv[0] | ((uint32_t)hv[1] << 16), (uint32_t)hv[2] | ((uint32_t)hv[3] << 16)};
-                *reinterpret_cast<u32x2_t *>(Pl + o) =
-                    u32x2_t{(uint32_t)lv[0] | ((uint32_t)lv[1] << 16), (uint32_t)lv[2] | ((uint32_t)lv[3] << 16)};
+                uint32_t hw[2], lw[2];
+                split2(v[0], v[1], hw[0], lw[0]);
+                split2(v[2], v[3], hw[1], lw[1]);
+                *reinterpret_cast<u32x2_t *>(Ph + o) = u32x2_t{hw[0], hw[1]};
+                *reinterpret_cast<u32x2_t *>(Pl + o) = u32x2_t{lw[0], lw[1]};
             }
         }
     }
