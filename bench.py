@@ -84,8 +84,10 @@ def parse():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint and backward timings")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a hipGraph")
     ap.add_argument("--launch-variants", default="",
-                    help="comma-separated BF16x3 kernel variant per forward launch under the fused schedule, "
+                    help="comma-separated BF16x3 kernel variant per forward launch under --schedule, "
                          "replacing the autotuner's choice (to profile a fixed configuration)")
+    ap.add_argument("--schedule", default="fused", choices=["fused", "split", "fused_balanced", "split_balanced"],
+                    help="forward schedule used with --launch-variants")
     ap.add_argument("--dropin", action="store_true",
                     help="also time the drop-in HGEMM entry point on host arrays (PCIe-inclusive)")
     return ap.parse_args()
@@ -569,7 +571,7 @@ def main():
             tuning[p]["launch_variants"] = {d: plan.launch_variants(d)
                                             for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
     if args.launch_variants and "bf16x3" in plan.precisions:
-        plan.set_schedule("forward", "bf16x3", "fused")
+        plan.set_schedule("forward", "bf16x3", args.schedule)
         plan.set_launch_variants([int(v) for v in args.launch_variants.split(",")], "forward")
         tuning["bf16x3"]["forward"] = plan.get_schedule("forward", "bf16x3")
         tuning["bf16x3"]["launch_variants"]["forward"] = plan.launch_variants("forward")
