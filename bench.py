@@ -570,6 +570,9 @@ def main():
         if p == "bf16x3":
             tuning[p]["launch_variants"] = {d: plan.launch_variants(d)
                                             for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
+        else:
+            tuning[p]["seg_variants"] = {d: plan.seg_variants(p, d)
+                                         for d in (["forward", "adjoint"] if plan.adjoint else ["forward"])}
     if args.launch_variants and "bf16x3" in plan.precisions:
         plan.set_schedule("forward", "bf16x3", args.schedule)
         plan.set_launch_variants([int(v) for v in args.launch_variants.split(",")], "forward")

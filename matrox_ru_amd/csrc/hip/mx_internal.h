@@ -101,9 +101,10 @@ struct mx_plan {
     // per-launch BF16x3 variants [direction][schedule] chosen by the
     // autotuner (empty: x3_variant for every launch)
     std::vector<int> lvar[2][mx::NSCHED];
-    // per-launch prefetch depth of the FP64 / FP32 kernel [direction][0: FP64,
-    // 1: FP32][schedule] chosen by the autotuner (empty: depth 1)
-    std::vector<int> ldepth[2][2][mx::NSCHED];
+    // per-launch variant of the FP64 / FP32 kernel (tile size, prefetch
+    // depth) [direction][0: FP64, 1: FP32][schedule] chosen by the autotuner
+    // (empty: 64 x 64 tiles, depth 1)
+    std::vector<int> lsg[2][2][mx::NSCHED];
     mx::X3Rows *d_wrows = nullptr;
     mx::X3Rows *d_srows = nullptr;      // skeleton blocks (slen > 0) for panel export
     int n_srows = 0;

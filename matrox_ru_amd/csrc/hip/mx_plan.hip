@@ -98,9 +98,12 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         for (int sc = 0; sc < NSCHED; ++sc) {
             TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
             compute_launch_flops(L);
+            pair_tiles(L);
             Program &pg = P->prog[dir][0][sc];
             upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
             pg.launches = L.launches;
+            pg.tiles2 = upload(L.tiles2.data(), L.tiles2.size());
+            pg.launches2 = L.launches2;
             pg.flops = L.flops;
             pg.built = true;
         }
@@ -304,26 +307,37 @@ void ensure_panels(mx_plan *P, int nrhs) {
     }
 }
 
+// FP64/FP32 kernel variants of a launch: 64 x 64 tiles with one or two
+// chunks of loads in flight, or (FP32) 128 x 128 tiles on the plan's 128-row
+// tile list
+enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3 };
+
 template <typename T>
-static void run_seg_launch(const Launch &ln, const Task *tasks, const Seg *segs, const int2 *tiles, int nrhs,
-                           const Bufs<T> &b, int depth, hipStream_t st) {
-    int nnt = (nrhs + SG_BN - 1) / SG_BN;
+static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T> &b, int variant, hipStream_t st) {
+    const bool wide = variant == SEG_V128 && sizeof(T) == sizeof(float);
+    const Launch &ln = wide ? pg.launches2[li] : pg.launches[li];
+    const int bn = wide ? SW_BN : SG_BN;
+    int nnt = (nrhs + bn - 1) / bn;
     int64_t total = (int64_t)ln.ntiles * nnt;
     if (total <= 0) return;
     if (total > INT32_MAX) throw MxError(MX_ERR_ARG, "launch too large");
-    if (depth == 2)
-        hipLaunchKernelGGL((seg_gemm_kernel<T, 2>), dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
-                           tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+    const dim3 grid((unsigned)total);
+    if (wide)
+        hipLaunchKernelGGL(seg_gemm_wide_kernel<float>, grid, dim3(SG_THREADS), 0, st, pg.tasks, pg.segs,
+                           pg.tiles2 + ln.tile_offset, ln.ntiles, nrhs, nnt, reinterpret_cast<const Bufs<float> &>(b));
+    else if (variant == SEG_V64_D2)
+        hipLaunchKernelGGL((seg_gemm_kernel<T, 2>), grid, dim3(SG_THREADS), 0, st, pg.tasks, pg.segs,
+                           pg.tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
     else
-        hipLaunchKernelGGL((seg_gemm_kernel<T, 1>), dim3((unsigned)total), dim3(SG_THREADS), 0, st, tasks, segs,
-                           tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
+        hipLaunchKernelGGL((seg_gemm_kernel<T, 1>), grid, dim3(SG_THREADS), 0, st, pg.tasks, pg.segs,
+                           pg.tiles + ln.tile_offset, ln.ntiles, nrhs, nnt, b);
 }
 
-// prefetch depth of launch i of the FP64/FP32 program in use (1 unless the
-// autotuner chose otherwise)
-static int seg_depth(const mx_plan *P, int dir, int kd, size_t i) {
-    const std::vector<int> &d = P->ldepth[dir][kd][P->sched[dir][0]];
-    return i < d.size() ? d[i] : 1;
+// kernel variant of launch i of the FP64/FP32 program in use (SEG_V64_D1
+// unless the autotuner chose otherwise)
+static int seg_variant(const mx_plan *P, int dir, int kd, size_t i) {
+    const std::vector<int> &d = P->lsg[dir][kd][P->sched[dir][0]];
+    return i < d.size() ? d[i] : SEG_V64_D1;
 }
 
 static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
@@ -375,8 +389,7 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         b.vec[VEC_Y] = (double *)Y;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F64, true);
-            run_seg_launch<double>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, seg_depth(P, dir, 0, i),
-                                   st);
+            run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st);
             prof_mark(P, st, MX_KIND_SEG_F64, false);
         }
     } else if (prec == MX_FP32) {
@@ -404,8 +417,7 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         b.vec[VEC_Y] = Yf;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             prof_mark(P, st, MX_KIND_SEG_F32, true);
-            run_seg_launch<float>(pg0.launches[i], pg0.tasks, pg0.segs, pg0.tiles, nrhs, b, seg_depth(P, dir, 1, i),
-                                  st);
+            run_seg_launch<float>(pg0, i, nrhs, b, seg_variant(P, dir, 1, i), st);
             prof_mark(P, st, MX_KIND_SEG_F32, false);
         }
         if (io == MX_IO_F64) {
@@ -718,18 +730,20 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         P->sched[0][kd] = best_pick[0];
         P->sched[1][kd] = best_pick[1];
         if (prec != MX_BF16X3) {
-            // per launch: prefetch depth 2 for latency-bound launches, depth
-            // 1 where more workgroups per CU pay
+            // per launch: 64 x 64 tiles with prefetch depth 2 for
+            // latency-bound launches, depth 1 where more workgroups per CU
+            // pay, 128 x 128 tiles (FP32) for the wide launches
             const int seg_kind = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
             const int pk = prec == MX_FP64 ? 0 : 1;
+            const int nv = prec == MX_FP64 ? SEG_V64_D2 : SEG_V128;
             P->profiling = 1;
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][0];
                 const size_t nl = P->prog[dir][0][sc].launches.size();
-                std::vector<int> &ld = P->ldepth[dir][pk][sc];
-                std::vector<int> pick(nl, 1);
+                std::vector<int> &ld = P->lsg[dir][pk][sc];
+                std::vector<int> pick(nl, SEG_V64_D1);
                 std::vector<float> best(nl, 0.f);
-                for (int depth = 1; depth <= 2; ++depth) {
+                for (int depth = SEG_V64_D1; depth <= nv; ++depth) {
                     ld.assign(nl, depth);
                     eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
                     std::vector<float> t(nl, 0.f);
@@ -745,7 +759,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                         }
                     }
                     for (size_t i = 0; i < nl; ++i)
-                        if (depth == 1 || t[i] < 0.97f * best[i]) {
+                        if (depth == SEG_V64_D1 || t[i] < 0.97f * best[i]) {
                             best[i] = t[i];
                             pick[i] = depth;
                         }
@@ -931,8 +945,12 @@ void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf
         b.mat[MAT_D] = Kr;
         b.vec[VEC_W] = const_cast<double *>(W);
         b.vec[VEC_Y] = out;
-        Launch ln{0, (int)L.tiles.size()};
-        run_seg_launch<double>(ln, dt, ds, dtl, nrhs, b, 1, st);
+        Program pg;   // views the lists uploaded above; they are freed below
+        pg.tasks = dt;
+        pg.segs = ds;
+        pg.tiles = dtl;
+        pg.launches.push_back(Launch{0, (int)L.tiles.size()});
+        run_seg_launch<double>(pg, 0, nrhs, b, SEG_V64_D1, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(st));
     } catch (...) {
@@ -1327,6 +1345,47 @@ mx_status mx_plan_set_launch_variants(mx_plan *plan, int32_t direction, const in
                       "count must equal the number of launches of the current schedule");
             for (int32_t i = 0; i < count; ++i)
                 arg_check(variants[i] >= 1 && variants[i] <= X3_NVARIANT, "bad BF16x3 kernel variant");
+            lv.assign(variants, variants + count);
+        }
+        drop_graphs(plan);
+        return 0;
+    });
+}
+
+mx_status mx_plan_seg_variants(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *variants,
+                               int32_t cap, int32_t *count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && count && (direction == 0 || direction == 1) && cap >= 0 && (cap == 0 || variants),
+                  "bad argument");
+        arg_check(prec == MX_FP64 || prec == MX_FP32, "FP64 or FP32 only");
+        const int pk = prec == MX_FP64 ? 0 : 1;
+        const int sc = plan->sched[direction][0];
+        const size_t nl = plan->prog[direction][0][sc].launches.size();
+        *count = (int32_t)nl;
+        for (int32_t i = 0; i < (int32_t)nl && i < cap; ++i) variants[i] = seg_variant(plan, direction, pk, i);
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_seg_variants(mx_plan *plan, int32_t direction, mx_precision prec, const int32_t *variants,
+                                   int32_t count) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && (direction == 0 || direction == 1), "bad argument");
+        arg_check(prec == MX_FP64 || prec == MX_FP32, "FP64 or FP32 only");
+        if (direction == DIR_ADJ && !plan->adjoint)
+            throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+        const int pk = prec == MX_FP64 ? 0 : 1;
+        const int sc = plan->sched[direction][0];
+        std::vector<int> &lv = plan->lsg[direction][pk][sc];
+        if (count == 0) {
+            lv.clear();
+        } else {
+            arg_check(variants != nullptr, "null variants");
+            arg_check(count == (int32_t)plan->prog[direction][0][sc].launches.size(),
+                      "count must equal the number of launches of the current schedule");
+            const int nv = prec == MX_FP64 ? SEG_V64_D2 : SEG_V128;
+            for (int32_t i = 0; i < count; ++i)
+                arg_check(variants[i] >= SEG_V64_D1 && variants[i] <= nv, "bad FP64/FP32 kernel variant");
             lv.assign(variants, variants + count);
         }
         drop_graphs(plan);

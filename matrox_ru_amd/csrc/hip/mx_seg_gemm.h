@@ -9,9 +9,14 @@
 // double buffer; the global loads of chunk i+1 (DEPTH 1) or of chunks i+1
 // and i+2 (DEPTH 2, two register stages) are in flight while chunk i is
 // multiplied.  DEPTH 2 serves launches whose chains of chunks are
-// latency-bound (few, deep tasks); DEPTH 1 keeps more workgroups per CU.  Ragged edges (block sizes that are not multiples of the
-// tile) load zeros.  The finished tile goes through LDS so that the global
-// stores (and, for beta = 1, loads) of C are 64 consecutive rows per column.
+// latency-bound (few, deep tasks); DEPTH 1 keeps more workgroups per CU.
+// Ragged edges (block sizes that are not multiples of the tile) load zeros.
+// The finished tile goes through LDS so that the global stores (and, for
+// beta = 1, loads) of C are 64 consecutive rows per column.
+//
+// seg_gemm_wide_kernel is the FP32 form with 128 x 128 tiles (4 waves of
+// 64 x 64, 4 x 4 MFMA blocks each): half the L2 traffic per flop of the
+// 64 x 64 tile, for the wide launches.
 //
 // MFMA 16x16x4 operand layout (lane l of 64): A element (row l%16, k l/16),
 // B element (k l/16, col l%16).  Result register r holds C(4*(l/16)+r, l%16)
@@ -377,6 +382,158 @@ __global__ __launch_bounds__(SG_THREADS) void seg_gemm_kernel(const Task *__rest
         if (gm < M && gn < nrhs) {
             T v = Cs[col * SG_CP + row];
             T *p = C + (int64_t)gn * M + gm;
+            *p = task.beta ? *p + v : v;
+        }
+    }
+}
+
+// ---- FP32, 128 x 128 tiles --------------------------------------------------
+
+constexpr int SW_BM = 128, SW_BN = 128, SW_BK = 16;
+constexpr int SW_AP = SW_BM + 16;   // A chunk [k][m]: 16 rows x 4 k of a fragment read hit distinct banks
+constexpr int SW_BP = SW_BK + 4;    // B chunk [n][k]: likewise for 16 columns x 4 k
+constexpr int SW_CP = SW_BM + 1;    // C tile [n][m] in the epilogue
+
+struct StageW {
+    float a[8];
+    float b[8];
+};
+
+__device__ __forceinline__ void load_chunk_w(StageW &st, const Seg &sg, int k0, int m0, int M, int n0, int nrhs,
+                                             const Bufs<float> &bufs, int tid) {
+    const float *A = bufs.mat[sg.a_buf] + sg.a_off;
+    const int K = sg.k;
+    if (!sg.a_trans) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int e = r * SG_THREADS + tid;
+            int kk = e / SW_BM, mm = e % SW_BM;
+            int gm = m0 + mm, gk = k0 + kk;
+            st.a[r] = (gm < M && gk < K) ? A[(int64_t)gk * sg.lda + gm] : 0.f;
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            int e = r * SG_THREADS + tid;
+            int mm = e / SW_BK, kk = e % SW_BK;
+            int gm = m0 + mm, gk = k0 + kk;
+            st.a[r] = (gm < M && gk < K) ? A[(int64_t)gm * sg.lda + gk] : 0.f;
+        }
+    }
+    const float *Bv = bufs.vec[sg.b_buf] + sg.b_row * (int64_t)nrhs;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        int e = r * SG_THREADS + tid;
+        int nn = e / SW_BK, kk = e % SW_BK;
+        int gn = n0 + nn, gk = k0 + kk;
+        st.b[r] = (gn < nrhs && gk < K) ? Bv[(int64_t)gn * sg.b_ld + gk] : 0.f;
+    }
+}
+
+__device__ __forceinline__ void store_chunk_w(const StageW &st, int a_trans, float *As, float *Bs, int tid) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        int e = r * SG_THREADS + tid;
+        if (!a_trans) As[(e / SW_BM) * SW_AP + (e % SW_BM)] = st.a[r];
+        else As[(e % SW_BK) * SW_AP + (e / SW_BK)] = st.a[r];
+        Bs[(e / SW_BK) * SW_BP + (e % SW_BK)] = st.b[r];
+    }
+}
+
+// (a template, instantiated for float only, so that every translation unit
+// that includes this header shares one definition)
+template <typename T>
+__global__ __launch_bounds__(SG_THREADS) void seg_gemm_wide_kernel(const Task *__restrict__ tasks,
+                                                                   const Seg *__restrict__ segs,
+                                                                   const int2 *__restrict__ tiles, int ntiles,
+                                                                   int nrhs, int n_ntiles, Bufs<T> bufs) {
+    static_assert(sizeof(T) == sizeof(float), "FP32 only");
+    constexpr int A_ELEMS = SW_BK * SW_AP;
+    constexpr int STAGE = A_ELEMS + SW_BN * SW_BP;
+    constexpr int SMEM = (2 * STAGE > SW_BN * SW_CP) ? 2 * STAGE : SW_BN * SW_CP;
+    __shared__ float smem[SMEM];
+
+    const int tid = threadIdx.x;
+    const int total = ntiles * n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / n_ntiles, nt = w % n_ntiles;
+    const int2 tl = tiles[tile_id];
+    const Task task = tasks[tl.x];
+    const int M = task.m;
+    const int m0 = tl.y * SW_BM, n0 = nt * SW_BN;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f4_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4_t{0.f, 0.f, 0.f, 0.f};
+
+    int seg, k0;
+    bool have = first_chunk(segs, task.seg_begin, task.seg_end, seg, k0);
+    StageW st;
+    int cur = 0;
+    if (have) {
+        Seg sg = segs[seg];
+        load_chunk_w(st, sg, k0, m0, M, n0, nrhs, bufs, tid);
+        store_chunk_w(st, sg.a_trans, smem, smem + A_ELEMS, tid);
+    }
+    __syncthreads();
+    while (have) {
+        int nseg = seg, nk0 = k0;
+        bool nhave = next_chunk(segs, task.seg_end, nseg, nk0);
+        Seg nsg;
+        if (nhave) {
+            nsg = segs[nseg];
+            load_chunk_w(st, nsg, nk0, m0, M, n0, nrhs, bufs, tid);
+        }
+        const float *As = smem + cur * STAGE;
+        const float *Bs = As + A_ELEMS;
+#pragma unroll
+        for (int kk = 0; kk < SW_BK; kk += 4) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[(kk + lk) * SW_AP + wr * 64 + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[(wc * 64 + j * 16 + lr) * SW_BP + kk + lk];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = mfma16x16x4(a[i], b[j], acc[i][j]);
+        }
+        if (nhave) {
+            float *Asn = smem + (cur ^ 1) * STAGE;
+            store_chunk_w(st, nsg.a_trans, Asn, Asn + A_ELEMS, tid);
+        }
+        __syncthreads();
+        cur ^= 1;
+        seg = nseg;
+        k0 = nk0;
+        have = nhave;
+    }
+
+    float *Cs = smem;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                int row = wr * 64 + i * 16 + MfmaAcc<float>::row(lk, r);
+                int col = wc * 64 + j * 16 + lr;
+                Cs[col * SW_CP + row] = acc[i][j][r];
+            }
+    __syncthreads();
+    float *C = bufs.vec[task.c_buf] + task.c_row * (int64_t)nrhs;
+#pragma unroll 4
+    for (int e = tid; e < SW_BM * SW_BN; e += SG_THREADS) {
+        int col = e / SW_BM, row = e % SW_BM;
+        int gm = m0 + row, gn = n0 + col;
+        if (gm < M && gn < nrhs) {
+            float v = Cs[col * SW_CP + row];
+            float *p = C + (int64_t)gn * M + gm;
             *p = task.beta ? *p + v : v;
         }
     }

@@ -87,6 +87,10 @@ def lib():
         L.mx_plan_launch_variants.restype = ctypes.c_int
         L.mx_plan_set_launch_variants.argtypes = [P, I32, P, I32]
         L.mx_plan_set_launch_variants.restype = ctypes.c_int
+        L.mx_plan_seg_variants.argtypes = [P, I32, ctypes.c_int, P, I32, ctypes.POINTER(I32)]
+        L.mx_plan_seg_variants.restype = ctypes.c_int
+        L.mx_plan_set_seg_variants.argtypes = [P, I32, ctypes.c_int, P, I32]
+        L.mx_plan_set_seg_variants.restype = ctypes.c_int
         L.mx_batch_create.argtypes = [ctypes.POINTER(P), P, P, I32, ctypes.c_uint32]
         L.mx_batch_create.restype = ctypes.c_int
         L.mx_batch_destroy.argtypes = [P]
@@ -315,6 +319,22 @@ class Plan:
         an empty list returns to the plan-wide variant."""
         v = (I32 * max(len(variants), 1))(*variants)
         _check(lib().mx_plan_set_launch_variants(self._h, DIRECTIONS[direction], v, len(variants)))
+
+    def seg_variants(self, precision, direction="forward"):
+        """FP64/FP32 kernel variant of each launch (mx_plan_seg_variants):
+        1 = 64x64 tiles, one chunk in flight; 2 = two; 3 = 128x128 (FP32)."""
+        cap = 4096
+        v = (I32 * cap)()
+        cnt = I32()
+        _check(lib().mx_plan_seg_variants(self._h, DIRECTIONS[direction], PRECISIONS[precision], v, cap,
+                                          ctypes.byref(cnt)))
+        return [v[i] for i in range(min(cnt.value, cap))]
+
+    def set_seg_variants(self, variants, precision, direction="forward"):
+        """Fix the FP64/FP32 kernel variant of each launch (mx_plan_set_seg_variants)."""
+        v = (I32 * max(len(variants), 1))(*variants)
+        _check(lib().mx_plan_set_seg_variants(self._h, DIRECTIONS[direction], PRECISIONS[precision], v,
+                                              len(variants)))
 
     def grad_flops(self, nrhs):
         f = ctypes.c_double()

@@ -115,6 +115,41 @@ def test_pinned_launch_variants(gpu):
     assert rel(Yf.cpu().numpy(), Yo) <= 1e-5
 
 
+@pytest.mark.parametrize("case", ["h2b_ragged", "h2b_tall"])
+def test_fp64_fp32_kernel_variants(gpu, case):
+    """Every FP64/FP32 kernel variant (64x64 tiles with one or two chunks of
+    loads in flight, 128x128 tiles for FP32) on every launch, both
+    directions, against the oracle."""
+    import torch
+    _say(f"[seg variants] {case}")
+    X, hm = make_problem(**CASES[case])
+    nrhs = 150
+    TW = host.normal((hm.N * nrhs,), seed=29)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("fp64", "fp32"), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    for prec, variants in (("fp64", (1, 2)), ("fp32", (1, 2, 3))):
+        for v in variants:
+            for d in ("forward", "adjoint"):
+                n = len(plan.seg_variants(prec, d))
+                plan.set_seg_variants([v] * n, prec, d)
+                assert plan.seg_variants(prec, d) == [v] * n
+            Y = torch.full_like(W, float("nan"))
+            G = torch.full_like(W, float("nan"))
+            plan.eval(W, Y, nrhs, prec)
+            plan.eval_adjoint(W, G, nrhs, prec)
+            torch.cuda.synchronize()
+            tol = 1e-12 if prec == "fp64" else 1e-5
+            ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
+            _say(f"  {prec} variant {v}: forward {ey:.2e} adjoint {eg:.2e}")
+            assert ey <= tol and eg <= tol, (prec, v)
+    with pytest.raises(hgemm.MxError):
+        plan.set_seg_variants([3] * len(plan.seg_variants("fp64")), "fp64")
+    with pytest.raises(hgemm.MxError):
+        plan.set_seg_variants([1], "fp32")
+
+
 def test_balanced_schedule_moves_near_work(gpu):
     """The balanced schedules move the leading near blocks of leaves with long
     near lists into the UP launches: on the transposed structure (whose row
