@@ -84,6 +84,36 @@ def test_dropin_hgemm_hss_overwrites_like_reference(gpu, nrhs):
     assert rel(Yg, Yo) <= 1e-12 and rel(wg, wo) <= 1e-12 and rel(ug, uo) <= 1e-12
 
 
+@pytest.mark.parametrize("n", [128, 200, 700])
+def test_smallest_trees(gpu, n):
+    """Two, four and sixteen leaves of 64 points (the smallest trees the
+    builder makes; with so few leaves the budget near lists degenerate to
+    HSS): every precision and schedule, forward and adjoint, against the
+    oracle."""
+    import torch
+    _say(f"[small tree] n={n}")
+    X, hm = make_problem(n=n, d=3, leaf=64, rank=20, tol=1e-6, budget=0.5, h=1.0)
+    nrhs = 7
+    TW = host.normal((hm.N * nrhs,), seed=31)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
+    plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
+    W = torch.from_numpy(TW).cuda()
+    for prec in ("fp64", "fp32", "bf16x3"):
+        for sc in sorted(hgemm.SCHEDULES):
+            for d in ("forward", "adjoint"):
+                plan.set_schedule(d, prec, sc)
+            Y = torch.full_like(W, float("nan"))
+            G = torch.full_like(W, float("nan"))
+            plan.eval(W, Y, nrhs, prec)
+            plan.eval_adjoint(W, G, nrhs, prec)
+            torch.cuda.synchronize()
+            tol = 1e-12 if prec == "fp64" else 1e-5
+            ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
+            assert ey <= tol and eg <= tol, (prec, sc, ey, eg)
+        _say(f"  {prec}: nleaf={hm.nleaf} forward {ey:.2e} adjoint {eg:.2e}")
+
+
 def test_fp64_closed_form_equals_dense(gpu):
     _say("[fp64 closed form]")
     X, hm = make_problem(**CASES["h2b_fullrank"])
