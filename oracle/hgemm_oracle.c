@@ -21,15 +21,46 @@
  * to MKL cblas_dgemm.  Parallel loops follow the reference's OpenMP loops,
  * which are race-free by construction of the row-blocking.
  */
+#include <dlfcn.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
+/* Optional BLAS for the CPU baseline.  The reference hands every block
+ * product to MKL cblas_dgemm (HGEMM_gen.h:134-232) with sequential MKL
+ * inside its OpenMP loops; orc_use_blas loads a CBLAS library at run time
+ * (e.g. the MKL runtime, libmkl_rt), selects MKL's sequential threading
+ * layer when it has one, and routes gemm() through cblas_dgemm.  The parity
+ * tests never call it: they check the GPU against the GEMM written below. */
+typedef void (*cblas_dgemm_fn)(int layout, int transa, int transb, int m, int n, int k, double alpha,
+                               const double *a, int lda, const double *b, int ldb, double beta, double *c,
+                               int ldc);
+static cblas_dgemm_fn blas_dgemm = NULL;
+
+int orc_use_blas(const char *path) {
+    if (!path) {
+        blas_dgemm = NULL;
+        return 0;
+    }
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) return -1;
+    /* MKL_Set_Threading_Layer(MKL_THREADING_SEQUENTIAL = 1) must come before
+     * any other MKL call; other CBLAS libraries lack the symbol */
+    int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+    if (set_layer) set_layer(1);
+    blas_dgemm = (cblas_dgemm_fn)dlsym(h, "cblas_dgemm");
+    return blas_dgemm ? 1 : -2;
+}
+
 /* C (m x n, ldc) = beta*C + op(A) * B ; op(A) = A (m x k, lda) or A^T (A is k x m, lda). */
 static void gemm(int transA, int m, int n, int k, const double *A, int lda, const double *Bm, int ldb,
                  double beta, double *C, int ldc) {
     if (m <= 0 || n <= 0) return;
+    if (blas_dgemm && k > 0) {   /* CblasColMajor = 102, CblasNoTrans = 111, CblasTrans = 112 */
+        blas_dgemm(102, transA ? 112 : 111, 111, m, n, k, 1.0, A, lda, Bm, ldb, beta, C, ldc);
+        return;
+    }
     if (beta == 0.0) {
         for (int j = 0; j < n; ++j) memset(C + (size_t)j * ldc, 0, sizeof(double) * (size_t)m);
     }

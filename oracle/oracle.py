@@ -53,8 +53,31 @@ def lib():
     L.orc_assemble.argtypes = [I32, I32, P, P, P, P, P, P, P, P, I64, P, P, P, P, I64, P, P, P, P, P]
     L.orc_gemm.restype = None
     L.orc_gemm.argtypes = [I32, I32, I32, I32, P, I32, P, I32, F64, P, I32]
+    L.orc_use_blas.restype = ctypes.c_int
+    L.orc_use_blas.argtypes = [ctypes.c_char_p]
     _lib = L
     return L
+
+
+MKL_CANDIDATES = ("libmkl_rt.so", "libmkl_rt.so.1", "/opt/conda/lib/libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1")
+
+
+def use_blas(path):
+    """Route the oracle's block products through cblas_dgemm of the CBLAS
+    library at `path` (MKL: sequential threading layer, inside the oracle's
+    OpenMP loops, as the reference runs), or back to the built-in GEMM with
+    None.  Returns True when the library was loaded.  For the CPU baseline
+    only; the parity checks use the built-in GEMM."""
+    rc = lib().orc_use_blas(path.encode() if path else None)
+    return rc == 1
+
+
+def use_mkl():
+    """use_blas on the first MKL runtime found; the path, or None."""
+    for c in MKL_CANDIDATES:
+        if use_blas(c):
+            return c
+    return None
 
 
 def _p(a):

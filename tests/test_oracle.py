@@ -131,6 +131,27 @@ def test_gemm_restatement_matches_numpy():
     assert rel(oracle.gemm(At, Bm, transA=True, C=C, beta=1.0), C + At.T @ Bm) < 1e-15
 
 
+def test_mkl_route_of_the_cpu_baseline_matches():
+    """The CPU baseline's MKL route (cblas_dgemm inside the same sweeps) gives
+    the built-in GEMM's results up to rounding, and switching back restores
+    the built-in GEMM bit for bit."""
+    X, hm = make_problem(n=2048, d=6, leaf=64, rank=40, tol=1e-6, budget=0.2, h=1.0)
+    TW = host.normal((hm.N * 9,), seed=77)
+    Y0, w0, u0 = oracle.hgemm(hm, TW, 9)
+    path = oracle.use_mkl()
+    if path is None:
+        pytest.skip("no MKL runtime in this image")
+    try:
+        Y1, w1, u1 = oracle.hgemm(hm, TW, 9)
+        G1 = oracle.gemm(np.eye(5)[:, :3], np.arange(6.0).reshape(3, 2))
+    finally:
+        oracle.use_blas(None)
+    assert rel(Y1, Y0) < 1e-13 and rel(w1, w0) < 1e-13 and rel(u1, u0) < 1e-13
+    assert np.array_equal(G1, np.eye(5)[:, :3] @ np.arange(6.0).reshape(3, 2))
+    Y2, _, _ = oracle.hgemm(hm, TW, 9)
+    assert np.array_equal(Y2, Y0)
+
+
 def test_acc_and_flops_definitions():
     X, hm = make_problem(n=1024, d=5, leaf=64, rank=100000, tol=0.0, budget=0.25, h=1.3)
     W = host.normal((1024, 4), seed=14)
