@@ -246,13 +246,17 @@ __device__ __forceinline__ void x3_issue(X3Loader<S> &L, const X3Args &a, const 
 constexpr int vmcnt_enc(int v) { return 0xF70 | (v & 15) | ((v >> 4) << 14); }
 
 // wait until at most j * PER of this wave's LDS-DMA loads are outstanding
-template <int PER>
+// (j <= J, the chunks that may stay in flight; vmcnt takes an immediate, so
+// each j has its own instruction)
+template <int PER, int J>
 __device__ __forceinline__ void wait_dma(int j) {
-    static_assert(3 * PER < 64, "vmcnt is 6 bits");
-    if (j >= 3) __builtin_amdgcn_s_waitcnt(vmcnt_enc(3 * PER));
-    else if (j == 2) __builtin_amdgcn_s_waitcnt(vmcnt_enc(2 * PER));
-    else if (j == 1) __builtin_amdgcn_s_waitcnt(vmcnt_enc(PER));
-    else __builtin_amdgcn_s_waitcnt(vmcnt_enc(0));
+    static_assert(J * PER < 64, "vmcnt is 6 bits");
+    if constexpr (J == 0) {
+        __builtin_amdgcn_s_waitcnt(vmcnt_enc(0));
+    } else {
+        if (j >= J) __builtin_amdgcn_s_waitcnt(vmcnt_enc(J * PER));
+        else wait_dma<PER, J - 1>(j);
+    }
 }
 
 // workgroup barrier without the memory fences of __syncthreads (which would
@@ -584,7 +588,7 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     for (int c = 0; c < nch; ++c) {
         // chunk c has landed (this wave's part, then everyone's), and every
         // wave is done with the stage chunk c-1 used, which chunk c+NST-1 refills
-        wait_dma<S::PER>(issued - c - 1);
+        wait_dma<S::PER, NST - 1>(issued - c - 1);
         wg_barrier();
         issue();
         x3_compute<NJ>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
@@ -600,6 +604,8 @@ int x3_variant_bn(int variant) {
     case X3_V128x256_S3: return 256;
     case X3_V128x64_S2: return 64;
     case X3_V128x32_S2: return 32;
+    case X3_V128x32_S6: return 32;
+    case X3_V128x64_S5: return 64;
     default: return 128;
     }
 }
@@ -632,6 +638,12 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V256x128_W8S3:
         hipLaunchKernelGGL((x3_kernel<4, 2, 3, 4>), grid, dim3(512), 0, st, a);
+        break;
+    case X3_V128x32_S6:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 6, 1>), grid, dim3(256), 0, st, a);
+        break;
+    case X3_V128x64_S5:
+        hipLaunchKernelGGL((x3_kernel<2, 2, 5, 2>), grid, dim3(256), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);

@@ -21,7 +21,7 @@ PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
-               7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3"}
+               7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5"}
 IO_F64, IO_F32 = 0, 1
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
