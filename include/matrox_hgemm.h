@@ -324,6 +324,22 @@ mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W
  * in the reference. */
 mx_status mx_batch_eval_adjoint(mx_batch *batch, mx_precision prec, const void *const *dY, void *const *dW,
                                 mx_io_type io, void *stream);
+/* Backward pass of every group (mx_backward on each group's plan): dW[g] =
+ * K~_g^T dY[g] and, where dD / dB / dVT (arrays of ngroups device pointers,
+ * or NULL) give a buffer, the gradients of the group's stored blocks, from
+ * the skeletons of the batch's last forward evaluation in this precision
+ * (of the same W).  A group's block buffers hold its instances' blocks one
+ * after another (mx_batch_group_blocks gives their lengths,
+ * mx_batch_block_layout an instance's first elements), in each instance's
+ * own CDS layout.  Needs MX_PLAN_ADJOINT.  No counterpart in the
+ * reference. */
+mx_status mx_batch_backward(mx_batch *batch, mx_precision prec, const void *const *W, const void *const *dY,
+                            void *const *dW, mx_io_type io, void *const *dD, void *const *dB, void *const *dVT,
+                            void *stream);
+mx_status mx_batch_group_blocks(const mx_batch *batch, int32_t group, int64_t *D_len, int64_t *B_len,
+                                int64_t *VT_len);
+mx_status mx_batch_block_layout(const mx_batch *batch, int32_t instance, int64_t *d_off, int64_t *b_off,
+                                int64_t *vt_off);
 /* Capture the whole batch (all groups and their fork/join) as one hipGraph
  * per (direction, precision, io, W, Y) and replay it. */
 mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable);

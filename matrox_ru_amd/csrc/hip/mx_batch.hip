@@ -18,6 +18,7 @@ namespace mx {
 struct BatchInst {
     int group;
     int64_t row_off;                 // first row in the group's W / Y
+    int64_t d_off = 0, b_off = 0, vt_off = 0;   // first element of its D, B, VT in the group's blocks
     int64_t N;
     int nleaf;
     std::vector<int32_t> wrow, Ddim; // instance-local leaf layout
@@ -31,6 +32,7 @@ struct GroupStructure {
     std::vector<int32_t> lc, rc, lm, Ddim, slen, npx, npy, fpx, fpy, VTptr, wrow, skelrow;
     std::vector<uint64_t> Dptr, Bptr;
     std::vector<double> D, B, VT;
+    std::vector<int64_t> dbase, bbase, vbase;   // per member: first element of its D, B, VT
     int64_t N = 0;
     mx_structure view() {
         mx_structure s{};
@@ -102,6 +104,9 @@ static void combine(const std::vector<const mx_structure *> &in, GroupStructure 
     }
     arg_check(rb < INT32_MAX && sb < INT32_MAX && vb < INT32_MAX, "batch group too large for 32-bit offsets");
     G.N = rb;
+    G.dbase = dbase;
+    G.bbase = bbase;
+    G.vbase = vbase;
     // dummy nodes 0..k-2 in heap order; child ids >= k-1 are instance roots
     for (int j = 0; j < k - 1; ++j) {
         int c[2] = {2 * j + 1, 2 * j + 2};
@@ -222,6 +227,12 @@ static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs
         }
         GroupStructure G;
         combine(members, G);
+        for (size_t m = 0; m < kv.second.size(); ++m) {
+            BatchInst &bi = B->inst[kv.second[m]];
+            bi.d_off = G.dbase[m];
+            bi.b_off = G.bbase[m];
+            bi.vt_off = G.vbase[m];
+        }
         mx_structure v = G.view();
         B->plans.push_back(create_plan(v, precisions, 0, 0));
         B->group_nrhs.push_back(kv.first);
@@ -418,6 +429,46 @@ mx_status mx_batch_eval(mx_batch *batch, mx_precision prec, const void *const *W
 mx_status mx_batch_eval_adjoint(mx_batch *batch, mx_precision prec, const void *const *dY, void *const *dW,
                                 mx_io_type io, void *stream) {
     return (mx_status)batch_run(batch, DIR_ADJ, prec, dY, dW, io, stream);
+}
+
+mx_status mx_batch_backward(mx_batch *batch, mx_precision prec, const void *const *W, const void *const *dY,
+                            void *const *dW, mx_io_type io, void *const *dD, void *const *dB, void *const *dVT,
+                            void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && W && dY && dW, "null argument");
+        if (!batch->plans[0]->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
+        const hipStream_t st = (hipStream_t)stream;
+        for (size_t g = 0; g < batch->plans.size(); ++g)
+            backward(batch->plans[g], prec, W[g], dY[g], dW[g], batch->group_nrhs[g], io, nullptr, nullptr,
+                     dD ? dD[g] : nullptr, dB ? dB[g] : nullptr, dVT ? dVT[g] : nullptr, st);
+        return 0;
+    });
+}
+
+mx_status mx_batch_group_blocks(const mx_batch *batch, int32_t group, int64_t *D_len, int64_t *B_len,
+                                int64_t *VT_len) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && D_len && B_len && VT_len && group >= 0 && group < (int32_t)batch->plans.size(),
+                  "bad argument");
+        const mx_plan *P = batch->plans[group];
+        *D_len = P->D_len;
+        *B_len = P->B_len;
+        *VT_len = P->VT_len;
+        return 0;
+    });
+}
+
+mx_status mx_batch_block_layout(const mx_batch *batch, int32_t instance, int64_t *d_off, int64_t *b_off,
+                                int64_t *vt_off) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && d_off && b_off && vt_off && instance >= 0 && instance < (int32_t)batch->inst.size(),
+                  "bad argument");
+        const BatchInst &bi = batch->inst[instance];
+        *d_off = bi.d_off;
+        *b_off = bi.b_off;
+        *vt_off = bi.vt_off;
+        return 0;
+    });
 }
 
 mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable) {

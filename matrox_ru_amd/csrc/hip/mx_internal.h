@@ -286,6 +286,11 @@ struct GenState {
 mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y,
                      const GenInfo *gen = nullptr);
 void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io, hipStream_t st);
+// dW = K~^T dY and (for non-null outputs) the block gradients, from the
+// forward skeletons wskel/uskel (null: the plan's own, of its last forward
+// evaluation in this precision)
+void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
+              const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st);
 void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);

@@ -485,8 +485,8 @@ static void export_panels(mx_plan *P, int dir, int nrhs, float *wdst, float *uds
 // forward skeletons (w^, u^: caller copies, or the plan's own buffers) and the
 // adjoint ones (y^, g^).  FP64, FP32 or split-BF16x3 compute (the latter on
 // FP32 copies of the operands); outputs in the I/O type.
-static void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
-                     const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
+void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
+              const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
     arg_check(W && dY && dW, "null W, dY or dW");
     arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
     arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");

@@ -107,6 +107,12 @@ def lib():
         L.mx_batch_eval.restype = ctypes.c_int
         L.mx_batch_eval_adjoint.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, P]
         L.mx_batch_eval_adjoint.restype = ctypes.c_int
+        L.mx_batch_backward.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_int, P, P, P, P]
+        L.mx_batch_backward.restype = ctypes.c_int
+        L.mx_batch_group_blocks.argtypes = [P, I32, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
+        L.mx_batch_group_blocks.restype = ctypes.c_int
+        L.mx_batch_block_layout.argtypes = [P, I32, ctypes.POINTER(I64), ctypes.POINTER(I64), ctypes.POINTER(I64)]
+        L.mx_batch_block_layout.restype = ctypes.c_int
         L.mx_batch_set_graph.argtypes = [P, I32]
         L.mx_batch_set_graph.restype = ctypes.c_int
         L.mx_batch_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, I32, P]
@@ -518,6 +524,44 @@ class Batch:
         """dW[g] = K~^T dY[g], instance by instance (mx_batch_eval_adjoint;
         the batch must be built with adjoint=True)."""
         return self._run(lib().mx_batch_eval_adjoint, dY, dW, precision, stream)
+
+    def group_blocks(self, g):
+        """(D_len, B_len, VT_len) of group g's stored blocks."""
+        d, b, v = I64(), I64(), I64()
+        _check(lib().mx_batch_group_blocks(self._h, g, ctypes.byref(d), ctypes.byref(b), ctypes.byref(v)))
+        return d.value, b.value, v.value
+
+    def block_offsets(self, i):
+        """First elements of instance i's D, B and VT in its group's blocks."""
+        d, b, v = I64(), I64(), I64()
+        _check(lib().mx_batch_block_layout(self._h, i, ctypes.byref(d), ctypes.byref(b), ctypes.byref(v)))
+        return d.value, b.value, v.value
+
+    def backward(self, W, dY, precision="fp32", blocks=True, stream=None):
+        """mx_batch_backward after a forward evaluation of W: (dW, dD, dB,
+        dVT) as per-group lists of tensors in W's dtype (the block lists are
+        None when blocks is False); block_view slices an instance out."""
+        import torch
+        assert self.adjoint, "batch was created without adjoint=True"
+        dt = W[0].dtype
+        dW = [torch.empty_like(w) for w in W]
+        io = IO_F64 if dt == torch.float64 else IO_F32
+        arr = lambda ts: ctypes.cast((P * len(ts))(*[t.data_ptr() for t in ts]), P)
+        grads = [None, None, None]
+        if blocks:
+            lens = [self.group_blocks(g) for g in range(len(self.groups))]
+            grads = [[torch.empty(max(l[w], 1), dtype=dt, device="cuda") for l in lens] for w in range(3)]
+        _check(lib().mx_batch_backward(self._h, PRECISIONS[precision], arr(W), arr(dY), arr(dW), io,
+                                       arr(grads[0]) if blocks else None, arr(grads[1]) if blocks else None,
+                                       arr(grads[2]) if blocks else None, _stream_handle(stream)))
+        return dW, grads[0], grads[1], grads[2]
+
+    def block_view(self, bufs, i, which, length):
+        """Instance i's `length` elements of its D (which=0), B (1) or VT (2)
+        inside the group buffers bufs."""
+        g = self.layout[i][0]
+        off = self.block_offsets(i)[which]
+        return bufs[g][off:off + length]
 
     def set_graph(self, enable=True):
         _check(lib().mx_batch_set_graph(self._h, int(enable)))

@@ -136,6 +136,44 @@ def test_batch_adjoint_matches_oracle_on_transposed(gpu, batch_case, prec):
     assert e.value.code == -4
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_batch_backward_matches_single_plans(gpu, batch_case, prec):
+    """mx_batch_backward: each instance's dW and block gradients equal those
+    of a plan of that instance alone (FP64 1e-12; BF16x3 within 5e-5, the
+    products being split to 16 bits in both), and dW matches the oracle on
+    the transposed structure."""
+    import torch
+    _say(f"[batch backward] prec={prec}")
+    hms, nrhs, TW, ref = batch_case
+    b = hgemm.Batch(hms, nrhs, precisions=("fp64", "bf16x3"), adjoint=True)
+    dt = torch.float64
+    W = _fill(b, TW, dt)
+    dY = _fill(b, [host.normal((hm.N * n,), seed=950 + i) for i, (hm, n) in enumerate(zip(hms, nrhs))], dt)
+    Y = b.buffers(dt)
+    b.eval(W, Y, prec)
+    dW, dD, dB, dVT = b.backward(W, dY, prec)
+    torch.cuda.synchronize()
+    tol = 1e-12 if prec == "fp64" else 5e-5
+    for i, (hm, n) in enumerate(zip(hms, nrhs)):
+        plan = hgemm.Plan(hm, precisions=(prec,), adjoint=True)
+        Wi = b.instance_view(W, i).contiguous()
+        dYi = b.instance_view(dY, i).contiguous()
+        Yi = torch.empty_like(Wi)
+        plan.eval(Wi, Yi, n, prec)
+        ref_g = plan.backward(Wi, dYi, n, prec)
+        got = [b.instance_view(dW, i)] + [b.block_view(buf, i, w, ref_g[w + 1].numel())
+                                          for w, buf in enumerate((dD, dB, dVT))]
+        for nm, a_, r_ in zip(("dW", "dD", "dB", "dVT"), got, ref_g):
+            if r_.numel() == 0:
+                continue
+            e = rel(a_.double().cpu().numpy(), r_.double().cpu().numpy())
+            assert e <= tol, (i, nm, e)
+        go = oracle.hgemm(hm.transposed(), dYi.cpu().numpy(), n)[0]
+        assert rel(b.instance_view(dW, i).double().cpu().numpy(), go) <= (1e-12 if prec == "fp64" else 2e-5)
+        plan.close()
+    _say(f"  {len(hms)} instances checked")
+
+
 def test_batch_dense_rows(gpu, batch_case):
     import torch
     _say("[batch dense rows]")
