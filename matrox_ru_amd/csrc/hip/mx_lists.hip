@@ -112,13 +112,70 @@ static int add_tiles(TaskLists &L, int task, int m) {
 
 // Order of the tiles of every launch.  xcd_remap hands XCD x the x-th
 // contiguous eighth of a launch's work items (the column tiles of a tile are
-// adjacent).  The tiles, in structural order (tree order: neighbouring
-// leaves and nodes share W and skeleton blocks, which then stay in one XCD's
-// L2), are cut into eight contiguous groups of about equal work; each group
-// is sorted longest first (so long lists, such as the adjoint's
-// column-grouped couplings at the top of the tree, start early) and padded
-// with empty tiles to the size of the largest, so that group x is exactly
-// the eighth xcd_remap gives XCD x.
+// adjacent), and an XCD runs about 64 work items at a time: at nrhs = 1024
+// the row and column tiles of four 256-row tasks.  Its 4 MB L2 holds little
+// more than one K-chunk of their operands, so a B operand (a W or skeleton
+// panel, read by every task whose list names it) is served from L2 to a
+// second task only when both read it at about the same time.
+//
+// Launches of at least 64 tasks (the big ones) are therefore ordered in
+// groups of four tasks that share B operands: greedily, the task with the
+// most segments, then repeatedly the task sharing the most operands with the
+// group.  Within a group the segments are reordered so that a shared operand
+// sits at the same position in every task of the group that reads it (slot
+// by slot, the operand the most unplaced tasks still need).  Groups go to
+// eight bins of about equal work (most work first, to the least loaded bin),
+// each bin runs its groups longest task first (long lists start early), and
+// the bins are padded with empty tiles to the size of the largest, so that
+// bin x is exactly the eighth xcd_remap gives XCD x.
+//
+// Smaller launches keep the tiles in structural order (tree order:
+// neighbouring nodes share skeleton blocks), cut into eight contiguous
+// groups of about equal work, each sorted longest first and padded the same
+// way; this spreads a few tasks' tiles over all eight XCDs.
+static void reorder_group_segments(TaskLists &L, const std::vector<int> &tasks) {
+    auto key = [](const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; };
+    const size_t G = tasks.size();
+    std::vector<std::vector<Seg>> rem(G), seq(G);
+    for (size_t j = 0; j < G; ++j) {
+        const Task &t = L.tasks[tasks[j]];
+        rem[j].assign(L.segs.begin() + t.seg_begin, L.segs.begin() + t.seg_end);
+    }
+    for (;;) {
+        std::vector<size_t> open;
+        for (size_t j = 0; j < G; ++j)
+            if (!rem[j].empty()) open.push_back(j);
+        if (open.empty()) break;
+        while (!open.empty()) {   // one position of every open task
+            std::map<int64_t, int> cnt;
+            for (size_t j : open) {
+                std::set<int64_t> mine;
+                for (const Seg &g : rem[j]) mine.insert(key(g));
+                for (int64_t k : mine) ++cnt[k];
+            }
+            int64_t best = 0;
+            int bc = 0;
+            for (const auto &kc : cnt)   // ascending keys: ties go to the smallest
+                if (kc.second > bc) {
+                    best = kc.first;
+                    bc = kc.second;
+                }
+            std::vector<size_t> still;
+            for (size_t j : open) {
+                auto it = std::find_if(rem[j].begin(), rem[j].end(), [&](const Seg &g) { return key(g) == best; });
+                if (it == rem[j].end()) {
+                    still.push_back(j);
+                    continue;
+                }
+                seq[j].push_back(*it);
+                rem[j].erase(it);
+            }
+            open.swap(still);
+        }
+    }
+    for (size_t j = 0; j < G; ++j) std::copy(seq[j].begin(), seq[j].end(), L.segs.begin() + L.tasks[tasks[j]].seg_begin);
+}
+
 static void order_tiles(TaskLists &L, int bk) {
     if (L.dummy < 0) {
         Task tk{};
@@ -127,40 +184,138 @@ static void order_tiles(TaskLists &L, int bk) {
         L.tasks.push_back(tk);
         L.task_kind.push_back(L.task_kind.empty() ? 0 : L.task_kind.back());
     }
+    const int ng = 8, G = 4, min_tasks = 64;
+    auto tile_cost = [&](int task) {
+        const Task &tk = L.tasks[task];
+        int64_t cost = 2;   // prologue and epilogue
+        for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
+        return cost;
+    };
+    auto key = [](const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; };
     std::vector<int2> out;
     std::vector<Launch> lns;
+    std::vector<int> local(L.tasks.size(), -1);
     for (const Launch &ln : L.launches) {
         const int T = ln.ntiles;
-        std::vector<std::pair<int64_t, int2>> v;
-        v.reserve(T);
-        int64_t total = 0;
+        // the launch's tasks in structural order, with their row tiles
+        std::vector<int> tk;
+        std::vector<std::vector<int>> rts;
         for (int i = 0; i < T; ++i) {
-            int2 tl = L.tiles[ln.tile_offset + i];
-            const Task &tk = L.tasks[tl.x];
-            int64_t cost = 2;   // prologue and epilogue
-            for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
-            v.push_back({cost, tl});
-            total += cost;
+            const int2 tl = L.tiles[ln.tile_offset + i];
+            if (local[tl.x] < 0) {
+                local[tl.x] = (int)tk.size();
+                tk.push_back(tl.x);
+                rts.emplace_back();
+            }
+            rts[local[tl.x]].push_back(tl.y);
         }
-        const int ng = 8;
-        std::vector<int> cut(ng + 1, T);
-        cut[0] = 0;
-        int64_t acc = 0;
-        int g = 1;
-        for (int i = 0; i < T && g < ng; ++i) {
-            acc += v[i].first;
-            while (g < ng && acc * ng >= total * g) cut[g++] = i + 1;
+        for (int t : tk) local[t] = -1;
+        const int n = (int)tk.size();
+        std::vector<std::vector<int2>> bins(ng);
+        if (n < min_tasks) {
+            std::vector<std::pair<int64_t, int2>> v;
+            v.reserve(T);
+            int64_t total = 0;
+            for (int i = 0; i < T; ++i) {
+                const int2 tl = L.tiles[ln.tile_offset + i];
+                v.push_back({tile_cost(tl.x), tl});
+                total += v.back().first;
+            }
+            std::vector<int> cut(ng + 1, T);
+            cut[0] = 0;
+            int64_t acc = 0;
+            int g = 1;
+            for (int i = 0; i < T && g < ng; ++i) {
+                acc += v[i].first;
+                while (g < ng && acc * ng >= total * g) cut[g++] = i + 1;
+            }
+            for (int q = 0; q < ng; ++q) {
+                std::stable_sort(v.begin() + cut[q], v.begin() + cut[q + 1],
+                                 [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
+                                     return a.first > b.first;
+                                 });
+                for (int i = cut[q]; i < cut[q + 1]; ++i) bins[q].push_back(v[i].second);
+            }
+        } else {
+            std::vector<std::vector<int64_t>> src(n);
+            std::map<int64_t, std::vector<int>> users;
+            std::vector<int64_t> cost(n);
+            for (int i = 0; i < n; ++i) {
+                const Task &t = L.tasks[tk[i]];
+                for (int sg = t.seg_begin; sg < t.seg_end; ++sg) src[i].push_back(key(L.segs[sg]));
+                std::sort(src[i].begin(), src[i].end());
+                src[i].erase(std::unique(src[i].begin(), src[i].end()), src[i].end());
+                for (int64_t k : src[i]) users[k].push_back(i);
+                cost[i] = tile_cost(tk[i]);
+            }
+            std::vector<int> seeds(n);
+            for (int i = 0; i < n; ++i) seeds[i] = i;
+            std::stable_sort(seeds.begin(), seeds.end(), [&](int a, int b) { return src[a].size() > src[b].size(); });
+            std::vector<char> used(n, 0);
+            std::vector<int> score(n, 0);
+            std::vector<std::vector<int>> groups;
+            size_t sp = 0;
+            int lowest = 0;
+            for (;;) {
+                while (sp < seeds.size() && used[seeds[sp]]) ++sp;
+                if (sp == seeds.size()) break;
+                std::vector<int> g{seeds[sp]};
+                used[seeds[sp]] = 1;
+                std::set<int64_t> U(src[g[0]].begin(), src[g[0]].end());
+                while ((int)g.size() < G) {
+                    std::vector<int> touched;
+                    for (int64_t k : U)
+                        for (int c : users[k])
+                            if (!used[c] && score[c]++ == 0) touched.push_back(c);
+                    int best = -1;
+                    for (int c : touched)
+                        if (best < 0 || score[c] > score[best] || (score[c] == score[best] && c < best)) best = c;
+                    for (int c : touched) score[c] = 0;
+                    if (best < 0) {   // nothing shared: the next task in structural order
+                        while (lowest < n && used[lowest]) ++lowest;
+                        if (lowest == n) break;
+                        best = lowest;
+                    }
+                    used[best] = 1;
+                    g.push_back(best);
+                    U.insert(src[best].begin(), src[best].end());
+                }
+                groups.push_back(g);
+            }
+            const int ngr = (int)groups.size();
+            std::vector<int64_t> work(ngr, 0), longest(ngr, 0);
+            for (int q = 0; q < ngr; ++q) {
+                std::vector<int> ids;
+                for (int j : groups[q]) {
+                    work[q] += cost[j] * (int64_t)rts[j].size();
+                    longest[q] = std::max(longest[q], cost[j]);
+                    ids.push_back(tk[j]);
+                }
+                reorder_group_segments(L, ids);
+            }
+            std::vector<int> order(ngr);
+            for (int q = 0; q < ngr; ++q) order[q] = q;
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+            std::vector<int64_t> load(ng, 0);
+            std::vector<std::vector<int>> bg(ng);
+            for (int q : order) {
+                const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                bg[b].push_back(q);
+                load[b] += work[q];
+            }
+            for (int b = 0; b < ng; ++b) {
+                std::stable_sort(bg[b].begin(), bg[b].end(), [&](int a, int c) { return longest[a] > longest[c]; });
+                for (int q : bg[b])
+                    for (int j : groups[q])
+                        for (int r : rts[j]) bins[b].push_back(make_int2(tk[j], r));
+            }
         }
-        int gmax = 0;
-        for (int q = 0; q < ng; ++q) gmax = std::max(gmax, cut[q + 1] - cut[q]);
-        Launch nl{(int)out.size(), gmax * ng};
-        for (int q = 0; q < ng; ++q) {
-            std::stable_sort(v.begin() + cut[q], v.begin() + cut[q + 1],
-                             [](const std::pair<int64_t, int2> &a, const std::pair<int64_t, int2> &b) {
-                                 return a.first > b.first;
-                             });
-            for (int i = cut[q]; i < cut[q + 1]; ++i) out.push_back(v[i].second);
-            for (int i = cut[q + 1] - cut[q]; i < gmax; ++i) out.push_back(make_int2(L.dummy, 0));
+        size_t gmax = 0;
+        for (const auto &b : bins) gmax = std::max(gmax, b.size());
+        Launch nl{(int)out.size(), (int)gmax * ng};
+        for (const auto &b : bins) {
+            out.insert(out.end(), b.begin(), b.end());
+            for (size_t i = b.size(); i < gmax; ++i) out.push_back(make_int2(L.dummy, 0));
         }
         lns.push_back(nl);
     }
