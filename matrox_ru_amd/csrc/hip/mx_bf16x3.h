@@ -70,6 +70,7 @@ struct X3Args {
 
 constexpr int X3_BM = 128, X3_BK = 32;   // tile rows of every variant, chunk depth
 constexpr int X3_MAXSEG = 256;      // segments per task the kernel holds in LDS
+constexpr int X3_FRAG = 512;        // bf16 per 16 x 32 fragment (1 KB)
 
 __host__ __device__ inline int64_t x3_packed_elems(int M, int K) {
     return (int64_t)((M + 15) / 16) * ((K + 31) / 32) * 512;

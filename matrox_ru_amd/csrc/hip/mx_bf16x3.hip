@@ -1,5 +1,6 @@
 // Split-BF16 x3 kernels (layouts and arithmetic described in mx_bf16x3.h).
 #include "mx_bf16x3.h"
+#include "mx_grad.h"
 
 #include <type_traits>
 
@@ -166,7 +167,6 @@ __device__ __forceinline__ x3_f4 mfma_bf16(bf16x8_t a, bf16x8_t b, x3_f4 c) {
 // discarded.  A stage holds A fragments [row group][hi, lo] followed by B
 // fragments [column group][hi, lo], so each wave's 8 A and 8 B fragments are
 // two contiguous 8 KB runs.
-constexpr int X3_FRAG = 512;              // uint16 per fragment (1 KB)
 
 // Tile geometry: BM x BN output tile, NW waves.
 template <int BM_, int BN_, int NW_>
@@ -650,5 +650,246 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     }
 }
+
+// ---- split-BF16x3 gradient products -------------------------------------
+//
+// K-fragment packing (layout in mx_grad.h).  Column blocks: one wave per
+// fragment, consecutive waves on consecutive row groups of one chunk, so the
+// 16-row runs a wave reads per column continue in its neighbours' loads.
+template <typename T>
+__global__ __launch_bounds__(256) void kpack_cols_kernel(const T *__restrict__ src, const KBlock *__restrict__ blocks,
+                                                         int nrhs, uint16_t *__restrict__ hi,
+                                                         uint16_t *__restrict__ lo) {
+    const KBlock b = blocks[blockIdx.y];
+    const int nfr = (b.m + 15) / 16, nfk = (nrhs + 31) / 32;
+    const int64_t nfrag = (int64_t)nfr * nfk;
+    const int lane = threadIdx.x & 63;
+    const int64_t step = (int64_t)gridDim.x * 4;
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < nfrag; w += step) {
+        const int g = (int)(w % nfr), c = (int)(w / nfr);
+        const int i = g * 16 + (lane & 15);
+        const int k0 = c * 32 + 8 * (lane >> 4);
+        const T *p = src + b.row * (int64_t)nrhs + i;
+        T v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = (i < b.m && k0 + e < nrhs) ? p[(int64_t)(k0 + e) * b.m] : T(0);
+        uint32_t h[4], l[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) split2(v[2 * e], v[2 * e + 1], h[e], l[e]);
+        const int64_t o = ((b.fr + g) * nfk + c) * X3_FRAG + lane * 8;
+        *reinterpret_cast<u32x4_t *>(hi + o) = u32x4_t{h[0], h[1], h[2], h[3]};
+        *reinterpret_cast<u32x4_t *>(lo + o) = u32x4_t{l[0], l[1], l[2], l[3]};
+    }
+}
+
+template <typename T>
+void launch_kpack_cols(const T *src, const KBlock *blocks, int nblocks, int nrhs, uint16_t *hi, uint16_t *lo,
+                       hipStream_t st) {
+    const int gx = std::min(1024, ((nrhs + 31) / 32) * 64);   // 64 fragments per column chunk cover 1024 rows
+    for (int b0 = 0; b0 < nblocks; b0 += 65535) {
+        const dim3 grid((unsigned)gx, (unsigned)std::min(65535, nblocks - b0));
+        hipLaunchKernelGGL(kpack_cols_kernel<T>, grid, dim3(256), 0, st, src, blocks + b0, nrhs, hi, lo);
+    }
+}
+template void launch_kpack_cols<float>(const float *, const KBlock *, int, int, uint16_t *, uint16_t *, hipStream_t);
+template void launch_kpack_cols<double>(const double *, const KBlock *, int, int, uint16_t *, uint16_t *,
+                                        hipStream_t);
+
+// Split panels: a 32-row panel block holds each column's 32 rows as 64
+// contiguous bytes, so a 128-column tile is 8 KB per half.  A workgroup
+// reads the tile (16-byte loads, coalesced), transposes it through LDS and
+// writes its 2 row groups x 4 chunks of fragments per half (one chunk per
+// wave).  Columns past nrhs and rows past the block are zero.
+__global__ __launch_bounds__(256) void kpack_panels_kernel(const uint16_t *__restrict__ phi,
+                                                           const uint16_t *__restrict__ plo,
+                                                           const KBlock *__restrict__ blocks, int nrhs,
+                                                           uint16_t *__restrict__ hi, uint16_t *__restrict__ lo) {
+    __shared__ __attribute__((aligned(16))) uint16_t t[2][128 * 32];   // [half][column][row]
+    const KBlock b = blocks[blockIdx.y];
+    const int nfr = (b.m + 15) / 16, nfk = (nrhs + 31) / 32;
+    const int npb = (b.m + 31) / 32, nct = (nrhs + 127) / 128;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int u = blockIdx.x; u < npb * nct; u += gridDim.x) {
+        const int pb = u % npb, ct = u / npb;
+        const int j0 = ct * 128;
+        const int64_t base = (b.row + (int64_t)pb * 32) * nrhs + (int64_t)j0 * 32;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int piece = r * 256 + tid;   // 16 bytes: rows rr..rr+7 of column q
+            const int q = piece >> 2, rr = (piece & 3) * 8;
+            u32x4_t vh = u32x4_t{0, 0, 0, 0}, vl = u32x4_t{0, 0, 0, 0};
+            if (j0 + q < nrhs) {
+                vh = *reinterpret_cast<const u32x4_t *>(phi + base + q * 32 + rr);
+                vl = *reinterpret_cast<const u32x4_t *>(plo + base + q * 32 + rr);
+            }
+            *reinterpret_cast<u32x4_t *>(&t[0][q * 32 + rr]) = vh;
+            *reinterpret_cast<u32x4_t *>(&t[1][q * 32 + rr]) = vl;
+        }
+        __syncthreads();
+        const int c = ct * 4 + wv;
+        if (c < nfk) {
+#pragma unroll
+            for (int gg = 0; gg < 2; ++gg) {
+                const int g = 2 * pb + gg;
+                if (g >= nfr) break;
+                const int r = gg * 16 + (lane & 15);
+                const bool rok = pb * 32 + r < b.m;
+                const int q0 = wv * 32 + 8 * (lane >> 4);
+                const int64_t o = ((b.fr + g) * nfk + c) * X3_FRAG + lane * 8;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    uint32_t w[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const uint32_t a0 = rok ? t[h][(q0 + 2 * e) * 32 + r] : 0u;
+                        const uint32_t a1 = rok ? t[h][(q0 + 2 * e + 1) * 32 + r] : 0u;
+                        w[e] = a0 | (a1 << 16);
+                    }
+                    *reinterpret_cast<u32x4_t *>((h ? lo : hi) + o) = u32x4_t{w[0], w[1], w[2], w[3]};
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+void launch_kpack_panels(const uint16_t *phi, const uint16_t *plo, const KBlock *blocks, int nblocks, int nrhs,
+                         uint16_t *hi, uint16_t *lo, hipStream_t st) {
+    const int gx = std::min(256, 8 * ((nrhs + 127) / 128));   // 8 panel blocks (256 rows) per column tile
+    for (int b0 = 0; b0 < nblocks; b0 += 65535) {
+        const dim3 grid((unsigned)gx, (unsigned)std::min(65535, nblocks - b0));
+        hipLaunchKernelGGL(kpack_panels_kernel, grid, dim3(256), 0, st, phi, plo, blocks + b0, nrhs, hi, lo);
+    }
+}
+
+// Gradient products dC = sum_s X_s Z_s^T on the evaluation kernel's pipeline
+// (128 x 128 tiles, 4 waves of 64 x 64, two LDS stages, LDS-DMA loads): A
+// fragments are X's K-fragments (row groups of the tile), B fragments Z's
+// (row groups of the tile's columns), each advancing one fragment per
+// 32-deep chunk.  Row groups past X are zeroed at the MFMA (as in the
+// evaluation); column groups past Z load a valid neighbour and their results
+// are not stored.  The tile leaves through LDS as 64 consecutive rows per
+// column.
+template <typename S>
+struct KLoader {
+    const uint16_t *p[S::PER];
+    int seg, left;
+};
+
+template <typename S, typename O>
+__device__ __forceinline__ void kg_seg_begin(KLoader<S> &L, const GradX3Args<O> &a, const KSeg &sg, int wv, int lane,
+                                             int mtile16, int nfm, int ntile16, int nfn, int nfk) {
+#pragma unroll
+    for (int t = 0; t < S::PER; ++t) {
+        const int f = wv * S::PER + t;   // wave-uniform
+        if (f < S::FA) {
+            const int64_t g = sg.x_fr + min(mtile16 + (f >> 1), nfm - 1);
+            L.p[t] = ((f & 1) ? a.klo[sg.x_buf] : a.khi[sg.x_buf]) + g * nfk * X3_FRAG + lane * 8;
+        } else {
+            const int q = f - S::FA;
+            const int64_t g = sg.z_fr + min(ntile16 + (q >> 1), nfn - 1);
+            L.p[t] = ((q & 1) ? a.klo[sg.z_buf] : a.khi[sg.z_buf]) + g * nfk * X3_FRAG + lane * 8;
+        }
+    }
+    L.left = nfk;
+}
+
+template <typename S, typename O>
+__device__ __forceinline__ void kg_issue(KLoader<S> &L, const GradX3Args<O> &a, const KSeg *sseg, uint16_t *stage,
+                                         int wv, int lane, int mtile16, int nfm, int ntile16, int nfn, int nfk) {
+    if (L.left == 0) {
+        ++L.seg;
+        kg_seg_begin<S, O>(L, a, sseg[L.seg], wv, lane, mtile16, nfm, ntile16, nfn, nfk);
+    }
+#pragma unroll
+    for (int t = 0; t < S::PER; ++t) {
+        lds_dma16(L.p[t], stage + (wv * S::PER + t) * X3_FRAG);
+        L.p[t] += X3_FRAG;
+    }
+    --L.left;
+}
+
+template <typename O>
+__global__ __launch_bounds__(256) void x3_grad_kernel(GradX3Args<O> a) {
+    typedef X3Shape<2, 2> S;
+    constexpr int NST = 2;
+    static_assert(S::BM == GRAD_X3_TILE && S::BN == GRAD_X3_TILE, "tile size");
+    static_assert(S::NW * 64 * 64 * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
+    __shared__ KSeg sseg[GRAD_KMAXSEG];
+
+    const int tid = threadIdx.x;
+    const int4 tl = a.tiles[xcd_remap(blockIdx.x, a.ntiles)];
+    const GTask task = a.tasks[tl.x];
+    O *out = a.out[task.c_buf];
+    if (out == nullptr) return;   // this block type's gradient was not requested (uniform)
+    const int M = task.m, N = task.n;
+    const int m0 = tl.y * S::BM, n0 = tl.z * S::BN;
+    const int lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 1, wc = wv & 1;
+    const int nfm = (M + 15) / 16, nfn = (N + 15) / 16;
+    const int mtile16 = m0 / 16, ntile16 = n0 / 16;
+    const int nfk = (a.nrhs + 31) / 32;
+
+    const int nseg = task.seg_end - task.seg_begin;   // <= GRAD_KMAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += S::THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+    const int nch = nseg * nfk;
+
+    x3_f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+
+    KLoader<S> ld;
+    ld.seg = -1;
+    ld.left = 0;
+    int issued = 0;
+    auto issue = [&]() {
+        if (issued < nch) {
+            kg_issue<S, O>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, ntile16, nfn, nfk);
+            ++issued;
+        }
+    };
+#pragma unroll
+    for (int p = 0; p < NST - 1; ++p) issue();
+    for (int c = 0; c < nch; ++c) {
+        wait_dma<S::PER, NST - 1>(issued - c - 1);
+        wg_barrier();
+        issue();
+        x3_compute<4>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+    }
+    __syncthreads();
+
+    // C(row, col) of block (i, j), register q: row 16 i + 4 (lane / 16) + q,
+    // column 16 j + lane % 16 of the wave's 64 x 64
+    float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + q;
+                Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[i][j][q];
+            }
+    __builtin_amdgcn_wave_barrier();
+    O *C = out + task.c_off;
+    const int rb = m0 + wr * 64, cb = n0 + wc * 64;
+    for (int e = lane; e < 64 * 64; e += 64) {
+        const int col = e >> 6, row = e & 63;
+        const int gm = rb + row, gn = cb + col;
+        if (gm < M && gn < N) C[(int64_t)gn * task.ldc + gm] = (O)Cs[col * 64 + (row ^ ((col & 15) << 2))];
+    }
+}
+
+template <typename O>
+void launch_grad_x3(const GradX3Args<O> &a, hipStream_t st) {
+    if (a.ntiles <= 0) return;
+    hipLaunchKernelGGL((x3_grad_kernel<O>), dim3((unsigned)a.ntiles), dim3(256), 0, st, a);
+}
+template void launch_grad_x3<float>(const GradX3Args<float> &, hipStream_t);
+template void launch_grad_x3<double>(const GradX3Args<double> &, hipStream_t);
 
 }  // namespace mx

@@ -55,10 +55,53 @@ struct GradArgs {
 template <typename T, typename O>
 void launch_grad(const GradArgs<T, O> &a, hipStream_t st);
 
-// Split-BF16x3 products of FP32 operands with FP32 accumulation: the same
-// task list, cut into 128 x 128 tiles (GRAD_X3_TILE).
+// Split-BF16x3 products (MX_BF16X3 backward).  Both operands are read as
+// K-fragment blocks: an m x nrhs column block, split as x = hi + lo (bf16),
+// is stored as 16-row groups by 32-column chunks, fragment (g, c) at
+// ((fr + g) * nfk + c) * 512 elements of its buffer (nfk = ceil(nrhs / 32),
+// fr the block's first group); lane l of a fragment holds row 16 g + l % 16,
+// columns 32 c + 8 (l / 16) .. +7, zero past the block.  A fragment is then
+// an MFMA A operand (16 rows x 32 deep) of X and a B operand (32 deep x 16
+// columns) of Z alike, and a gradient product runs on the LDS-DMA pipeline
+// of the BF16x3 evaluation kernel, 128 x 128 tiles (GRAD_X3_TILE).
 constexpr int GRAD_X3_TILE = 128;
+constexpr int GRAD_KMAXSEG = 8;     // segments per gradient task (2 are used)
+
+struct alignas(16) KSeg {
+    int64_t x_fr, z_fr;     // first 16-row group of X_s and of Z_s in their K-fragment buffers
+    int32_t x_buf, z_buf;   // GradBuf
+};
+
+struct alignas(16) KBlock {   // a column block to pack into K-fragments
+    int64_t row;    // its first row in the source (column buffer, or split panel row)
+    int64_t fr;     // its first 16-row group in the K-fragment buffer
+    int32_t m;      // rows
+    int32_t pad;
+};
+
 template <typename O>
-void launch_grad_x3(const GradArgs<float, O> &a, hipStream_t st);
+struct GradX3Args {
+    const GTask *tasks;
+    const KSeg *segs;
+    const int4 *tiles;        // (task, row tile, column tile, unused), GRAD_X3_TILE tiles
+    int ntiles, nrhs;
+    const uint16_t *khi[GB_COUNT];   // K-fragment buffers of the operands
+    const uint16_t *klo[GB_COUNT];
+    O *out[3];                // dD, dB, dVT
+};
+
+template <typename O>
+void launch_grad_x3(const GradX3Args<O> &a, hipStream_t st);
+
+// Column blocks (T = float or double, element (i, k) of a block at
+// row * nrhs + k * m + i) into K-fragments.
+template <typename T>
+void launch_kpack_cols(const T *src, const KBlock *blocks, int nblocks, int nrhs, uint16_t *hi, uint16_t *lo,
+                       hipStream_t st);
+
+// Split panels (element (i, k) of a block at ((row + i / 32 * 32) * nrhs +
+// k * 32 + i % 32, hi and lo apart) into K-fragments.
+void launch_kpack_panels(const uint16_t *phi, const uint16_t *plo, const KBlock *blocks, int nblocks, int nrhs,
+                         uint16_t *hi, uint16_t *lo, hipStream_t st);
 
 }  // namespace mx

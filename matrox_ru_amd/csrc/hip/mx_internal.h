@@ -128,6 +128,16 @@ struct mx_plan {
     int g_ntiles = 0;
     int4 *g_tiles_x3 = nullptr;         // GRAD_X3_TILE tiles
     int g_ntiles_x3 = 0;
+    // split-BF16x3 gradient products: segments over K-fragment blocks, the
+    // blocks to pack (leaf blocks of W / dY, skeleton blocks in the column
+    // layout and as split panels), and the K-fragment buffers of the six
+    // operands (hi, lo), sized for k_nrhs columns
+    mx::KSeg *g_ksegs = nullptr;
+    mx::KBlock *kb_leaf = nullptr, *kb_skel = nullptr, *kb_panel = nullptr;
+    int n_kb_leaf = 0, n_kb_skel = 0;
+    int64_t kfr_leaf = 0, kfr_skel = 0;   // 16-row groups of a leaf-blocked and a skeleton operand
+    uint16_t *kfrag = nullptr;
+    size_t kfrag_bytes = 0;
     double g_flops = 0.0;               // algorithmic flops per nrhs column
     // packing lists of the split-BF16 operands, kept to repack new blocks
     mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
@@ -174,7 +184,7 @@ struct mx_plan {
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
                         d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, d_xblk[0], d_xblk[1], d_xblk[2],
-                        wskel_adj, uskel_adj, io_g};
+                        wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);

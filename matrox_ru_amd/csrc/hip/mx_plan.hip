@@ -192,6 +192,49 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         P->g_tiles_x3 = upload(G.tiles_x3.data(), G.tiles_x3.size());
         P->g_ntiles_x3 = (int)G.tiles_x3.size();
         P->g_flops = G.flops;
+        if (precisions & MX_PLAN_BF16X3) {
+            // K-fragment blocks: leaves (W, dY) and skeleton nodes, each
+            // starting at its own 16-row group, in index order
+            std::vector<KBlock> kl, ks, kp;
+            std::map<int64_t, int64_t> leaf_fr, skel_fr;   // first row -> first group (non-empty blocks)
+            int64_t fr = 0;
+            for (int l = 0; l < s.nleaf; ++l) {
+                if (s.Ddim[l] == 0) continue;
+                kl.push_back(KBlock{s.wrow[l], fr, s.Ddim[l], 0});
+                leaf_fr[s.wrow[l]] = fr;
+                fr += (s.Ddim[l] + 15) / 16;
+            }
+            P->kfr_leaf = fr;
+            fr = 0;
+            for (int v = 0; v < s.numnodes; ++v) {
+                if (s.slen[v] == 0) continue;
+                ks.push_back(KBlock{s.skelrow[v], fr, s.slen[v], 0});
+                kp.push_back(KBlock{P->sprow[v], fr, s.slen[v], 0});
+                skel_fr[s.skelrow[v]] = fr;
+                fr += (s.slen[v] + 15) / 16;
+            }
+            P->kfr_skel = fr;
+            std::vector<KSeg> kseg(G.segs.size());
+            for (size_t i = 0; i < G.segs.size(); ++i) {
+                const GSeg &g = G.segs[i];
+                auto group = [&](int buf, int64_t row) {
+                    const auto &m = (buf == GB_W || buf == GB_DY) ? leaf_fr : skel_fr;
+                    auto it = m.find(row);
+                    if (it == m.end()) throw MxError(MX_ERR_ARG, "internal: gradient operand without a block");
+                    return it->second;
+                };
+                kseg[i] = KSeg{group(g.x_buf, g.x_row), group(g.z_buf, g.z_row), g.x_buf, g.z_buf};
+            }
+            for (const GTask &t : G.tasks)
+                if (t.seg_end - t.seg_begin > GRAD_KMAXSEG)
+                    throw MxError(MX_ERR_ARG, "internal: a gradient task with too many segments");
+            P->g_ksegs = upload(kseg.data(), kseg.size());
+            P->kb_leaf = upload(kl.data(), kl.size());
+            P->kb_skel = upload(ks.data(), ks.size());
+            P->kb_panel = upload(kp.data(), kp.size());
+            P->n_kb_leaf = (int)kl.size();
+            P->n_kb_skel = (int)ks.size();
+        }
     }
     // stored blocks: uploaded, or generated from the points straight into
     // each precision's representation (the FP64 blocks of a large matrix
@@ -480,11 +523,58 @@ static void export_panels(mx_plan *P, int dir, int nrhs, float *wdst, float *uds
     HIPCHK(hipGetLastError());
 }
 
+// K-fragment buffers of the six gradient operands (hi, lo) for nrhs columns:
+// leaf-blocked operands (W, dY) hold kfr_leaf 16-row groups, skeleton ones
+// kfr_skel.
+static void ensure_kfrag(mx_plan *P, int nrhs) {
+    const size_t nfk = (size_t)(nrhs + 31) / 32;
+    const size_t need = (2 * (size_t)P->kfr_leaf + 4 * (size_t)P->kfr_skel) * nfk * X3_FRAG * 2 * sizeof(uint16_t);
+    if (need > P->kfrag_bytes) {
+        if (P->kfrag) HIPCHK(hipFree(P->kfrag));
+        P->kfrag = nullptr;
+        HIPCHK(hipMalloc(&P->kfrag, std::max<size_t>(need, 1)));
+        P->kfrag_bytes = need;
+    }
+}
+
+// hi and lo K-fragments of gradient operand b (GradBuf) for nrhs columns
+static void kfrag_of(mx_plan *P, int b, int nrhs, uint16_t **hi, uint16_t **lo) {
+    const int64_t nfk = (nrhs + 31) / 32;
+    const int64_t leaf = P->kfr_leaf * nfk * X3_FRAG, skel = P->kfr_skel * nfk * X3_FRAG;
+    const int64_t off = b < GB_WHAT ? 2 * leaf * b : 4 * leaf + 2 * skel * (b - GB_WHAT);
+    *hi = P->kfrag + off;
+    *lo = *hi + (b < GB_WHAT ? leaf : skel);
+}
+
+// Skeleton panels of the last BF16x3 evaluation (direction dir) into the
+// K-fragments of operands bw (its wskel panel) and bu (its uskel panel).
+static void kpack_skel_panels(mx_plan *P, int dir, int nrhs, int bw, int bu, hipStream_t st) {
+    if (P->panels_dir != dir || P->panels_nrhs != nrhs)
+        throw MxError(MX_ERR_STATE, "the skeleton panels do not hold that evaluation");
+    const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
+    const uint16_t *pw = P->panels;
+    uint16_t *hi, *lo;
+    kfrag_of(P, bw, nrhs, &hi, &lo);
+    launch_kpack_panels(pw + 2 * wp, pw + 2 * wp + sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo, st);
+    kfrag_of(P, bu, nrhs, &hi, &lo);
+    launch_kpack_panels(pw + 2 * wp + 2 * sp, pw + 2 * wp + 3 * sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo, st);
+}
+
+template <typename T>
+static void kpack_cols(mx_plan *P, const void *src, bool leaf, int b, int nrhs, hipStream_t st) {
+    uint16_t *hi, *lo;
+    kfrag_of(P, b, nrhs, &hi, &lo);
+    launch_kpack_cols((const T *)src, leaf ? P->kb_leaf : P->kb_skel, leaf ? P->n_kb_leaf : P->n_kb_skel, nrhs, hi,
+                      lo, st);
+}
+
 // Backward pass of Y = K~ W for the upstream gradient dY: dW = K~^T dY by the
 // adjoint evaluation, then (when requested) the block gradients from the
 // forward skeletons (w^, u^: caller copies, or the plan's own buffers) and the
-// adjoint ones (y^, g^).  FP64, FP32 or split-BF16x3 compute (the latter on
-// FP32 copies of the operands); outputs in the I/O type.
+// adjoint ones (y^, g^).  FP64, FP32 or split-BF16x3 compute; outputs in the
+// I/O type.  Split-BF16x3 products read every operand as K-fragments, packed
+// from the I/O arrays, from caller FP32 skeletons, or straight from the
+// evaluations' skeleton panels.
 void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
               const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st) {
     arg_check(W && dY && dW, "null W, dY or dW");
@@ -494,16 +584,24 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     if (!P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const bool grads = dD || dB || dVT;
-    const bool x3 = prec == MX_BF16X3;   // BF16x3 evaluations and gradient products (FP32 operands)
+    const bool x3 = prec == MX_BF16X3;
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
-    if (grads && (!wskel || !uskel)) {
-        if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
-        if (x3) {   // the forward skeletons are split panels: FP32 copies before the adjoint reuses the panels
-            ensure_work(P, nrhs, sizeof(float), false);
-            export_panels(P, DIR_FWD, nrhs, (float *)P->wskel, (float *)P->uskel, st);
-        } else if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes) {
-            throw MxError(MX_ERR_STATE, "the last forward evaluation had fewer columns");
+    if (grads && x3) {
+        ensure_kfrag(P, nrhs);
+        if (wskel && uskel) {   // caller FP32 skeletons
+            kpack_cols<float>(P, wskel, false, GB_WHAT, nrhs, st);
+            kpack_cols<float>(P, uskel, false, GB_UHAT, nrhs, st);
+        } else {                // the forward's panels, before the adjoint reuses them
+            if (P->last_prec != prec)
+                throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
+            prof_mark(P, st, MX_KIND_CONVERT, true);
+            kpack_skel_panels(P, DIR_FWD, nrhs, GB_WHAT, GB_UHAT, st);
+            prof_mark(P, st, MX_KIND_CONVERT, false);
         }
+    } else if (grads && (!wskel || !uskel)) {
+        if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
+        if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes)
+            throw MxError(MX_ERR_STATE, "the last forward evaluation had fewer columns");
         wskel = P->wskel;
         uskel = P->uskel;
     }
@@ -519,15 +617,48 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     if (!grads) return;
     const int64_t nel = P->N * (int64_t)nrhs;
     const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : (prec == MX_FP32 ? MX_KIND_GRAD_F32 : MX_KIND_GRAD_X3);
-    if (x3) {   // adjoint skeletons to FP32; W and dY to FP32 when the I/O is FP64
-        ensure_adj_work(P, nrhs, sizeof(float));
-        export_panels(P, DIR_ADJ, nrhs, (float *)P->wskel_adj, (float *)P->uskel_adj, st);
+    if (x3) {
+        prof_mark(P, st, MX_KIND_CONVERT, true);
+        kpack_skel_panels(P, DIR_ADJ, nrhs, GB_YHAT, GB_GHAT, st);
         if (io == MX_IO_F64) {
-            ensure_work(P, nrhs, sizeof(float), true);
-            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)dY, (float *)P->io_w, nel);
+            kpack_cols<double>(P, W, true, GB_W, nrhs, st);
+            kpack_cols<double>(P, dY, true, GB_DY, nrhs, st);
+        } else {
+            kpack_cols<float>(P, W, true, GB_W, nrhs, st);
+            kpack_cols<float>(P, dY, true, GB_DY, nrhs, st);
         }
-    }
-    if (prec == MX_FP64) {
+        prof_mark(P, st, MX_KIND_CONVERT, false);
+        auto fill = [&](auto &g) {
+            g.tasks = P->g_tasks;
+            g.segs = P->g_ksegs;
+            g.tiles = P->g_tiles_x3;
+            g.ntiles = P->g_ntiles_x3;
+            g.nrhs = nrhs;
+            for (int b = 0; b < GB_COUNT; ++b) {
+                uint16_t *hi, *lo;
+                kfrag_of(P, b, nrhs, &hi, &lo);
+                g.khi[b] = hi;
+                g.klo[b] = lo;
+            }
+        };
+        prof_mark(P, st, kind, true);
+        if (io == MX_IO_F64) {
+            GradX3Args<double> g{};
+            fill(g);
+            g.out[MAT_D] = (double *)dD;
+            g.out[MAT_B] = (double *)dB;
+            g.out[MAT_VT] = (double *)dVT;
+            launch_grad_x3(g, st);
+        } else {
+            GradX3Args<float> g{};
+            fill(g);
+            g.out[MAT_D] = (float *)dD;
+            g.out[MAT_B] = (float *)dB;
+            g.out[MAT_VT] = (float *)dVT;
+            launch_grad_x3(g, st);
+        }
+        prof_mark(P, st, kind, false);
+    } else if (prec == MX_FP64) {
         GradArgs<double, double> g{};
         g.tasks = P->g_tasks;
         g.segs = P->g_segs;
@@ -559,8 +690,8 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
         auto fill = [&](auto &g) {
             g.tasks = P->g_tasks;
             g.segs = P->g_segs;
-            g.tiles = x3 ? P->g_tiles_x3 : P->g_tiles;
-            g.ntiles = x3 ? P->g_ntiles_x3 : P->g_ntiles;
+            g.tiles = P->g_tiles;
+            g.ntiles = P->g_ntiles;
             g.nrhs = nrhs;
             g.src[GB_W] = Wf;
             g.src[GB_DY] = dYf;
@@ -576,16 +707,14 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
             g.out[MAT_D] = (double *)dD;
             g.out[MAT_B] = (double *)dB;
             g.out[MAT_VT] = (double *)dVT;
-            if (x3) launch_grad_x3(g, st);
-            else launch_grad(g, st);
+            launch_grad(g, st);
         } else {
             GradArgs<float, float> g{};
             fill(g);
             g.out[MAT_D] = (float *)dD;
             g.out[MAT_B] = (float *)dB;
             g.out[MAT_VT] = (float *)dVT;
-            if (x3) launch_grad_x3(g, st);
-            else launch_grad(g, st);
+            launch_grad(g, st);
         }
         prof_mark(P, st, kind, false);
     }

@@ -323,8 +323,12 @@ def test_bf16x3_backward_close_to_fp64(gpu, case, nrhs, io):
     assert rel(ws.double().cpu().numpy(), ws64.cpu().numpy()) <= 2e-5
     assert rel(us.double().cpu().numpy(), us64.cpu().numpy()) <= 2e-5
     got = plan.backward(W, dY, nrhs, "bf16x3")
-    for a, b, nm in zip(got, ref, ["dW", "dD", "dB", "dVT"]):
+    # the same products from caller FP32 skeletons (packed from the column
+    # layout instead of the plan's panels)
+    got2 = plan.backward(W, dY, nrhs, "bf16x3", skeletons=(ws, us))
+    for a, a2, b, nm in zip(got, got2, ref, ["dW", "dD", "dB", "dVT"]):
         assert a.dtype == dt
         e = rel(a.double().cpu().numpy(), b.cpu().numpy())
-        _say(f"  {nm}: {e:.2e}")
-        assert e <= 5e-5, nm
+        e2 = rel(a2.double().cpu().numpy(), b.cpu().numpy())
+        _say(f"  {nm}: {e:.2e} (explicit skeletons {e2:.2e})")
+        assert e <= 5e-5 and e2 <= 5e-5, nm
