@@ -547,17 +547,23 @@ static void kfrag_of(mx_plan *P, int b, int nrhs, uint16_t **hi, uint16_t **lo) 
 }
 
 // Skeleton panels of the last BF16x3 evaluation (direction dir) into the
-// K-fragments of operands bw (its wskel panel) and bu (its uskel panel).
+// K-fragments of operands bw (its wskel panel) and bu (its uskel panel); a
+// negative operand is skipped.
 static void kpack_skel_panels(mx_plan *P, int dir, int nrhs, int bw, int bu, hipStream_t st) {
     if (P->panels_dir != dir || P->panels_nrhs != nrhs)
         throw MxError(MX_ERR_STATE, "the skeleton panels do not hold that evaluation");
     const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
     const uint16_t *pw = P->panels;
     uint16_t *hi, *lo;
-    kfrag_of(P, bw, nrhs, &hi, &lo);
-    launch_kpack_panels(pw + 2 * wp, pw + 2 * wp + sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo, st);
-    kfrag_of(P, bu, nrhs, &hi, &lo);
-    launch_kpack_panels(pw + 2 * wp + 2 * sp, pw + 2 * wp + 3 * sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo, st);
+    if (bw >= 0) {
+        kfrag_of(P, bw, nrhs, &hi, &lo);
+        launch_kpack_panels(pw + 2 * wp, pw + 2 * wp + sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo, st);
+    }
+    if (bu >= 0) {
+        kfrag_of(P, bu, nrhs, &hi, &lo);
+        launch_kpack_panels(pw + 2 * wp + 2 * sp, pw + 2 * wp + 3 * sp, P->kb_panel, P->n_kb_skel, nrhs, hi, lo,
+                            st);
+    }
 }
 
 template <typename T>
@@ -586,16 +592,19 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     const bool grads = dD || dB || dVT;
     const bool x3 = prec == MX_BF16X3;
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
+    // operands of the requested products: dD = dY W^T, dB = y^ w^^T,
+    // dVT = g^ [W | w^]^T + u^ [dY | y^]^T
+    const bool need_dw = dD || dVT, need_skel_w = dB || dVT, need_skel_u = dVT != nullptr;
     if (grads && x3) {
         ensure_kfrag(P, nrhs);
         if (wskel && uskel) {   // caller FP32 skeletons
-            kpack_cols<float>(P, wskel, false, GB_WHAT, nrhs, st);
-            kpack_cols<float>(P, uskel, false, GB_UHAT, nrhs, st);
+            if (need_skel_w) kpack_cols<float>(P, wskel, false, GB_WHAT, nrhs, st);
+            if (need_skel_u) kpack_cols<float>(P, uskel, false, GB_UHAT, nrhs, st);
         } else {                // the forward's panels, before the adjoint reuses them
             if (P->last_prec != prec)
                 throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
             prof_mark(P, st, MX_KIND_CONVERT, true);
-            kpack_skel_panels(P, DIR_FWD, nrhs, GB_WHAT, GB_UHAT, st);
+            kpack_skel_panels(P, DIR_FWD, nrhs, need_skel_w ? GB_WHAT : -1, need_skel_u ? GB_UHAT : -1, st);
             prof_mark(P, st, MX_KIND_CONVERT, false);
         }
     } else if (grads && (!wskel || !uskel)) {
@@ -619,13 +628,15 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     const int kind = prec == MX_FP64 ? MX_KIND_GRAD_F64 : (prec == MX_FP32 ? MX_KIND_GRAD_F32 : MX_KIND_GRAD_X3);
     if (x3) {
         prof_mark(P, st, MX_KIND_CONVERT, true);
-        kpack_skel_panels(P, DIR_ADJ, nrhs, GB_YHAT, GB_GHAT, st);
-        if (io == MX_IO_F64) {
-            kpack_cols<double>(P, W, true, GB_W, nrhs, st);
-            kpack_cols<double>(P, dY, true, GB_DY, nrhs, st);
-        } else {
-            kpack_cols<float>(P, W, true, GB_W, nrhs, st);
-            kpack_cols<float>(P, dY, true, GB_DY, nrhs, st);
+        kpack_skel_panels(P, DIR_ADJ, nrhs, need_skel_w ? GB_YHAT : -1, need_skel_u ? GB_GHAT : -1, st);
+        if (need_dw) {
+            if (io == MX_IO_F64) {
+                kpack_cols<double>(P, W, true, GB_W, nrhs, st);
+                kpack_cols<double>(P, dY, true, GB_DY, nrhs, st);
+            } else {
+                kpack_cols<float>(P, W, true, GB_W, nrhs, st);
+                kpack_cols<float>(P, dY, true, GB_DY, nrhs, st);
+            }
         }
         prof_mark(P, st, MX_KIND_CONVERT, false);
         auto fill = [&](auto &g) {

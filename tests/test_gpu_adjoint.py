@@ -299,7 +299,8 @@ def test_backward_errors(gpu):
 
 
 @pytest.mark.parametrize("case,nrhs,io", [("h2b_ragged", 48, "f64"), ("h2b_ragged", 48, "f32"),
-                                          ("h2b_wide", 160, "f64"), ("h2b_wide", 77, "f32")])
+                                          ("h2b_ragged", 1, "f32"), ("h2b_wide", 160, "f64"),
+                                          ("h2b_wide", 77, "f32")])
 def test_bf16x3_backward_close_to_fp64(gpu, case, nrhs, io):
     """Split-BF16x3 forward, adjoint and block products: gradients within
     5e-5 of the FP64 ones (operands split to 16 significant bits carry ~1e-5
@@ -332,3 +333,9 @@ def test_bf16x3_backward_close_to_fp64(gpu, case, nrhs, io):
         e2 = rel(a2.double().cpu().numpy(), b.cpu().numpy())
         _say(f"  {nm}: {e:.2e} (explicit skeletons {e2:.2e})")
         assert e <= 5e-5 and e2 <= 5e-5, nm
+    # one block gradient at a time packs only its operands; same products
+    for k in range(3):
+        flags = tuple(i == k for i in range(3))
+        sub = plan.backward(W, dY, nrhs, "bf16x3", skeletons=(ws, us), blocks=flags)
+        assert all(sub[1 + i] is None for i in range(3) if i != k)
+        assert torch.equal(sub[1 + k], got2[1 + k]), k
