@@ -739,6 +739,49 @@ GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
             }
         }
     }
+    // xcd_remap gives XCD x the x-th contiguous eighth of the tiles.  Each
+    // gradient type (D, B, VT: their tiles sit in that order, VT tiles with
+    // twice the segments) is cut into eight contiguous parts of equal work,
+    // and eighth x holds part x of every type, longest tiles first, padded
+    // with empty tiles to the longest eighth: the work is even across the
+    // XCDs whichever gradients a call requests, and each XCD still walks
+    // neighbouring blocks.
+    GTask empty{};
+    empty.ldc = 1;
+    empty.c_buf = MAT_D;
+    empty.seg_begin = empty.seg_end = 0;
+    const int dummy = (int)G.tasks.size();
+    G.tasks.push_back(empty);
+    auto balance = [&](std::vector<int4> &tiles) {
+        const int ng = 8;
+        std::vector<std::vector<int4>> eighth(ng);
+        for (int type : {MAT_VT, MAT_D, MAT_B}) {
+            std::vector<int4> v;
+            int64_t total = 0;
+            for (const int4 &tl : tiles)
+                if (G.tasks[tl.x].c_buf == type) {
+                    v.push_back(tl);
+                    total += G.tasks[tl.x].seg_end - G.tasks[tl.x].seg_begin;
+                }
+            int64_t acc = 0;
+            int q = 0;
+            for (const int4 &tl : v) {
+                const int64_t c = G.tasks[tl.x].seg_end - G.tasks[tl.x].seg_begin;
+                while (q < ng - 1 && (acc + c / 2) * ng >= total * (q + 1)) ++q;   // part by the tile's midpoint
+                eighth[q].push_back(tl);
+                acc += c;
+            }
+        }
+        size_t longest = 0;
+        for (const auto &e : eighth) longest = std::max(longest, e.size());
+        tiles.clear();
+        for (const auto &e : eighth) {
+            tiles.insert(tiles.end(), e.begin(), e.end());
+            for (size_t i = e.size(); i < longest; ++i) tiles.push_back(make_int4(dummy, 0, 0, 0));
+        }
+    };
+    balance(G.tiles);
+    balance(G.tiles_x3);
     return G;
 }
 
