@@ -133,47 +133,67 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // neighbouring nodes share skeleton blocks), cut into eight contiguous
 // groups of about equal work, each sorted longest first and padded the same
 // way; this spreads a few tasks' tiles over all eight XCDs.
+static int64_t seg_key(const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; }
+
+// Slot by slot, the B operand that the most tasks of the group still needing
+// a segment at this slot share (ties: the smallest key) goes next in each of
+// them.  Tasks are bits of a mask (a group has at most 32).
 static void reorder_group_segments(TaskLists &L, const std::vector<int> &tasks) {
-    auto key = [](const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; };
-    const size_t G = tasks.size();
-    std::vector<std::vector<Seg>> rem(G), seq(G);
-    for (size_t j = 0; j < G; ++j) {
+    const int G = (int)tasks.size();
+    std::vector<int64_t> keys;
+    for (int t : tasks)
+        for (int sg = L.tasks[t].seg_begin; sg < L.tasks[t].seg_end; ++sg) keys.push_back(seg_key(L.segs[sg]));
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const int K = (int)keys.size();
+    // per task: its segments by key index (ascending), a cursor per key
+    std::vector<std::vector<std::pair<int, int>>> by(G);   // (key index, segment)
+    std::vector<uint32_t> holders(K, 0);
+    std::vector<int> left(G);
+    for (int j = 0; j < G; ++j) {
         const Task &t = L.tasks[tasks[j]];
-        rem[j].assign(L.segs.begin() + t.seg_begin, L.segs.begin() + t.seg_end);
+        for (int sg = t.seg_begin; sg < t.seg_end; ++sg) {
+            const int ki = (int)(std::lower_bound(keys.begin(), keys.end(), seg_key(L.segs[sg])) - keys.begin());
+            by[j].push_back({ki, sg});
+            holders[ki] |= 1u << j;
+        }
+        std::stable_sort(by[j].begin(), by[j].end(),
+                         [](const std::pair<int, int> &x, const std::pair<int, int> &y) { return x.first < y.first; });
+        left[j] = (int)by[j].size();
     }
+    std::vector<std::vector<char>> used(G);
+    for (int j = 0; j < G; ++j) used[j].assign(by[j].size(), 0);
+    std::vector<std::vector<Seg>> seq(G);
+    auto take = [&](int j, int ki) {   // the first unused segment of task j with key ki
+        auto it = std::lower_bound(by[j].begin(), by[j].end(), std::make_pair(ki, -1));
+        size_t i = (size_t)(it - by[j].begin());
+        while (used[j][i]) ++i;
+        used[j][i] = 1;
+        seq[j].push_back(L.segs[by[j][i].second]);
+        --left[j];
+        if (i + 1 >= by[j].size() || by[j][i + 1].first != ki || used[j][i + 1]) holders[ki] &= ~(1u << j);
+    };
     for (;;) {
-        std::vector<size_t> open;
-        for (size_t j = 0; j < G; ++j)
-            if (!rem[j].empty()) open.push_back(j);
-        if (open.empty()) break;
-        while (!open.empty()) {   // one position of every open task
-            std::map<int64_t, int> cnt;
-            for (size_t j : open) {
-                std::set<int64_t> mine;
-                for (const Seg &g : rem[j]) mine.insert(key(g));
-                for (int64_t k : mine) ++cnt[k];
-            }
-            int64_t best = 0;
-            int bc = 0;
-            for (const auto &kc : cnt)   // ascending keys: ties go to the smallest
-                if (kc.second > bc) {
-                    best = kc.first;
-                    bc = kc.second;
+        uint32_t open = 0;
+        for (int j = 0; j < G; ++j)
+            if (left[j] > 0) open |= 1u << j;
+        if (!open) break;
+        while (open) {   // one position of every open task
+            int best = -1, bc = 0;
+            for (int ki = 0; ki < K; ++ki) {
+                const int c = __builtin_popcount(holders[ki] & open);
+                if (c > bc) {
+                    bc = c;
+                    best = ki;
                 }
-            std::vector<size_t> still;
-            for (size_t j : open) {
-                auto it = std::find_if(rem[j].begin(), rem[j].end(), [&](const Seg &g) { return key(g) == best; });
-                if (it == rem[j].end()) {
-                    still.push_back(j);
-                    continue;
-                }
-                seq[j].push_back(*it);
-                rem[j].erase(it);
             }
-            open.swap(still);
+            const uint32_t placed = holders[best] & open;
+            for (int j = 0; j < G; ++j)
+                if (placed >> j & 1u) take(j, best);
+            open &= ~placed;
         }
     }
-    for (size_t j = 0; j < G; ++j) std::copy(seq[j].begin(), seq[j].end(), L.segs.begin() + L.tasks[tasks[j]].seg_begin);
+    for (int j = 0; j < G; ++j) std::copy(seq[j].begin(), seq[j].end(), L.segs.begin() + L.tasks[tasks[j]].seg_begin);
 }
 
 static void order_tiles(TaskLists &L, int bk) {
@@ -191,7 +211,6 @@ static void order_tiles(TaskLists &L, int bk) {
         for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) cost += (L.segs[sg].k + bk - 1) / bk;
         return cost;
     };
-    auto key = [](const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; };
     std::vector<int2> out;
     std::vector<Launch> lns;
     std::vector<int> local(L.tasks.size(), -1);
@@ -237,21 +256,29 @@ static void order_tiles(TaskLists &L, int bk) {
                 for (int i = cut[q]; i < cut[q + 1]; ++i) bins[q].push_back(v[i].second);
             }
         } else {
-            std::vector<std::vector<int64_t>> src(n);
-            std::map<int64_t, std::vector<int>> users;
+            std::vector<std::vector<int>> src(n);   // distinct B operands (dense indices) of each task
+            std::vector<int64_t> all;
             std::vector<int64_t> cost(n);
             for (int i = 0; i < n; ++i) {
                 const Task &t = L.tasks[tk[i]];
-                for (int sg = t.seg_begin; sg < t.seg_end; ++sg) src[i].push_back(key(L.segs[sg]));
+                for (int sg = t.seg_begin; sg < t.seg_end; ++sg) all.push_back(seg_key(L.segs[sg]));
+                cost[i] = tile_cost(tk[i]);
+            }
+            std::sort(all.begin(), all.end());
+            all.erase(std::unique(all.begin(), all.end()), all.end());
+            std::vector<std::vector<int>> users(all.size());
+            for (int i = 0; i < n; ++i) {
+                const Task &t = L.tasks[tk[i]];
+                for (int sg = t.seg_begin; sg < t.seg_end; ++sg)
+                    src[i].push_back((int)(std::lower_bound(all.begin(), all.end(), seg_key(L.segs[sg])) - all.begin()));
                 std::sort(src[i].begin(), src[i].end());
                 src[i].erase(std::unique(src[i].begin(), src[i].end()), src[i].end());
-                for (int64_t k : src[i]) users[k].push_back(i);
-                cost[i] = tile_cost(tk[i]);
+                for (int k : src[i]) users[k].push_back(i);
             }
             std::vector<int> seeds(n);
             for (int i = 0; i < n; ++i) seeds[i] = i;
             std::stable_sort(seeds.begin(), seeds.end(), [&](int a, int b) { return src[a].size() > src[b].size(); });
-            std::vector<char> used(n, 0);
+            std::vector<char> used(n, 0), inU(all.size(), 0);
             std::vector<int> score(n, 0);
             std::vector<std::vector<int>> groups;
             size_t sp = 0;
@@ -261,10 +288,18 @@ static void order_tiles(TaskLists &L, int bk) {
                 if (sp == seeds.size()) break;
                 std::vector<int> g{seeds[sp]};
                 used[seeds[sp]] = 1;
-                std::set<int64_t> U(src[g[0]].begin(), src[g[0]].end());
+                std::vector<int> U;   // operands of the group
+                auto add = [&](int task) {
+                    for (int k : src[task])
+                        if (!inU[k]) {
+                            inU[k] = 1;
+                            U.push_back(k);
+                        }
+                };
+                add(g[0]);
                 while ((int)g.size() < G) {
                     std::vector<int> touched;
-                    for (int64_t k : U)
+                    for (int k : U)
                         for (int c : users[k])
                             if (!used[c] && score[c]++ == 0) touched.push_back(c);
                     int best = -1;
@@ -278,8 +313,9 @@ static void order_tiles(TaskLists &L, int bk) {
                     }
                     used[best] = 1;
                     g.push_back(best);
-                    U.insert(src[best].begin(), src[best].end());
+                    add(best);
                 }
+                for (int k : U) inU[k] = 0;
                 groups.push_back(g);
             }
             const int ngr = (int)groups.size();
