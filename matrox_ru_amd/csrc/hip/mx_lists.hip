@@ -118,8 +118,8 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // panel, read by every task whose list names it) is served from L2 to a
 // second task only when both read it at about the same time.
 //
-// Launches of at least 64 tasks (the big ones) are therefore ordered in
-// groups of four tasks that share B operands: greedily, the task with the
+// The BF16x3 kernel's launches of at least 64 tasks (the big ones) are
+// therefore ordered in groups of four tasks that share B operands: greedily, the task with the
 // most segments, then repeatedly the task sharing the most operands with the
 // group.  Within a group the segments are reordered so that a shared operand
 // sits at the same position in every task of the group that reads it (slot
@@ -129,10 +129,13 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // the bins are padded with empty tiles to the size of the largest, so that
 // bin x is exactly the eighth xcd_remap gives XCD x.
 //
-// Smaller launches keep the tiles in structural order (tree order:
-// neighbouring nodes share skeleton blocks), cut into eight contiguous
-// groups of about equal work, each sorted longest first and padded the same
-// way; this spreads a few tasks' tiles over all eight XCDs.
+// Smaller launches, and the FP64 / FP32 kernel's (64 x 64 tiles: a window
+// of 64 work items holds about two 256-row tasks, half a group; the grouping
+// measured 2-4% slower there), keep the tiles in
+// structural order (tree order: neighbouring nodes share skeleton blocks),
+// cut into eight contiguous groups of about equal work, each sorted longest
+// first and padded the same way; this spreads a few tasks' tiles over all
+// eight XCDs.
 static int64_t seg_key(const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; }
 
 // Slot by slot, the B operand that the most tasks of the group still needing
@@ -196,7 +199,7 @@ static void reorder_group_segments(TaskLists &L, const std::vector<int> &tasks) 
     for (int j = 0; j < G; ++j) std::copy(seq[j].begin(), seq[j].end(), L.segs.begin() + L.tasks[tasks[j]].seg_begin);
 }
 
-static void order_tiles(TaskLists &L, int bk) {
+static void order_tiles(TaskLists &L, int bk, bool group_by_operand) {
     if (L.dummy < 0) {
         Task tk{};
         tk.c_buf = VEC_USKEL;
@@ -231,7 +234,7 @@ static void order_tiles(TaskLists &L, int bk) {
         for (int t : tk) local[t] = -1;
         const int n = (int)tk.size();
         std::vector<std::vector<int2>> bins(ng);
-        if (n < min_tasks) {
+        if (n < min_tasks || !group_by_operand) {
             std::vector<std::pair<int64_t, int2>> v;
             v.reserve(T);
             int64_t total = 0;
@@ -657,7 +660,7 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
-    order_tiles(L, kind == 1 ? X3_BK : SG_BK);
+    order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1);
     return L;
 }
 
