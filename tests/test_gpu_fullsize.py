@@ -1,0 +1,109 @@
+"""Checks at the headline size: N = 65,536 points in 28 dimensions (16-
+component Gaussian mixture), Gaussian kernel h = 1.5, leaves of 256, ranks
+up to 256 (tolerance 1e-5), near budget 0.03: 256 leaves, 1,536 near blocks,
+3,263 far blocks.
+
+* FP64 evaluation and adjoint against the oracle (the forward structure and
+  its transpose) at a few columns, where the oracle takes well under a
+  second: <= 1e-12 relative Frobenius error.
+* At nrhs = 1,024, properties that hold at any size: linearity, the adjoint
+  identity <K~x, y> = <x, K~^T y>, column independence (the first columns of
+  a 1,024-column evaluation equal a 37-column evaluation of those columns),
+  and split-BF16x3 within 1e-5 of FP64 in both directions.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel
+from matrox_ru_amd import hgemm, host
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+HEADLINE = dict(n=65536, d=28, ncomp=16, sig_lo=0.5, sig_hi=2.0, diag=True, seed=1002,
+                leaf=256, rank=256, tol=1e-5, budget=0.03, h=1.5)
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def headline(gpu):
+    p = HEADLINE
+    _say("[fullsize] building the headline matrix")
+    X = host.gaussian_mixture(p["n"], p["d"], p["ncomp"], sig_lo=p["sig_lo"], sig_hi=p["sig_hi"], diag=p["diag"],
+                              seed=p["seed"])
+    hm = host.build_hmatrix(X, leaf_size=p["leaf"], max_rank=p["rank"], tol=p["tol"], budget=p["budget"], h=p["h"])
+    assert hm.nleaf == 256 and hm.N == 65536
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
+    return hm, plan
+
+
+def _columns(hm, TW, nrhs, k):
+    """The first k columns of a leaf-blocked N x nrhs array, leaf-blocked."""
+    out = np.empty(hm.N * k)
+    for l in range(hm.nleaf):
+        r, m = int(hm.wrow[l]), int(hm.Ddim[l])
+        blk = TW[r * nrhs:(r + m) * nrhs].reshape(nrhs, m)   # column-major m x nrhs
+        out[r * k:(r + m) * k] = blk[:k].reshape(-1)
+    return out
+
+
+def test_fullsize_fp64_matches_oracle(headline):
+    import torch
+    hm, plan = headline
+    nrhs = 8
+    _say(f"[fullsize oracle] nrhs={nrhs}")
+    oracle.use_mkl()
+    TW = host.normal((hm.N * nrhs,), seed=71)
+    Yo, _, _ = oracle.hgemm(hm, TW, nrhs)
+    Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
+    W = torch.from_numpy(TW).cuda()
+    Y = torch.full_like(W, float("nan"))
+    G = torch.full_like(W, float("nan"))
+    plan.eval(W, Y, nrhs, "fp64")
+    plan.eval_adjoint(W, G, nrhs, "fp64")
+    ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
+    _say(f"  forward {ey:.2e} adjoint {eg:.2e}")
+    assert ey <= 1e-12 and eg <= 1e-12
+
+
+def test_fullsize_properties(headline):
+    import torch
+    hm, plan = headline
+    nrhs = 1024
+    _say(f"[fullsize properties] nrhs={nrhs}")
+    TX = host.normal((hm.N * nrhs,), seed=72)
+    x = torch.from_numpy(TX).cuda()
+    z = torch.from_numpy(host.normal((hm.N * nrhs,), seed=73)).cuda()
+    Kx, Kz, Kxz = torch.empty_like(x), torch.empty_like(x), torch.empty_like(x)
+    plan.eval(x, Kx, nrhs, "fp64")
+    plan.eval(z, Kz, nrhs, "fp64")
+    plan.eval(0.75 * x - 1.25 * z, Kxz, nrhs, "fp64")
+    e_lin = float(torch.linalg.vector_norm(Kxz - (0.75 * Kx - 1.25 * Kz)) / torch.linalg.vector_norm(Kxz))
+    _say(f"  linearity {e_lin:.2e}")
+    assert e_lin <= 1e-13
+    Ktz = torch.empty_like(z)
+    plan.eval_adjoint(z, Ktz, nrhs, "fp64")
+    a, b = float((Kx * z).sum()), float((x * Ktz).sum())
+    scale = float(torch.linalg.vector_norm(Kx) * torch.linalg.vector_norm(z))
+    _say(f"  adjoint identity {abs(a - b) / scale:.2e}")
+    assert abs(a - b) <= 1e-12 * scale
+    k = 37
+    x37 = torch.from_numpy(_columns(hm, TX, nrhs, k)).cuda()
+    K37 = torch.empty_like(x37)
+    plan.eval(x37, K37, k, "fp64")
+    e_col = rel(K37.cpu().numpy(), _columns(hm, Kx.cpu().numpy(), nrhs, k))
+    _say(f"  columns {e_col:.2e}")
+    assert e_col <= 1e-14
+    Y3, G3 = torch.empty_like(x), torch.empty_like(z)
+    plan.eval(x, Y3, nrhs, "bf16x3")
+    plan.eval_adjoint(z, G3, nrhs, "bf16x3")
+    ey = float(torch.linalg.vector_norm(Y3 - Kx) / torch.linalg.vector_norm(Kx))
+    eg = float(torch.linalg.vector_norm(G3 - Ktz) / torch.linalg.vector_norm(Ktz))
+    _say(f"  bf16x3 vs fp64: forward {ey:.2e} adjoint {eg:.2e}")
+    assert ey <= 1e-5 and eg <= 1e-5
