@@ -1,4 +1,4 @@
-"""Checks at the headline size: N = 65,536 points in 28 dimensions (16-
+"""Checks at full size.  The headline matrix: N = 65,536 points in 28 dimensions (16-
 component Gaussian mixture), Gaussian kernel h = 1.5, leaves of 256, ranks
 up to 256 (tolerance 1e-5), near budget 0.03: 256 leaves, 1,536 near blocks,
 3,263 far blocks.
@@ -103,6 +103,57 @@ def test_fullsize_properties(headline):
     Y3, G3 = torch.empty_like(x), torch.empty_like(z)
     plan.eval(x, Y3, nrhs, "bf16x3")
     plan.eval_adjoint(z, G3, nrhs, "bf16x3")
+    ey = float(torch.linalg.vector_norm(Y3 - Kx) / torch.linalg.vector_norm(Kx))
+    eg = float(torch.linalg.vector_norm(G3 - Ktz) / torch.linalg.vector_norm(Ktz))
+    _say(f"  bf16x3 vs fp64: forward {ey:.2e} adjoint {eg:.2e}")
+    assert ey <= 1e-5 and eg <= 1e-5
+
+
+SCALE = dict(n=524288, d=27, ncomp=64, sig_lo=0.5, sig_hi=1.5, diag=False, seed=1005,
+             leaf=256, rank=256, tol=1e-5, budget=0.03, h=1.2)
+
+
+def test_scale_properties(gpu):
+    """N = 524,288 points in 27 dimensions (64 components), h = 1.2: 2,048
+    leaves, 122,880 near and 194,637 far blocks, whose FP64 values (150 GB)
+    are generated on the device and never exist on the host.  At nrhs = 64:
+    FP64 linearity and adjoint identity, then split-BF16x3 within 1e-5 of
+    FP64 in both directions (one plan at a time)."""
+    import torch
+    p = SCALE
+    _say("[scale] building the structure")
+    X = host.gaussian_mixture(p["n"], p["d"], p["ncomp"], sig_lo=p["sig_lo"], sig_hi=p["sig_hi"], diag=p["diag"],
+                              seed=p["seed"])
+    hm = host.build_hmatrix(X, leaf_size=p["leaf"], max_rank=p["rank"], tol=p["tol"], budget=p["budget"], h=p["h"],
+                            skip_blocks=1)
+    assert hm.nleaf == 2048
+    nrhs = 64
+    x = torch.from_numpy(host.normal((hm.N * nrhs,), seed=81)).cuda()
+    z = torch.from_numpy(host.normal((hm.N * nrhs,), seed=82)).cuda()
+    plan = hgemm.Plan(hm, precisions=("fp64",), adjoint=True, generate=(X, p["h"]))
+    Kx, Kz, Kxz, Ktz = (torch.empty_like(x) for _ in range(4))
+    plan.eval(x, Kx, nrhs, "fp64")
+    plan.eval(z, Kz, nrhs, "fp64")
+    plan.eval(0.75 * x - 1.25 * z, Kxz, nrhs, "fp64")
+    plan.eval_adjoint(z, Ktz, nrhs, "fp64")
+    torch.cuda.synchronize()
+    plan.close()
+    del plan
+    torch.cuda.empty_cache()
+    e_lin = float(torch.linalg.vector_norm(Kxz - (0.75 * Kx - 1.25 * Kz)) / torch.linalg.vector_norm(Kxz))
+    a, b = float((Kx * z).sum()), float((x * Ktz).sum())
+    scale = float(torch.linalg.vector_norm(Kx) * torch.linalg.vector_norm(z))
+    _say(f"  linearity {e_lin:.2e} adjoint identity {abs(a - b) / scale:.2e}")
+    assert e_lin <= 1e-13
+    assert abs(a - b) <= 1e-12 * scale
+    plan = hgemm.Plan(hm, precisions=("bf16x3",), adjoint=True, generate=(X, p["h"]))
+    Y3, G3 = torch.empty_like(x), torch.empty_like(z)
+    plan.eval(x, Y3, nrhs, "bf16x3")
+    plan.eval_adjoint(z, G3, nrhs, "bf16x3")
+    torch.cuda.synchronize()
+    plan.close()
+    del plan
+    torch.cuda.empty_cache()
     ey = float(torch.linalg.vector_norm(Y3 - Kx) / torch.linalg.vector_norm(Kx))
     eg = float(torch.linalg.vector_norm(G3 - Ktz) / torch.linalg.vector_norm(Ktz))
     _say(f"  bf16x3 vs fp64: forward {ey:.2e} adjoint {eg:.2e}")
