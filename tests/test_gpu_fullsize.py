@@ -12,8 +12,9 @@ up to 256 (tolerance 1e-5), near budget 0.03: 256 leaves, 1,536 near blocks,
   and split-BF16x3 within 1e-5 of FP64 in both directions.
 
 The other workloads at their full sizes: N = 524,288 with device-generated
-blocks (properties), HSS at N = 16,384 (oracle, symmetry), and the
-N = 32,768 adjoint workload (gradients against central differences).
+blocks (properties), HSS at N = 16,384 (oracle, symmetry), the N = 32,768
+adjoint workload (gradients against central differences), and the batch of
+64 matrices (oracle per group, every instance's mixed precision).
 """
 import sys
 
@@ -260,3 +261,61 @@ def test_adjoint_workload_gradients(gpu):
         e = rel(np.array(an[gp]), fd)
         _say(f"  {gp} gradients vs central differences {e:.2e}")
         assert e <= bar, gp
+
+
+BATCH = dict(count=64, n_lo=2048, n_hi=32768, dims=(8, 18, 28, 54), h_lo=0.8, h_hi=2.0, leaves=(128, 256),
+             rank=128, tol=1e-5, budget=0.03, nrhs=(32, 128, 512), ncomp=16, sig_lo=0.5, sig_hi=2.0, seed=3,
+             seed_points=1003, seed_w=2003)
+
+
+def test_batched_workload(gpu):
+    """64 independent matrices (N log-uniform in [2,048, 32,768], d in {8, 18,
+    28, 54}, h in [0.8, 2.0], leaves of 128 or 256, ranks up to 128; those of
+    at most 64 leaves are HSS-structured), nrhs in {32, 128, 512}, as one
+    batch of three nrhs groups: the smallest instance of each group against
+    the oracle in FP64; every instance's split-BF16x3 forward and adjoint
+    within 2e-5 of FP64; the adjoint identity of each group in FP64."""
+    import torch
+    w = BATCH
+    _say("[batched workload] building 64 instances")
+    rng = np.random.default_rng(w["seed"])
+    inst, hms = [], []
+    for i in range(w["count"]):
+        n = int(round(np.exp(rng.uniform(np.log(w["n_lo"]), np.log(w["n_hi"])))))
+        it = dict(n=n, d=int(rng.choice(w["dims"])), h=float(rng.uniform(w["h_lo"], w["h_hi"])),
+                  leaf=int(rng.choice(w["leaves"])), nrhs=int(rng.choice(w["nrhs"])))
+        X = host.gaussian_mixture(n, it["d"], w["ncomp"], sig_lo=w["sig_lo"], sig_hi=w["sig_hi"], diag=True,
+                                  seed=w["seed_points"] + 100 * i)
+        hms.append(host.build_hmatrix(X, leaf_size=it["leaf"], max_rank=w["rank"], tol=w["tol"], budget=w["budget"],
+                                      h=it["h"]))
+        inst.append(it)
+    b = hgemm.Batch(hms, [it["nrhs"] for it in inst], precisions=("fp64", "bf16x3"), adjoint=True)
+    assert len(b.groups) == 3
+    W = b.buffers(torch.float64, fill=0.0)
+    TW = []
+    for i, (hm, it) in enumerate(zip(hms, inst)):
+        TW.append(host.normal((hm.N * it["nrhs"],), seed=w["seed_w"] + 100 * i))
+        b.instance_view(W, i).copy_(torch.from_numpy(TW[-1]))
+    Y64, Y3, G64, G3 = (b.buffers(torch.float64) for _ in range(4))
+    b.eval(W, Y64, "fp64")
+    b.eval(W, Y3, "bf16x3")
+    b.eval_adjoint(W, G64, "fp64")
+    b.eval_adjoint(W, G3, "bf16x3")
+    torch.cuda.synchronize()
+    oracle.use_mkl()
+    for g in range(3):
+        i = min((i for i in range(len(hms)) if b.layout[i][0] == g), key=lambda i: hms[i].N)
+        Yo, _, _ = oracle.hgemm(hms[i], TW[i], inst[i]["nrhs"])
+        e = rel(b.instance_view(Y64, i).cpu().numpy(), Yo)
+        _say(f"  group {g} (nrhs {b.groups[g][0]}): instance {i}, N {hms[i].N}, fp64 vs oracle {e:.2e}")
+        assert e <= 1e-12
+        a = float((Y64[g] * W[g]).sum())
+        c = float((W[g] * G64[g]).sum())
+        assert abs(a - c) <= 1e-12 * float(torch.linalg.vector_norm(Y64[g]) * torch.linalg.vector_norm(W[g]))
+    worst_y = worst_g = 0.0
+    for i in range(len(hms)):
+        y64, g64 = b.instance_view(Y64, i), b.instance_view(G64, i)
+        worst_y = max(worst_y, float(torch.linalg.vector_norm(b.instance_view(Y3, i) - y64) / torch.linalg.vector_norm(y64)))
+        worst_g = max(worst_g, float(torch.linalg.vector_norm(b.instance_view(G3, i) - g64) / torch.linalg.vector_norm(g64)))
+    _say(f"  bf16x3 vs fp64, worst instance: forward {worst_y:.2e} adjoint {worst_g:.2e}")
+    assert worst_y <= 2e-5 and worst_g <= 2e-5
