@@ -119,23 +119,23 @@ static int add_tiles(TaskLists &L, int task, int m) {
 // second task only when both read it at about the same time.
 //
 // The BF16x3 kernel's launches of at least 64 tasks (the big ones) are
-// therefore ordered in groups of four tasks that share B operands: greedily, the task with the
-// most segments, then repeatedly the task sharing the most operands with the
-// group.  Within a group the segments are reordered so that a shared operand
-// sits at the same position in every task of the group that reads it (slot
-// by slot, the operand the most unplaced tasks still need).  Groups go to
-// eight bins of about equal work (most work first, to the least loaded bin),
-// each bin runs its groups longest task first (long lists start early), and
-// the bins are padded with empty tiles to the size of the largest, so that
-// bin x is exactly the eighth xcd_remap gives XCD x.
+// therefore ordered in groups of four tasks that share B operands: greedily,
+// the task with the most segments, then repeatedly the task sharing the most
+// operands with the group.  Within a group the segments are reordered so
+// that a shared operand sits at the same position in every task of the
+// group that reads it (slot by slot, the operand the most unplaced tasks
+// still need).  Groups go to eight bins of about equal work (most work
+// first, to the least loaded bin), each bin runs its groups longest task
+// first (long lists start early), and the bins are padded with empty tiles
+// to the size of the largest, so that bin x is exactly the eighth xcd_remap
+// gives XCD x.
 //
-// Smaller launches, and the FP64 / FP32 kernel's (64 x 64 tiles: a window
-// of 64 work items holds about two 256-row tasks, half a group; the grouping
-// measured 2-4% slower there), keep the tiles in
-// structural order (tree order: neighbouring nodes share skeleton blocks),
-// cut into eight contiguous groups of about equal work, each sorted longest
-// first and padded the same way; this spreads a few tasks' tiles over all
-// eight XCDs.
+// Smaller launches, and the FP64 / FP32 kernel's (64 x 64 tiles: a window of
+// 64 work items holds about two 256-row tasks, half a group; the grouping
+// measured 2-4% slower there), keep the tiles in structural order (tree
+// order: neighbouring nodes share skeleton blocks), cut into eight
+// contiguous groups of about equal work, each sorted longest first and
+// padded the same way; this spreads a few tasks' tiles over all eight XCDs.
 static int64_t seg_key(const Seg &g) { return ((int64_t)g.b_buf << 48) ^ g.b_row; }
 
 // Slot by slot, the B operand that the most tasks of the group still needing
