@@ -594,7 +594,7 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
     // operands of the requested products: dD = dY W^T, dB = y^ w^^T,
     // dVT = g^ [W | w^]^T + u^ [dY | y^]^T
-    const bool need_dw = dD || dVT, need_skel_w = dB || dVT, need_skel_u = dVT != nullptr;
+    const bool need_w_dy = dD || dVT, need_skel_w = dB || dVT, need_skel_u = dVT != nullptr;
     if (grads && x3) {
         ensure_kfrag(P, nrhs);
         if (wskel && uskel) {   // caller FP32 skeletons
@@ -629,7 +629,7 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     if (x3) {
         prof_mark(P, st, MX_KIND_CONVERT, true);
         kpack_skel_panels(P, DIR_ADJ, nrhs, need_skel_w ? GB_YHAT : -1, need_skel_u ? GB_GHAT : -1, st);
-        if (need_dw) {
+        if (need_w_dy) {
             if (io == MX_IO_F64) {
                 kpack_cols<double>(P, W, true, GB_W, nrhs, st);
                 kpack_cols<double>(P, dY, true, GB_DY, nrhs, st);
