@@ -64,3 +64,27 @@ def test_hgemm_before_configure_fails_cleanly():
     args = [None] * 12 + [4] + [None] * 19
     lib.HGEMM.restype = ctypes.c_int32
     assert lib.HGEMM(*args) == -1
+
+
+def _enum_values(prefix):
+    text = open(os.path.join(ROOT, "include", "matrox_hgemm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"\b(" + prefix + r"[A-Za-z0-9_]*)\s*=\s*(\d+)", text)}
+
+
+def test_binding_constants_match_header():
+    """The Python binding's precision, I/O, schedule, kernel-variant and
+    launch-kind numbers are the header's."""
+    from matrox_ru_amd import hgemm
+    prec = _enum_values("MX_(?:FP64|FP32|BF16X3)")
+    assert (prec["MX_FP64"], prec["MX_FP32"], prec["MX_BF16X3"]) == (hgemm.FP64, hgemm.FP32, hgemm.BF16X3)
+    io = _enum_values("MX_IO_")
+    assert (io["MX_IO_F64"], io["MX_IO_F32"]) == (hgemm.IO_F64, hgemm.IO_F32)
+    sched = _enum_values("MX_SCHED_")
+    assert sorted(sched.values()) == sorted(hgemm.SCHEDULES.values())
+    assert sched["MX_SCHED_FUSED"] == hgemm.SCHEDULES["fused"]
+    assert sched["MX_SCHED_SPLIT_BALANCED"] == hgemm.SCHEDULES["split_balanced"]
+    variants = _enum_values("MX_X3_")
+    assert sorted(variants.values()) == sorted(hgemm.X3_VARIANTS)
+    kinds = _enum_values("MX_KIND_")
+    assert sorted(kinds.values()) == sorted(hgemm.KIND_NAMES)
