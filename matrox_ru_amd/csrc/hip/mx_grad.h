@@ -45,13 +45,13 @@ template <typename T, typename O>
 struct GradArgs {
     const GTask *tasks;
     const GSeg *segs;
-    const int4 *tiles;        // (task, row tile, column tile, unused), 64 x 64 or GRAD_X3_TILE tiles
+    const int4 *tiles;        // (task, row tile, column tile, unused), 64 x 64 tiles
     int ntiles, nrhs;
     const T *src[GB_COUNT];
     O *out[3];                // dD, dB, dVT
 };
 
-// One launch over every tile of the gradient task list.
+// One launch over every tile of the gradient task list (FP64 or FP32 MFMA).
 template <typename T, typename O>
 void launch_grad(const GradArgs<T, O> &a, hipStream_t st);
 
