@@ -183,6 +183,7 @@ struct mx_batch {
         std::vector<void *> Y;
         hipGraphExec_t exec;
         uint64_t used;
+        uint64_t gen;   // batch_gen at capture
     };
     std::vector<GraphEntry> graphs;
     uint64_t clock = 0;
@@ -283,19 +284,49 @@ static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type 
     }
 }
 
+// Sum of the group plans' buffer generations: it changes whenever any work
+// buffer of any group is reallocated (also through mx_batch_plan).
+static uint64_t batch_gen(const mx_batch *B) {
+    uint64_t g = 0;
+    for (const mx_plan *P : B->plans) g += P->buf_gen;
+    return g;
+}
+
+static void batch_drop_stale_graphs(mx_batch *B) {
+    const uint64_t gen = batch_gen(B);
+    for (size_t i = 0; i < B->graphs.size();) {
+        if (B->graphs[i].gen != gen) {
+            HIPCHK(hipGraphExecDestroy(B->graphs[i].exec));
+            B->graphs.erase(B->graphs.begin() + (std::ptrdiff_t)i);
+        } else {
+            ++i;
+        }
+    }
+}
+
+static void batch_note_eval(mx_batch *B, int dir, mx_precision prec) {
+    for (size_t g = 0; g < B->plans.size(); ++g) note_eval(B->plans[g], dir, prec, B->group_nrhs[g]);
+}
+
+// The whole batch as one captured hipGraph (all groups, forked and joined
+// inside it), replayed while no group plan's work buffers have moved.
 static void batch_eval_graph(mx_batch *B, int dir, mx_precision prec, const void *const *W, void *const *Y,
                              mx_io_type io, hipStream_t st) {
     const int ng = (int)B->plans.size();
+    batch_drop_stale_graphs(B);
     for (auto &g : B->graphs) {
         bool same = g.dir == dir && g.prec == prec && g.io == io;
         for (int i = 0; same && i < ng; ++i) same = g.W[i] == W[i] && g.Y[i] == Y[i];
         if (same) {
             g.used = ++B->clock;
             HIPCHK(hipGraphLaunch(g.exec, st));
+            batch_note_eval(B, dir, prec);
             return;
         }
     }
     batch_size_work(B, dir, prec, io);
+    batch_drop_stale_graphs(B);
+    const uint64_t gen = batch_gen(B);
     for (auto *P : B->plans) P->profiling = 0;
     hipStream_t cs = st;
     bool own = false;
@@ -325,6 +356,10 @@ static void batch_eval_graph(mx_batch *B, int dir, mx_precision prec, const void
         throw;
     }
     if (own) HIPCHK(hipStreamDestroy(cs));
+    if (batch_gen(B) != gen) {   // sized above; a capture must not move them
+        (void)hipGraphExecDestroy(exec);
+        throw MxError(MX_ERR_STATE, "work buffers moved during graph capture");
+    }
     if (B->graphs.size() >= 4) {
         auto lru = std::min_element(B->graphs.begin(), B->graphs.end(),
                                     [](const mx_batch::GraphEntry &a, const mx_batch::GraphEntry &b) {
@@ -341,8 +376,10 @@ static void batch_eval_graph(mx_batch *B, int dir, mx_precision prec, const void
     e.Y.assign(Y, Y + ng);
     e.exec = exec;
     e.used = ++B->clock;
+    e.gen = gen;
     B->graphs.push_back(e);
     HIPCHK(hipGraphLaunch(exec, st));
+    batch_note_eval(B, dir, prec);
 }
 
 static void batch_drop_graphs(mx_batch *B) {

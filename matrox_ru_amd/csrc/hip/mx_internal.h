@@ -150,7 +150,13 @@ struct mx_plan {
     size_t skel_bytes = 0, skel_adj_bytes = 0;
     void *io_w = nullptr, *io_y = nullptr, *io_g = nullptr;   // FP32 staging when io is FP64
     size_t io_bytes = 0, io_g_bytes = 0;
-    int last_prec = -1;
+    // the last forward evaluation (precision, nrhs) whose skeletons the work
+    // buffers hold; -1 / 0 when none, or when its buffers were reallocated
+    int last_prec = -1, last_nrhs = 0;
+    // bumped whenever a buffer an evaluation reads or writes (wskel, uskel,
+    // their adjoint copies, io_w / io_y, panels) is freed and allocated again;
+    // graphs captured under an older generation hold freed addresses
+    uint64_t buf_gen = 0;
 
 
     // points for the dense reference rows
@@ -165,6 +171,7 @@ struct mx_plan {
         void *Y;
         hipGraphExec_t exec;
         uint64_t used;
+        uint64_t gen;   // buf_gen at capture
     };
     int use_graph = 0;
     std::vector<GraphEntry> graphs;
@@ -304,6 +311,9 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
 void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);
+// Record what an evaluation (eager, captured or replayed) leaves in the work
+// buffers: the skeletons of direction dir at nrhs columns.
+void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs);
 void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st, float *best_ms);
 void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,
                      const int32_t *Ddim, const int32_t *rows, int nrows, const double *W, int nrhs, double *out,

@@ -295,12 +295,18 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     return P.release();
 }
 
+// Work buffers grow to the largest request and are never shrunk.  Every
+// reallocation of a buffer that an evaluation reads or writes bumps
+// buf_gen, so that graphs captured with the old address are dropped before
+// the next replay (eval_graph), and forgets the skeletons the old buffer held.
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem) {
     size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
     if (sb > P->skel_adj_bytes) {
         if (P->wskel_adj) HIPCHK(hipFree(P->wskel_adj));
         if (P->uskel_adj) HIPCHK(hipFree(P->uskel_adj));
         P->wskel_adj = P->uskel_adj = nullptr;
+        P->skel_adj_bytes = 0;
+        ++P->buf_gen;
         HIPCHK(hipMalloc(&P->wskel_adj, sb));
         HIPCHK(hipMalloc(&P->uskel_adj, sb));
         P->skel_adj_bytes = sb;
@@ -323,6 +329,12 @@ void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
         if (P->wskel) HIPCHK(hipFree(P->wskel));
         if (P->uskel) HIPCHK(hipFree(P->uskel));
         P->wskel = P->uskel = nullptr;
+        P->skel_bytes = 0;
+        ++P->buf_gen;
+        if (P->last_prec != MX_BF16X3) {   // the FP64/FP32 forward skeletons are gone
+            P->last_prec = -1;
+            P->last_nrhs = 0;
+        }
         HIPCHK(hipMalloc(&P->wskel, sb));
         HIPCHK(hipMalloc(&P->uskel, sb));
         P->skel_bytes = sb;
@@ -333,6 +345,8 @@ void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
             if (P->io_w) HIPCHK(hipFree(P->io_w));
             if (P->io_y) HIPCHK(hipFree(P->io_y));
             P->io_w = P->io_y = nullptr;
+            P->io_bytes = 0;
+            ++P->buf_gen;
             HIPCHK(hipMalloc(&P->io_w, ib));
             HIPCHK(hipMalloc(&P->io_y, ib));
             P->io_bytes = ib;
@@ -345,8 +359,23 @@ void ensure_panels(mx_plan *P, int nrhs) {
     if (bytes > P->panel_bytes) {
         if (P->panels) HIPCHK(hipFree(P->panels));
         P->panels = nullptr;
+        P->panel_bytes = 0;
+        ++P->buf_gen;
+        P->panels_dir = -1;   // the split skeleton panels are gone
+        P->panels_nrhs = 0;
         HIPCHK(hipMalloc(&P->panels, bytes));
         P->panel_bytes = bytes;
+    }
+}
+
+void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs) {
+    if (prec == MX_BF16X3) {   // forward and adjoint share the panels
+        P->panels_dir = dir;
+        P->panels_nrhs = nrhs;
+    }
+    if (dir == DIR_FWD) {
+        P->last_prec = prec;
+        P->last_nrhs = nrhs;
     }
 }
 
@@ -504,11 +533,9 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
             launch_x3(xa, v, st);
             prof_mark(P, st, MX_KIND_BF16X3, false);
         }
-        P->panels_dir = dir;
-        P->panels_nrhs = nrhs;
     }
     HIPCHK(hipGetLastError());
-    if (dir == DIR_FWD) P->last_prec = prec;
+    note_eval(P, dir, prec, nrhs);
 }
 
 // FP32 copies of the skeleton panels of the last BF16x3 evaluation (of
@@ -591,7 +618,6 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     const bool grads = dD || dB || dVT;
     const bool x3 = prec == MX_BF16X3;
-    const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
     // operands of the requested products: dD = dY W^T, dB = y^ w^^T,
     // dVT = g^ [W | w^]^T + u^ [dY | y^]^T
     const bool need_w_dy = dD || dVT, need_skel_w = dB || dVT, need_skel_u = dVT != nullptr;
@@ -609,8 +635,8 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
         }
     } else if (grads && (!wskel || !uskel)) {
         if (P->last_prec != prec) throw MxError(MX_ERR_STATE, "no forward evaluation in this precision to take skeletons from");
-        if ((size_t)P->skel_rows * nrhs * elem > P->skel_bytes)
-            throw MxError(MX_ERR_STATE, "the last forward evaluation had fewer columns");
+        if (P->last_nrhs != nrhs)
+            throw MxError(MX_ERR_STATE, "the last forward evaluation had another number of columns");
         wskel = P->wskel;
         uskel = P->uskel;
     }
@@ -955,17 +981,33 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
     drop_graphs(P);
 }
 
+// Graphs captured before the last reallocation of a work buffer write
+// through freed addresses: destroy them.
+static void drop_stale_graphs(mx_plan *P) {
+    for (size_t i = 0; i < P->graphs.size();) {
+        if (P->graphs[i].gen != P->buf_gen) {
+            HIPCHK(hipGraphExecDestroy(P->graphs[i].exec));
+            P->graphs.erase(P->graphs.begin() + (std::ptrdiff_t)i);
+        } else {
+            ++i;
+        }
+    }
+}
+
 // Evaluation through a captured hipGraph: the launch sequence of one
 // (precision, I/O type, nrhs, W, Y) is captured once and replayed; work
 // buffers are sized before the capture and the sequence holds no memsets.
-// Up to four graphs are kept per plan (least recently used is dropped).
+// A graph is replayed only while the work buffers it was captured with are
+// still allocated (buf_gen unchanged).  Up to four graphs are kept per plan
+// (least recently used is dropped).
 static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
                        hipStream_t st) {
+    drop_stale_graphs(P);
     for (auto &g : P->graphs) {
         if (g.dir == dir && g.prec == prec && g.io == io && g.nrhs == nrhs && g.W == W && g.Y == Y) {
             g.used = ++P->graph_clock;
             HIPCHK(hipGraphLaunch(g.exec, st));
-            if (dir == DIR_FWD) P->last_prec = prec;
+            note_eval(P, dir, prec, nrhs);
             return;
         }
     }
@@ -975,6 +1017,8 @@ static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, vo
         ensure_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
         if (dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
     }
+    drop_stale_graphs(P);
+    const uint64_t gen = P->buf_gen;
     const int prof = P->profiling;
     P->profiling = 0;
     hipStream_t cs = st;
@@ -1007,6 +1051,10 @@ static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, vo
     }
     P->profiling = prof;
     if (own) HIPCHK(hipStreamDestroy(cs));
+    if (P->buf_gen != gen) {   // the buffers were sized above; a capture must not move them
+        (void)hipGraphExecDestroy(exec);
+        throw MxError(MX_ERR_STATE, "work buffers moved during graph capture");
+    }
     if (P->graphs.size() >= 4) {
         auto lru = std::min_element(P->graphs.begin(), P->graphs.end(),
                                     [](const mx_plan::GraphEntry &a, const mx_plan::GraphEntry &b) {
@@ -1015,9 +1063,9 @@ static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, vo
         HIPCHK(hipGraphExecDestroy(lru->exec));
         P->graphs.erase(lru);
     }
-    P->graphs.push_back(mx_plan::GraphEntry{dir, prec, io, nrhs, W, Y, exec, ++P->graph_clock});
+    P->graphs.push_back(mx_plan::GraphEntry{dir, prec, io, nrhs, W, Y, exec, ++P->graph_clock, gen});
     HIPCHK(hipGraphLaunch(exec, st));
-    P->last_prec = prec;
+    note_eval(P, dir, prec, nrhs);
 }
 
 // Kr (nrows x N, column-major) = exp(-|x_rows[r] - x_j|^2 / (2 h^2)), points in
@@ -1397,9 +1445,10 @@ mx_status mx_plan_copy_skeletons(mx_plan *plan, void *wskel_dst, void *uskel_dst
             export_panels(plan, DIR_FWD, nrhs, (float *)wskel_dst, (float *)uskel_dst, (hipStream_t)stream);
             return 0;
         }
+        if (plan->last_nrhs != nrhs)
+            throw MxError(MX_ERR_STATE, "the last forward evaluation had another number of columns");
         size_t elem = plan->last_prec == MX_FP64 ? sizeof(double) : sizeof(float);
         size_t bytes = (size_t)plan->skel_rows * nrhs * elem;
-        arg_check(bytes <= plan->skel_bytes, "nrhs exceeds the last evaluation's buffers");
         HIPCHK(hipMemcpyAsync(wskel_dst, plan->wskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
         HIPCHK(hipMemcpyAsync(uskel_dst, plan->uskel, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
         return 0;
