@@ -135,6 +135,30 @@ mx_status mx_plan_destroy(mx_plan *plan);
 mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
                   void *stream);
 
+/* Size of the static work area of mx_eval_noalloc, per GPU (bytes). */
+#define MX_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* mx_eval without any device allocation: no counterpart in the reference
+ * (whose generated HGEMM takes caller-owned work buffers); for callers that
+ * cannot allocate device memory — code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc.  Same arguments, kernels and
+ * results as mx_eval (forward direction), but the work buffers (skeletons,
+ * the FP32 staging of FP64 I/O, the split panels) come from one static
+ * device array of MX_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends, instead of the plan's own buffers.  Needs, in bytes, about
+ *     FP64:    16 * skel_rows * nrhs
+ *     FP32:     8 * skel_rows * nrhs  (+ 8 * N * nrhs with FP64 I/O)
+ *     BF16x3:   4 * (padded N + 2 * padded skel_rows) * nrhs
+ * (each part rounded up to 256 bytes); a larger evaluation returns
+ * MX_ERR_ARG with the size it needs and does nothing.  Every call on one
+ * GPU uses the same array, which keeps the last call's contents: calls that
+ * overlap in time on one GPU (other streams, other threads) race.  The
+ * plan's own work buffers and skeleton state are not touched, so
+ * mx_backward and mx_plan_copy_skeletons do not see these evaluations.
+ * Launch timing (mx_plan_set_profiling) does not record them. */
+mx_status mx_eval_noalloc(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
+                          void *stream);
+
 /* Adjoint evaluation dW = K~^T dY (same layouts and rules as mx_eval): the
  * upward pass of dY, couplings grouped by their column node through B^T,
  * downward pass, near blocks grouped by their column leaf through D^T.  This

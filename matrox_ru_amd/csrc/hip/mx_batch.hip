@@ -274,13 +274,7 @@ static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type 
         mx_plan *P = B->plans[g];
         if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
         if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
-        const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
-        if (prec == MX_BF16X3) {
-            ensure_panels(P, B->group_nrhs[g]);
-        } else {
-            ensure_work(P, B->group_nrhs[g], elem, io == MX_IO_F64);
-            if (dir == DIR_ADJ) ensure_adj_work(P, B->group_nrhs[g], elem);
-        }
+        size_work(P, dir, prec, io, B->group_nrhs[g]);
     }
 }
 

@@ -308,7 +308,18 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
 // evaluation in this precision)
 void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void *dW, int nrhs, mx_io_type io,
               const void *wskel, const void *uskel, void *dD, void *dB, void *dVT, hipStream_t st);
-void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);
+// Work buffers of one evaluation: the FP64/FP32 skeletons of its direction,
+// the FP32 staging of FP64 I/O (FP32 compute), the split panels (BF16x3).
+struct Work {
+    void *wsk = nullptr, *usk = nullptr;
+    void *io_w = nullptr, *io_y = nullptr;
+    uint16_t *panels = nullptr;
+    bool prof = false;   // record launch events
+};
+// Size the plan's own work buffers for one evaluation (may reallocate).
+void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs);
+void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);   // forward skeletons (+ staging)
+void ensure_io(mx_plan *P, int nrhs);                                 // FP32 staging of FP64 W and Y
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);
 // Record what an evaluation (eager, captured or replayed) leaves in the work

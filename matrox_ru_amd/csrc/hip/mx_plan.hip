@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <tuple>
 
 namespace mx {
@@ -339,18 +340,20 @@ void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io) {
         HIPCHK(hipMalloc(&P->uskel, sb));
         P->skel_bytes = sb;
     }
-    if (need_io) {
-        size_t ib = (size_t)std::max<int64_t>(P->N, 1) * nrhs * sizeof(float);
-        if (ib > P->io_bytes) {
-            if (P->io_w) HIPCHK(hipFree(P->io_w));
-            if (P->io_y) HIPCHK(hipFree(P->io_y));
-            P->io_w = P->io_y = nullptr;
-            P->io_bytes = 0;
-            ++P->buf_gen;
-            HIPCHK(hipMalloc(&P->io_w, ib));
-            HIPCHK(hipMalloc(&P->io_y, ib));
-            P->io_bytes = ib;
-        }
+    if (need_io) ensure_io(P, nrhs);
+}
+
+void ensure_io(mx_plan *P, int nrhs) {
+    size_t ib = (size_t)std::max<int64_t>(P->N, 1) * nrhs * sizeof(float);
+    if (ib > P->io_bytes) {
+        if (P->io_w) HIPCHK(hipFree(P->io_w));
+        if (P->io_y) HIPCHK(hipFree(P->io_y));
+        P->io_w = P->io_y = nullptr;
+        P->io_bytes = 0;
+        ++P->buf_gen;
+        HIPCHK(hipMalloc(&P->io_w, ib));
+        HIPCHK(hipMalloc(&P->io_y, ib));
+        P->io_bytes = ib;
     }
 }
 
@@ -430,8 +433,9 @@ static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
     }
 }
 
-void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
-          hipStream_t st) {
+// Checks shared by every evaluation entry point.
+static void eval_checks(const mx_plan *P, int dir, mx_precision prec, const void *W, const void *Y, int nrhs,
+                        mx_io_type io) {
     arg_check(W && Y, "null W or Y");
     arg_check(nrhs >= 1, "nrhs must be >= 1");
     arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
@@ -440,41 +444,57 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
     if (dev != P->device) throw MxError(MX_ERR_STATE, "plan was created on another device");
     if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
+    if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
+}
+
+void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs) {
+    if (prec == MX_BF16X3) {
+        ensure_panels(P, nrhs);
+        return;
+    }
+    const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
+    const bool need_io = prec == MX_FP32 && io == MX_IO_F64;
+    if (dir == DIR_ADJ) {
+        ensure_adj_work(P, nrhs, elem);
+        if (need_io) ensure_io(P, nrhs);
+    } else {
+        ensure_work(P, nrhs, elem, need_io);
+    }
+}
+
+// The launch sequence of one evaluation on the work buffers wk (no
+// allocation, no synchronisation).
+static void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+                      hipStream_t st, const Work &wk) {
     const Program &pg0 = P->prog[dir][0][P->sched[dir][0]];
     const Program &pg1 = P->prog[dir][1][P->sched[dir][1]];
     const int64_t nel = P->N * (int64_t)nrhs;
-    if (prec != MX_BF16X3 && dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
-    void *wsk = dir == DIR_ADJ ? P->wskel_adj : P->wskel;
-    void *usk = dir == DIR_ADJ ? P->uskel_adj : P->uskel;
-    if (!P->keep_prof) P->ev_used = 0;
+    auto mark = [&](int kind, bool begin) {
+        if (wk.prof) prof_mark(P, st, kind, begin);
+    };
     if (prec == MX_FP64) {
-        if (io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
-        ensure_work(P, nrhs, sizeof(double), false);
-        if (dir == DIR_FWD) wsk = P->wskel, usk = P->uskel;
         Bufs<double> b;
         b.mat[MAT_D] = P->D64;
         b.mat[MAT_B] = P->B64;
         b.mat[MAT_VT] = P->VT64;
         b.vec[VEC_W] = const_cast<double *>(static_cast<const double *>(W));
-        b.vec[VEC_WSKEL] = (double *)wsk;
-        b.vec[VEC_USKEL] = (double *)usk;
+        b.vec[VEC_WSKEL] = (double *)wk.wsk;
+        b.vec[VEC_USKEL] = (double *)wk.usk;
         b.vec[VEC_Y] = (double *)Y;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
-            prof_mark(P, st, MX_KIND_SEG_F64, true);
+            mark(MX_KIND_SEG_F64, true);
             run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st);
-            prof_mark(P, st, MX_KIND_SEG_F64, false);
+            mark(MX_KIND_SEG_F64, false);
         }
     } else if (prec == MX_FP32) {
-        ensure_work(P, nrhs, sizeof(float), io == MX_IO_F64);
-        if (dir == DIR_FWD) wsk = P->wskel, usk = P->uskel;
         const float *Wf;
         float *Yf;
         if (io == MX_IO_F64) {
-            prof_mark(P, st, MX_KIND_CONVERT, true);
-            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)P->io_w, nel);
-            prof_mark(P, st, MX_KIND_CONVERT, false);
-            Wf = (const float *)P->io_w;
-            Yf = (float *)P->io_y;
+            mark(MX_KIND_CONVERT, true);
+            cvt_f64_f32<<<cvt_grid(nel), 256, 0, st>>>((const double *)W, (float *)wk.io_w, nel);
+            mark(MX_KIND_CONVERT, false);
+            Wf = (const float *)wk.io_w;
+            Yf = (float *)wk.io_y;
         } else {
             Wf = (const float *)W;
             Yf = (float *)Y;
@@ -484,21 +504,20 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
         b.mat[MAT_B] = P->B32;
         b.mat[MAT_VT] = P->VT32;
         b.vec[VEC_W] = const_cast<float *>(Wf);
-        b.vec[VEC_WSKEL] = (float *)wsk;
-        b.vec[VEC_USKEL] = (float *)usk;
+        b.vec[VEC_WSKEL] = (float *)wk.wsk;
+        b.vec[VEC_USKEL] = (float *)wk.usk;
         b.vec[VEC_Y] = Yf;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
-            prof_mark(P, st, MX_KIND_SEG_F32, true);
+            mark(MX_KIND_SEG_F32, true);
             run_seg_launch<float>(pg0, i, nrhs, b, seg_variant(P, dir, 1, i), st);
-            prof_mark(P, st, MX_KIND_SEG_F32, false);
+            mark(MX_KIND_SEG_F32, false);
         }
         if (io == MX_IO_F64) {
-            prof_mark(P, st, MX_KIND_CONVERT, true);
+            mark(MX_KIND_CONVERT, true);
             cvt_f32_f64<<<cvt_grid(nel), 256, 0, st>>>(Yf, (double *)Y, nel);
-            prof_mark(P, st, MX_KIND_CONVERT, false);
+            mark(MX_KIND_CONVERT, false);
         }
     } else {
-        ensure_panels(P, nrhs);
         const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
         X3Args xa{};
         xa.nrhs = nrhs;
@@ -511,16 +530,16 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
             xa.ahi[w] = P->xhi[w];
             xa.alo[w] = P->xlo[w];
         }
-        uint16_t *pw = P->panels;
+        uint16_t *pw = wk.panels;
         xa.phi[VEC_W] = pw;
         xa.plo[VEC_W] = pw + wp;
         xa.phi[VEC_WSKEL] = pw + 2 * wp;
         xa.plo[VEC_WSKEL] = pw + 2 * wp + sp;
         xa.phi[VEC_USKEL] = pw + 2 * wp + 2 * sp;
         xa.plo[VEC_USKEL] = pw + 2 * wp + 3 * sp;
-        prof_mark(P, st, MX_KIND_CONVERT, true);
+        mark(MX_KIND_CONVERT, true);
         launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
-        prof_mark(P, st, MX_KIND_CONVERT, false);
+        mark(MX_KIND_CONVERT, false);
         for (size_t i = 0; i < pg1.launches.size(); ++i) {
             const int v = lv.size() == pg1.launches.size() ? lv[i] : P->x3_variant;
             const int bn = x3_variant_bn(v);
@@ -529,12 +548,27 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
             xa.n_ntiles = (nrhs + bn - 1) / bn;
             xa.tiles = (pairs ? pg1.tiles2 : pg1.tiles) + ln.tile_offset;
             xa.ntiles = ln.ntiles;
-            prof_mark(P, st, MX_KIND_BF16X3, true);
+            mark(MX_KIND_BF16X3, true);
             launch_x3(xa, v, st);
-            prof_mark(P, st, MX_KIND_BF16X3, false);
+            mark(MX_KIND_BF16X3, false);
         }
     }
     HIPCHK(hipGetLastError());
+}
+
+void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+          hipStream_t st) {
+    eval_checks(P, dir, prec, W, Y, nrhs, io);
+    size_work(P, dir, prec, io, nrhs);
+    Work wk;
+    wk.wsk = dir == DIR_ADJ ? P->wskel_adj : P->wskel;
+    wk.usk = dir == DIR_ADJ ? P->uskel_adj : P->uskel;
+    wk.io_w = P->io_w;
+    wk.io_y = P->io_y;
+    wk.panels = P->panels;
+    wk.prof = P->profiling != 0;
+    if (!P->keep_prof) P->ev_used = 0;
+    eval_core(P, dir, prec, W, Y, nrhs, io, st, wk);
     note_eval(P, dir, prec, nrhs);
 }
 
@@ -1011,12 +1045,8 @@ static void eval_graph(mx_plan *P, int dir, mx_precision prec, const void *W, vo
             return;
         }
     }
-    if (prec == MX_BF16X3) {
-        ensure_panels(P, nrhs);
-    } else {
-        ensure_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float), io == MX_IO_F64);
-        if (dir == DIR_ADJ) ensure_adj_work(P, nrhs, prec == MX_FP64 ? sizeof(double) : sizeof(float));
-    }
+    eval_checks(P, dir, prec, W, Y, nrhs, io);
+    size_work(P, dir, prec, io, nrhs);
     drop_stale_graphs(P);
     const uint64_t gen = P->buf_gen;
     const int prof = P->profiling;
@@ -1289,6 +1319,79 @@ static int32_t compat_eval(bool hss, double *D, double *B, double *VT, uint64_t 
     return 0;
 }
 
+// ---- evaluation without allocation (mx_eval_noalloc) ----------------------
+//
+// Every work buffer of the forward evaluation is carved out of one
+// file-scope device array of MX_NOALLOC_SCRATCH_BYTES.  Each GPU the library
+// runs on holds its own copy until the process ends; its address on the
+// current device comes from hipGetSymbolAddress, taken once per device on the
+// first call there (host code never uses the array's name as an address).
+// Calls on one GPU share the array: calls that overlap in time race.  No
+// part of the array needs clearing: every skeleton block and panel row
+// (padding included) is written before it is read in every evaluation.
+
+static __device__ __align__(256) unsigned char scratch[MX_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, void *> g_scratch_ptr;   // device -> address of its scratch
+
+static void *scratch_base(int dev) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        throw MxError(MX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+    return ptr;
+}
+
+static size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Work buffers of a forward evaluation laid out in the scratch area; returns
+// the bytes needed (nothing is written when base is null).
+static size_t noalloc_layout(const mx_plan *P, mx_precision prec, mx_io_type io, int nrhs, unsigned char *base,
+                             Work *wk) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        void *p = base ? base + off : nullptr;
+        off += round256(std::max<size_t>(bytes, 1));
+        return p;
+    };
+    if (prec == MX_BF16X3) {
+        wk->panels = (uint16_t *)take(((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t));
+    } else {
+        const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
+        const size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
+        wk->wsk = take(sb);
+        wk->usk = take(sb);
+        if (prec == MX_FP32 && io == MX_IO_F64) {
+            const size_t ib = (size_t)std::max<int64_t>(P->N, 1) * nrhs * sizeof(float);
+            wk->io_w = take(ib);
+            wk->io_y = take(ib);
+        }
+    }
+    return off;
+}
+
+static void eval_noalloc(mx_plan *P, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+                         hipStream_t st) {
+    eval_checks(P, DIR_FWD, prec, W, Y, nrhs, io);
+    Work probe;
+    const size_t need = noalloc_layout(P, prec, io, nrhs, nullptr, &probe);
+    if (need > MX_NOALLOC_SCRATCH_BYTES)
+        throw MxError(MX_ERR_ARG, "mx_eval_noalloc: this evaluation needs " + std::to_string(need) +
+                                      " bytes of work space; the static scratch holds " +
+                                      std::to_string((size_t)MX_NOALLOC_SCRATCH_BYTES));
+    int dev;
+    HIPCHK(hipGetDevice(&dev));
+    Work wk;
+    noalloc_layout(P, prec, io, nrhs, static_cast<unsigned char *>(scratch_base(dev)), &wk);
+    wk.prof = false;
+    eval_core(P, DIR_FWD, prec, W, Y, nrhs, io, st, wk);
+}
+
 }  // namespace mx
 
 using namespace mx;
@@ -1382,6 +1485,16 @@ mx_status mx_eval(mx_plan *plan, mx_precision prec, const void *W, void *Y, int3
         arg_check(W && Y && nrhs >= 1, "bad W, Y or nrhs");
         if (plan->use_graph && !plan->profiling) eval_graph(plan, DIR_FWD, prec, W, Y, nrhs, io, (hipStream_t)stream);
         else eval(plan, DIR_FWD, prec, W, Y, nrhs, io, (hipStream_t)stream);
+        return 0;
+    });
+}
+
+mx_status mx_eval_noalloc(mx_plan *plan, mx_precision prec, const void *W, void *Y, int32_t nrhs, mx_io_type io,
+                          void *stream) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        arg_check(prec == MX_FP64 || prec == MX_FP32 || prec == MX_BF16X3, "bad precision");
+        eval_noalloc(plan, prec, W, Y, nrhs, io, (hipStream_t)stream);
         return 0;
     });
 }

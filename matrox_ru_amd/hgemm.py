@@ -23,6 +23,7 @@ DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5"}
 IO_F64, IO_F32 = 0, 1
+NOALLOC_SCRATCH_BYTES = 256 << 20   # MX_NOALLOC_SCRATCH_BYTES
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
 P = ctypes.c_void_p
@@ -69,6 +70,8 @@ def lib():
         L.mx_plan_destroy.restype = ctypes.c_int
         L.mx_eval.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
         L.mx_eval.restype = ctypes.c_int
+        L.mx_eval_noalloc.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
+        L.mx_eval_noalloc.restype = ctypes.c_int
         L.mx_eval_adjoint.argtypes = [P, ctypes.c_int, P, P, I32, ctypes.c_int, P]
         L.mx_eval_adjoint.restype = ctypes.c_int
         L.mx_backward.argtypes = [P, ctypes.c_int, P, P, P, I32, ctypes.c_int, P, P, P, P, P, P]
@@ -246,6 +249,19 @@ class Plan:
         io = IO_F64 if W.dtype == torch.float64 else IO_F32
         _check(lib().mx_eval(self._h, PRECISIONS[precision], W.data_ptr(), Y.data_ptr(), nrhs, io,
                              _stream_handle(stream)))
+        return Y
+
+    def eval_noalloc(self, W, Y, nrhs, precision="fp64", stream=None):
+        """Y = K~ W through mx_eval_noalloc: the same evaluation as ``eval``
+        with its work buffers in the library's static scratch array (no
+        device allocation; calls overlapping on one GPU race)."""
+        import torch
+        assert W.is_cuda and Y.is_cuda and W.is_contiguous() and Y.is_contiguous()
+        assert W.numel() == self.rows * nrhs and Y.numel() == self.rows * nrhs
+        assert W.dtype == Y.dtype and W.dtype in (torch.float64, torch.float32)
+        io = IO_F64 if W.dtype == torch.float64 else IO_F32
+        _check(lib().mx_eval_noalloc(self._h, PRECISIONS[precision], W.data_ptr(), Y.data_ptr(), nrhs, io,
+                                     _stream_handle(stream)))
         return Y
 
     def eval_adjoint(self, dY, dW, nrhs, precision="fp64", stream=None):

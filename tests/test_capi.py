@@ -88,3 +88,6 @@ def test_binding_constants_match_header():
     assert sorted(variants.values()) == sorted(hgemm.X3_VARIANTS)
     kinds = _enum_values("MX_KIND_")
     assert sorted(kinds.values()) == sorted(hgemm.KIND_NAMES)
+    text = open(os.path.join(ROOT, "include", "matrox_hgemm.h")).read()
+    m = re.search(r"#define MX_NOALLOC_SCRATCH_BYTES \(\(size_t\)(\d+) << (\d+)\)", text)
+    assert m and int(m.group(1)) << int(m.group(2)) == hgemm.NOALLOC_SCRATCH_BYTES
