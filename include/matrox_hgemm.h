@@ -58,6 +58,30 @@ typedef enum mx_io_type { MX_IO_F64 = 0, MX_IO_F32 = 1 } mx_io_type;
  * (symGen/HGEMM_gen.h:126,142,180,197).  Process-wide; call before HGEMM. */
 mx_status mx_hgemm_configure(int32_t nleaf, int32_t nl, int32_t fl, int32_t cdepth);
 
+/* Whether HGEMM / HGEMM_HSS keep their device plan (the uploaded structure
+ * and blocks) between calls; no counterpart in the reference, whose
+ * generated code reads the host arrays directly.  A call reuses the kept
+ * plan when it passes the same D, B and VT pointers and structure arrays of
+ * the same contents (always compared in full: they are small); W, apres,
+ * uskel and nrhs may change.
+ *   MX_HGEMM_CACHE_OFF     build and free a plan in every call;
+ *   MX_HGEMM_CACHE_VERIFY  (default) keep it, and check every call, by a
+ *                          hash of every byte of D, B and VT computed on
+ *                          the host threads while the device evaluates,
+ *                          that the blocks are unchanged (a changed block
+ *                          rebuilds the plan and evaluates again);
+ *   MX_HGEMM_CACHE_TRUST   keep it without reading D, B and VT: the caller
+ *                          promises that blocks at the same addresses hold
+ *                          the same values (or calls mx_hgemm_cache again,
+ *                          which drops the kept plan).
+ * The kept plan holds the FP64 blocks on the device until it is dropped. */
+typedef enum mx_hgemm_cache_mode {
+    MX_HGEMM_CACHE_OFF = 0,
+    MX_HGEMM_CACHE_VERIFY = 1,
+    MX_HGEMM_CACHE_TRUST = 2
+} mx_hgemm_cache_mode;
+mx_status mx_hgemm_cache(int32_t mode);
+
 /* H2-b evaluation; replaces symGen/HGEMM_gen.h:117-238.  Argument meaning,
  * layout and accumulation are the reference's; returns 0 on success or a
  * negative mx_status (the reference always returns 0). */

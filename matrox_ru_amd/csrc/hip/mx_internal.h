@@ -2,6 +2,7 @@
 //   mx_lists.hip     structure checks and per-destination task lists
 //   mx_generate.hip  kernel blocks computed on the device from the points
 //   mx_plan.hip      plans, evaluation, backward, autotuner, C-ABI
+//   mx_dropin.hip    the drop-in HGEMM / HGEMM_HSS entry points
 //   mx_batch.hip     many independent instances
 #pragma once
 #include "../../../include/matrox_hgemm.h"
@@ -55,6 +56,7 @@ struct Program {
     int2 *tiles2 = nullptr;
     std::vector<Launch> launches2;
     std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
+    int last_write[4] = {-1, -1, -1, -1};   // per VecBuf: the last launch writing it (-1: none)
     bool built = false;
     void release() {
         if (tasks) (void)hipFree(tasks);
@@ -209,6 +211,7 @@ struct TaskLists {
     std::vector<int> kinds;       // per launch: 0 = FP32/FP64 seg kernel, 1 = BF16x3 kernel
     std::vector<int> task_kind;   // per task
     std::vector<double> flops;    // per launch, algorithmic flops per nrhs column
+    int last_write[4] = {-1, -1, -1, -1};   // per VecBuf: the last launch writing it
     int dummy = -1;               // an empty task (no rows, no segments) padding launches
     std::vector<int2> tiles2;     // tiles of twice the rows (pair_tiles)
     std::vector<Launch> launches2;
@@ -315,7 +318,16 @@ struct Work {
     void *io_w = nullptr, *io_y = nullptr;
     uint16_t *panels = nullptr;
     bool prof = false;   // record launch events
+    // FP64/FP32: recorded on the stream once the last launch writing wskel
+    // (resp. uskel) is enqueued, so copies of them can start early
+    hipEvent_t after_wskel = nullptr, after_uskel = nullptr;
 };
+// Argument and state checks of an evaluation (throws).
+void eval_checks(const mx_plan *P, int dir, mx_precision prec, const void *W, const void *Y, int nrhs, mx_io_type io);
+// The launch sequence of one evaluation on the work buffers wk (no
+// allocation, no synchronisation).
+void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+               hipStream_t st, const Work &wk);
 // Size the plan's own work buffers for one evaluation (may reallocate).
 void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs);
 void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);   // forward skeletons (+ staging)

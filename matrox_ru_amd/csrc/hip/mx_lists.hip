@@ -9,6 +9,7 @@ namespace mx {
 
 void compute_launch_flops(TaskLists &L) {
     L.flops.assign(L.launches.size(), 0.0);
+    for (int &w : L.last_write) w = -1;
     std::vector<int> seen(L.tasks.size(), -1);
     for (size_t li = 0; li < L.launches.size(); ++li) {
         for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
@@ -16,6 +17,7 @@ void compute_launch_flops(TaskLists &L) {
             if (seen[task] == (int)li) continue;   // count each task once per launch
             seen[task] = (int)li;
             const Task &tk = L.tasks[task];
+            if (tk.seg_end > tk.seg_begin) L.last_write[tk.c_buf] = (int)li;
             for (int sg = tk.seg_begin; sg < tk.seg_end; ++sg) L.flops[li] += 2.0 * tk.m * (double)L.segs[sg].k;
         }
     }

@@ -106,6 +106,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pg.tiles2 = upload(L.tiles2.data(), L.tiles2.size());
             pg.launches2 = L.launches2;
             pg.flops = L.flops;
+            std::copy(L.last_write, L.last_write + 4, pg.last_write);
             pg.built = true;
         }
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
@@ -179,6 +180,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pg.tiles2 = upload(X.tiles2.data(), X.tiles2.size());
             pg.launches2 = X.launches2;
             pg.flops = X.flops;
+            std::copy(X.last_write, X.last_write + 4, pg.last_write);
             pg.built = true;
         }
         for (int w = 0; w < 3; ++w) P->x3_elems[w] = next[w];
@@ -434,8 +436,8 @@ static void prof_mark(mx_plan *P, hipStream_t st, int kind, bool begin) {
 }
 
 // Checks shared by every evaluation entry point.
-static void eval_checks(const mx_plan *P, int dir, mx_precision prec, const void *W, const void *Y, int nrhs,
-                        mx_io_type io) {
+void eval_checks(const mx_plan *P, int dir, mx_precision prec, const void *W, const void *Y, int nrhs,
+                 mx_io_type io) {
     arg_check(W && Y, "null W or Y");
     arg_check(nrhs >= 1, "nrhs must be >= 1");
     arg_check(io == MX_IO_F64 || io == MX_IO_F32, "bad io type");
@@ -462,15 +464,17 @@ void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs) 
     }
 }
 
-// The launch sequence of one evaluation on the work buffers wk (no
-// allocation, no synchronisation).
-static void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
-                      hipStream_t st, const Work &wk) {
+void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
+               hipStream_t st, const Work &wk) {
     const Program &pg0 = P->prog[dir][0][P->sched[dir][0]];
     const Program &pg1 = P->prog[dir][1][P->sched[dir][1]];
     const int64_t nel = P->N * (int64_t)nrhs;
     auto mark = [&](int kind, bool begin) {
         if (wk.prof) prof_mark(P, st, kind, begin);
+    };
+    auto phase = [&](size_t i) {   // events after the last writer of each skeleton buffer
+        if (wk.after_wskel && (int)i == pg0.last_write[VEC_WSKEL]) HIPCHK(hipEventRecord(wk.after_wskel, st));
+        if (wk.after_uskel && (int)i == pg0.last_write[VEC_USKEL]) HIPCHK(hipEventRecord(wk.after_uskel, st));
     };
     if (prec == MX_FP64) {
         Bufs<double> b;
@@ -485,6 +489,7 @@ static void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, voi
             mark(MX_KIND_SEG_F64, true);
             run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st);
             mark(MX_KIND_SEG_F64, false);
+            phase(i);
         }
     } else if (prec == MX_FP32) {
         const float *Wf;
@@ -511,6 +516,7 @@ static void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, voi
             mark(MX_KIND_SEG_F32, true);
             run_seg_launch<float>(pg0, i, nrhs, b, seg_variant(P, dir, 1, i), st);
             mark(MX_KIND_SEG_F32, false);
+            phase(i);
         }
         if (io == MX_IO_F64) {
             mark(MX_KIND_CONVERT, true);
@@ -1186,139 +1192,6 @@ static void dense_rows(mx_plan *P, const int32_t *rows, int nrows, const double 
                     nrhs, out, st);
 }
 
-// ---- drop-in entry points -------------------------------------------------
-
-struct CompatDims {
-    bool set = false;
-    int32_t nleaf = 0, nl = 0, fl = 0, cdepth = 0;
-};
-static CompatDims g_dims;
-
-static std::vector<int64_t> visit_blocks(const int32_t *set, const int32_t *blocks, int32_t rows) {
-    std::vector<int64_t> ks;
-    for (int i = 0; i < rows; ++i)
-        for (int j = set[i]; j < set[i + 1]; ++j)
-            for (int k = blocks[j]; k < blocks[j + 1]; ++k) ks.push_back(k);
-    return ks;
-}
-
-static int32_t compat_eval(bool hss, double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr,
-                           int32_t *VTptr, int32_t *lc, int32_t *rc, double *mrhs, double *apres, int32_t nrhs,
-                           int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel, int32_t *wso,
-                           double *uskel, int32_t *uso, int32_t *lm, int32_t *slen, int32_t *nblockSet,
-                           int32_t *nblocks, int32_t *npairx, int32_t *npairy, int32_t *fblockSet,
-                           int32_t *fblocks, int32_t *fpairx, int32_t *fpairy) {
-    if (!g_dims.set) throw MxError(MX_ERR_STATE, "call mx_hgemm_configure before HGEMM");
-    arg_check(nrhs >= 1, "nrhs must be >= 1");
-    arg_check(D && B && VT && Dptr && Bptr && VTptr && lc && rc && mrhs && apres && Ddim && wptr && uptr &&
-                  wskel && wso && uskel && uso && lm && slen,
-              "null argument");
-    const int nleaf = g_dims.nleaf, nn = 2 * nleaf - 1;
-    mx_structure s{};
-    s.nleaf = nleaf;
-    s.numnodes = nn;
-    s.is_hss = hss ? 1 : 0;
-    s.lchildren = lc;
-    s.rchildren = rc;
-    s.lm = lm;
-    s.Ddim = Ddim;
-    s.slen = slen;
-    s.VTptr = VTptr;
-    std::vector<int32_t> wrow(nleaf), skelrow(nn);
-    for (int l = 0; l < nleaf; ++l) {
-        arg_check(wptr[l] % nrhs == 0 && uptr[l] == wptr[l], "wptr/uptr must be equal multiples of nrhs");
-        wrow[l] = wptr[l] / nrhs;
-    }
-    for (int v = 0; v < nn; ++v) {
-        arg_check(wso[v] % nrhs == 0 && uso[v] == wso[v], "skeleton offsets must be equal multiples of nrhs");
-        skelrow[v] = wso[v] / nrhs;
-    }
-    s.wrow = wrow.data();
-    s.skelrow = skelrow.data();
-    std::vector<int32_t> px, py, fx, fy;
-    std::vector<uint64_t> dp, bp;
-    if (hss) {
-        s.ncount = nleaf;
-        s.fcount = nn - 1;
-        s.Dptr = Dptr;
-        s.Bptr = Bptr;
-    } else {
-        arg_check(nblockSet && nblocks && npairx && npairy && fblockSet && fblocks && fpairx && fpairy,
-                  "null blocking array");
-        for (int64_t k : visit_blocks(nblockSet, nblocks, g_dims.nl)) {
-            px.push_back(npairx[k]);
-            py.push_back(npairy[k]);
-            dp.push_back(Dptr[k]);
-        }
-        for (int64_t k : visit_blocks(fblockSet, fblocks, g_dims.fl)) {
-            fx.push_back(fpairx[k]);
-            fy.push_back(fpairy[k]);
-            bp.push_back(Bptr[k]);
-        }
-        s.ncount = (int64_t)px.size();
-        s.fcount = (int64_t)fx.size();
-        s.npairx = px.data();
-        s.npairy = py.data();
-        s.fpairx = fx.data();
-        s.fpairy = fy.data();
-        s.Dptr = dp.data();
-        s.Bptr = bp.data();
-    }
-    // buffer extents implied by the offsets
-    int64_t dl = 0, bl = 0, vl = 0;
-    for (int64_t k = 0; k < s.ncount; ++k) {
-        int x = hss ? (int)k : s.npairx[k], y = hss ? (int)k : s.npairy[k];
-        arg_check(x >= 0 && x < nleaf && y >= 0 && y < nleaf, "near pair leaf out of range");
-        dl = std::max<int64_t>(dl, (int64_t)s.Dptr[k] + (int64_t)Ddim[x] * Ddim[y]);
-    }
-    for (int64_t k = 0; k < s.fcount; ++k) {
-        int x, y;
-        if (hss) {
-            x = (int)k + 1;
-            y = (x & 1) ? x + 1 : x - 1;
-        } else {
-            x = s.fpairx[k];
-            y = s.fpairy[k];
-        }
-        arg_check(x > 0 && x < nn && y > 0 && y < nn, "far pair node out of range");
-        bl = std::max<int64_t>(bl, (int64_t)s.Bptr[k] + (int64_t)slen[x] * slen[y]);
-    }
-    for (int v = 1; v < nn; ++v) {
-        arg_check((lc[v] == -1) == (rc[v] == -1), "node with one child");
-        if (lc[v] == -1) arg_check(lm[v] >= 0 && lm[v] < nleaf, "leaf map out of range");
-        else arg_check(lc[v] > 0 && lc[v] < nn && rc[v] > 0 && rc[v] < nn, "child out of range");
-        int cols = lc[v] == -1 ? Ddim[lm[v]] : slen[lc[v]] + slen[rc[v]];
-        vl = std::max<int64_t>(vl, (int64_t)VTptr[v] + (int64_t)slen[v] * cols);
-    }
-    s.D = D;
-    s.B = B;
-    s.VT = VT;
-    s.D_len = dl;
-    s.B_len = bl;
-    s.VT_len = vl;
-    int beta = hss ? 0 : 1;
-    std::unique_ptr<mx_plan> P(create_plan(s, MX_PLAN_FP64, beta, beta));
-    int64_t N = P->N, SK = P->skel_rows;
-    double *dW = upload(mrhs, (size_t)(N * nrhs));
-    double *dY = upload(apres, (size_t)(N * nrhs));
-    ensure_work(P.get(), nrhs, sizeof(double), false);
-    HIPCHK(hipMemcpy(P->uskel, uskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyHostToDevice));
-    try {
-        eval(P.get(), DIR_FWD, MX_FP64, dW, dY, nrhs, MX_IO_F64, nullptr);
-        HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMemcpy(apres, dY, (size_t)(N * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(wskel, P->wskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(uskel, P->uskel, (size_t)(SK * nrhs) * sizeof(double), hipMemcpyDeviceToHost));
-    } catch (...) {
-        (void)hipFree(dW);
-        (void)hipFree(dY);
-        throw;
-    }
-    HIPCHK(hipFree(dW));
-    HIPCHK(hipFree(dY));
-    return 0;
-}
-
 // ---- evaluation without allocation (mx_eval_noalloc) ----------------------
 //
 // Every work buffer of the forward evaluation is carved out of one
@@ -1397,51 +1270,6 @@ static void eval_noalloc(mx_plan *P, mx_precision prec, const void *W, void *Y, 
 using namespace mx;
 
 extern "C" {
-
-mx_status mx_hgemm_configure(int32_t nleaf, int32_t nl, int32_t fl, int32_t cdepth) {
-    return (mx_status)guard([&]() {
-        arg_check(nleaf >= 1 && nl >= 0 && fl >= 0 && cdepth >= 0, "bad dimensions");
-        g_dims.nleaf = nleaf;
-        g_dims.nl = nl;
-        g_dims.fl = fl;
-        g_dims.cdepth = cdepth;
-        g_dims.set = true;
-        return 0;
-    });
-}
-
-int32_t HGEMM(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
-              int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
-              double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
-              int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
-              int32_t *nblockSet, int32_t *nblocks, int32_t *npairx, int32_t *npairy, int32_t *fblockSet,
-              int32_t *fblocks, int32_t *fpairx, int32_t *fpairy, int32_t *wpart, int32_t *clevelset) {
-    (void)levelset;
-    (void)idx;
-    (void)wpart;
-    (void)clevelset;
-    return guard([&]() {
-        return compat_eval(false, D, B, VT, Dptr, Bptr, VTptr, lchildren, rchildren, mrhs, apres, nrhs, Ddim,
-                           wptr, uptr, wskel, wskeloffset, uskel, uskeloffset, lm, slen, nblockSet, nblocks,
-                           npairx, npairy, fblockSet, fblocks, fpairx, fpairy);
-    });
-}
-
-int32_t HGEMM_HSS(double *D, double *B, double *VT, uint64_t *Dptr, uint64_t *Bptr, int32_t *VTptr,
-                  int32_t *lchildren, int32_t *rchildren, int32_t *levelset, int32_t *idx, double *mrhs,
-                  double *apres, int32_t nrhs, int32_t *Ddim, int32_t *wptr, int32_t *uptr, double *wskel,
-                  int32_t *wskeloffset, double *uskel, int32_t *uskeloffset, int32_t *lm, int32_t *slen,
-                  int32_t *wpart, int32_t *clevelset) {
-    (void)levelset;
-    (void)idx;
-    (void)wpart;
-    (void)clevelset;
-    return guard([&]() {
-        return compat_eval(true, D, B, VT, Dptr, Bptr, VTptr, lchildren, rchildren, mrhs, apres, nrhs, Ddim, wptr,
-                           uptr, wskel, wskeloffset, uskel, uskeloffset, lm, slen, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, nullptr, nullptr);
-    });
-}
 
 mx_status mx_plan_create(mx_plan **plan, const mx_structure *s, uint32_t precisions) {
     return (mx_status)guard([&]() {

@@ -57,6 +57,8 @@ def lib():
         L.mx_last_error.restype = ctypes.c_char_p
         L.mx_hgemm_configure.argtypes = [I32, I32, I32, I32]
         L.mx_hgemm_configure.restype = ctypes.c_int
+        L.mx_hgemm_cache.argtypes = [I32]
+        L.mx_hgemm_cache.restype = ctypes.c_int
         L.HGEMM.argtypes = [P] * 12 + [I32] + [P] * 19
         L.HGEMM.restype = I32
         L.HGEMM_HSS.argtypes = [P] * 12 + [I32] + [P] * 11
@@ -626,6 +628,15 @@ class Batch:
             pass
 
 
+HGEMM_CACHE = {"off": 0, "verify": 1, "trust": 2}
+
+
+def set_hgemm_cache(mode):
+    """mx_hgemm_cache: whether the drop-in entry points keep their device
+    plan between calls ("off", "verify" (default) or "trust")."""
+    _check(lib().mx_hgemm_cache(HGEMM_CACHE[mode]))
+
+
 def configure(nleaf, nl, fl, cdepth):
     """mx_hgemm_configure: the loop bounds the generated HGEMM bakes in."""
     _check(lib().mx_hgemm_configure(nleaf, nl, fl, cdepth))
@@ -635,52 +646,62 @@ def _hp(x):
     return x.ctypes.data
 
 
+class Dropin:
+    """One matrix's argument list for the drop-in HGEMM / HGEMM_HSS
+    (symGen/HGEMM_gen.h:117-124, symGen/HGEMM_v1.h:117-124) at a given nrhs,
+    built once; ``__call__`` runs the entry point on host arrays (TW, apres,
+    wskel, uskel: float64, N*nrhs and skel_rows*nrhs elements), updating
+    apres, wskel and uskel in place."""
+
+    _INT = ("VTptr", "lchildren", "rchildren", "levelset", "idx", "Ddim", "lm", "slen", "nblockSet", "nblocks",
+            "npairx", "npairy", "fblockSet", "fblocks", "fpairx", "fpairy", "wpart", "clevelset")
+
+    def __init__(self, hm, nrhs, hss=None):
+        a = hm.arrays
+        self.hss = bool(hm.is_hss) if hss is None else bool(hss)
+        self.nrhs = nrhs
+        self.N = hm.N
+        self.sk = int(hm.scalars["skel_rows"]) * nrhs
+        self.dims = (int(hm.scalars["nleaf"]), 0 if self.hss else int(hm.scalars["nl"]),
+                     0 if self.hss else int(hm.scalars["fl"]), int(hm.scalars["cdepth"]))
+        self.arrs = {k: _np(a[k], np.int32) for k in self._INT}
+        self.D, self.B, self.VT = (_np(a[k], np.float64) for k in ("D", "B", "VT"))
+        self.Dptr, self.Bptr = _np(a["Dptr"], np.uint64), _np(a["Bptr"], np.uint64)
+        self.wptr = hm.wptr(nrhs)
+        self.so = hm.skeloffset(nrhs)
+
+    def __call__(self, TW, apres, wskel, uskel):
+        for x, n in ((TW, self.N * self.nrhs), (apres, self.N * self.nrhs), (wskel, self.sk), (uskel, self.sk)):
+            assert x.dtype == np.float64 and x.flags.c_contiguous and x.size >= n
+        configure(*self.dims)
+        r = self.arrs
+        head = [_hp(self.D), _hp(self.B), _hp(self.VT), _hp(self.Dptr), _hp(self.Bptr), _hp(r["VTptr"]),
+                _hp(r["lchildren"]), _hp(r["rchildren"]), _hp(r["levelset"]), _hp(r["idx"]), _hp(TW), _hp(apres),
+                self.nrhs, _hp(r["Ddim"]), _hp(self.wptr), _hp(self.wptr), _hp(wskel), _hp(self.so), _hp(uskel),
+                _hp(self.so), _hp(r["lm"]), _hp(r["slen"])]
+        if self.hss:
+            rc = lib().HGEMM_HSS(*head, _hp(r["wpart"]), _hp(r["clevelset"]))
+        else:
+            rc = lib().HGEMM(*head, *(_hp(r[k]) for k in ("nblockSet", "nblocks", "npairx", "npairy", "fblockSet",
+                                                           "fblocks", "fpairx", "fpairy", "wpart", "clevelset")))
+        _check(rc)
+
+
+def _dropin(hm, TW, nrhs, apres, uskel, hss):
+    call = Dropin(hm, nrhs, hss)
+    TW = _np(TW, np.float64)
+    Y = np.zeros(hm.N * nrhs) if apres is None else _np(apres, np.float64).copy()
+    ws = np.zeros(max(call.sk, 1))
+    us = np.zeros(max(call.sk, 1)) if uskel is None else _np(uskel, np.float64).copy()
+    call(TW, Y, ws, us)
+    return Y, ws[:call.sk], us[:call.sk]
+
+
 def hgemm(hm, TW, nrhs, apres=None, uskel=None):
     """Drop-in H2-b HGEMM on host arrays; returns (apres, wskel, uskel)."""
-    a = hm.arrays
-    configure(int(hm.scalars["nleaf"]), int(hm.scalars["nl"]), int(hm.scalars["fl"]), int(hm.scalars["cdepth"]))
-    N = hm.N
-    sk = int(hm.scalars["skel_rows"]) * nrhs
-    TW = _np(TW, np.float64)
-    Y = np.zeros(N * nrhs) if apres is None else _np(apres, np.float64).copy()
-    ws = np.zeros(max(sk, 1))
-    us = np.zeros(max(sk, 1)) if uskel is None else _np(uskel, np.float64).copy()
-    wptr = hm.wptr(nrhs)
-    so = hm.skeloffset(nrhs)
-    arrs = {k: _np(a[k], np.int32) for k in ("VTptr", "lchildren", "rchildren", "levelset", "idx", "Ddim", "lm",
-                                              "slen", "nblockSet", "nblocks", "npairx", "npairy", "fblockSet",
-                                              "fblocks", "fpairx", "fpairy", "wpart", "clevelset")}
-    D, B, VT = _np(a["D"], np.float64), _np(a["B"], np.float64), _np(a["VT"], np.float64)
-    Dptr, Bptr = _np(a["Dptr"], np.uint64), _np(a["Bptr"], np.uint64)
-    rc = lib().HGEMM(_hp(D), _hp(B), _hp(VT), _hp(Dptr), _hp(Bptr), _hp(arrs["VTptr"]), _hp(arrs["lchildren"]),
-                     _hp(arrs["rchildren"]), _hp(arrs["levelset"]), _hp(arrs["idx"]), _hp(TW), _hp(Y), nrhs,
-                     _hp(arrs["Ddim"]), _hp(wptr), _hp(wptr), _hp(ws), _hp(so), _hp(us), _hp(so), _hp(arrs["lm"]),
-                     _hp(arrs["slen"]), _hp(arrs["nblockSet"]), _hp(arrs["nblocks"]), _hp(arrs["npairx"]),
-                     _hp(arrs["npairy"]), _hp(arrs["fblockSet"]), _hp(arrs["fblocks"]), _hp(arrs["fpairx"]),
-                     _hp(arrs["fpairy"]), _hp(arrs["wpart"]), _hp(arrs["clevelset"]))
-    _check(rc)
-    return Y, ws[:sk], us[:sk]
+    return _dropin(hm, TW, nrhs, apres, uskel, False)
 
 
 def hgemm_hss(hm, TW, nrhs, apres=None, uskel=None):
     """Drop-in HSS HGEMM (HGEMM_v1.h signature) on host arrays."""
-    a = hm.arrays
-    configure(int(hm.scalars["nleaf"]), 0, 0, int(hm.scalars["cdepth"]))
-    N = hm.N
-    sk = int(hm.scalars["skel_rows"]) * nrhs
-    TW = _np(TW, np.float64)
-    Y = np.zeros(N * nrhs) if apres is None else _np(apres, np.float64).copy()
-    ws = np.zeros(max(sk, 1))
-    us = np.zeros(max(sk, 1)) if uskel is None else _np(uskel, np.float64).copy()
-    wptr = hm.wptr(nrhs)
-    so = hm.skeloffset(nrhs)
-    arrs = {k: _np(a[k], np.int32) for k in ("VTptr", "lchildren", "rchildren", "levelset", "idx", "Ddim", "lm",
-                                              "slen", "wpart", "clevelset")}
-    D, B, VT = _np(a["D"], np.float64), _np(a["B"], np.float64), _np(a["VT"], np.float64)
-    Dptr, Bptr = _np(a["Dptr"], np.uint64), _np(a["Bptr"], np.uint64)
-    rc = lib().HGEMM_HSS(_hp(D), _hp(B), _hp(VT), _hp(Dptr), _hp(Bptr), _hp(arrs["VTptr"]),
-                         _hp(arrs["lchildren"]), _hp(arrs["rchildren"]), _hp(arrs["levelset"]), _hp(arrs["idx"]),
-                         _hp(TW), _hp(Y), nrhs, _hp(arrs["Ddim"]), _hp(wptr), _hp(wptr), _hp(ws), _hp(so), _hp(us),
-                         _hp(so), _hp(arrs["lm"]), _hp(arrs["slen"]), _hp(arrs["wpart"]), _hp(arrs["clevelset"]))
-    _check(rc)
-    return Y, ws[:sk], us[:sk]
+    return _dropin(hm, TW, nrhs, apres, uskel, True)

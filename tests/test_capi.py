@@ -91,3 +91,14 @@ def test_binding_constants_match_header():
     text = open(os.path.join(ROOT, "include", "matrox_hgemm.h")).read()
     m = re.search(r"#define MX_NOALLOC_SCRATCH_BYTES \(\(size_t\)(\d+) << (\d+)\)", text)
     assert m and int(m.group(1)) << int(m.group(2)) == hgemm.NOALLOC_SCRATCH_BYTES
+
+
+def test_hgemm_cache_mode_checked_without_gpu():
+    """mx_hgemm_cache rejects an unknown mode before any device work; the
+    valid modes only drop what is kept (nothing, here)."""
+    lib = ctypes.CDLL(_native.HGEMM_LIB)
+    lib.mx_last_error.restype = ctypes.c_char_p
+    assert lib.mx_hgemm_cache(7) == -1
+    assert b"cache mode" in lib.mx_last_error()
+    for mode in (0, 2, 1):
+        assert lib.mx_hgemm_cache(mode) == 0
