@@ -358,6 +358,23 @@ mx_status mx_plan_dims(const mx_plan *plan, int64_t *N, int64_t *skel_rows);
 typedef struct mx_batch mx_batch;
 mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances, const int32_t *nrhs, int32_t count,
                           uint32_t precisions);
+
+/* Split-BF16x3 accuracy guard of a batch.  When `precisions` holds both
+ * MX_PLAN_BF16X3 and MX_PLAN_FP32, mx_batch_create estimates every
+ * instance's relative error in BF16x3: each group is evaluated once in
+ * BF16x3 and in FP64 (else FP32) on a unit-variance pseudo-random W with its
+ * nrhs, forward and, with MX_PLAN_ADJOINT, adjoint.  Instances whose
+ * estimate exceeds MX_BATCH_X3_GUARD are put in groups of their own that
+ * evaluate in FP32 whenever BF16x3 is requested (their plans hold no split
+ * blocks), so that every instance stays within 1e-5 of FP64.  The error of
+ * the split products grows with the depth of the tree and the weight of the
+ * far field: small, smooth instances can exceed 1e-5.  Add
+ * MX_BATCH_NO_X3_GUARD to `precisions` to switch the guard off. */
+#define MX_BATCH_X3_GUARD 8e-6
+#define MX_BATCH_NO_X3_GUARD (1u << 9)
+/* The guard's estimate for an instance (-1 when no estimate was made) and
+ * whether its group evaluates in FP32 when BF16x3 is requested. */
+mx_status mx_batch_instance_guard(const mx_batch *batch, int32_t instance, double *x3_err, int32_t *fp32);
 mx_status mx_batch_destroy(mx_batch *batch);
 mx_status mx_batch_groups(const mx_batch *batch, int32_t *ngroups);
 mx_status mx_batch_group_info(const mx_batch *batch, int32_t group, int32_t *nrhs, int64_t *rows);

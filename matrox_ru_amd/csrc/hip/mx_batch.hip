@@ -2,6 +2,8 @@
 // include/matrox_hgemm.h).
 #include "mx_internal.h"
 
+#include <cmath>
+
 namespace mx {
 
 // ---- many independent instances (mx_batch) -------------------------------
@@ -172,6 +174,11 @@ static void combine(const std::vector<const mx_structure *> &in, GroupStructure 
 struct mx_batch {
     std::vector<mx_plan *> plans;          // one per group
     std::vector<int> group_nrhs;
+    // groups whose instances the split-BF16x3 accuracy guard moved: they
+    // evaluate in FP32 when BF16x3 is requested (their plans hold no split
+    // blocks)
+    std::vector<int> group_fp32;
+    std::vector<double> x3_err;            // per instance: estimated BF16x3 error (-1: not estimated)
     std::vector<int64_t> group_rows;
     std::vector<hipStream_t> streams;      // one per group (when there are several)
     std::vector<hipEvent_t> events;        // fork + one join event per group
@@ -199,16 +206,20 @@ struct mx_batch {
 
 namespace mx {
 
-static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs, int count, uint32_t precisions) {
+// Instances grouped by (nrhs, fp32[i]); a group of fp32 instances is built
+// without the split-BF16x3 blocks.
+static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs, int count, uint32_t precisions,
+                              const std::vector<char> &fp32) {
     arg_check(in && nrhs && count >= 1, "bad batch arguments");
     std::unique_ptr<mx_batch> B(new mx_batch());
-    std::map<int, std::vector<int>> by_nrhs;
+    std::map<std::pair<int, int>, std::vector<int>> by_key;
     for (int i = 0; i < count; ++i) {
         arg_check(in[i] != nullptr && nrhs[i] >= 1, "instance " + std::to_string(i) + ": null structure or bad nrhs");
-        by_nrhs[nrhs[i]].push_back(i);
+        by_key[{nrhs[i], fp32[i] ? 1 : 0}].push_back(i);
     }
     B->inst.resize(count);
-    for (auto &kv : by_nrhs) {
+    B->x3_err.assign(count, -1.0);
+    for (auto &kv : by_key) {
         const int g = (int)B->plans.size();
         std::vector<const mx_structure *> members;
         int64_t row = 0;
@@ -235,9 +246,11 @@ static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs
             bi.vt_off = G.vbase[m];
         }
         mx_structure v = G.view();
-        B->plans.push_back(create_plan(v, precisions, 0, 0));
-        B->group_nrhs.push_back(kv.first);
+        const bool routed = kv.first.second != 0;
+        B->plans.push_back(create_plan(v, routed ? (precisions & ~(uint32_t)MX_PLAN_BF16X3) : precisions, 0, 0));
+        B->group_nrhs.push_back(kv.first.first);
         B->group_rows.push_back(G.N);
+        B->group_fp32.push_back(routed ? 1 : 0);
     }
     const int ng = (int)B->plans.size();
     if (ng > 1) {
@@ -249,6 +262,119 @@ static mx_batch *create_batch(const mx_structure *const *in, const int32_t *nrhs
     return B.release();
 }
 
+// The precision group g evaluates in when prec is requested.
+static mx_precision group_prec(const mx_batch *B, int g, mx_precision prec) {
+    return prec == MX_BF16X3 && B->group_fp32[g] ? MX_FP32 : prec;
+}
+
+// Per instance i of `ranges` (element ranges [b, e) of two group buffers):
+// out[2i] += sum (x - y)^2, out[2i+1] += sum y^2.
+__global__ void instance_err_kernel(const double *__restrict__ x, const double *__restrict__ y,
+                                    const int64_t *__restrict__ ranges, double *out) {
+    const int64_t b = ranges[2 * blockIdx.y], e = ranges[2 * blockIdx.y + 1];
+    double d2 = 0.0, y2 = 0.0;
+    for (int64_t k = b + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < e; k += (int64_t)gridDim.x * blockDim.x) {
+        const double t = x[k] - y[k];
+        d2 += t * t;
+        y2 += y[k] * y[k];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        d2 += __shfl_down(d2, o);
+        y2 += __shfl_down(y2, o);
+    }
+    __shared__ double red[2][8];
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) {
+        red[0][w] = d2;
+        red[1][w] = y2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int i = 0; i < (int)(blockDim.x / 64); ++i) {
+            a += red[0][i];
+            c += red[1][i];
+        }
+        atomicAdd(&out[2 * blockIdx.y], a);
+        atomicAdd(&out[2 * blockIdx.y + 1], c);
+    }
+}
+
+// Estimated relative error of every instance's split-BF16x3 evaluation
+// (forward, and adjoint when built): each group evaluated once on a
+// unit-variance pseudo-random W in BF16x3 and in the reference precision
+// (FP64 when built, else FP32, whose own error is ~1e-6 of that).
+static std::vector<double> estimate_x3_error(mx_batch *B, hipStream_t st) {
+    std::vector<double> err(B->inst.size(), 0.0);
+    for (size_t g = 0; g < B->plans.size(); ++g) {
+        mx_plan *P = B->plans[g];
+        const int n = B->group_nrhs[g];
+        const mx_precision ref = (P->precisions & MX_PLAN_FP64) ? MX_FP64 : MX_FP32;
+        std::vector<int> members;
+        std::vector<int64_t> ranges;
+        for (size_t i = 0; i < B->inst.size(); ++i)
+            if (B->inst[i].group == (int)g) {
+                members.push_back((int)i);
+                ranges.push_back(B->inst[i].row_off * n);
+                ranges.push_back((B->inst[i].row_off + B->inst[i].N) * n);
+            }
+        const int64_t nel = std::max<int64_t>(B->group_rows[g], 1) * n;
+        double *buf = nullptr, *acc = nullptr;
+        int64_t *dr = nullptr;
+        try {
+            HIPCHK(hipMalloc(&buf, 3 * (size_t)nel * sizeof(double)));
+            HIPCHK(hipMalloc(&acc, 2 * members.size() * sizeof(double)));
+            dr = upload(ranges.data(), ranges.size());
+            double *W = buf, *Y3 = buf + nel, *Yr = buf + 2 * nel;
+            launch_fill_uniform(W, nel, true, st);
+            for (int dir = 0; dir < (P->adjoint ? 2 : 1); ++dir) {
+                eval(P, dir, MX_BF16X3, W, Y3, n, MX_IO_F64, st);
+                eval(P, dir, ref, W, Yr, n, MX_IO_F64, st);
+                HIPCHK(hipMemsetAsync(acc, 0, 2 * members.size() * sizeof(double), st));
+                instance_err_kernel<<<dim3(16, (unsigned)members.size()), 256, 0, st>>>(Y3, Yr, dr, acc);
+                HIPCHK(hipGetLastError());
+                std::vector<double> h(2 * members.size());
+                HIPCHK(hipMemcpyAsync(h.data(), acc, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                for (size_t m = 0; m < members.size(); ++m)
+                    err[members[m]] = std::max(err[members[m]], h[2 * m + 1] > 0 ? std::sqrt(h[2 * m] / h[2 * m + 1])
+                                                                                  : 0.0);
+            }
+        } catch (...) {
+            for (void *p : {(void *)buf, (void *)acc, (void *)dr})
+                if (p) (void)hipFree(p);
+            throw;
+        }
+        for (void *p : {(void *)buf, (void *)acc, (void *)dr}) HIPCHK(hipFree(p));
+    }
+    return err;
+}
+
+// Build the batch; with BF16x3 and FP32 both requested (and the guard not
+// switched off), estimate every instance's BF16x3 error and rebuild with the
+// instances above MX_BATCH_X3_GUARD in FP32 groups.
+static mx_batch *create_guarded_batch(const mx_structure *const *in, const int32_t *nrhs, int count,
+                                      uint32_t precisions) {
+    const bool guard = (precisions & MX_PLAN_BF16X3) && (precisions & MX_PLAN_FP32) &&
+                       !(precisions & MX_BATCH_NO_X3_GUARD);
+    precisions &= ~(uint32_t)MX_BATCH_NO_X3_GUARD;
+    std::vector<char> fp32(std::max(count, 0), 0);
+    std::unique_ptr<mx_batch> B(create_batch(in, nrhs, count, precisions, fp32));
+    if (!guard) return B.release();
+    const std::vector<double> err = estimate_x3_error(B.get(), nullptr);
+    bool any = false;
+    for (int i = 0; i < count; ++i) {
+        fp32[i] = err[i] > MX_BATCH_X3_GUARD;
+        any = any || fp32[i];
+    }
+    if (any) {
+        B.reset();
+        B.reset(create_batch(in, nrhs, count, precisions, fp32));
+    }
+    B->x3_err = err;
+    return B.release();
+}
+
 // all groups, forked from and joined back to st; dir DIR_FWD (Y = K~ W) or
 // DIR_ADJ (Y = K~^T W)
 static void batch_eval(mx_batch *B, int dir, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
@@ -256,13 +382,13 @@ static void batch_eval(mx_batch *B, int dir, mx_precision prec, const void *cons
     const int ng = (int)B->plans.size();
     for (int g = 0; g < ng; ++g) arg_check(W[g] && Y[g], "null W or Y of group " + std::to_string(g));
     if (ng == 1) {
-        eval(B->plans[0], dir, prec, W[0], Y[0], B->group_nrhs[0], io, st);
+        eval(B->plans[0], dir, group_prec(B, 0, prec), W[0], Y[0], B->group_nrhs[0], io, st);
         return;
     }
     HIPCHK(hipEventRecord(B->events[0], st));
     for (int g = 0; g < ng; ++g) {
         HIPCHK(hipStreamWaitEvent(B->streams[g], B->events[0], 0));
-        eval(B->plans[g], dir, prec, W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
+        eval(B->plans[g], dir, group_prec(B, g, prec), W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
         HIPCHK(hipEventRecord(B->events[g + 1], B->streams[g]));
     }
     for (int g = 0; g < ng; ++g) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
@@ -272,9 +398,11 @@ static void batch_eval(mx_batch *B, int dir, mx_precision prec, const void *cons
 static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type io) {
     for (size_t g = 0; g < B->plans.size(); ++g) {
         mx_plan *P = B->plans[g];
-        if (!(P->precisions & (1u << prec))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
+        const mx_precision gp = group_prec(B, (int)g, prec);
+        if (!(P->precisions & (1u << gp))) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the batch");
         if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
-        size_work(P, dir, prec, io, B->group_nrhs[g]);
+        if (gp == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
+        size_work(P, dir, gp, io, B->group_nrhs[g]);
     }
 }
 
@@ -299,7 +427,7 @@ static void batch_drop_stale_graphs(mx_batch *B) {
 }
 
 static void batch_note_eval(mx_batch *B, int dir, mx_precision prec) {
-    for (size_t g = 0; g < B->plans.size(); ++g) note_eval(B->plans[g], dir, prec, B->group_nrhs[g]);
+    for (size_t g = 0; g < B->plans.size(); ++g) note_eval(B->plans[g], dir, group_prec(B, (int)g, prec), B->group_nrhs[g]);
 }
 
 // The whole batch as one captured hipGraph (all groups, forked and joined
@@ -392,7 +520,7 @@ mx_status mx_batch_create(mx_batch **batch, const mx_structure *const *instances
     return (mx_status)guard([&]() {
         arg_check(batch != nullptr, "null argument");
         *batch = nullptr;
-        *batch = create_batch(instances, nrhs, count, precisions);
+        *batch = create_guarded_batch(instances, nrhs, count, precisions);
         return 0;
     });
 }
@@ -425,6 +553,15 @@ mx_status mx_batch_layout(const mx_batch *batch, int32_t instance, int32_t *grou
                   "bad argument");
         *group = batch->inst[instance].group;
         *row_offset = batch->inst[instance].row_off;
+        return 0;
+    });
+}
+
+mx_status mx_batch_instance_guard(const mx_batch *batch, int32_t instance, double *x3_err, int32_t *fp32) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && x3_err && fp32 && instance >= 0 && instance < (int32_t)batch->inst.size(), "bad argument");
+        *x3_err = batch->x3_err[instance];
+        *fp32 = batch->group_fp32[batch->inst[instance].group];
         return 0;
     });
 }
@@ -470,7 +607,8 @@ mx_status mx_batch_backward(mx_batch *batch, mx_precision prec, const void *cons
         if (!batch->plans[0]->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
         const hipStream_t st = (hipStream_t)stream;
         for (size_t g = 0; g < batch->plans.size(); ++g)
-            backward(batch->plans[g], prec, W[g], dY[g], dW[g], batch->group_nrhs[g], io, nullptr, nullptr,
+            backward(batch->plans[g], group_prec(batch, (int)g, prec), W[g], dY[g], dW[g], batch->group_nrhs[g], io,
+                     nullptr, nullptr,
                      dD ? dD[g] : nullptr, dB ? dB[g] : nullptr, dVT ? dVT[g] : nullptr, st);
         return 0;
     });
@@ -516,7 +654,8 @@ mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, i
         arg_check(batch != nullptr, "null batch");
         if (prec == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 I/O");
         for (size_t g = 0; g < batch->plans.size(); ++g)
-            autotune(batch->plans[g], prec, batch->group_nrhs[g], io, reps, (hipStream_t)stream, nullptr);
+            autotune(batch->plans[g], group_prec(batch, (int)g, prec), batch->group_nrhs[g], io, reps,
+                     (hipStream_t)stream, nullptr);
         batch_drop_graphs(batch);
         return 0;
     });
@@ -528,7 +667,7 @@ mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total
         *total = 0.0;
         for (size_t g = 0; g < batch->plans.size(); ++g) {
             const mx_plan *P = batch->plans[g];
-            const int kd = prec == MX_BF16X3 ? 1 : 0;
+            const int kd = group_prec(batch, (int)g, prec) == MX_BF16X3 ? 1 : 0;
             for (double f : P->prog[DIR_FWD][kd][P->sched[DIR_FWD][kd]].flops) *total += f * batch->group_nrhs[g];
         }
         return 0;

@@ -338,6 +338,8 @@ void ensure_panels(mx_plan *P, int nrhs);
 // buffers: the skeletons of direction dir at nrhs columns.
 void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs);
 void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, hipStream_t st, float *best_ms);
+// n unit-variance pseudo-random values (splitmix64 of the index), FP64 or FP32
+void launch_fill_uniform(void *p, int64_t n, bool f64, hipStream_t st);
 void dense_rows_core(const double *X_leaf, int64_t N, int d, double h, int nleaf, const int32_t *wrow,
                      const int32_t *Ddim, const int32_t *rows, int nrows, const double *W, int nrhs, double *out,
                      hipStream_t st);

@@ -835,6 +835,11 @@ __global__ void fill_uniform(void *p, int64_t n, int f64) {
     }
 }
 
+void launch_fill_uniform(void *p, int64_t n, bool f64, hipStream_t st) {
+    fill_uniform<<<cvt_grid(n), 256, 0, st>>>(p, n, f64 ? 1 : 0);
+    HIPCHK(hipGetLastError());
+}
+
 static void drop_graphs(mx_plan *P) {
     for (auto &g : P->graphs) HIPCHK(hipGraphExecDestroy(g.exec));
     P->graphs.clear();

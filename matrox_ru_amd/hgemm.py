@@ -24,6 +24,8 @@ X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_re
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5"}
 IO_F64, IO_F32 = 0, 1
 NOALLOC_SCRATCH_BYTES = 256 << 20   # MX_NOALLOC_SCRATCH_BYTES
+BATCH_X3_GUARD = 8e-6               # MX_BATCH_X3_GUARD
+BATCH_NO_X3_GUARD = 1 << 9          # MX_BATCH_NO_X3_GUARD
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3}
 
 P = ctypes.c_void_p
@@ -106,6 +108,8 @@ def lib():
         L.mx_batch_group_info.restype = ctypes.c_int
         L.mx_batch_layout.argtypes = [P, I32, ctypes.POINTER(I32), ctypes.POINTER(I64)]
         L.mx_batch_layout.restype = ctypes.c_int
+        L.mx_batch_instance_guard.argtypes = [P, I32, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(I32)]
+        L.mx_batch_instance_guard.restype = ctypes.c_int
         L.mx_batch_plan.argtypes = [P, I32, ctypes.POINTER(P)]
         L.mx_batch_plan.restype = ctypes.c_int
         L.mx_batch_eval.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int, P]
@@ -471,9 +475,12 @@ class Batch:
     element range [off_i * nrhs_g, (off_i + N_i) * nrhs_g) in its own
     leaf-blocked layout (``instance_view``)."""
 
-    def __init__(self, hms, nrhs, precisions=("fp32",), adjoint=False):
+    def __init__(self, hms, nrhs, precisions=("fp32",), adjoint=False, x3_guard=True):
+        """x3_guard (with "bf16x3" and "fp32" in precisions): instances whose
+        estimated split-BF16x3 error exceeds BATCH_X3_GUARD evaluate in FP32
+        when BF16x3 is requested (``x3_err``, ``fp32_instances``)."""
         assert len(hms) == len(nrhs) and len(hms) >= 1
-        mask = PLAN_ADJOINT if adjoint else 0
+        mask = (PLAN_ADJOINT if adjoint else 0) | (0 if x3_guard else BATCH_NO_X3_GUARD)
         for p in precisions:
             mask |= 1 << PRECISIONS[p]
         structs, keep = [], []
@@ -499,10 +506,17 @@ class Batch:
             _check(lib().mx_batch_group_info(h, i, ctypes.byref(nr_), ctypes.byref(rows)))
             self.groups.append((nr_.value, rows.value))
         self.layout = []
+        self.x3_err = []
+        self.fp32_instances = []
         for i in range(len(hms)):
             gi, off = I32(), I64()
             _check(lib().mx_batch_layout(h, i, ctypes.byref(gi), ctypes.byref(off)))
             self.layout.append((gi.value, off.value))
+            e, f = ctypes.c_double(), I32()
+            _check(lib().mx_batch_instance_guard(h, i, ctypes.byref(e), ctypes.byref(f)))
+            self.x3_err.append(e.value)
+            if f.value:
+                self.fp32_instances.append(i)
 
     def buffers(self, dtype, fill=None):
         """One CUDA tensor per group (rows_g * nrhs_g elements)."""

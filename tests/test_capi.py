@@ -91,6 +91,10 @@ def test_binding_constants_match_header():
     text = open(os.path.join(ROOT, "include", "matrox_hgemm.h")).read()
     m = re.search(r"#define MX_NOALLOC_SCRATCH_BYTES \(\(size_t\)(\d+) << (\d+)\)", text)
     assert m and int(m.group(1)) << int(m.group(2)) == hgemm.NOALLOC_SCRATCH_BYTES
+    m = re.search(r"#define MX_BATCH_X3_GUARD ([0-9.e-]+)", text)
+    assert m and float(m.group(1)) == hgemm.BATCH_X3_GUARD
+    m = re.search(r"#define MX_BATCH_NO_X3_GUARD \(1u << (\d+)\)", text)
+    assert m and 1 << int(m.group(1)) == hgemm.BATCH_NO_X3_GUARD
 
 
 def test_hgemm_cache_mode_checked_without_gpu():

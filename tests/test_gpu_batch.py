@@ -51,7 +51,7 @@ def test_batch_instances_match_oracle(gpu, batch_case, prec):
     _say(f"[batch] prec={prec}")
     hms, nrhs, TW, ref = batch_case
     b = hgemm.Batch(hms, nrhs, precisions=("fp64", "fp32", "bf16x3"))
-    assert sorted(nr for nr, _ in b.groups) == [5, 8, 32]
+    assert sorted({nr for nr, _ in b.groups}) == [5, 8, 32]
     dt = torch.float64 if prec == "fp64" else torch.float32
     W = _fill(b, TW, dt)
     Y = b.buffers(dt, fill=float("nan"))
@@ -64,10 +64,9 @@ def test_batch_instances_match_oracle(gpu, batch_case, prec):
             ro = oracle.hgemm(hms[i], b.instance_view(W, i).double().cpu().numpy(), nrhs[i])[0]
         e = rel(y, ro)
         _say(f"  instance {i}: nrhs={nrhs[i]} N={hms[i].N} err={e:.2e}")
-        # split-BF16x3 keeps 16 significant bits per operand and re-splits
-        # every skeleton level: ~1e-5 on these small, far-field-heavy
-        # instances (the 65,536-point workload is checked at 1e-5)
-        assert e <= {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 2e-5}[prec], i
+        # split-BF16x3: the batch's accuracy guard evaluates in FP32 the
+        # instances whose BF16x3 error it estimates above 8e-6
+        assert e <= {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 1e-5}[prec], i
 
 
 def test_batch_graph_replays_and_autotune(gpu, batch_case):
@@ -91,7 +90,7 @@ def test_batch_graph_replays_and_autotune(gpu, batch_case):
     b.eval(W, Y2, "fp32")
     torch.cuda.synchronize()
     for i in range(len(hms)):
-        assert rel(b.instance_view(Y2, i).cpu().numpy(), b.instance_view(Y1, i).cpu().numpy()) <= 2e-5
+        assert rel(b.instance_view(Y2, i).cpu().numpy(), b.instance_view(Y1, i).cpu().numpy()) <= 1e-5
     f = b.flops("fp32")
     assert f == pytest.approx(sum(oracle.flops(hm, n) for hm, n in zip(hms, nrhs)), rel=1e-12)
 
@@ -115,7 +114,7 @@ def test_batch_adjoint_matches_oracle_on_transposed(gpu, batch_case, prec):
         go = oracle.hgemm(hms[i].transposed(), b.instance_view(dY, i).double().cpu().numpy(), nrhs[i])[0]
         e = rel(g, go)
         _say(f"  instance {i}: nrhs={nrhs[i]} N={hms[i].N} err={e:.2e}")
-        assert e <= {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 2e-5}[prec], i
+        assert e <= {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 1e-5}[prec], i
     b.set_graph(True)
     Yf = b.buffers(dt)
     G2 = b.buffers(dt)
@@ -128,7 +127,7 @@ def test_batch_adjoint_matches_oracle_on_transposed(gpu, batch_case, prec):
     for i in range(len(hms)):
         assert rel(b.instance_view(Yf, i).double().cpu().numpy(), ref[i] if prec == "fp64" else
                    oracle.hgemm(hms[i], b.instance_view(dY, i).double().cpu().numpy(), nrhs[i])[0]) <= \
-            {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 2e-5}[prec]
+            {"fp64": 1e-12, "fp32": 1e-5, "bf16x3": 1e-5}[prec]
     fwd_only = hgemm.Batch(hms[:2], nrhs[:2], precisions=("fp32",))
     Wf = fwd_only.buffers(torch.float32, fill=0.0)
     with pytest.raises(hgemm.MxError) as e:
@@ -145,7 +144,7 @@ def test_batch_backward_matches_single_plans(gpu, batch_case, prec):
     import torch
     _say(f"[batch backward] prec={prec}")
     hms, nrhs, TW, ref = batch_case
-    b = hgemm.Batch(hms, nrhs, precisions=("fp64", "bf16x3"), adjoint=True)
+    b = hgemm.Batch(hms, nrhs, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     dt = torch.float64
     W = _fill(b, TW, dt)
     dY = _fill(b, [host.normal((hm.N * n,), seed=950 + i) for i, (hm, n) in enumerate(zip(hms, nrhs))], dt)
@@ -155,12 +154,14 @@ def test_batch_backward_matches_single_plans(gpu, batch_case, prec):
     torch.cuda.synchronize()
     tol = 1e-12 if prec == "fp64" else 5e-5
     for i, (hm, n) in enumerate(zip(hms, nrhs)):
-        plan = hgemm.Plan(hm, precisions=(prec,), adjoint=True)
+        # the guard's FP32 instances run their backward in FP32
+        ip = "fp32" if prec == "bf16x3" and i in b.fp32_instances else prec
+        plan = hgemm.Plan(hm, precisions=(ip,), adjoint=True)
         Wi = b.instance_view(W, i).contiguous()
         dYi = b.instance_view(dY, i).contiguous()
         Yi = torch.empty_like(Wi)
-        plan.eval(Wi, Yi, n, prec)
-        ref_g = plan.backward(Wi, dYi, n, prec)
+        plan.eval(Wi, Yi, n, ip)
+        ref_g = plan.backward(Wi, dYi, n, ip)
         got = [b.instance_view(dW, i)] + [b.block_view(buf, i, w, ref_g[w + 1].numel())
                                           for w, buf in enumerate((dD, dB, dVT))]
         for nm, a_, r_ in zip(("dW", "dD", "dB", "dVT"), got, ref_g):
@@ -169,7 +170,7 @@ def test_batch_backward_matches_single_plans(gpu, batch_case, prec):
             e = rel(a_.double().cpu().numpy(), r_.double().cpu().numpy())
             assert e <= tol, (i, nm, e)
         go = oracle.hgemm(hm.transposed(), dYi.cpu().numpy(), n)[0]
-        assert rel(b.instance_view(dW, i).double().cpu().numpy(), go) <= (1e-12 if prec == "fp64" else 2e-5)
+        assert rel(b.instance_view(dW, i).double().cpu().numpy(), go) <= (1e-12 if prec == "fp64" else 1e-5)
         plan.close()
     _say(f"  {len(hms)} instances checked")
 
