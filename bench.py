@@ -77,7 +77,8 @@ def parse():
     ap.add_argument("--nrhs", type=int, default=0, help="override the workload's nrhs")
     ap.add_argument("--precision", default="", choices=["", "bf16x3", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-nrhs", type=int, default=128)
+    ap.add_argument("--cpu-sample-nrhs", type=int, default=0,
+                    help="columns of the CPU baseline's evaluation (0: the workload's nrhs)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain 1-buffer copy of this size (HBM-counter calibration)")
     ap.add_argument("--skip-checks", action="store_true", help="skip the dense-row error checks")
@@ -184,43 +185,64 @@ def dense_checks(plan, hm, wl, Wd, nrhs, results, np, torch):
     return checks
 
 
-def cpu_baseline(hm, wl, nrhs, ns, host):
-    """The oracle (restated reference evaluation, OpenMP loops) on the host
-    cores, on a bounded sample: the same compressed matrix with ns columns.
-    Its block products go through MKL cblas_dgemm (sequential, inside the
-    OpenMP loops, as the reference runs) when the MKL runtime is present,
-    else through the oracle's own GEMM; both are timed when MKL is there."""
-    from oracle import oracle
-    Ws = host.normal((hm.N, ns), seed=wl["seed_w"] + 1)
-    TWs = hm.transform(Ws)
-    fl = oracle.flops(hm, ns)
+def host_cores():
+    """(threads the oracle's OpenMP loops use, physical cores among the CPUs
+    this process may run on)."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/core_id") as f:
+                core = f.read().strip()
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/physical_package_id") as f:
+                pkg = f.read().strip()
+            cores.add((pkg, core))
+        except OSError:
+            cores.add(("?", str(c)))
+    threads = int(os.environ.get("OMP_NUM_THREADS", len(cpus)))
+    return threads, len(cores)
 
-    def timed():
-        oracle.hgemm(hm, TWs, ns)
+
+def cpu_baseline(hm, wl, nrhs, ns, host, own_ns=128):
+    """The oracle (restated reference evaluation, OpenMP loops) on the host
+    cores, on the same compressed matrix with ns columns (the workload's
+    nrhs unless a smaller sample is asked for).  Its block products go
+    through MKL cblas_dgemm (sequential, inside the OpenMP loops, as the
+    reference runs) when the MKL runtime is present, else through the
+    oracle's own GEMM; the own GEMM is also timed, on own_ns columns."""
+    from oracle import oracle
+
+    def timed(cols):
+        Ws = host.normal((hm.N, cols), seed=wl["seed_w"] + 1)
+        TWs = hm.transform(Ws)
+        oracle.hgemm(hm, TWs, cols)
         ts = []
         for _ in range(3):
             t1 = time.time()
-            oracle.hgemm(hm, TWs, ns)
+            oracle.hgemm(hm, TWs, cols)
             ts.append(time.time() - t1)
-        return statistics.median(ts)
+        return statistics.median(ts), oracle.flops(hm, cols)
 
-    t_own = timed()
+    own_ns = min(own_ns, ns)
+    t_own, fl_own = timed(own_ns)
     mkl = oracle.use_mkl()
-    t_mkl = None
+    t_mkl = fl = None
     if mkl:
         try:
-            t_mkl = timed()
+            t_mkl, fl = timed(ns)
         finally:
             oracle.use_blas(None)
-    tcpu = t_mkl if t_mkl is not None else t_own
-    blas = f"MKL cblas_dgemm ({mkl}, sequential)" if t_mkl is not None else "own column-major GEMM"
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    cpu = {"value": round(fl / tcpu / 1e12, 5), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-           "sample": f"oracle FP64 evaluation (restated HGEMM_gen.h sweeps, OpenMP, {blas}) of the same "
-                     f"compressed matrix with nrhs={ns} instead of {nrhs}; median of 3 runs = {tcpu:.3f} s",
+    if t_mkl is None:
+        t_mkl, fl = timed(ns)   # the own GEMM at the full sample
+    blas = f"MKL cblas_dgemm ({mkl}, sequential)" if mkl else "own column-major GEMM"
+    threads, phys = host_cores()
+    cpu = {"value": round(fl / t_mkl / 1e12, 5), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+           "physical_cores": phys,
+           "sample": f"oracle FP64 evaluation (restated HGEMM_gen.h sweeps, OpenMP on {threads} threads, {blas}) "
+                     f"of the same compressed matrix with nrhs={ns}; median of 3 runs after one = {t_mkl:.3f} s",
            "blas": blas,
-           "ms_per_eval_scaled_to_nrhs": round(tcpu * 1e3 * nrhs / ns, 1),
-           "own_gemm_tflops": round(fl / t_own / 1e12, 5)}
+           "ms_per_eval": round(t_mkl * 1e3 * nrhs / ns, 1),
+           "own_gemm_tflops": round(fl_own / t_own / 1e12, 5), "own_gemm_nrhs": own_ns}
     log(f"[bench] cpu baseline: {cpu}")
     return cpu
 
@@ -682,7 +704,7 @@ def main():
     if not args.no_dropin:
         dropin = dropin_timings(hm, Wd, nrhs, flops64, np, hgemm)
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(hm, wl, nrhs, min(args.cpu_sample_nrhs, nrhs), host)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(hm, wl, nrhs, min(args.cpu_sample_nrhs or nrhs, nrhs), host)
 
     metric = HEADLINE_METRIC if args.workload == "h2b_n65536" else \
         f"HGEMM eval TFLOP/s+ms, {prec} vs FP64 plain, {wl['name']} nrhs={nrhs}"
