@@ -51,6 +51,9 @@ typedef struct mxh_config {
     int32_t  skip_blocks;    /* 1: lay out D and B (Dptr, Bptr, sizes in the
                                 scalars "D_len", "B_len") without computing
                                 them, for evaluators that generate them      */
+    int32_t  all_rows;       /* 1: factor every node's ID on all n points
+                                instead of the sampled rows (a control for
+                                the sampling's accuracy; O(n) rows a node)   */
 } mxh_config;
 
 /* Fill a config with the reference's defaults (m=256, rank 256, tol 1e-5,
@@ -80,9 +83,12 @@ int64_t mxh_get_scalar(const mxh_handle *h, const char *name);
  *   nblockSet nblocks npairx npairy fblockSet fblocks fpairx fpairy
  *   wpart clevelset lids lidsoffset lidslen leaf proj_column
  *   skels skelsoffset flat_nx flat_ny flat_fx flat_fy
+ *   id_rows   (per node: rows of the block its ID factored)
  *   wrow urow skelrow   (row offsets: multiply by nrhs for wptr/uptr/wskeloffset)
  * uint64 arrays: Dptr Bptr
  * double arrays: D B VT X
+ *   id_ratio  (per node: |R_ss / R_00| of the first column the ID left out,
+ *              with s its rank; 0 when it kept every column)
  * For an HSS structure (budget giving one near leaf per leaf) D/B and the
  * pair arrays are in the flat order the HSS evaluator indexes (Dptr[leaf],
  * Bptr[node-1]) and the blocked arrays describe that order as one row-block.

@@ -52,6 +52,8 @@ struct Handle {
     std::vector<std::vector<int>> skels;
     std::vector<std::vector<double>> proj;
     std::vector<int> projcol;
+    std::vector<double> id_ratio;   // per node: |R_ss / R_00| of the first column the ID left out (0: none)
+    std::vector<int> id_rows;       // per node: rows of the block the ID factored
 
     std::map<std::string, std::vector<int32_t>> i32;
     std::map<std::string, std::vector<uint64_t>> u64;
@@ -482,7 +484,8 @@ static void build_knn(Handle &H) {
 // stops once s is known: the trailing rows are never needed.
 static int interpolative_decomposition(double *A, int m, int n, double tol, int max_rank,
                                        std::vector<int> &skel_cols, std::vector<double> &proj,
-                                       bool inner_parallel) {
+                                       bool inner_parallel, double *left_out_ratio) {
+    *left_out_ratio = 0.0;
     std::vector<int> jpvt(n);
     for (int j = 0; j < n; ++j) jpvt[j] = j;
     std::vector<double> vn1(n), vn2(n);
@@ -529,6 +532,7 @@ static int interpolative_decomposition(double *A, int m, int n, double tol, int 
             r00_zero = (r00 == 0.0);
         } else if (!r00_zero && std::abs(rjj / r00) < tol) {
             s = std::min(j, max_rank);
+            *left_out_ratio = std::abs(rjj / r00);
             break;
         }
         if (j + 1 >= std::min(n, max_rank)) {
@@ -548,7 +552,20 @@ static int interpolative_decomposition(double *A, int m, int n, double tol, int 
                 for (int i = j + 1; i < m; ++i) ac[i] -= w * aj[i];
             }
         }
-        if (j + 1 >= std::min(n, max_rank)) break;
+        if (j + 1 >= std::min(n, max_rank)) {
+            // the cap stops here: the next pivot would be the largest
+            // trailing column norm of rows j+1..m
+            if (j + 1 < n && j + 1 < m && !r00_zero) {
+                double mx = 0.0;
+                for (int c = j + 1; c < n; ++c) {
+                    double ss = 0;
+                    for (int i = j + 1; i < m; ++i) ss += A[(size_t)c * m + i] * A[(size_t)c * m + i];
+                    mx = std::max(mx, ss);
+                }
+                *left_out_ratio = std::sqrt(mx) / std::abs(r00);
+            }
+            break;
+        }
         // downdate column norms (LAPACK dlaqp2 recipe)
         for (int c = j + 1; c < n; ++c) {
             if (vn1[c] != 0.0) {
@@ -644,7 +661,7 @@ static void skeletonize(Handle &H, int v, std::vector<NodeSampling> &samp, bool 
     std::sort(ordered.begin(), ordered.end());
 
     std::vector<int> amap;
-    if (nsamples < (size_t)(n - (int)own.size())) {
+    if (!H.cfg.all_rows && nsamples < (size_t)(n - (int)own.size())) {
         std::unordered_set<int> inA;
         for (size_t i = 0; i < ordered.size() && amap.size() < nsamples; ++i) {
             amap.push_back(ordered[i].second);
@@ -667,7 +684,9 @@ static void skeletonize(Handle &H, int v, std::vector<NodeSampling> &samp, bool 
     kernel_block(H, amap.data(), m1, bmap.data(), n1, Kab.data());
     std::vector<int> cols;
     std::vector<double> proj;
-    interpolative_decomposition(Kab.data(), m1, n1, H.cfg.tol, H.cfg.max_rank, cols, proj, inner_parallel);
+    interpolative_decomposition(Kab.data(), m1, n1, H.cfg.tol, H.cfg.max_rank, cols, proj, inner_parallel,
+                                &H.id_ratio[v]);
+    H.id_rows[v] = m1;
     H.skels[v].resize(cols.size());
     for (size_t i = 0; i < cols.size(); ++i) H.skels[v][i] = bmap[cols[i]];
     H.proj[v].swap(proj);
@@ -683,6 +702,8 @@ static void compress(Handle &H) {
     H.skels.assign(H.numnodes, {});
     H.proj.assign(H.numnodes, {});
     H.projcol.assign(H.numnodes, 0);
+    H.id_ratio.assign(H.numnodes, 0.0);
+    H.id_rows.assign(H.numnodes, 0);
     std::vector<NodeSampling> samp(H.numnodes);
     const int nthr = omp_get_max_threads();
     for (int l = 0; l < H.depth; ++l) {
@@ -937,6 +958,8 @@ static void emit_cds(Handle &H) {
         sko[v] = (int)sk.size();
         sk.insert(sk.end(), H.skels[v].begin(), H.skels[v].end());
     }
+    H.f64["id_ratio"] = H.id_ratio;
+    I["id_rows"].assign(H.id_rows.begin(), H.id_rows.end());
     H.scal["skel_rows"] = soff;
     H.scal["ncount"] = (int64_t)H.u64["Dptr"].size();
     H.scal["fcount"] = (int64_t)H.u64["Bptr"].size();
@@ -1245,6 +1268,7 @@ void mxh_config_default(mxh_config *c) {
     c->nthreads = 0;
     c->seed = 0x4D617452ULL;
     c->skip_blocks = 0;
+    c->all_rows = 0;
 }
 
 static mxh_handle *build_c(const double *X, int32_t n, int32_t d, const mxh_config *cfg, bool full) {

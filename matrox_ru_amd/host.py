@@ -17,9 +17,9 @@ _I32 = ["VTptr", "lchildren", "rchildren", "levelset", "idx_level", "idx", "Ddim
         "slen", "nblockSet", "nblocks", "npairx", "npairy", "fblockSet", "fblocks",
         "fpairx", "fpairy", "wpart", "clevelset", "lids", "lidsoffset", "lidslen", "leaf",
         "proj_column", "skels", "skelsoffset", "flat_nx", "flat_ny", "flat_fx", "flat_fy",
-        "wrow", "urow", "skelrow"]
+        "wrow", "urow", "skelrow", "id_rows"]
 _U64 = ["Dptr", "Bptr"]
-_F64 = ["D", "B", "VT", "X"]
+_F64 = ["D", "B", "VT", "X", "id_ratio"]
 _SCALARS = ["n", "d", "nleaf", "numnodes", "depth", "cdepth", "nl", "fl", "ncount",
             "fcount", "is_hss", "N_total", "skel_rows", "D_len", "B_len"]
 
@@ -31,7 +31,7 @@ class MxhConfig(ctypes.Structure):
                 ("near_block", ctypes.c_int32), ("far_block", ctypes.c_int32),
                 ("agg_levels", ctypes.c_int32), ("bottom_levels", ctypes.c_int32),
                 ("nbins", ctypes.c_int32), ("nthreads", ctypes.c_int32),
-                ("seed", ctypes.c_uint64), ("skip_blocks", ctypes.c_int32)]
+                ("seed", ctypes.c_uint64), ("skip_blocks", ctypes.c_int32), ("all_rows", ctypes.c_int32)]
 
 
 _configured = False
