@@ -252,6 +252,35 @@ mx_status mx_plan_set_graph(mx_plan *plan, int32_t enable);
  * order.  Defaults: fused forward, split adjoint.  x3_variant: BF16x3 kernel
  * tile and pipeline depth (one per plan, 0 keeps the current): */
 enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1, MX_SCHED_FUSED_BALANCED = 2, MX_SCHED_SPLIT_BALANCED = 3 };
+/* MX_SCHED_SWEEP (MX_BF16X3 only): one persistent launch per evaluation runs
+ * every tile of the fused schedule, each as soon as the skeleton blocks it
+ * reads are written (per-block counters; a tile reads its parent's transfer
+ * last, so the couplings and near blocks run while the top-down chain is in
+ * progress).  x3_variant picks its tile width: MX_X3_128x128_S2,
+ * MX_X3_128x64_S2 or MX_X3_128x32_S2.  The autotuner picks it when it is the
+ * faster; any other schedule turns it off.  No counterpart in the reference,
+ * whose sweeps synchronise per level group (symGen/HGEMM_gen.h:142-236). */
+enum { MX_SCHED_SWEEP = 4 };
+
+/* Error word of the last sweep this plan enqueued (after synchronising
+ * `stream`): 0 = none or clean; 1 = a wait for a producer, 2 = claiming
+ * a work item, exceeded one second, after which the launch drained without
+ * waiting (its outputs are then incomplete).  No counterpart in the
+ * reference. */
+mx_status mx_plan_sweep_status(const mx_plan *plan, void *stream, int32_t *error);
+
+/* Host-only: the sweep's work lists of one direction of a structure (no
+ * device work, for checks of the scheduling protocol).  With tiles == NULL
+ * it returns the counts.  tiles: 12 int32 per tile (task, row tile, rank,
+ * 0, the last producer tile per queue x 8), queues back to back; qinfo: per
+ * queue its first tile and tile count (16); tasks: 8 per task (early waits
+ * [b, e), late waits [b, e), index of the first u^ segment or -1, counter
+ * block signalled or -1, 0, 0); waits: 2 per wait (counter block, target
+ * count of finished (tile, wave) writes per column tile).  Counter block v
+ * is the w^ block of node v, numnodes + v its u^ block.  No counterpart in
+ * the reference. */
+mx_status mx_sweep_describe(const mx_structure *s, int32_t direction, int32_t *ntiles, int32_t *ntasks,
+                            int32_t *nwaits, int32_t *tiles, int32_t *qinfo, int32_t *tasks, int32_t *waits);
 enum { MX_DIR_FORWARD = 0, MX_DIR_ADJOINT = 1 };
 enum {
     MX_X3_128x128_S2 = 1, /* 4 waves, 2 stages, two workgroups per CU (default) */

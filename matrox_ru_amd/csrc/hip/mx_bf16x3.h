@@ -133,4 +133,84 @@ int x3_variant_bn(int variant);   // output columns of a tile
 int x3_variant_bm(int variant);   // output rows of a tile (X3_BM or 2 X3_BM)
 void launch_x3(const X3Args &a, int variant, hipStream_t st);
 
+// ---- dependency-driven sweep: one persistent launch per evaluation --------
+//
+// Every tile (task, row tile) of the level launches, times every column
+// tile, is a work item.  The work items sit in SWEEP_NQ queues, one per XCD:
+// queue z holds the z-th XCD group of every level launch, in launch order,
+// so a queue keeps the tile order that shares operands through its XCD's
+// L2.  A work item starts as soon as the skeleton blocks it reads are
+// written: per (skeleton block, column tile) a counter holds the number of
+// finished (tile, wave) writes.  A task waits for the producers of its
+// leading segments before its first load ("early": children for UP, far
+// sources for the couplings) and for those of its trailing segments, the
+// ones reading u^ (the parent's transfer, the leaf's VT^T u), only before
+// the first of them ("late"), so coupling and near work runs while the
+// top-down chain is still in progress.
+//
+// Progress does not depend on how many workgroups are resident: a
+// workgroup claims the next item of its XCD's queue (fetch-and-add), but
+// runs it only once every producer of that item is claimed by some
+// workgroup; until then it claims and runs the first untaken item in rank
+// order (a topological order of all items), whose producers are all
+// claimed.  A held item ranks above everything its holder runs first, so
+// waits only point to lower ranks and the lowest unfinished item waits for
+// nothing.  Every wait is also bounded in time: past SWEEP_TIMEOUT_TICKS it
+// sets the error word, after which every workgroup drains without waiting.
+//
+// Coherence across the XCDs' L2 caches without whole-cache maintenance: a
+// producer writes its skeleton panel tile with device-scope stores (sc1:
+// written through its XCD's L2), waits for them to complete, then adds to
+// the counter (device-scope atomic); a consumer polls the counter with
+// device-scope loads and only then loads the tile.  Within one launch no
+// cache line of a panel is read before its one write (blocks, row groups
+// and column tiles never share a line, and the caches are invalidated at
+// every kernel start), so the consumer's first access misses and reads the
+// written data.  The agent-scope fences that would make this formal write
+// back and invalidate whole L2s at every item and cost more than the
+// evaluation itself.
+constexpr int SWEEP_NQ = 8;
+constexpr int SWEEP_WAVES = 4;            // waves per workgroup: each signals its own writes
+constexpr int SWEEP_HEAD_STRIDE = 32;     // int32 per queue head (one 128-byte line each)
+constexpr int SWEEP_ERR = SWEEP_NQ * SWEEP_HEAD_STRIDE;   // error word
+constexpr int SWEEP_CNT = SWEEP_ERR + SWEEP_HEAD_STRIDE;  // first counter
+constexpr uint64_t SWEEP_TIMEOUT_TICKS = 100000000ull;   // 1 s of the 100 MHz realtime clock
+constexpr int SWEEP_MAX_POLLS = 1 << 24;   // and at most this many polls per wait or claim (a second bound)
+
+struct alignas(16) SweepTile {
+    int32_t task, rt;     // task and row tile
+    int32_t g;            // rank (work item rank = g * ncols + column tile)
+    int32_t pad;
+    int32_t maxtile[SWEEP_NQ];   // per queue: the last tile index of a producer (-1: none)
+};
+
+struct alignas(16) SweepTask {
+    int32_t ew_b, ew_e;   // early waits [ew_b, ew_e)
+    int32_t lw_b, lw_e;   // late waits, before segment late_seg
+    int32_t late_seg;     // index within the task's segments, -1: none
+    int32_t sig;          // counter block the task's tiles signal (-1: Y, none)
+    int32_t pad[2];
+};
+
+struct SweepWait {
+    int32_t block;    // counter block (skeleton node; u^ blocks after the w^ ones)
+    int32_t target;   // finished (tile, wave) writes of the block per column tile
+};
+
+struct X3SweepArgs {
+    X3Args x;                    // tasks, segs, operands, panels, Y (tiles unused)
+    const SweepTile *q;          // the queues, concatenated
+    const int2 *qinfo;           // per queue: first entry, tiles
+    const SweepTask *st;         // per task
+    const SweepWait *w;
+    int32_t *ctl;                // heads, error word, counters (zeroed before the launch)
+    int ncols;                   // column tiles
+};
+
+int sweep_bn(int variant);               // columns of a tile: X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2
+bool sweep_variant_ok(int variant);
+// Zero the control block, then run the sweep on `grid` workgroups.
+void launch_x3_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st);
+int sweep_max_grid(int variant);         // resident workgroups the device holds (occupancy x CUs)
+
 }  // namespace mx

@@ -224,12 +224,14 @@ __device__ __forceinline__ void x3_seg_begin(X3Loader<S> &L, const X3Args &a, co
     L.left = nfk;
 }
 
-// DMA of the next chunk into a stage (segments with k == 0 are skipped)
-template <typename S>
+// DMA of the next chunk into a stage (segments with k == 0 are skipped);
+// on_seg(si) runs before the first load of segment si
+template <typename S, typename OnSeg>
 __device__ __forceinline__ void x3_issue(X3Loader<S> &L, const X3Args &a, const Seg *sseg, uint16_t *stage, int wv,
-                                         int lane, int mtile16, int nfm, int n0) {
+                                         int lane, int mtile16, int nfm, int n0, OnSeg &&on_seg) {
     while (L.left == 0) {
         ++L.seg;
+        on_seg(L.seg);
         x3_seg_begin(L, a, sseg[L.seg], wv, lane, mtile16, nfm, n0);
     }
     const int64_t bstep = (int64_t)a.nrhs * 32;
@@ -385,7 +387,18 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][NJ], const uint16_t *
 //    up to the padded height; padding rows are 0), split with the hardware
 //    round-to-nearest-even conversion; beta = 1 adds the panel's current
 //    value first.
-template <int NI, int NJ>
+// 8-byte panel store: plain, or (DEV) a device-scope store (sc1), written
+// through to where every XCD's L2 misses find it
+template <bool DEV>
+__device__ __forceinline__ void st_panel(uint16_t *p, u32x2_t v) {
+    if constexpr (DEV)
+        __hip_atomic_store(reinterpret_cast<uint64_t *>(p), __builtin_bit_cast(uint64_t, v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *reinterpret_cast<u32x2_t *>(p) = v;
+}
+
+template <int NI, int NJ, bool DEV = false>
 __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg, const X3Args &a, const Task &task,
                                             int lane, int wv, int wr, int wc, int m0, int n0) {
     constexpr int WC = 16 * NJ;   // columns of the wave
@@ -449,8 +462,8 @@ __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg,
                 uint32_t hw[2], lw[2];
                 split2(v[0], v[1], hw[0], lw[0]);
                 split2(v[2], v[3], hw[1], lw[1]);
-                *reinterpret_cast<u32x2_t *>(Ph + o) = u32x2_t{hw[0], hw[1]};
-                *reinterpret_cast<u32x2_t *>(Pl + o) = u32x2_t{lw[0], lw[1]};
+                st_panel<DEV>(Ph + o, u32x2_t{hw[0], hw[1]});
+                st_panel<DEV>(Pl + o, u32x2_t{lw[0], lw[1]});
             }
         }
     }
@@ -537,25 +550,18 @@ __global__ __launch_bounds__(256) void x3r_kernel(X3Args a) {
     x3_epilogue<4, 4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
-// The pipeline depth NST and the wave grid are template parameters; the
-// autotuner picks among the instantiated variants (launch_x3).
-template <int WM, int WN, int NST, int NJ = 4>
-__global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
-    typedef X3Shape<WM, WN, NJ> S;
+// One output tile (rows m0.., columns n0.. of task): the task's segment
+// list staged in LDS, then the NST-stage pipeline and the epilogue (DEV:
+// device-scope skeleton panel stores).  on_seg as in x3_issue.  Ends with the
+// epilogue's stores issued; the caller synchronises before reusing stg or
+// sseg.
+template <typename S, int NST, int NJ, bool DEV = false, typename OnSeg>
+__device__ __forceinline__ void x3_tile(const X3Args &a, const Task &task, int m0, int n0, uint16_t *stg,
+                                        Seg *sseg, OnSeg &&on_seg) {
     static_assert(S::NW * 64 * 16 * NJ * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
-    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
-    // the task's segment list, read from LDS inside the pipeline (a vector
-    // load there would wait for every outstanding LDS-DMA load)
-    __shared__ Seg sseg[X3_MAXSEG];
-
+    constexpr int WN = S::BN / (16 * NJ);
     const int tid = threadIdx.x;
-    const int total = a.ntiles * a.n_ntiles;
-    const int w = xcd_remap(blockIdx.x, total);
-    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
-    const int2 tl = a.tiles[tile_id];
-    const Task task = a.tasks[tl.x];
     const int M = task.m;
-    const int m0 = tl.y * S::BM, n0 = nt * S::BN;
     const int lane = tid & 63, wv = tid >> 6;
     const int wr = wv / WN, wc = wv % WN;
     const int nfm = (M + 15) / 16;
@@ -579,7 +585,7 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     int issued = 0;
     auto issue = [&]() {
         if (issued < nch) {
-            x3_issue<S>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, n0);
+            x3_issue<S>(ld, a, sseg, stg + (issued % NST) * S::STAGE, wv, lane, mtile16, nfm, n0, on_seg);
             ++issued;
         }
     };
@@ -594,7 +600,25 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
         x3_compute<NJ>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
     }
     __syncthreads();
-    x3_epilogue<4, NJ>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
+    x3_epilogue<4, NJ, DEV>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
+}
+
+// The pipeline depth NST and the wave grid are template parameters; the
+// autotuner picks among the instantiated variants (launch_x3).
+template <int WM, int WN, int NST, int NJ = 4>
+__global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
+    typedef X3Shape<WM, WN, NJ> S;
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
+    // the task's segment list, read from LDS inside the pipeline (a vector
+    // load there would wait for every outstanding LDS-DMA load)
+    __shared__ Seg sseg[X3_MAXSEG];
+
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    x3_tile<S, NST, NJ>(a, task, tl.y * S::BM, nt * S::BN, stg, sseg, [](int) {});
 }
 
 int x3_variant_bm(int variant) { return variant == X3_V256x128_W8S3 ? 2 * X3_BM : X3_BM; }
@@ -647,6 +671,312 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);
+        break;
+    }
+}
+
+// ---- dependency-driven sweep (protocol in mx_bf16x3.h) ----------------------
+
+__device__ __forceinline__ int sweep_load(const int32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int sweep_cas(int32_t *p, int expect, int want) {
+    __hip_atomic_compare_exchange_strong(p, &expect, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+    return expect;   // the value found
+}
+
+// XCD of the executing workgroup (hardware register; only the choice of
+// queue, and so L2 locality, depends on it)
+__device__ __forceinline__ int sweep_home() {
+    int x;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(x));
+    return (x & 15) % SWEEP_NQ;
+}
+
+// Wave-wide: wait until every counter of waits [b, e) at column tile ct has
+// reached its target.  The loads of the block that follow are issued after
+// the counters are seen (no cache invalidation is needed: see the coherence
+// note in mx_bf16x3.h).  Gives up on an error or after SWEEP_TIMEOUT_TICKS
+// (setting the error word).
+__device__ __forceinline__ void sweep_wait(const X3SweepArgs &sa, int b, int e, int ct, int lane) {
+    if (b >= e) return;
+    const int32_t *cnt = sa.ctl + SWEEP_CNT;
+    for (int base = b; base < e; base += 64) {
+        const int i = base + lane;
+        int idx = 0, target = 0;
+        if (i < e) {
+            const SweepWait w = sa.w[i];
+            idx = w.block * sa.ncols + ct;
+            target = w.target;
+        }
+        uint64_t t0 = 0;
+        for (int it = 0;; ++it) {
+            const bool ok = i >= e || sweep_load(cnt + idx) >= target;
+            if (__ballot(!ok) == 0) break;
+            if (sweep_load(sa.ctl + SWEEP_ERR)) break;
+            const uint64_t now = __builtin_amdgcn_s_memrealtime();
+            if (it == 0) {
+                t0 = now;
+            } else if (now - t0 > SWEEP_TIMEOUT_TICKS || it > SWEEP_MAX_POLLS) {
+                if (lane == 0) __hip_atomic_store(sa.ctl + SWEEP_ERR, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");   // no memory access moves above the polls
+}
+
+// Claims (wave 0; lanes 0..7 each look after one queue).  A work item is
+// claimed by fetch-and-add on a queue head: the home queue's, or once that
+// queue is exhausted, the head of the queue whose next item ranks lowest.
+// Such an item may still have a producer nobody has claimed.  Before it
+// runs, its holder therefore checks that every producer is claimed (the
+// heads have passed the producers' positions) and, while one is not,
+// claims the untaken item of lowest rank over all queues, if that ranks
+// below the held item (compare-and-swap on that queue's head; its
+// producers rank lower, so all are claimed), and runs that first.  A held
+// item thus ranks above everything its holder runs before it, so waits only
+// ever point to lower ranks and the lowest unfinished item waits for
+// nothing.  `heads` (lanes 0..7)
+// caches the last heads read: they only grow, so a producer seen claimed
+// stays claimed.  Error word: 1 = a wait timed out, 2 = claiming timed out.
+
+// position of the item after `h` in queue z is in range and its rank
+__device__ __forceinline__ int sweep_rank(const X3SweepArgs &sa, int2 qi, int h) {
+    return h < qi.y * sa.ncols ? sa.q[qi.x + h / sa.ncols].g * sa.ncols + h % sa.ncols : INT_MAX;
+}
+
+// lanes 0..7: the queue (z) of the lowest rank among lanes 0..7, and that rank
+__device__ __forceinline__ void sweep_argmin(int &rank, int &bz) {
+#pragma unroll
+    for (int o = SWEEP_NQ / 2; o >= 1; o >>= 1) {
+        const int r2 = __shfl_xor(rank, o), z2 = __shfl_xor(bz, o);
+        if (r2 < rank || (r2 == rank && z2 < bz)) {
+            rank = r2;
+            bz = z2;
+        }
+    }
+    rank = __shfl(rank, 0);
+    bz = __shfl(bz, 0);
+}
+
+__device__ __forceinline__ bool sweep_timed_out(const X3SweepArgs &sa, uint64_t t0, int it, int lane, int code) {
+    if (sweep_load(sa.ctl + SWEEP_ERR)) return true;
+    if (__builtin_amdgcn_s_memrealtime() - t0 > SWEEP_TIMEOUT_TICKS || it > SWEEP_MAX_POLLS) {
+        if (lane == 0) __hip_atomic_store(sa.ctl + SWEEP_ERR, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ int sweep_fetch_head(const X3SweepArgs &sa, int q) {
+    return __hip_atomic_fetch_add(sa.ctl + q * SWEEP_HEAD_STRIDE, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Any next item: q = -1 when every queue is exhausted (or on an error).
+// pre (lane 0): the result of a fetch-and-add on the home queue's head that
+// the caller has already issued, or -1.
+__device__ __forceinline__ void sweep_claim(const X3SweepArgs &sa, int home, int lane, int pre, int &heads,
+                                            int &q_out, int &pos_out) {
+    const int z = lane & (SWEEP_NQ - 1);
+    const int2 qi = sa.qinfo[z];
+    const int len = qi.y * sa.ncols;
+    int cand = home;
+    pre = __shfl(pre, 0);
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (int it = 0;; ++it) {
+        if (sweep_timed_out(sa, t0, it, lane, 2)) {
+            q_out = -1;
+            return;
+        }
+        int got = 0;
+        if (it == 0 && pre >= 0) got = pre;
+        else if (lane == 0) got = sweep_fetch_head(sa, cand);
+        got = __shfl(got, 0);
+        if (got < __shfl(len, cand)) {
+            q_out = cand;
+            pos_out = got;
+            return;
+        }
+        // that queue is exhausted: the queue whose next item ranks lowest
+        heads = lane < SWEEP_NQ ? sweep_load(sa.ctl + z * SWEEP_HEAD_STRIDE) : 0;
+        int rank = lane < SWEEP_NQ ? sweep_rank(sa, qi, heads) : INT_MAX, bz = z;
+        sweep_argmin(rank, bz);
+        if (rank == INT_MAX) {
+            q_out = -1;
+            return;
+        }
+        cand = bz;
+    }
+}
+
+// Are all producers of item (q, pos) claimed?  Reads the heads again only
+// when the cached ones do not show it.
+__device__ __forceinline__ bool sweep_producers_claimed(const X3SweepArgs &sa, int q, int pos, int lane, int &heads) {
+    const int tix = pos / sa.ncols, ct = pos % sa.ncols;
+    const int need = lane < SWEEP_NQ ? sa.q[sa.qinfo[q].x + tix].maxtile[lane] : -1;
+    if (__ballot(!(need < 0 || heads > need * sa.ncols + ct)) == 0) return true;
+    if (lane < SWEEP_NQ) heads = sweep_load(sa.ctl + lane * SWEEP_HEAD_STRIDE);
+    return __ballot(!(need < 0 || heads > need * sa.ncols + ct)) == 0;
+}
+
+// The untaken item of lowest rank, if it ranks below `limit` (the held
+// item's rank): compare-and-swap on its queue's head.  q = -1 when there is
+// none (then every item below the held one, its producers included, is
+// claimed) or on an error.  The bound matters: the held item's unclaimed
+// producer seen a moment ago may have been claimed since, and the lowest
+// untaken item may then depend on the held item itself.
+__device__ __forceinline__ void sweep_claim_lowest(const X3SweepArgs &sa, int lane, int limit, int &heads, int &q_out,
+                                                   int &pos_out) {
+    const int z = lane & (SWEEP_NQ - 1);
+    const int2 qi = sa.qinfo[z];
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    for (int it = 0;; ++it) {
+        if (sweep_timed_out(sa, t0, it, lane, 2)) {
+            q_out = -1;
+            return;
+        }
+        heads = lane < SWEEP_NQ ? sweep_load(sa.ctl + z * SWEEP_HEAD_STRIDE) : 0;
+        int rank = lane < SWEEP_NQ ? sweep_rank(sa, qi, heads) : INT_MAX, bz = z;
+        sweep_argmin(rank, bz);
+        if (rank >= limit) {
+            q_out = -1;
+            return;
+        }
+        const int hb = __shfl(heads, bz);
+        int got = 0;
+        if (lane == 0) got = sweep_cas(sa.ctl + bz * SWEEP_HEAD_STRIDE, hb, hb + 1);
+        if (__shfl(got, 0) == hb) {
+            q_out = bz;
+            pos_out = hb;
+            return;
+        }
+    }
+}
+
+// Persistent workgroups of 4 waves (2 x 2, tiles of 128 x 16 NJ x 2
+// columns, 2 stages): claim an item, wait for its early producers, run the
+// tile with the late wait before its first u^ segment, signal the written
+// block (every wave waits for its own panel stores, then adds 1), repeat.  Two
+// workgroups per CU, as the level kernel: the accumulators then live in
+// VGPRs.
+template <int NJ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x3_sweep_kernel(X3SweepArgs sa) {
+    typedef X3Shape<2, 2, NJ> S;
+    constexpr int NST = 2;
+    static_assert(S::NW == SWEEP_WAVES, "every wave signals once per tile");
+    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
+    __shared__ Seg sseg[X3_MAXSEG];
+    // the item to run; wave 0's claim state between items (in LDS, so that
+    // no register stays live across a tile): the claimed item waiting to
+    // run, whether to claim a new one, the cached queue heads
+    __shared__ int s_take[2], s_held[3], s_heads[SWEEP_NQ];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int home = sweep_home();
+    // wave 0: pick the next item to run into s_take (pre as in sweep_claim)
+    auto pick = [&](int pre) {
+        int held_q = s_held[0], held_pos = s_held[1], heads = s_heads[lane & (SWEEP_NQ - 1)];
+        if (s_held[2]) sweep_claim(sa, home, lane, pre, heads, held_q, held_pos);
+        int q = held_q, pos = held_pos, fresh = 1;
+        if (held_q >= 0 && !sweep_producers_claimed(sa, held_q, held_pos, lane, heads)) {
+            // run the lowest untaken item first (if it ranks below the held one), keep the held one
+            const int limit = sweep_rank(sa, sa.qinfo[held_q], held_pos);
+            sweep_claim_lowest(sa, lane, limit, heads, q, pos);
+            if (q >= 0) fresh = 0;
+            else if (!sweep_load(sa.ctl + SWEEP_ERR)) q = held_q, pos = held_pos;   // its producers are claimed
+        }
+        if (lane < SWEEP_NQ) s_heads[lane] = heads;
+        if (lane == 0) {
+            s_take[0] = q;
+            s_take[1] = pos;
+            s_held[0] = held_q;
+            s_held[1] = held_pos;
+            s_held[2] = fresh;
+        }
+    };
+    if (wv == 0) {
+        if (lane == 0) {
+            s_held[0] = -1;
+            s_held[2] = 1;
+        }
+        if (lane < SWEEP_NQ) s_heads[lane] = 0;
+        pick(-1);
+    }
+    __syncthreads();
+    for (;;) {
+        // uniform across the workgroup: scalar registers and loads from here
+        const int q = __builtin_amdgcn_readfirstlane(s_take[0]), pos = __builtin_amdgcn_readfirstlane(s_take[1]);
+        if (q < 0) break;
+        const int tix = pos / sa.ncols, ct = pos % sa.ncols;
+        const SweepTile tile = sa.q[sa.qinfo[q].x + tix];
+        const Task task = sa.x.tasks[tile.task];
+        const SweepTask stk = sa.st[tile.task];
+        sweep_wait(sa, stk.ew_b, stk.ew_e, ct, lane);
+        x3_tile<S, NST, NJ, true>(sa.x, task, tile.rt * S::BM, ct * S::BN, stg, sseg, [&](int si) {
+            if (si == stk.late_seg) sweep_wait(sa, stk.lw_b, stk.lw_e, ct, lane);
+        });
+        // wave 0 starts the next claim now, so that its round trip overlaps
+        // the completion of the panel stores (every wave reads q, pos and
+        // the task above, before this point)
+        int pre = -1;
+        const bool claim_next = wv == 0 && s_held[2];
+        if (claim_next && lane == 0) pre = sweep_fetch_head(sa, home);
+        if (stk.sig >= 0) {
+            // this wave's device-scope panel stores have completed before its count
+            __builtin_amdgcn_s_waitcnt(0);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (lane == 0)
+                __hip_atomic_fetch_add(sa.ctl + SWEEP_CNT + stk.sig * sa.ncols + ct, 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (wv == 0) pick(claim_next ? pre : -1);
+        __syncthreads();   // s_take is ready; stages and segment list are free
+    }
+}
+
+__global__ void sweep_reset_kernel(int32_t *ctl, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) ctl[i] = 0;
+}
+
+bool sweep_variant_ok(int variant) {
+    return variant == X3_V128x128_S2 || variant == X3_V128x64_S2 || variant == X3_V128x32_S2;
+}
+
+int sweep_bn(int variant) { return x3_variant_bn(variant); }
+
+template <int NJ>
+static int sweep_grid_of() {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, x3_sweep_kernel<NJ>, 256, 0) != hipSuccess) return 0;
+    return std::max(1, per) * cus;
+}
+
+int sweep_max_grid(int variant) {
+    switch (variant) {
+    case X3_V128x64_S2: return sweep_grid_of<2>();
+    case X3_V128x32_S2: return sweep_grid_of<1>();
+    default: return sweep_grid_of<4>();
+    }
+}
+
+void launch_x3_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st) {
+    hipLaunchKernelGGL(sweep_reset_kernel, dim3((unsigned)std::min(64, (ctl_ints + 255) / 256)), dim3(256), 0, st,
+                       a.ctl, ctl_ints);
+    if (grid <= 0) return;
+    switch (variant) {
+    case X3_V128x64_S2:
+        hipLaunchKernelGGL(x3_sweep_kernel<2>, dim3((unsigned)grid), dim3(256), 0, st, a);
+        break;
+    case X3_V128x32_S2:
+        hipLaunchKernelGGL(x3_sweep_kernel<1>, dim3((unsigned)grid), dim3(256), 0, st, a);
+        break;
+    default:
+        hipLaunchKernelGGL(x3_sweep_kernel<4>, dim3((unsigned)grid), dim3(256), 0, st, a);
         break;
     }
 }

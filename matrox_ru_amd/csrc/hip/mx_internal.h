@@ -100,6 +100,34 @@ struct mx_plan {
     std::vector<int64_t> wprow, sprow;  // padded panel row offsets (W leaves, skeleton nodes)
     int64_t wp_rows = 0, sp_rows = 0;
     int x3_variant = mx::X3_V128x128_S2;   // BF16x3 kernel variant (tile, pipeline depth)
+    // dependency-driven sweeps (BF16x3) per direction: device copies of the
+    // SweepLists, whether evaluations use them (instead of the level
+    // launches of sched[dir][1]) and their tile width
+    struct Sweep {
+        mx::Task *tasks = nullptr;
+        mx::Seg *segs = nullptr;
+        mx::SweepTask *st = nullptr;
+        mx::SweepWait *w = nullptr;
+        mx::SweepTile *q = nullptr;
+        int2 *qinfo = nullptr;
+        std::vector<int2> qinfo_h;
+        bool ok = false;
+        void release() {
+            void *ptrs[] = {tasks, segs, st, w, q, qinfo};
+            for (void *p : ptrs)
+                if (p) (void)hipFree(p);
+            tasks = nullptr;
+            segs = nullptr;
+            st = nullptr;
+            w = nullptr;
+            q = nullptr;
+            qinfo = nullptr;
+        }
+    } sweep[2];
+    int sweep_on[2] = {0, 0};
+    int sweep_var[2] = {mx::X3_V128x128_S2, mx::X3_V128x128_S2};
+    int sweep_grid[3] = {0, 0, 0};   // resident workgroups per tile width (128, 64, 32 columns), 0: not queried
+    int32_t *sweep_last_ctl = nullptr;   // control block of the last sweep enqueued (mx_plan_sweep_status)
     // per-launch BF16x3 variants [direction][schedule] chosen by the
     // autotuner (empty: x3_variant for every launch)
     std::vector<int> lvar[2][mx::NSCHED];
@@ -187,6 +215,7 @@ struct mx_plan {
     int ev_used = 0;
 
     ~mx_plan() {
+        for (auto &sw : sweep) sw.release();
         for (auto &pd : prog)
             for (auto &pk : pd)
                 for (auto &ps : pk) ps.release();
@@ -244,6 +273,26 @@ void pair_tiles(TaskLists &L);
 TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir, int sched,
                       const int64_t *wrow_in, const int64_t *skel_in);
 GradLists build_grad_lists(const mx_structure &s, const Topology &t);
+
+// The dependency-driven sweep of one direction (protocol in mx_bf16x3.h),
+// built from the BF16x3 fused-schedule lists: the same tasks with each
+// task's u^-reading segments moved last, its early and late waits, and the
+// launches' XCD groups as queues.
+struct SweepLists {
+    std::vector<Task> tasks;
+    std::vector<Seg> segs;
+    std::vector<SweepTask> stasks;
+    std::vector<SweepWait> waits;
+    std::vector<SweepTile> tiles;   // queue 0, then queue 1, ...
+    std::vector<int2> qinfo;        // per queue: first tile, tiles
+    int nblocks = 0;                // counter blocks
+    bool ok = false;                // false: the lists have a block with several writers
+};
+// X: fused-schedule BF16x3 lists after order_tiles / split_long_tasks;
+// sprow: first panel row of every skeleton node.
+SweepLists build_sweep(const TaskLists &X, int numnodes, const std::vector<int64_t> &sprow);
+// Ints of a sweep control block for nrhs columns (the widest tiling).
+int64_t sweep_ctl_ints(int numnodes, int nrhs);
 
 // ---- device buffers ----------------------------------------------------------
 
@@ -317,6 +366,7 @@ struct Work {
     void *wsk = nullptr, *usk = nullptr;
     void *io_w = nullptr, *io_y = nullptr;
     uint16_t *panels = nullptr;
+    int32_t *sweep_ctl = nullptr;   // BF16x3: control block of the sweep (sweep_ctl_ints)
     bool prof = false;   // record launch events
     // FP64/FP32: recorded on the stream once the last launch writing wskel
     // (resp. uskel) is enqueued, so copies of them can start early
@@ -334,6 +384,13 @@ void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);   // forward 
 void ensure_io(mx_plan *P, int nrhs);                                 // FP32 staging of FP64 W and Y
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);
+// Bytes of the split panels for nrhs columns, and of panels plus the sweep
+// control block after them (at offset round256(panel bytes)).
+size_t panel_bytes_of(const mx_plan *P, int nrhs);
+size_t panel_ctl_bytes_of(const mx_plan *P, int nrhs);
+// The sweep's workgroup count for an evaluation of nrhs columns (queried
+// once per tile width).
+int sweep_grid_for(mx_plan *P, int dir, int nrhs);
 // Record what an evaluation (eager, captured or replayed) leaves in the work
 // buffers: the skeletons of direction dir at nrhs columns.
 void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs);

@@ -666,6 +666,125 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
     return L;
 }
 
+int64_t sweep_ctl_ints(int numnodes, int nrhs) {
+    return SWEEP_CNT + (int64_t)2 * numnodes * ((nrhs + 31) / 32);
+}
+
+SweepLists build_sweep(const TaskLists &X, int numnodes, const std::vector<int64_t> &sprow) {
+    SweepLists S;
+    S.nblocks = 2 * numnodes;
+    std::map<int64_t, int> node_at;   // first panel row -> skeleton node
+    for (int v = 0; v < numnodes; ++v) node_at[sprow[v]] = v;
+    // counter block of a panel block: w^ of node v is v, u^ of node v is numnodes + v
+    auto block_of = [&](int buf, int64_t row) {
+        auto it = node_at.find(row);
+        if (it == node_at.end()) throw MxError(MX_ERR_ARG, "internal: a skeleton block without a node");
+        return buf == VEC_WSKEL ? it->second : numnodes + it->second;
+    };
+    const int ntask = (int)X.tasks.size();
+    auto rowtiles = [&](int t) { return (X.tasks[t].m + X3_BM - 1) / X3_BM; };
+    // the tasks of the launches, and the one writer of every output block
+    std::vector<char> used(ntask, 0);
+    for (const int2 &tl : X.tiles)
+        if (tl.x != X.dummy) used[tl.x] = 1;
+    std::vector<int> writer(S.nblocks, -1);
+    std::set<int64_t> y_rows;
+    for (int t = 0; t < ntask; ++t) {
+        if (!used[t]) continue;
+        const Task &tk = X.tasks[t];
+        if (tk.c_buf == VEC_Y) {
+            if (!y_rows.insert(tk.c_row).second) return S;   // several writers of a Y block
+        } else {
+            const int b = block_of(tk.c_buf, tk.c_row);
+            if (writer[b] >= 0) return S;                      // several writers of a skeleton block
+            writer[b] = t;
+        }
+    }
+    // tasks with their u^ segments last, and their waits
+    S.tasks = X.tasks;
+    S.stasks.assign(ntask, SweepTask{0, 0, 0, 0, -1, -1, {0, 0}});
+    for (int t = 0; t < ntask; ++t) {
+        Task &tk = S.tasks[t];
+        const int b0 = (int)S.segs.size();
+        std::vector<Seg> early, late;
+        for (int i = X.tasks[t].seg_begin; i < X.tasks[t].seg_end && t != X.dummy; ++i)
+            (X.segs[i].b_buf == VEC_USKEL ? late : early).push_back(X.segs[i]);
+        SweepTask &st = S.stasks[t];
+        auto add_waits = [&](const std::vector<Seg> &segs) {
+            std::set<int> seen;
+            for (const Seg &g : segs) {
+                if (g.b_buf != VEC_WSKEL && g.b_buf != VEC_USKEL) continue;
+                const int b = block_of(g.b_buf, g.b_row);
+                if (!seen.insert(b).second) continue;
+                if (writer[b] < 0) throw MxError(MX_ERR_ARG, "internal: a skeleton block read but never written");
+                S.waits.push_back(SweepWait{b, SWEEP_WAVES * rowtiles(writer[b])});
+            }
+        };
+        st.ew_b = (int)S.waits.size();
+        add_waits(early);
+        st.ew_e = (int)S.waits.size();
+        st.lw_b = st.ew_e;
+        add_waits(late);
+        st.lw_e = (int)S.waits.size();
+        st.late_seg = late.empty() ? -1 : (int)early.size();
+        st.sig = tk.c_buf == VEC_Y || t == X.dummy ? -1 : block_of(tk.c_buf, tk.c_row);
+        for (const Seg &g : early) S.segs.push_back(g);
+        for (const Seg &g : late) S.segs.push_back(g);
+        tk.seg_begin = b0;
+        tk.seg_end = (int)S.segs.size();
+    }
+    // queues: the XCD groups of every launch (eight contiguous parts, as
+    // xcd_remap deals them), launch by launch; rank g interleaves the
+    // groups position by position
+    std::vector<std::vector<SweepTile>> q(SWEEP_NQ);
+    std::map<std::pair<int, int>, std::pair<int, int>> at;   // (task, row tile) -> (queue, index)
+    int gbase = 0;
+    for (const Launch &ln : X.launches) {
+        const int n = ln.ntiles, per = n / SWEEP_NQ, rem = n % SWEEP_NQ;
+        int longest = 0;
+        for (int z = 0; z < SWEEP_NQ; ++z) {
+            const int b = z * per + std::min(z, rem), e = b + per + (z < rem ? 1 : 0);
+            int p = 0;
+            for (int i = b; i < e; ++i) {
+                const int2 tl = X.tiles[ln.tile_offset + i];
+                if (tl.x == X.dummy) continue;   // (a task without segments still writes zeros)
+                SweepTile st{};
+                st.task = tl.x;
+                st.rt = tl.y;
+                st.g = gbase + p * SWEEP_NQ + z;
+                at[{tl.x, tl.y}] = {z, (int)q[z].size()};
+                q[z].push_back(st);
+                ++p;
+            }
+            longest = std::max(longest, p);
+        }
+        gbase += longest * SWEEP_NQ;
+    }
+    // producers: every row tile of the writer of each block a task reads;
+    // each must rank below the consumer (a producer is in an earlier launch)
+    for (int z = 0; z < SWEEP_NQ; ++z)
+        for (SweepTile &st : q[z]) {
+            for (int y = 0; y < SWEEP_NQ; ++y) st.maxtile[y] = -1;
+            const SweepTask &sk = S.stasks[st.task];
+            for (int wi = sk.ew_b; wi < sk.lw_e; ++wi) {
+                const int p = writer[S.waits[wi].block];
+                for (int r = 0; r < rowtiles(p); ++r) {
+                    auto it = at.find({p, r});
+                    if (it == at.end()) throw MxError(MX_ERR_ARG, "internal: a producer tile outside the launches");
+                    const int y = it->second.first, j = it->second.second;
+                    if (q[y][j].g >= st.g) throw MxError(MX_ERR_ARG, "internal: a producer ranked after its consumer");
+                    st.maxtile[y] = std::max(st.maxtile[y], j);
+                }
+            }
+        }
+    for (int z = 0; z < SWEEP_NQ; ++z) {
+        S.qinfo.push_back(make_int2((int)S.tiles.size(), (int)q[z].size()));
+        S.tiles.insert(S.tiles.end(), q[z].begin(), q[z].end());
+    }
+    S.ok = true;
+    return S;
+}
+
 // Row tile r of a task maps to row pair r / 2; each pair appears once per
 // launch, where its first tile was.  A launch that order_tiles cut into
 // eight padded XCD groups keeps that cut (each group re-padded to the

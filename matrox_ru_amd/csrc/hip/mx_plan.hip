@@ -59,6 +59,26 @@ static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     }
 }
 
+// First padded panel row of every W leaf block and skeleton block (BF16x3
+// split panels), and the rows of each panel buffer.
+static void panel_rows(const mx_structure &s, std::vector<int64_t> &wprow, std::vector<int64_t> &sprow,
+                       int64_t *wp_rows, int64_t *sp_rows) {
+    wprow.assign(s.nleaf, 0);
+    sprow.assign(s.numnodes, 0);
+    int64_t r = 0;
+    for (int l = 0; l < s.nleaf; ++l) {
+        wprow[l] = r;
+        r += x3_pad_rows(s.Ddim[l]);
+    }
+    *wp_rows = r;
+    r = 0;
+    for (int v = 0; v < s.numnodes; ++v) {
+        sprow[v] = r;
+        r += x3_pad_rows(s.slen[v]);
+    }
+    *sp_rows = r;
+}
+
 mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int beta_y, const GenInfo *gen) {
     arg_check(s.nleaf > 0 && s.numnodes >= 1, "empty structure");
     arg_check(s.numnodes == 2 * s.nleaf - 1, "numnodes must be 2*nleaf-1 for a binary tree");
@@ -112,20 +132,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     if (precisions & MX_PLAN_BF16X3) {
         // split panels: padded row offsets of W leaf blocks and skeleton blocks
-        P->wprow.assign(s.nleaf, 0);
-        P->sprow.assign(s.numnodes, 0);
-        int64_t r = 0;
-        for (int l = 0; l < s.nleaf; ++l) {
-            P->wprow[l] = r;
-            r += x3_pad_rows(s.Ddim[l]);
-        }
-        P->wp_rows = r;
-        r = 0;
-        for (int v = 0; v < s.numnodes; ++v) {
-            P->sprow[v] = r;
-            r += x3_pad_rows(s.slen[v]);
-        }
-        P->sp_rows = r;
+        panel_rows(s, P->wprow, P->sprow, &P->wp_rows, &P->sp_rows);
         std::vector<X3Rows> wr(s.nleaf);
         for (int l = 0; l < s.nleaf; ++l) wr[l] = X3Rows{s.wrow[l], P->wprow[l], s.Ddim[l], 0};
         P->d_wrows = upload(wr.data(), wr.size());
@@ -170,6 +177,20 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                     sg.a_off = dst;
                     sg.a_trans = 0;
                     sg.lda = (sg.k + 31) / 32;
+                }
+            }
+            if (sc == SCHED_FUSED) {
+                SweepLists SW = build_sweep(X, s.numnodes, P->sprow);
+                if (SW.ok) {
+                    mx_plan::Sweep &sw = P->sweep[dir];
+                    sw.tasks = upload(SW.tasks.data(), SW.tasks.size());
+                    sw.segs = upload(SW.segs.data(), SW.segs.size());
+                    sw.st = upload(SW.stasks.data(), SW.stasks.size());
+                    sw.w = upload(SW.waits.data(), SW.waits.size());
+                    sw.q = upload(SW.tiles.data(), SW.tiles.size());
+                    sw.qinfo = upload(SW.qinfo.data(), SW.qinfo.size());
+                    sw.qinfo_h = SW.qinfo;
+                    sw.ok = true;
                 }
             }
             compute_launch_flops(X);
@@ -359,8 +380,29 @@ void ensure_io(mx_plan *P, int nrhs) {
     }
 }
 
+size_t panel_bytes_of(const mx_plan *P, int nrhs) {
+    return ((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t);
+}
+
+size_t panel_ctl_bytes_of(const mx_plan *P, int nrhs) {
+    return ((panel_bytes_of(P, nrhs) + 255) & ~(size_t)255) + (size_t)sweep_ctl_ints(P->numnodes, nrhs) * sizeof(int32_t);
+}
+
+int sweep_grid_for(mx_plan *P, int dir, int nrhs) {
+    const int v = P->sweep_var[dir];
+    const int k = v == X3_V128x64_S2 ? 1 : v == X3_V128x32_S2 ? 2 : 0;
+    if (P->sweep_grid[k] == 0) P->sweep_grid[k] = sweep_max_grid(v);
+    if (P->sweep_grid[k] <= 0) throw MxError(MX_ERR_HIP, "sweep: no occupancy for the kernel");
+    const int bn = sweep_bn(v), ncols = (nrhs + bn - 1) / bn;
+    int64_t items = 0;
+    for (const int2 &qi : P->sweep[dir].qinfo_h) items += (int64_t)qi.y * ncols;
+    return (int)std::min<int64_t>(P->sweep_grid[k], std::max<int64_t>(items, 1));
+}
+
+// The panels buffer holds the split panels, then (from a 256-byte boundary)
+// the sweep's control block.
 void ensure_panels(mx_plan *P, int nrhs) {
-    size_t bytes = ((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t);
+    size_t bytes = panel_ctl_bytes_of(P, nrhs);
     if (bytes > P->panel_bytes) {
         if (P->panels) HIPCHK(hipFree(P->panels));
         P->panels = nullptr;
@@ -452,6 +494,7 @@ void eval_checks(const mx_plan *P, int dir, mx_precision prec, const void *W, co
 void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs) {
     if (prec == MX_BF16X3) {
         ensure_panels(P, nrhs);
+        if (P->sweep_on[dir] && P->sweep[dir].ok) (void)sweep_grid_for(P, dir, nrhs);   // occupancy, before any capture
         return;
     }
     const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
@@ -546,6 +589,29 @@ void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, i
         mark(MX_KIND_CONVERT, true);
         launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
         mark(MX_KIND_CONVERT, false);
+        if (P->sweep_on[dir] && P->sweep[dir].ok) {
+            const mx_plan::Sweep &sw = P->sweep[dir];
+            if (!wk.sweep_ctl) throw MxError(MX_ERR_STATE, "internal: no sweep control block");
+            X3SweepArgs sa{};
+            sa.x = xa;
+            sa.x.tasks = sw.tasks;
+            sa.x.segs = sw.segs;
+            sa.q = sw.q;
+            sa.qinfo = sw.qinfo;
+            sa.st = sw.st;
+            sa.w = sw.w;
+            sa.ctl = wk.sweep_ctl;
+            const int v = P->sweep_var[dir], bn = sweep_bn(v);
+            sa.ncols = (nrhs + bn - 1) / bn;
+            const int ctl_ints = SWEEP_CNT + 2 * P->numnodes * sa.ncols;
+            const int grid = sweep_grid_for(P, dir, nrhs);
+            mark(MX_KIND_BF16X3, true);
+            launch_x3_sweep(sa, v, grid, ctl_ints, st);
+            mark(MX_KIND_BF16X3, false);
+            P->sweep_last_ctl = wk.sweep_ctl;
+            HIPCHK(hipGetLastError());
+            return;
+        }
         for (size_t i = 0; i < pg1.launches.size(); ++i) {
             const int v = lv.size() == pg1.launches.size() ? lv[i] : P->x3_variant;
             const int bn = x3_variant_bn(v);
@@ -572,6 +638,9 @@ void eval(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nr
     wk.io_w = P->io_w;
     wk.io_y = P->io_y;
     wk.panels = P->panels;
+    if (prec == MX_BF16X3)
+        wk.sweep_ctl = reinterpret_cast<int32_t *>(reinterpret_cast<unsigned char *>(P->panels) +
+                                                   ((panel_bytes_of(P, nrhs) + 255) & ~(size_t)255));
     wk.prof = P->profiling != 0;
     if (!P->keep_prof) P->ev_used = 0;
     eval_core(P, dir, prec, W, Y, nrhs, io, st, wk);
@@ -847,10 +916,23 @@ static void drop_graphs(mx_plan *P) {
 
 static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int variant) {
     arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
-    arg_check(sched >= 0 && sched < NSCHED, "bad schedule");
+    arg_check((sched >= 0 && sched < NSCHED) || sched == MX_SCHED_SWEEP, "bad schedule");
     arg_check(variant >= 0 && variant <= X3_NVARIANT, "bad BF16x3 kernel variant (0 keeps)");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     const int kd = prec == MX_BF16X3 ? 1 : 0;
+    if (sched == MX_SCHED_SWEEP) {
+        if (prec != MX_BF16X3) throw MxError(MX_ERR_UNSUPPORTED, "the sweep schedule is for MX_BF16X3");
+        if (!(P->precisions & MX_PLAN_BF16X3)) throw MxError(MX_ERR_UNSUPPORTED, "precision not built into the plan");
+        if (!P->sweep[dir].ok)
+            throw MxError(MX_ERR_UNSUPPORTED, "no sweep for this plan (a block with several writers)");
+        arg_check(variant == 0 || sweep_variant_ok(variant),
+                  "sweep tile: MX_X3_128x128_S2, MX_X3_128x64_S2 or MX_X3_128x32_S2 (0 keeps)");
+        P->sweep_on[dir] = 1;
+        if (variant) P->sweep_var[dir] = variant;
+        drop_graphs(P);
+        return;
+    }
+    if (kd == 1) P->sweep_on[dir] = 0;
     P->sched[dir][kd] = sched;
     if (variant) {
         P->x3_variant = variant;
@@ -894,6 +976,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         const int kd = prec == MX_BF16X3 ? 1 : 0;
         const int ndir = P->adjoint ? 2 : 1;
         const int nvar = prec == MX_BF16X3 ? X3_NVARIANT : 1;
+        if (prec == MX_BF16X3) P->sweep_on[0] = P->sweep_on[1] = 0;   // level launches first
         const int cur_variant = P->x3_variant;
         const int cur_sched[2] = {P->sched[0][kd], P->sched[1][kd]};
         int best_variant = cur_variant;
@@ -1013,6 +1096,36 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                 lv = pick;
             }
             P->profiling = 0;
+            // the dependency-driven sweep (three tile widths) against the
+            // tuned level launches
+            for (int dir = 0; dir < ndir; ++dir) {
+                if (!P->sweep[dir].ok) continue;
+                auto time_eval = [&]() {
+                    eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
+                    HIPCHK(hipEventRecord(e0, st));
+                    for (int r = 0; r < reps; ++r) eval(P, dir, prec, W, Y, nrhs, io, st);
+                    HIPCHK(hipEventRecord(e1, st));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    return ms / reps;
+                };
+                P->sweep_on[dir] = 0;
+                float best = time_eval();
+                int pick = 0;
+                for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2}) {
+                    P->sweep_on[dir] = 1;
+                    P->sweep_var[dir] = v;
+                    const float ms = time_eval();
+                    if (ms < 0.98f * best) {
+                        best = ms;
+                        pick = v;
+                    }
+                }
+                P->sweep_on[dir] = pick != 0;
+                if (pick) P->sweep_var[dir] = pick;
+                best_dir[dir] = best;
+            }
         }
         if (best_ms) {
             best_ms[0] = best_dir[0];
@@ -1238,7 +1351,8 @@ static size_t noalloc_layout(const mx_plan *P, mx_precision prec, mx_io_type io,
         return p;
     };
     if (prec == MX_BF16X3) {
-        wk->panels = (uint16_t *)take(((size_t)2 * P->wp_rows + (size_t)4 * P->sp_rows) * nrhs * sizeof(uint16_t));
+        wk->panels = (uint16_t *)take(panel_bytes_of(P, nrhs));
+        wk->sweep_ctl = (int32_t *)take((size_t)sweep_ctl_ints(P->numnodes, nrhs) * sizeof(int32_t));
     } else {
         const size_t elem = prec == MX_FP64 ? sizeof(double) : sizeof(float);
         const size_t sb = (size_t)std::max<int64_t>(P->skel_rows, 1) * nrhs * elem;
@@ -1440,12 +1554,61 @@ mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision pr
     });
 }
 
+mx_status mx_sweep_describe(const mx_structure *s, int32_t direction, int32_t *ntiles, int32_t *ntasks,
+                            int32_t *nwaits, int32_t *tiles, int32_t *qinfo, int32_t *tasks, int32_t *waits) {
+    return (mx_status)guard([&]() {
+        arg_check(s && ntiles && ntasks && nwaits && (direction == 0 || direction == 1), "bad argument");
+        Topology t = topology(*s);
+        validate(*s, t);
+        std::vector<int64_t> wprow, sprow;
+        int64_t wp = 0, sp = 0;
+        panel_rows(*s, wprow, sprow, &wp, &sp);
+        TaskLists X = build_lists(*s, t, 0, 0, 1, direction, SCHED_FUSED, wprow.data(), sprow.data());
+        split_long_tasks(X, X3_MAXSEG);
+        SweepLists SW = build_sweep(X, s->numnodes, sprow);
+        if (!SW.ok) throw MxError(MX_ERR_UNSUPPORTED, "no sweep for this structure (a block with several writers)");
+        const bool fill = tiles != nullptr;
+        arg_check(!fill || (*ntiles >= (int32_t)SW.tiles.size() && *ntasks >= (int32_t)SW.stasks.size() &&
+                            *nwaits >= (int32_t)SW.waits.size() && qinfo && tasks && waits),
+                  "arrays too small");
+        *ntiles = (int32_t)SW.tiles.size();
+        *ntasks = (int32_t)SW.stasks.size();
+        *nwaits = (int32_t)SW.waits.size();
+        if (!fill) return 0;
+        static_assert(sizeof(SweepTile) == 12 * sizeof(int32_t) && sizeof(SweepTask) == 8 * sizeof(int32_t),
+                      "descriptor layout");
+        std::memcpy(tiles, SW.tiles.data(), SW.tiles.size() * sizeof(SweepTile));
+        std::memcpy(tasks, SW.stasks.data(), SW.stasks.size() * sizeof(SweepTask));
+        for (size_t i = 0; i < SW.waits.size(); ++i) {
+            waits[2 * i] = SW.waits[i].block;
+            waits[2 * i + 1] = SW.waits[i].target;
+        }
+        for (int z = 0; z < SWEEP_NQ; ++z) {
+            qinfo[2 * z] = SW.qinfo[z].x;
+            qinfo[2 * z + 1] = SW.qinfo[z].y;
+        }
+        return 0;
+    });
+}
+
+mx_status mx_plan_sweep_status(const mx_plan *plan, void *stream, int32_t *error) {
+    return (mx_status)guard([&]() {
+        arg_check(plan && error, "null argument");
+        *error = 0;
+        HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+        if (plan->sweep_last_ctl)
+            HIPCHK(hipMemcpy(error, plan->sweep_last_ctl + SWEEP_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
+        return 0;
+    });
+}
+
 mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *schedule,
                                int32_t *x3_variant) {
     return (mx_status)guard([&]() {
         arg_check(plan && schedule && x3_variant && (direction == 0 || direction == 1), "bad argument");
-        *schedule = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
-        *x3_variant = plan->x3_variant;
+        const bool sw = prec == MX_BF16X3 && plan->sweep_on[direction];
+        *schedule = sw ? MX_SCHED_SWEEP : plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
+        *x3_variant = sw ? plan->sweep_var[direction] : plan->x3_variant;
         return 0;
     });
 }

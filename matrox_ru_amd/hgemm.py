@@ -18,7 +18,7 @@ FP64, FP32, BF16X3 = 0, 1, 2
 KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32",
               6: "grad_bf16x3"}
 PLAN_ADJOINT = 1 << 8
-SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3}
+SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5"}
@@ -88,6 +88,10 @@ def lib():
         L.mx_plan_set_schedule.restype = ctypes.c_int
         L.mx_plan_get_schedule.argtypes = [P, I32, ctypes.c_int, ctypes.POINTER(I32), ctypes.POINTER(I32)]
         L.mx_plan_get_schedule.restype = ctypes.c_int
+        L.mx_sweep_describe.argtypes = [P, I32] + [ctypes.POINTER(I32)] * 3 + [P] * 4
+        L.mx_sweep_describe.restype = ctypes.c_int
+        L.mx_plan_sweep_status.argtypes = [P, P, ctypes.POINTER(I32)]
+        L.mx_plan_sweep_status.restype = ctypes.c_int
         L.mx_plan_autotune.argtypes = [P, ctypes.c_int, I32, ctypes.c_int, I32, P, P]
         L.mx_plan_autotune.restype = ctypes.c_int
         L.mx_plan_launch_variants.argtypes = [P, I32, P, I32, ctypes.POINTER(I32)]
@@ -163,6 +167,25 @@ def _check(rc):
 
 def _np(a, dt):
     return np.ascontiguousarray(a, dtype=dt)
+
+
+def sweep_describe(hm, direction="forward"):
+    """Host-only description of the dependency-driven sweep's work lists
+    (mx_sweep_describe): dict of tiles (T x 12), qinfo (8 x 2), tasks
+    (n x 8) and waits (W x 2), int32."""
+    st, keep = structure_of(hm)
+    nt, nk, nw = I32(0), I32(0), I32(0)
+    sp = ctypes.byref(st)
+    _check(lib().mx_sweep_describe(sp, DIRECTIONS[direction], ctypes.byref(nt), ctypes.byref(nk), ctypes.byref(nw),
+                                   None, None, None, None))
+    out = dict(tiles=np.zeros((nt.value, 12), np.int32), qinfo=np.zeros((8, 2), np.int32),
+               tasks=np.zeros((nk.value, 8), np.int32), waits=np.zeros((max(nw.value, 1), 2), np.int32))
+    _check(lib().mx_sweep_describe(sp, DIRECTIONS[direction], ctypes.byref(nt), ctypes.byref(nk), ctypes.byref(nw),
+                                   out["tiles"].ctypes.data, out["qinfo"].ctypes.data, out["tasks"].ctypes.data,
+                                   out["waits"].ctypes.data))
+    out["waits"] = out["waits"][:nw.value]
+    del keep
+    return out
 
 
 def structure_of(hm):
@@ -314,9 +337,11 @@ class Plan:
         _check(lib().mx_plan_set_blocks(self._h, ps[0], ps[1], ps[2], _stream_handle(stream)))
 
     def set_schedule(self, direction, precision, schedule, x3_variant=0):
-        """Pick the coupling/downward schedule ("fused" or "split") of one
-        direction ("forward"/"adjoint") for a precision, and optionally the
-        BF16x3 kernel variant (X3_VARIANTS; mx_plan_set_schedule)."""
+        """Pick the coupling/downward schedule (SCHEDULES) of one direction
+        ("forward"/"adjoint") for a precision, and optionally the BF16x3
+        kernel variant (X3_VARIANTS; mx_plan_set_schedule).  "sweep" (BF16x3
+        only) runs the whole evaluation as one dependency-driven launch; its
+        x3_variant is the tile width (1, 5 or 6)."""
         _check(lib().mx_plan_set_schedule(self._h, DIRECTIONS[direction], PRECISIONS[precision],
                                           SCHEDULES[schedule], x3_variant))
 
@@ -326,6 +351,13 @@ class Plan:
                                           ctypes.byref(st)))
         names = {v: k for k, v in SCHEDULES.items()}
         return names[sc.value], st.value
+
+    def sweep_status(self, stream=None):
+        """Error word of the last sweep (mx_plan_sweep_status): 0 clean, 1 a
+        wait timed out, 2 claiming a work item timed out."""
+        e = I32()
+        _check(lib().mx_plan_sweep_status(self._h, _stream_handle(stream), ctypes.byref(e)))
+        return e.value
 
     def autotune(self, precision, nrhs, io="f64", reps=3, stream=None):
         """mx_plan_autotune; returns the chosen per-direction times in ms."""
