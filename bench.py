@@ -438,6 +438,18 @@ def run_batched(args):
     ms64 = timer(lambda: b.eval(W64, Y64, "fp64"), max(3, args.steps // 4), 1)
     ms32 = timer(lambda: b.eval(W32, Y32, "fp32"), args.steps, args.warmup)
     msmx = timer(lambda: b.eval(W32, Ymx, "bf16x3"), args.steps, args.warmup)
+    # the batched sweep (every BF16x3 group in one launch) against the
+    # groups' own concurrent launches, whichever the autotuner kept
+    sweep_pick = b.get_sweep("forward")
+    launch_forms = {}
+    Yt = b.buffers(torch.float32)
+    for v in (0, 1, 5, 6):
+        b.set_sweep("forward", v)
+        launch_forms["one_launch_w" + str(v) if v else "group_launches"] = round(
+            timer(lambda: b.eval(W32, Yt, "bf16x3"), args.steps, args.warmup), 4)
+    b.set_sweep("forward", sweep_pick)
+    del Yt
+    log(f"[bench] batch bf16x3 launch forms (ms): {launch_forms}; autotuned sweep width {sweep_pick}")
     fl = b.flops("fp64")
     log(f"[bench] batch fp64 {ms64:.3f} ms, fp32 {ms32:.3f} ms, bf16x3 {msmx:.3f} ms; {fl / 1e12:.4f} TFLOP")
     # kernel roofline per group, each group's plan run alone (HIP events per
@@ -448,7 +460,11 @@ def run_batched(args):
     x3_fl, x3_ms = 0.0, 0.0
     for g, (nr, rows) in enumerate(b.groups):
         gp = b.plan(g)
-        _, parts_g = gp.flops("bf16x3", nr)
+        try:
+            _, parts_g = gp.flops("bf16x3", nr)
+        except hgemm.MxError:   # a group the accuracy guard routed to FP32 has no BF16x3 lists
+            roof_groups.append({"group_nrhs": nr, "group_rows": rows, "precision": "fp32 (accuracy guard)"})
+            continue
         Yg = torch.empty_like(W32[g])
         roof, kinds = launch_profile(gp, lambda: gp.eval(W32[g], Yg, nr, "bf16x3"), "bf16x3", nr, parts_g, nprof=2)
         ceiling = min(MI355X_PEAK["bf16"] / 3, nr / 2 * MI355X_PEAK["hbm_gbs"] / 1e3)
@@ -494,6 +510,7 @@ def run_batched(args):
                    "groups": [{"nrhs": g[0], "rows": g[1]} for g in b.groups]},
         "algorithmic_tflop_per_eval": round(fl / 1e12, 5),
         "fp64_ms": round(ms64, 4), "fp32_ms": round(ms32, 4), "bf16x3_ms": round(msmx, 4),
+        "bf16x3_launch_forms_ms": launch_forms, "batched_sweep_width": sweep_pick,
         "fp32_tflops": round(fl / (ms32 * 1e-3) / 1e12, 3), "bf16x3_tflops": round(fl / (msmx * 1e-3) / 1e12, 3),
         "fp64_tflops": round(fl / (ms64 * 1e-3) / 1e12, 3),
         "max_rel_err_fp32_vs_fp64": max(errs32), "max_rel_err_bf16x3_vs_fp64": max(errsmx),

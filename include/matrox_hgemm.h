@@ -437,8 +437,23 @@ mx_status mx_batch_block_layout(const mx_batch *batch, int32_t instance, int64_t
 /* Capture the whole batch (all groups and their fork/join) as one hipGraph
  * per (direction, precision, io, W, Y) and replay it. */
 mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable);
-/* mx_plan_autotune on every group with its own nrhs. */
+/* mx_plan_autotune on every group with its own nrhs; with MX_BF16X3, also
+ * the batched sweep below (on or off, and its tile width) for the whole
+ * batch. */
 mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream);
+
+/* Batched sweep: every BF16x3 group of the batch (all nrhs) evaluated by one
+ * dependency-driven launch (MX_SCHED_SWEEP's protocol over the union of
+ * the groups' work lists), after each group's W split; FP32-routed groups
+ * run beside it.  x3_variant = tile width (MX_X3_128x128_S2,
+ * MX_X3_128x64_S2 or MX_X3_128x32_S2), 0 = off (every group runs its own
+ * plan's launches, concurrently).  Per direction; drops captured graphs.
+ * No counterpart in the reference, which evaluates one matrix per
+ * process (matroxTest-V1/main.cpp:22-115). */
+mx_status mx_batch_set_sweep(mx_batch *batch, int32_t direction, int32_t x3_variant);
+mx_status mx_batch_get_sweep(const mx_batch *batch, int32_t direction, int32_t *x3_variant);
+/* Error word of the batched sweeps (as mx_plan_sweep_status). */
+mx_status mx_batch_sweep_status(const mx_batch *batch, void *stream, int32_t *error);
 mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total);
 /* Per-instance dense check rows, as mx_plan_set_points / mx_dense_rows; W is
  * the instance's FP64 block inside its group buffer, out is nrows x nrhs. */

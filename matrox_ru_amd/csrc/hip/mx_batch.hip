@@ -194,7 +194,46 @@ struct mx_batch {
     };
     std::vector<GraphEntry> graphs;
     uint64_t clock = 0;
+    // Per direction: the BF16x3 groups evaluated together as one
+    // dependency-driven launch (sweep_var = its tile width, 0 = off: every
+    // group runs its own plan's launches).  The merged queues are built for
+    // one tile width; a group table (operands, panels, Y of every group) is
+    // kept per (Y pointers, I/O type, buffer generation).
+    int sweep_var[2] = {0, 0};
+    struct BSweep {
+        int var = 0;                     // tile width the lists were built for (0: none)
+        std::vector<int> groups;         // the batch group of each sweep group
+        std::vector<int> cbase, ncols;   // per sweep group: first counter, column tiles
+        mx::SweepTile *q = nullptr;
+        int2 *qinfo = nullptr;
+        int64_t items = 0;
+        int ctl_ints = 0;
+        int32_t *ctl = nullptr;
+        struct Table {
+            std::vector<void *> Y;
+            int io;
+            uint64_t gen;
+            mx::SweepGroup *d;
+        };
+        std::vector<Table> tables;
+        void release() {
+            for (void *p : {(void *)q, (void *)qinfo, (void *)ctl})
+                if (p) (void)hipFree(p);
+            for (auto &t : tables) (void)hipFree(t.d);
+            q = nullptr;
+            qinfo = nullptr;
+            ctl = nullptr;
+            tables.clear();
+            groups.clear();
+            cbase.clear();
+            ncols.clear();
+            items = 0;
+            ctl_ints = 0;
+            var = 0;
+        }
+    } bsweep[2];
     ~mx_batch() {
+        for (auto &b : bsweep) b.release();
         for (auto *p : plans) delete p;
         for (auto st : streams) (void)hipStreamDestroy(st);
         for (auto e : events) (void)hipEventDestroy(e);
@@ -375,27 +414,201 @@ static mx_batch *create_guarded_batch(const mx_structure *const *in, const int32
     return B.release();
 }
 
+static uint64_t batch_gen(const mx_batch *B);
+
+// ---- batched sweep: every BF16x3 group in one launch ------------------------
+//
+// The groups' sweep lists (each its plan's, as for mx_eval) are merged into
+// one set of eight queues.  Column tiles become explicit entries (row tile
+// | column tile << 16, ncols = 1), since the groups' nrhs differ.  The
+// groups' DAGs are independent, so any interleaving of their rank orders
+// is a rank order of the union: items are ordered by their rank within
+// their group divided by the group's item count, which spreads every group
+// over the whole launch.  Counters: one block per group, after the others.
+
+static bool batch_sweep_on(const mx_batch *B, int dir, mx_precision prec) {
+    return prec == MX_BF16X3 && B->sweep_var[dir] != 0;
+}
+
+static void build_batch_sweep(mx_batch *B, int dir, int var) {
+    mx_batch::BSweep &bs = B->bsweep[dir];
+    bs.release();
+    const int bn = sweep_bn(var);
+    struct Item {
+        double key;
+        int grp, r, z, j, ct;
+    };
+    std::vector<std::vector<Item>> perq(SWEEP_NQ);
+    std::vector<std::vector<std::vector<int>>> at;   // [sweep group][queue][tile * ncols + ct] -> merged index
+    int cb = SWEEP_CNT;
+    for (size_t g = 0; g < B->plans.size(); ++g) {
+        const mx_plan *P = B->plans[g];
+        if (B->group_fp32[g] || !(P->precisions & MX_PLAN_BF16X3)) continue;
+        const mx_plan::Sweep &sw = P->sweep[dir];
+        if (!sw.ok) throw MxError(MX_ERR_UNSUPPORTED, "batched sweep: a group without sweep lists");
+        const int si = (int)bs.groups.size();
+        const int nc = (B->group_nrhs[g] + bn - 1) / bn;
+        bs.groups.push_back((int)g);
+        bs.ncols.push_back(nc);
+        bs.cbase.push_back(cb);
+        cb += sw.nblocks * nc;
+        int maxg = 0;
+        for (const SweepTile &t : sw.h_tiles) maxg = std::max(maxg, t.g);
+        const double R = (double)(maxg + 1) * nc;
+        at.emplace_back(SWEEP_NQ);
+        for (int z = 0; z < SWEEP_NQ; ++z) {
+            const int2 qi = sw.qinfo_h[z];
+            at[si][z].assign((size_t)qi.y * nc, -1);
+            for (int j = 0; j < qi.y; ++j)
+                for (int ct = 0; ct < nc; ++ct) {
+                    const int r = sw.h_tiles[qi.x + j].g * nc + ct;
+                    perq[z].push_back(Item{(r + 0.5) / R, si, r, z, j, ct});
+                }
+        }
+    }
+    if (bs.groups.empty()) throw MxError(MX_ERR_UNSUPPORTED, "batched sweep: no BF16x3 group");
+    auto less = [](const Item &a, const Item &b) {
+        if (a.key != b.key) return a.key < b.key;
+        if (a.grp != b.grp) return a.grp < b.grp;
+        return a.r < b.r;
+    };
+    std::vector<Item> all;
+    for (int z = 0; z < SWEEP_NQ; ++z) {
+        std::sort(perq[z].begin(), perq[z].end(), less);
+        for (size_t p = 0; p < perq[z].size(); ++p) {
+            const Item &it = perq[z][p];
+            at[it.grp][z][(size_t)it.j * bs.ncols[it.grp] + it.ct] = (int)p;
+        }
+        all.insert(all.end(), perq[z].begin(), perq[z].end());
+    }
+    std::sort(all.begin(), all.end(), less);
+    std::map<std::tuple<int, int, int, int>, int> rank;   // (group, queue, tile, ct) -> global rank
+    for (size_t k = 0; k < all.size(); ++k) rank[std::make_tuple(all[k].grp, all[k].z, all[k].j, all[k].ct)] = (int)k;
+    std::vector<SweepTile> tiles;
+    std::vector<int2> qinfo;
+    for (int z = 0; z < SWEEP_NQ; ++z) {
+        qinfo.push_back(make_int2((int)tiles.size(), (int)perq[z].size()));
+        for (const Item &it : perq[z]) {
+            const mx_plan::Sweep &sw = B->plans[bs.groups[it.grp]]->sweep[dir];
+            const SweepTile &src = sw.h_tiles[sw.qinfo_h[it.z].x + it.j];
+            if (src.rt >= 65536 || it.ct >= 65536) throw MxError(MX_ERR_ARG, "batched sweep: too many tiles");
+            SweepTile t{};
+            t.task = src.task;
+            t.rt = src.rt | (it.ct << 16);
+            t.g = rank[std::make_tuple(it.grp, it.z, it.j, it.ct)];
+            t.grp = it.grp;
+            for (int y = 0; y < SWEEP_NQ; ++y) {
+                t.maxtile[y] = src.maxtile[y] < 0 ? -1 : at[it.grp][y][(size_t)src.maxtile[y] * bs.ncols[it.grp] + it.ct];
+                if (src.maxtile[y] >= 0 && (t.maxtile[y] < 0 || perq[y][t.maxtile[y]].key >= it.key))
+                    throw MxError(MX_ERR_ARG, "internal: batched sweep producer ranked after its consumer");
+            }
+            tiles.push_back(t);
+        }
+    }
+    bs.items = (int64_t)tiles.size();
+    bs.ctl_ints = cb;
+    bs.q = upload(tiles.data(), tiles.size());
+    bs.qinfo = upload(qinfo.data(), qinfo.size());
+    HIPCHK(hipMalloc(&bs.ctl, (size_t)cb * sizeof(int32_t)));
+    bs.var = var;
+}
+
+// The group table for outputs Y (batch groups) and io, built (and uploaded)
+// on first use; null when it does not exist yet and create is false.
+static SweepGroup *batch_sweep_table(mx_batch *B, int dir, mx_io_type io, void *const *Y, bool create,
+                                     uint64_t gen) {
+    mx_batch::BSweep &bs = B->bsweep[dir];
+    for (size_t i = 0; i < bs.tables.size();) {   // tables of moved buffers are stale
+        if (bs.tables[i].gen != gen) {
+            HIPCHK(hipFree(bs.tables[i].d));
+            bs.tables.erase(bs.tables.begin() + (std::ptrdiff_t)i);
+        } else {
+            ++i;
+        }
+    }
+    std::vector<void *> ys;
+    for (int g : bs.groups) ys.push_back(Y[g]);
+    for (auto &t : bs.tables)
+        if (t.Y == ys && t.io == (int)io) return t.d;
+    if (!create) return nullptr;
+    std::vector<SweepGroup> h(bs.groups.size());
+    for (size_t k = 0; k < bs.groups.size(); ++k) {
+        const int g = bs.groups[k];
+        mx_plan *P = B->plans[g];
+        SweepGroup &G = h[k];
+        G = SweepGroup{};
+        x3_args_of(P, B->group_nrhs[g], P->panels, G.x);
+        G.x.tasks = P->sweep[dir].tasks;
+        G.x.segs = P->sweep[dir].segs;
+        G.x.Y = Y[g];
+        G.x.y_f64 = io == MX_IO_F64;
+        G.st = P->sweep[dir].st;
+        G.w = P->sweep[dir].w;
+        G.ncols = bs.ncols[k];
+        G.cbase = bs.cbase[k];
+    }
+    if (bs.tables.size() >= 8) {
+        HIPCHK(hipFree(bs.tables.front().d));
+        bs.tables.erase(bs.tables.begin());
+    }
+    mx_batch::BSweep::Table t{ys, (int)io, gen, upload(h.data(), h.size())};
+    bs.tables.push_back(t);
+    return t.d;
+}
+
 // all groups, forked from and joined back to st; dir DIR_FWD (Y = K~ W) or
-// DIR_ADJ (Y = K~^T W)
+// DIR_ADJ (Y = K~^T W).  With the batched sweep on, the BF16x3 groups run
+// as one launch on st and the other groups beside it.
 static void batch_eval(mx_batch *B, int dir, mx_precision prec, const void *const *W, void *const *Y, mx_io_type io,
                        hipStream_t st) {
     const int ng = (int)B->plans.size();
     for (int g = 0; g < ng; ++g) arg_check(W[g] && Y[g], "null W or Y of group " + std::to_string(g));
-    if (ng == 1) {
+    const bool sw = batch_sweep_on(B, dir, prec);
+    std::vector<char> in_sweep(ng, 0);
+    if (sw) {
+        const mx_batch::BSweep &bs = B->bsweep[dir];
+        if (bs.var != B->sweep_var[dir]) throw MxError(MX_ERR_STATE, "internal: batched sweep lists not built");
+        for (int g : bs.groups) in_sweep[g] = 1;
+    }
+    if (ng == 1 && !sw) {
         eval(B->plans[0], dir, group_prec(B, 0, prec), W[0], Y[0], B->group_nrhs[0], io, st);
         return;
     }
-    HIPCHK(hipEventRecord(B->events[0], st));
+    if (ng > 1) HIPCHK(hipEventRecord(B->events[0], st));
     for (int g = 0; g < ng; ++g) {
+        if (in_sweep[g]) continue;
         HIPCHK(hipStreamWaitEvent(B->streams[g], B->events[0], 0));
         eval(B->plans[g], dir, group_prec(B, g, prec), W[g], Y[g], B->group_nrhs[g], io, B->streams[g]);
         HIPCHK(hipEventRecord(B->events[g + 1], B->streams[g]));
     }
-    for (int g = 0; g < ng; ++g) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
+    if (sw) {
+        mx_batch::BSweep &bs = B->bsweep[dir];
+        SweepGroup *tab = batch_sweep_table(B, dir, io, Y, false, batch_gen(B));
+        if (!tab) throw MxError(MX_ERR_STATE, "internal: batched sweep group table missing");
+        for (int g : bs.groups) {
+            mx_plan *P = B->plans[g];
+            const int n = B->group_nrhs[g];
+            const size_t wp = (size_t)P->wp_rows * n;
+            launch_x3_split_w(W[g], io == MX_IO_F64, P->d_wrows, P->nleaf, n, P->panels, P->panels + wp, st);
+        }
+        X3SweepArgs sa{};
+        sa.q = bs.q;
+        sa.qinfo = bs.qinfo;
+        sa.ctl = bs.ctl;
+        sa.ncols = 1;
+        sa.groups = tab;
+        const int grid = (int)std::min<int64_t>(std::max(sweep_max_grid(bs.var), 1), std::max<int64_t>(bs.items, 1));
+        launch_x3_sweep_batch(sa, bs.var, grid, bs.ctl_ints, st);
+        HIPCHK(hipGetLastError());
+    }
+    if (ng > 1)
+        for (int g = 0; g < ng; ++g)
+            if (!in_sweep[g]) HIPCHK(hipStreamWaitEvent(st, B->events[g + 1], 0));
 }
 
-// Allocate every work buffer an evaluation needs before it is captured.
-static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type io) {
+// Allocate every work buffer an evaluation needs before it is captured
+// (and, for the batched sweep, its lists and the group table of Y).
+static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type io, void *const *Y) {
     for (size_t g = 0; g < B->plans.size(); ++g) {
         mx_plan *P = B->plans[g];
         const mx_precision gp = group_prec(B, (int)g, prec);
@@ -403,6 +616,11 @@ static void batch_size_work(mx_batch *B, int dir, mx_precision prec, mx_io_type 
         if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
         if (gp == MX_FP64 && io != MX_IO_F64) throw MxError(MX_ERR_UNSUPPORTED, "FP64 evaluation needs FP64 W and Y");
         size_work(P, dir, gp, io, B->group_nrhs[g]);
+    }
+    if (batch_sweep_on(B, dir, prec)) {
+        if (B->bsweep[dir].var != B->sweep_var[dir]) build_batch_sweep(B, dir, B->sweep_var[dir]);
+        for (int g : B->bsweep[dir].groups) arg_check(Y[g] != nullptr, "null Y of group " + std::to_string(g));
+        (void)batch_sweep_table(B, dir, io, Y, true, batch_gen(B));
     }
 }
 
@@ -446,7 +664,7 @@ static void batch_eval_graph(mx_batch *B, int dir, mx_precision prec, const void
             return;
         }
     }
-    batch_size_work(B, dir, prec, io);
+    batch_size_work(B, dir, prec, io, Y);
     batch_drop_stale_graphs(B);
     const uint64_t gen = batch_gen(B);
     for (auto *P : B->plans) P->profiling = 0;
@@ -583,8 +801,50 @@ static int32_t batch_run(mx_batch *batch, int dir, mx_precision prec, const void
             throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
         bool prof = false;
         for (auto *P : batch->plans) prof = prof || P->profiling;
-        if (batch->use_graph && !prof) batch_eval_graph(batch, dir, prec, W, Y, io, (hipStream_t)stream);
-        else batch_eval(batch, dir, prec, W, Y, io, (hipStream_t)stream);
+        if (batch->use_graph && !prof) {
+            batch_eval_graph(batch, dir, prec, W, Y, io, (hipStream_t)stream);
+        } else {
+            batch_size_work(batch, dir, prec, io, Y);
+            batch_eval(batch, dir, prec, W, Y, io, (hipStream_t)stream);
+            if (batch_sweep_on(batch, dir, prec)) batch_note_eval(batch, dir, prec);
+        }
+        return 0;
+    });
+}
+
+mx_status mx_batch_set_sweep(mx_batch *batch, int32_t direction, int32_t x3_variant) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && (direction == 0 || direction == 1), "bad argument");
+        arg_check(x3_variant == 0 || sweep_variant_ok(x3_variant),
+                  "sweep tile: 0 (off), MX_X3_128x128_S2, MX_X3_128x64_S2 or MX_X3_128x32_S2");
+        if (direction == DIR_ADJ && !batch->plans[0]->adjoint)
+            throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
+        if (x3_variant && batch->sweep_var[direction] != x3_variant) build_batch_sweep(batch, direction, x3_variant);
+        batch->sweep_var[direction] = x3_variant;
+        batch_drop_graphs(batch);
+        return 0;
+    });
+}
+
+mx_status mx_batch_get_sweep(const mx_batch *batch, int32_t direction, int32_t *x3_variant) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && x3_variant && (direction == 0 || direction == 1), "bad argument");
+        *x3_variant = batch->sweep_var[direction];
+        return 0;
+    });
+}
+
+mx_status mx_batch_sweep_status(const mx_batch *batch, void *stream, int32_t *error) {
+    return (mx_status)guard([&]() {
+        arg_check(batch && error, "null argument");
+        *error = 0;
+        HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+        for (const auto &bs : batch->bsweep)
+            if (bs.ctl) {
+                int32_t e = 0;
+                HIPCHK(hipMemcpy(&e, bs.ctl + SWEEP_ERR, sizeof(int32_t), hipMemcpyDeviceToHost));
+                *error = std::max(*error, e);
+            }
         return 0;
     });
 }
@@ -649,6 +909,71 @@ mx_status mx_batch_set_graph(mx_batch *batch, int32_t enable) {
     });
 }
 
+// With BF16x3: time the whole batch with its groups' own launches and with
+// the batched sweep at each tile width, and keep the fastest (a choice
+// replaces the groups' own launches only when more than 2% faster).
+static void autotune_batch_sweep(mx_batch *B, mx_io_type io, int reps, hipStream_t st) {
+    bool any = false;
+    for (size_t g = 0; g < B->plans.size(); ++g)
+        any = any || (!B->group_fp32[g] && (B->plans[g]->precisions & MX_PLAN_BF16X3) && B->plans[g]->sweep[0].ok);
+    if (!any) return;
+    const int ng = (int)B->plans.size();
+    const size_t elem = io == MX_IO_F64 ? sizeof(double) : sizeof(float);
+    std::vector<void *> W(ng, nullptr), Y(ng, nullptr);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        for (void *p : W)
+            if (p) (void)hipFree(p);
+        for (void *p : Y)
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    try {
+        for (int g = 0; g < ng; ++g) {
+            const int64_t nel = std::max<int64_t>(B->group_rows[g], 1) * B->group_nrhs[g];
+            HIPCHK(hipMalloc(&W[g], (size_t)nel * elem));
+            HIPCHK(hipMalloc(&Y[g], (size_t)nel * elem));
+            launch_fill_uniform(W[g], nel, io == MX_IO_F64, st);
+        }
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        const int ndir = B->plans[0]->adjoint ? 2 : 1;
+        for (int dir = 0; dir < ndir; ++dir) {
+            auto time_batch = [&]() {
+                batch_size_work(B, dir, MX_BF16X3, io, Y.data());
+                batch_eval(B, dir, MX_BF16X3, W.data(), Y.data(), io, st);   // warm-up
+                HIPCHK(hipEventRecord(e0, st));
+                for (int r = 0; r < reps; ++r) batch_eval(B, dir, MX_BF16X3, W.data(), Y.data(), io, st);
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                return ms / reps;
+            };
+            B->sweep_var[dir] = 0;
+            float best = time_batch();
+            int pick = 0;
+            for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2}) {
+                if (B->bsweep[dir].var != v) build_batch_sweep(B, dir, v);
+                B->sweep_var[dir] = v;
+                const float ms = time_batch();
+                if (ms < 0.98f * best) {
+                    best = ms;
+                    pick = v;
+                }
+            }
+            if (pick && B->bsweep[dir].var != pick) build_batch_sweep(B, dir, pick);
+            B->sweep_var[dir] = pick;
+            batch_note_eval(B, dir, MX_BF16X3);
+        }
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+}
+
 mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, int32_t reps, void *stream) {
     return (mx_status)guard([&]() {
         arg_check(batch != nullptr, "null batch");
@@ -656,6 +981,7 @@ mx_status mx_batch_autotune(mx_batch *batch, mx_precision prec, mx_io_type io, i
         for (size_t g = 0; g < batch->plans.size(); ++g)
             autotune(batch->plans[g], group_prec(batch, (int)g, prec), batch->group_nrhs[g], io, reps,
                      (hipStream_t)stream, nullptr);
+        if (prec == MX_BF16X3) autotune_batch_sweep(batch, io, std::max(reps, 1), (hipStream_t)stream);
         batch_drop_graphs(batch);
         return 0;
     });

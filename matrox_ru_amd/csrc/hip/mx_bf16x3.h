@@ -178,9 +178,9 @@ constexpr uint64_t SWEEP_TIMEOUT_TICKS = 100000000ull;   // 1 s of the 100 MHz r
 constexpr int SWEEP_MAX_POLLS = 1 << 24;   // and at most this many polls per wait or claim (a second bound)
 
 struct alignas(16) SweepTile {
-    int32_t task, rt;     // task and row tile
+    int32_t task, rt;     // task and row tile (batched: row tile | column tile << 16)
     int32_t g;            // rank (work item rank = g * ncols + column tile)
-    int32_t pad;
+    int32_t grp;          // batched: group (index into X3SweepArgs::groups)
     int32_t maxtile[SWEEP_NQ];   // per queue: the last tile index of a producer (-1: none)
 };
 
@@ -197,6 +197,15 @@ struct SweepWait {
     int32_t target;   // finished (tile, wave) writes of the block per column tile
 };
 
+// One group of a batched sweep: its plan's operands, panels and lists and
+// its output (x.Y, x.y_f64).
+struct SweepGroup {
+    X3Args x;
+    const SweepTask *st;
+    const SweepWait *w;
+    int32_t ncols;               // column tiles of the group
+    int32_t cbase;               // its first counter in the control block
+};
 struct X3SweepArgs {
     X3Args x;                    // tasks, segs, operands, panels, Y (tiles unused)
     const SweepTile *q;          // the queues, concatenated
@@ -204,13 +213,18 @@ struct X3SweepArgs {
     const SweepTask *st;         // per task
     const SweepWait *w;
     int32_t *ctl;                // heads, error word, counters (zeroed before the launch)
-    int ncols;                   // column tiles
+    int ncols;                   // column tiles (batched: 1, every entry one work item)
+    // batched sweep (several plans' evaluations in one launch; null for one
+    // plan): per group its lists, buffers and output
+    const SweepGroup *groups;
 };
 
 int sweep_bn(int variant);               // columns of a tile: X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2
 bool sweep_variant_ok(int variant);
 // Zero the control block, then run the sweep on `grid` workgroups.
 void launch_x3_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st);
+// the same for a batched sweep (a.groups set)
+void launch_x3_sweep_batch(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st);
 int sweep_max_grid(int variant);         // resident workgroups the device holds (occupancy x CUs)
 
 }  // namespace mx

@@ -700,15 +700,16 @@ __device__ __forceinline__ int sweep_home() {
 // the counters are seen (no cache invalidation is needed: see the coherence
 // note in mx_bf16x3.h).  Gives up on an error or after SWEEP_TIMEOUT_TICKS
 // (setting the error word).
-__device__ __forceinline__ void sweep_wait(const X3SweepArgs &sa, int b, int e, int ct, int lane) {
+// cnt: the counters of the task's plan (ncols column tiles each), w its waits.
+__device__ __forceinline__ void sweep_wait(const X3SweepArgs &sa, const int32_t *cnt, const SweepWait *wl, int ncols,
+                                           int b, int e, int ct, int lane) {
     if (b >= e) return;
-    const int32_t *cnt = sa.ctl + SWEEP_CNT;
     for (int base = b; base < e; base += 64) {
         const int i = base + lane;
         int idx = 0, target = 0;
         if (i < e) {
-            const SweepWait w = sa.w[i];
-            idx = w.block * sa.ncols + ct;
+            const SweepWait w = wl[i];
+            idx = w.block * ncols + ct;
             target = w.target;
         }
         uint64_t t0 = 0;
@@ -863,7 +864,7 @@ __device__ __forceinline__ void sweep_claim_lowest(const X3SweepArgs &sa, int la
 // block (every wave waits for its own panel stores, then adds 1), repeat.  Two
 // workgroups per CU, as the level kernel: the accumulators then live in
 // VGPRs.
-template <int NJ>
+template <int NJ, bool BATCH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x3_sweep_kernel(X3SweepArgs sa) {
     typedef X3Shape<2, 2, NJ> S;
     constexpr int NST = 2;
@@ -910,14 +911,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
         // uniform across the workgroup: scalar registers and loads from here
         const int q = __builtin_amdgcn_readfirstlane(s_take[0]), pos = __builtin_amdgcn_readfirstlane(s_take[1]);
         if (q < 0) break;
-        const int tix = pos / sa.ncols, ct = pos % sa.ncols;
+        const int tix = pos / sa.ncols;
         const SweepTile tile = sa.q[sa.qinfo[q].x + tix];
-        const Task task = sa.x.tasks[tile.task];
-        const SweepTask stk = sa.st[tile.task];
-        sweep_wait(sa, stk.ew_b, stk.ew_e, ct, lane);
-        x3_tile<S, NST, NJ, true>(sa.x, task, tile.rt * S::BM, ct * S::BN, stg, sseg, [&](int si) {
-            if (si == stk.late_seg) sweep_wait(sa, stk.lw_b, stk.lw_e, ct, lane);
-        });
+        int ct = pos % sa.ncols, rt = tile.rt, gcols = sa.ncols;
+        const int32_t *cnt = sa.ctl + SWEEP_CNT;
+        const SweepTask *stl = sa.st;
+        const SweepWait *wl = sa.w;
+        const Task *tasks = sa.x.tasks;
+        if constexpr (BATCH) {
+            const SweepGroup &G = sa.groups[tile.grp];
+            ct = rt >> 16;
+            rt &= 0xffff;
+            gcols = G.ncols;
+            cnt = sa.ctl + G.cbase;
+            stl = G.st;
+            wl = G.w;
+            tasks = G.x.tasks;
+        }
+        const Task task = tasks[tile.task];
+        const SweepTask stk = stl[tile.task];
+        sweep_wait(sa, cnt, wl, gcols, stk.ew_b, stk.ew_e, ct, lane);
+        auto late = [&](int si) {
+            if (si == stk.late_seg) sweep_wait(sa, cnt, wl, gcols, stk.lw_b, stk.lw_e, ct, lane);
+        };
+        // the operands by reference, in kernel arguments or (batched) in the
+        // group table: a private copy would be indexed through scratch
+        if constexpr (BATCH)
+            x3_tile<S, NST, NJ, true>(sa.groups[tile.grp].x, task, rt * S::BM, ct * S::BN, stg, sseg, late);
+        else
+            x3_tile<S, NST, NJ, true>(sa.x, task, rt * S::BM, ct * S::BN, stg, sseg, late);
         // wave 0 starts the next claim now, so that its round trip overlaps
         // the completion of the panel stores (every wave reads q, pos and
         // the task above, before this point)
@@ -929,7 +951,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
             __builtin_amdgcn_s_waitcnt(0);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
             if (lane == 0)
-                __hip_atomic_fetch_add(sa.ctl + SWEEP_CNT + stk.sig * sa.ncols + ct, 1, __ATOMIC_RELAXED,
+                __hip_atomic_fetch_add(const_cast<int32_t *>(cnt) + stk.sig * gcols + ct, 1, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
         }
         if (wv == 0) pick(claim_next ? pre : -1);
@@ -952,7 +974,7 @@ static int sweep_grid_of() {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, x3_sweep_kernel<NJ>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, x3_sweep_kernel<NJ, false>, 256, 0) != hipSuccess) return 0;
     return std::max(1, per) * cus;
 }
 
@@ -964,21 +986,30 @@ int sweep_max_grid(int variant) {
     }
 }
 
-void launch_x3_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st) {
+template <bool BATCH>
+static void launch_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st) {
     hipLaunchKernelGGL(sweep_reset_kernel, dim3((unsigned)std::min(64, (ctl_ints + 255) / 256)), dim3(256), 0, st,
                        a.ctl, ctl_ints);
     if (grid <= 0) return;
     switch (variant) {
     case X3_V128x64_S2:
-        hipLaunchKernelGGL(x3_sweep_kernel<2>, dim3((unsigned)grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((x3_sweep_kernel<2, BATCH>), dim3((unsigned)grid), dim3(256), 0, st, a);
         break;
     case X3_V128x32_S2:
-        hipLaunchKernelGGL(x3_sweep_kernel<1>, dim3((unsigned)grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((x3_sweep_kernel<1, BATCH>), dim3((unsigned)grid), dim3(256), 0, st, a);
         break;
     default:
-        hipLaunchKernelGGL(x3_sweep_kernel<4>, dim3((unsigned)grid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((x3_sweep_kernel<4, BATCH>), dim3((unsigned)grid), dim3(256), 0, st, a);
         break;
     }
+}
+
+void launch_x3_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st) {
+    launch_sweep<false>(a, variant, grid, ctl_ints, st);
+}
+
+void launch_x3_sweep_batch(const X3SweepArgs &a, int variant, int grid, int ctl_ints, hipStream_t st) {
+    launch_sweep<true>(a, variant, grid, ctl_ints, st);
 }
 
 // ---- split-BF16x3 gradient products -------------------------------------

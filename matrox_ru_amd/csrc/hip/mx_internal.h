@@ -111,6 +111,11 @@ struct mx_plan {
         mx::SweepTile *q = nullptr;
         int2 *qinfo = nullptr;
         std::vector<int2> qinfo_h;
+        // host copies (batched sweeps merge the lists of several plans)
+        std::vector<mx::SweepTile> h_tiles;
+        std::vector<mx::SweepTask> h_st;
+        std::vector<mx::SweepWait> h_w;
+        int nblocks = 0;   // counter blocks
         bool ok = false;
         void release() {
             void *ptrs[] = {tasks, segs, st, w, q, qinfo};
@@ -384,6 +389,9 @@ void ensure_work(mx_plan *P, int nrhs, size_t elem, bool need_io);   // forward 
 void ensure_io(mx_plan *P, int nrhs);                                 // FP32 staging of FP64 W and Y
 void ensure_adj_work(mx_plan *P, int nrhs, size_t elem);   // skeletons of the adjoint evaluation
 void ensure_panels(mx_plan *P, int nrhs);
+// Packed operands and split-panel pointers of P's BF16x3 kernels for nrhs
+// columns on the panel buffer `panels` (sets nrhs, ahi, alo, phi, plo).
+void x3_args_of(const mx_plan *P, int nrhs, uint16_t *panels, X3Args &xa);
 // Bytes of the split panels for nrhs columns, and of panels plus the sweep
 // control block after them (at offset round256(panel bytes)).
 size_t panel_bytes_of(const mx_plan *P, int nrhs);

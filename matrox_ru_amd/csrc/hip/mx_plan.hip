@@ -190,6 +190,10 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                     sw.q = upload(SW.tiles.data(), SW.tiles.size());
                     sw.qinfo = upload(SW.qinfo.data(), SW.qinfo.size());
                     sw.qinfo_h = SW.qinfo;
+                    sw.h_tiles = SW.tiles;
+                    sw.h_st = SW.stasks;
+                    sw.h_w = SW.waits;
+                    sw.nblocks = SW.nblocks;
                     sw.ok = true;
                 }
             }
@@ -507,6 +511,21 @@ void size_work(mx_plan *P, int dir, mx_precision prec, mx_io_type io, int nrhs) 
     }
 }
 
+void x3_args_of(const mx_plan *P, int nrhs, uint16_t *panels, X3Args &xa) {
+    const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
+    xa.nrhs = nrhs;
+    for (int w = 0; w < 3; ++w) {
+        xa.ahi[w] = P->xhi[w];
+        xa.alo[w] = P->xlo[w];
+    }
+    xa.phi[VEC_W] = panels;
+    xa.plo[VEC_W] = panels + wp;
+    xa.phi[VEC_WSKEL] = panels + 2 * wp;
+    xa.plo[VEC_WSKEL] = panels + 2 * wp + sp;
+    xa.phi[VEC_USKEL] = panels + 2 * wp + 2 * sp;
+    xa.plo[VEC_USKEL] = panels + 2 * wp + 3 * sp;
+}
+
 void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, int nrhs, mx_io_type io,
                hipStream_t st, const Work &wk) {
     const Program &pg0 = P->prog[dir][0][P->sched[dir][0]];
@@ -567,25 +586,13 @@ void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, i
             mark(MX_KIND_CONVERT, false);
         }
     } else {
-        const size_t wp = (size_t)P->wp_rows * nrhs, sp = (size_t)P->sp_rows * nrhs;
         X3Args xa{};
-        xa.nrhs = nrhs;
+        x3_args_of(P, nrhs, wk.panels, xa);
         const std::vector<int> &lv = P->lvar[dir][P->sched[dir][1]];
         xa.y_f64 = io == MX_IO_F64;
         xa.Y = Y;
         xa.tasks = pg1.tasks;
         xa.segs = pg1.segs;
-        for (int w = 0; w < 3; ++w) {
-            xa.ahi[w] = P->xhi[w];
-            xa.alo[w] = P->xlo[w];
-        }
-        uint16_t *pw = wk.panels;
-        xa.phi[VEC_W] = pw;
-        xa.plo[VEC_W] = pw + wp;
-        xa.phi[VEC_WSKEL] = pw + 2 * wp;
-        xa.plo[VEC_WSKEL] = pw + 2 * wp + sp;
-        xa.phi[VEC_USKEL] = pw + 2 * wp + 2 * sp;
-        xa.plo[VEC_USKEL] = pw + 2 * wp + 3 * sp;
         mark(MX_KIND_CONVERT, true);
         launch_x3_split_w(W, io == MX_IO_F64, P->d_wrows, P->nleaf, nrhs, xa.phi[VEC_W], xa.plo[VEC_W], st);
         mark(MX_KIND_CONVERT, false);

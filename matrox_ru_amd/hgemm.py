@@ -128,6 +128,12 @@ def lib():
         L.mx_batch_block_layout.restype = ctypes.c_int
         L.mx_batch_set_graph.argtypes = [P, I32]
         L.mx_batch_set_graph.restype = ctypes.c_int
+        L.mx_batch_set_sweep.argtypes = [P, I32, I32]
+        L.mx_batch_set_sweep.restype = ctypes.c_int
+        L.mx_batch_get_sweep.argtypes = [P, I32, ctypes.POINTER(I32)]
+        L.mx_batch_get_sweep.restype = ctypes.c_int
+        L.mx_batch_sweep_status.argtypes = [P, P, ctypes.POINTER(I32)]
+        L.mx_batch_sweep_status.restype = ctypes.c_int
         L.mx_batch_autotune.argtypes = [P, ctypes.c_int, ctypes.c_int, I32, P]
         L.mx_batch_autotune.restype = ctypes.c_int
         L.mx_batch_flops.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
@@ -629,6 +635,22 @@ class Batch:
 
     def set_graph(self, enable=True):
         _check(lib().mx_batch_set_graph(self._h, int(enable)))
+
+    def set_sweep(self, direction, x3_variant):
+        """Evaluate every BF16x3 group in one dependency-driven launch
+        (mx_batch_set_sweep): x3_variant = tile width (1, 5 or 6), 0 = off."""
+        _check(lib().mx_batch_set_sweep(self._h, DIRECTIONS[direction], int(x3_variant)))
+
+    def get_sweep(self, direction):
+        v = I32()
+        _check(lib().mx_batch_get_sweep(self._h, DIRECTIONS[direction], ctypes.byref(v)))
+        return v.value
+
+    def sweep_status(self, stream=None):
+        """Error word of the batched sweeps (mx_batch_sweep_status)."""
+        e = I32()
+        _check(lib().mx_batch_sweep_status(self._h, _stream_handle(stream), ctypes.byref(e)))
+        return e.value
 
     def autotune(self, precision, io="f32", reps=3, stream=None):
         _check(lib().mx_batch_autotune(self._h, PRECISIONS[precision], IO_F64 if io == "f64" else IO_F32, reps,
