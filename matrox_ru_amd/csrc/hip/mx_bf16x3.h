@@ -114,7 +114,10 @@ void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Row
 //   use the 256-row tile lists of the plan);
 //   128 x 32 with 6 stages and 128 x 64 with 5 (120 KB, one workgroup per
 //   CU, five or four chunks of loads in flight): for the latency-bound
-//   launches of small nrhs and the upper tree levels.
+//   launches of small nrhs and the upper tree levels;
+//   128 x 32 with the K chunks split over 8 waves (register operands, the
+//   partial tiles summed through LDS in wave order): a task's chunks load
+//   in parallel instead of in a pipeline, for the chains of short tasks.
 enum X3Variant {
     X3_V128x128_S2 = 1,
     X3_V128x128_S4 = 2,
@@ -126,9 +129,10 @@ enum X3Variant {
     X3_V128x128_W8S4 = 8,
     X3_V256x128_W8S3 = 9,
     X3_V128x32_S6 = 10,
-    X3_V128x64_S5 = 11
+    X3_V128x64_S5 = 11,
+    X3_V128x32_K8 = 12
 };
-constexpr int X3_NVARIANT = 11;
+constexpr int X3_NVARIANT = 12;
 int x3_variant_bn(int variant);   // output columns of a tile
 int x3_variant_bm(int variant);   // output rows of a tile (X3_BM or 2 X3_BM)
 void launch_x3(const X3Args &a, int variant, hipStream_t st);

@@ -621,6 +621,133 @@ __global__ __launch_bounds__(64 * WM * WN) void x3_kernel(X3Args a) {
     x3_tile<S, NST, NJ>(a, task, tl.y * S::BM, nt * S::BN, stg, sseg, [](int) {});
 }
 
+// ---- split-K variant for latency-bound launches -----------------------------
+//
+// Tiles of 128 x 32.  The task's 32-deep K chunks are dealt to 8 waves (chunk
+// c to wave c % 8).  Each wave loads its chunk's 20 fragments (8 row groups
+// and 2 column groups, hi and lo) straight into registers, one chunk ahead,
+// and accumulates the whole 128 x 32 tile.  The eight partial tiles are
+// summed through LDS in wave order (so results do not depend on timing), and
+// two waves run the epilogue.  A task of K <= 256 then waits for one round
+// of loads instead of a pipeline of K/32 dependent chunks: the top of the
+// tree and small workloads are chains of such tasks.
+constexpr int X3SK_WAVES = 8;
+
+struct X3SkFrag {
+    u32x4_t a[16], b[4];   // row group i: a[2i] hi, a[2i+1] lo; column group j: b[2j] hi, b[2j+1] lo
+};
+
+__device__ __forceinline__ void x3sk_load(X3SkFrag &F, const X3Args &a, const Seg &sg, int ki, int lane, int mtile16,
+                                          int nfm, int n0) {
+    const int nfk = (sg.k + 31) / 32;
+    const int64_t ao = sg.a_off + (int64_t)ki * X3_FRAG + lane * 8;
+    const uint16_t *ah = a.ahi[sg.a_buf] + ao;
+    const uint16_t *al = a.alo[sg.a_buf] + ao;
+#pragma unroll
+    for (int rg = 0; rg < 8; ++rg) {
+        const int64_t o = (int64_t)min(mtile16 + rg, nfm - 1) * nfk * X3_FRAG;
+        F.a[2 * rg] = *reinterpret_cast<const u32x4_t *>(ah + o);
+        F.a[2 * rg + 1] = *reinterpret_cast<const u32x4_t *>(al + o);
+    }
+    const int64_t pb = (sg.b_row + (int64_t)ki * 32) * a.nrhs + 8 * (lane >> 4);
+#pragma unroll
+    for (int cg = 0; cg < 2; ++cg) {
+        const int64_t o = pb + (int64_t)min(n0 + cg * 16 + (lane & 15), a.nrhs - 1) * 32;
+        F.b[2 * cg] = *reinterpret_cast<const u32x4_t *>(a.phi[sg.b_buf] + o);
+        F.b[2 * cg + 1] = *reinterpret_cast<const u32x4_t *>(a.plo[sg.b_buf] + o);
+    }
+}
+
+__device__ __forceinline__ void x3sk_compute(x3_f4 (&acc)[8][2], const X3SkFrag &F, int mtile16, int nfm) {
+    const u32x4_t z = u32x4_t{0, 0, 0, 0};
+    bf16x8_t bh[2], bl[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        bh[j] = __builtin_bit_cast(bf16x8_t, F.b[2 * j]);
+        bl[j] = __builtin_bit_cast(bf16x8_t, F.b[2 * j + 1]);
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const bool ok = mtile16 + i < nfm;
+        const bf16x8_t ah = __builtin_bit_cast(bf16x8_t, ok ? F.a[2 * i] : z);
+        const bf16x8_t al = __builtin_bit_cast(bf16x8_t, ok ? F.a[2 * i + 1] : z);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            acc[i][j] = mfma_bf16(al, bh[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah, bl[j], acc[i][j]);
+            acc[i][j] = mfma_bf16(ah, bh[j], acc[i][j]);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
+    // per wave its partial tile, element-major (8 x 2 fragments x 4 values,
+    // then lane), so that a wave's stores and loads hit consecutive banks
+    __shared__ __attribute__((aligned(16))) float part[X3SK_WAVES * 64 * 64];
+    __shared__ Seg sseg[X3_MAXSEG];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    const int m0 = tl.y * X3_BM, n0 = nt * 32;
+    const int nfm = (task.m + 15) / 16, mtile16 = m0 / 16;
+    const int nseg = task.seg_end - task.seg_begin;   // <= X3_MAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += 64 * X3SK_WAVES) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+    int nch = 0;
+    for (int si = 0; si < nseg; ++si) nch += (sseg[si].k + 31) / 32;
+
+    x3_f4 acc[8][2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+    // segment cursor: chunk c lies in segment si, whose first chunk is cbase
+    int si = 0, cbase = 0;
+    auto load = [&](X3SkFrag &F, int c) {
+        while (c >= cbase + (sseg[si].k + 31) / 32) {
+            cbase += (sseg[si].k + 31) / 32;
+            ++si;
+        }
+        x3sk_load(F, a, sseg[si], c - cbase, lane, mtile16, nfm, n0);
+    };
+    X3SkFrag F0, F1;
+    if (wv < nch) load(F0, wv);
+    for (int c = wv; c < nch; c += 2 * X3SK_WAVES) {
+        if (c + X3SK_WAVES < nch) load(F1, c + X3SK_WAVES);
+        x3sk_compute(acc, F0, mtile16, nfm);
+        if (c + 2 * X3SK_WAVES < nch) load(F0, c + 2 * X3SK_WAVES);
+        if (c + X3SK_WAVES < nch) x3sk_compute(acc, F1, mtile16, nfm);
+    }
+    float *mine = part + wv * 64 * 64;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) mine[((i * 2 + j) * 4 + r) * 64 + lane] = acc[i][j][r];
+    __syncthreads();
+    x3_f4 fin[4][2];
+    if (wv < 2) {   // wave 0 sums row groups 0-3, wave 1 row groups 4-7, over the waves in order
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = (((wv * 4 + ii) * 2 + j) * 4 + r) * 64 + lane;
+                    float v = 0.f;
+#pragma unroll
+                    for (int q = 0; q < X3SK_WAVES; ++q) v += part[q * 64 * 64 + e];
+                    fin[ii][j][r] = v;
+                }
+    }
+    __syncthreads();   // the partials are read; the epilogue stages in the same LDS
+    if (wv < 2) x3_epilogue<4, 2>(fin, reinterpret_cast<uint16_t *>(part), a, task, lane, wv, wv, 0, m0, n0);
+}
+
 int x3_variant_bm(int variant) { return variant == X3_V256x128_W8S3 ? 2 * X3_BM : X3_BM; }
 
 int x3_variant_bn(int variant) {
@@ -630,6 +757,7 @@ int x3_variant_bn(int variant) {
     case X3_V128x32_S2: return 32;
     case X3_V128x32_S6: return 32;
     case X3_V128x64_S5: return 64;
+    case X3_V128x32_K8: return 32;
     default: return 128;
     }
 }
@@ -668,6 +796,9 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V128x64_S5:
         hipLaunchKernelGGL((x3_kernel<2, 2, 5, 2>), grid, dim3(256), 0, st, a);
+        break;
+    case X3_V128x32_K8:
+        hipLaunchKernelGGL(x3sk_kernel, grid, dim3(64 * X3SK_WAVES), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);
