@@ -816,7 +816,7 @@ mx_status mx_batch_set_sweep(mx_batch *batch, int32_t direction, int32_t x3_vari
     return (mx_status)guard([&]() {
         arg_check(batch && (direction == 0 || direction == 1), "bad argument");
         arg_check(x3_variant == 0 || sweep_variant_ok(x3_variant),
-                  "sweep tile: 0 (off), MX_X3_128x128_S2, MX_X3_128x64_S2 or MX_X3_128x32_S2");
+                  "sweep tile: 0 (off), MX_X3_128x128_S2, MX_X3_128x64_S2, MX_X3_128x32_S2 or MX_X3_128x32_K8");
         if (direction == DIR_ADJ && !batch->plans[0]->adjoint)
             throw MxError(MX_ERR_UNSUPPORTED, "batch was created without MX_PLAN_ADJOINT");
         if (x3_variant && batch->sweep_var[direction] != x3_variant) build_batch_sweep(batch, direction, x3_variant);
@@ -954,7 +954,7 @@ static void autotune_batch_sweep(mx_batch *B, mx_io_type io, int reps, hipStream
             B->sweep_var[dir] = 0;
             float best = time_batch();
             int pick = 0;
-            for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2}) {
+            for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2, X3_V128x32_K8}) {
                 if (B->bsweep[dir].var != v) build_batch_sweep(B, dir, v);
                 B->sweep_var[dir] = v;
                 const float ms = time_batch();

@@ -680,18 +680,17 @@ __device__ __forceinline__ void x3sk_compute(x3_f4 (&acc)[8][2], const X3SkFrag 
     }
 }
 
-__global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
-    // per wave its partial tile, element-major (8 x 2 fragments x 4 values,
-    // then lane), so that a wave's stores and loads hit consecutive banks
-    __shared__ __attribute__((aligned(16))) float part[X3SK_WAVES * 64 * 64];
-    __shared__ Seg sseg[X3_MAXSEG];
+// One 128 x 32 tile of task (rows m0.., columns n0..) by the 8 waves of the
+// workgroup; part: 8 x 64 x 64 floats of LDS, sseg: the segment list.
+// on_seg(si) runs in a wave before it loads from segment si, for every
+// segment its chunk cursor enters or passes over (the sweep's late wait);
+// DEV as in x3_tile.  AHEAD: load the wave's next chunk while computing
+// one (two register sets); without it the other resident wave of the SIMD
+// covers the loads.  The caller synchronises before reusing the LDS.
+template <bool DEV, bool AHEAD, typename OnSeg>
+__device__ __forceinline__ void x3sk_tile(const X3Args &a, const Task &task, int m0, int n0, float *part, Seg *sseg,
+                                          OnSeg &&on_seg) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int total = a.ntiles * a.n_ntiles;
-    const int w = xcd_remap(blockIdx.x, total);
-    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
-    const int2 tl = a.tiles[tile_id];
-    const Task task = a.tasks[tl.x];
-    const int m0 = tl.y * X3_BM, n0 = nt * 32;
     const int nfm = (task.m + 15) / 16, mtile16 = m0 / 16;
     const int nseg = task.seg_end - task.seg_begin;   // <= X3_MAXSEG (checked by the plan)
     for (int i = tid; i < nseg; i += 64 * X3SK_WAVES) sseg[i] = a.segs[task.seg_begin + i];
@@ -706,20 +705,34 @@ __global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
         for (int j = 0; j < 2; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
     // segment cursor: chunk c lies in segment si, whose first chunk is cbase
     int si = 0, cbase = 0;
+    bool entered = false;
     auto load = [&](X3SkFrag &F, int c) {
+        if (!entered) {
+            on_seg(0);
+            entered = true;
+        }
         while (c >= cbase + (sseg[si].k + 31) / 32) {
             cbase += (sseg[si].k + 31) / 32;
             ++si;
+            on_seg(si);
         }
         x3sk_load(F, a, sseg[si], c - cbase, lane, mtile16, nfm, n0);
     };
-    X3SkFrag F0, F1;
-    if (wv < nch) load(F0, wv);
-    for (int c = wv; c < nch; c += 2 * X3SK_WAVES) {
-        if (c + X3SK_WAVES < nch) load(F1, c + X3SK_WAVES);
-        x3sk_compute(acc, F0, mtile16, nfm);
-        if (c + 2 * X3SK_WAVES < nch) load(F0, c + 2 * X3SK_WAVES);
-        if (c + X3SK_WAVES < nch) x3sk_compute(acc, F1, mtile16, nfm);
+    if constexpr (AHEAD) {
+        X3SkFrag F0, F1;
+        if (wv < nch) load(F0, wv);
+        for (int c = wv; c < nch; c += 2 * X3SK_WAVES) {
+            if (c + X3SK_WAVES < nch) load(F1, c + X3SK_WAVES);
+            x3sk_compute(acc, F0, mtile16, nfm);
+            if (c + 2 * X3SK_WAVES < nch) load(F0, c + 2 * X3SK_WAVES);
+            if (c + X3SK_WAVES < nch) x3sk_compute(acc, F1, mtile16, nfm);
+        }
+    } else {
+        for (int c = wv; c < nch; c += X3SK_WAVES) {
+            X3SkFrag F;
+            load(F, c);
+            x3sk_compute(acc, F, mtile16, nfm);
+        }
     }
     float *mine = part + wv * 64 * 64;
 #pragma unroll
@@ -745,7 +758,20 @@ __global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
                 }
     }
     __syncthreads();   // the partials are read; the epilogue stages in the same LDS
-    if (wv < 2) x3_epilogue<4, 2>(fin, reinterpret_cast<uint16_t *>(part), a, task, lane, wv, wv, 0, m0, n0);
+    if (wv < 2) x3_epilogue<4, 2, DEV>(fin, reinterpret_cast<uint16_t *>(part), a, task, lane, wv, wv, 0, m0, n0);
+}
+
+__global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
+    // per wave its partial tile, element-major (8 x 2 fragments x 4 values,
+    // then lane), so that a wave's stores and loads hit consecutive banks
+    __shared__ __attribute__((aligned(16))) float part[X3SK_WAVES * 64 * 64];
+    __shared__ Seg sseg[X3_MAXSEG];
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    x3sk_tile<false, true>(a, task, tl.y * X3_BM, nt * 32, part, sseg, [](int) {});
 }
 
 int x3_variant_bm(int variant) { return variant == X3_V256x128_W8S3 ? 2 * X3_BM : X3_BM; }
@@ -995,12 +1021,26 @@ __device__ __forceinline__ void sweep_claim_lowest(const X3SweepArgs &sa, int la
 // block (every wave waits for its own panel stores, then adds 1), repeat.  Two
 // workgroups per CU, as the level kernel: the accumulators then live in
 // VGPRs.
-template <int NJ, bool BATCH>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x3_sweep_kernel(X3SweepArgs sa) {
+// NJ = 0: the split-K tile (8 waves; its first SWEEP_WAVES waves signal,
+// waves 0 and 1 after the epilogue's stores).
+template <int NJ>
+struct SweepTileShape {
     typedef X3Shape<2, 2, NJ> S;
-    constexpr int NST = 2;
-    static_assert(S::NW == SWEEP_WAVES, "every wave signals once per tile");
-    __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
+    static constexpr int WAVES = 4;
+    static constexpr int LDS16 = 2 * S::STAGE;   // uint16 of LDS for stages
+};
+template <>
+struct SweepTileShape<0> {
+    static constexpr int WAVES = X3SK_WAVES;
+    static constexpr int LDS16 = X3SK_WAVES * 64 * 64 * 2;   // the partial tiles
+};
+
+template <int NJ, bool BATCH>
+__global__ __launch_bounds__(64 * SweepTileShape<NJ>::WAVES) __attribute__((amdgpu_waves_per_eu(2))) void x3_sweep_kernel(
+    X3SweepArgs sa) {
+    typedef SweepTileShape<NJ> T;
+    static_assert(T::WAVES >= SWEEP_WAVES, "the first SWEEP_WAVES waves signal once per tile");
+    __shared__ __attribute__((aligned(16))) uint16_t stg[T::LDS16];
     __shared__ Seg sseg[X3_MAXSEG];
     // the item to run; wave 0's claim state between items (in LDS, so that
     // no register stays live across a tile): the claimed item waiting to
@@ -1067,17 +1107,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void x
         };
         // the operands by reference, in kernel arguments or (batched) in the
         // group table: a private copy would be indexed through scratch
-        if constexpr (BATCH)
-            x3_tile<S, NST, NJ, true>(sa.groups[tile.grp].x, task, rt * S::BM, ct * S::BN, stg, sseg, late);
-        else
-            x3_tile<S, NST, NJ, true>(sa.x, task, rt * S::BM, ct * S::BN, stg, sseg, late);
+        const X3Args &xr = BATCH ? sa.groups[tile.grp].x : sa.x;
+        if constexpr (NJ == 0) {
+            x3sk_tile<true, false>(xr, task, rt * X3_BM, ct * 32, reinterpret_cast<float *>(stg), sseg, late);
+        } else {
+            typedef typename T::S S;
+            x3_tile<S, 2, NJ, true>(xr, task, rt * S::BM, ct * S::BN, stg, sseg, late);
+        }
         // wave 0 starts the next claim now, so that its round trip overlaps
         // the completion of the panel stores (every wave reads q, pos and
         // the task above, before this point)
         int pre = -1;
         const bool claim_next = wv == 0 && s_held[2];
         if (claim_next && lane == 0) pre = sweep_fetch_head(sa, home);
-        if (stk.sig >= 0) {
+        if (stk.sig >= 0 && wv < SWEEP_WAVES) {
             // this wave's device-scope panel stores have completed before its count
             __builtin_amdgcn_s_waitcnt(0);
             __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1095,7 +1138,8 @@ __global__ void sweep_reset_kernel(int32_t *ctl, int n) {
 }
 
 bool sweep_variant_ok(int variant) {
-    return variant == X3_V128x128_S2 || variant == X3_V128x64_S2 || variant == X3_V128x32_S2;
+    return variant == X3_V128x128_S2 || variant == X3_V128x64_S2 || variant == X3_V128x32_S2 ||
+           variant == X3_V128x32_K8;
 }
 
 int sweep_bn(int variant) { return x3_variant_bn(variant); }
@@ -1105,7 +1149,9 @@ static int sweep_grid_of() {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess) return 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, x3_sweep_kernel<NJ, false>, 256, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, x3_sweep_kernel<NJ, false>, 64 * SweepTileShape<NJ>::WAVES,
+                                                     0) != hipSuccess)
+        return 0;
     return std::max(1, per) * cus;
 }
 
@@ -1113,6 +1159,7 @@ int sweep_max_grid(int variant) {
     switch (variant) {
     case X3_V128x64_S2: return sweep_grid_of<2>();
     case X3_V128x32_S2: return sweep_grid_of<1>();
+    case X3_V128x32_K8: return sweep_grid_of<0>();
     default: return sweep_grid_of<4>();
     }
 }
@@ -1128,6 +1175,9 @@ static void launch_sweep(const X3SweepArgs &a, int variant, int grid, int ctl_in
         break;
     case X3_V128x32_S2:
         hipLaunchKernelGGL((x3_sweep_kernel<1, BATCH>), dim3((unsigned)grid), dim3(256), 0, st, a);
+        break;
+    case X3_V128x32_K8:
+        hipLaunchKernelGGL((x3_sweep_kernel<0, BATCH>), dim3((unsigned)grid), dim3(64 * X3SK_WAVES), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_sweep_kernel<4, BATCH>), dim3((unsigned)grid), dim3(256), 0, st, a);

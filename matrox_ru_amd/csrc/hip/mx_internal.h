@@ -131,7 +131,7 @@ struct mx_plan {
     } sweep[2];
     int sweep_on[2] = {0, 0};
     int sweep_var[2] = {mx::X3_V128x128_S2, mx::X3_V128x128_S2};
-    int sweep_grid[3] = {0, 0, 0};   // resident workgroups per tile width (128, 64, 32 columns), 0: not queried
+    int sweep_grid[4] = {0, 0, 0, 0};   // resident workgroups per sweep tile (128, 64, 32 columns, split-K), 0: not queried
     int32_t *sweep_last_ctl = nullptr;   // control block of the last sweep enqueued (mx_plan_sweep_status)
     // per-launch BF16x3 variants [direction][schedule] chosen by the
     // autotuner (empty: x3_variant for every launch)

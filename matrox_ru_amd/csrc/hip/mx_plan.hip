@@ -394,7 +394,7 @@ size_t panel_ctl_bytes_of(const mx_plan *P, int nrhs) {
 
 int sweep_grid_for(mx_plan *P, int dir, int nrhs) {
     const int v = P->sweep_var[dir];
-    const int k = v == X3_V128x64_S2 ? 1 : v == X3_V128x32_S2 ? 2 : 0;
+    const int k = v == X3_V128x64_S2 ? 1 : v == X3_V128x32_S2 ? 2 : v == X3_V128x32_K8 ? 3 : 0;
     if (P->sweep_grid[k] == 0) P->sweep_grid[k] = sweep_max_grid(v);
     if (P->sweep_grid[k] <= 0) throw MxError(MX_ERR_HIP, "sweep: no occupancy for the kernel");
     const int bn = sweep_bn(v), ncols = (nrhs + bn - 1) / bn;
@@ -933,7 +933,7 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
         if (!P->sweep[dir].ok)
             throw MxError(MX_ERR_UNSUPPORTED, "no sweep for this plan (a block with several writers)");
         arg_check(variant == 0 || sweep_variant_ok(variant),
-                  "sweep tile: MX_X3_128x128_S2, MX_X3_128x64_S2 or MX_X3_128x32_S2 (0 keeps)");
+                  "sweep tile: MX_X3_128x128_S2, MX_X3_128x64_S2, MX_X3_128x32_S2 or MX_X3_128x32_K8 (0 keeps)");
         P->sweep_on[dir] = 1;
         if (variant) P->sweep_var[dir] = variant;
         drop_graphs(P);
@@ -1120,7 +1120,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                 P->sweep_on[dir] = 0;
                 float best = time_eval();
                 int pick = 0;
-                for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2}) {
+                for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2, X3_V128x32_K8}) {
                     P->sweep_on[dir] = 1;
                     P->sweep_var[dir] = v;
                     const float ms = time_eval();

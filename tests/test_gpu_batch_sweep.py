@@ -80,7 +80,7 @@ def test_batched_sweep_matches(gpu, case):
             b.plan(g).set_schedule(direction, "bf16x3", "fused", 1)
         b.set_sweep(direction, 0)
         Yl = _run(b, direction, W)
-        for width in (1, 5, 6):
+        for width in (1, 5, 6, 12):
             _say(f"[batched sweep] {direction} width={width}")
             b.set_sweep(direction, width)
             assert b.get_sweep(direction) == width
@@ -91,7 +91,8 @@ def test_batched_sweep_matches(gpu, case):
             for g in range(len(b.groups)):
                 assert torch.isfinite(Y1[g]).all()
                 assert torch.equal(Y1[g], Y2[g]), "not deterministic"
-                assert rel(Y1[g].cpu().numpy(), Yl[g].cpu().numpy()) <= 1e-6
+                # the split-K tile sums its partials in another order (see test_gpu_sweep.py)
+                assert rel(Y1[g].cpu().numpy(), Yl[g].cpu().numpy()) <= (5e-6 if width == 12 else 1e-6)
             errs = [rel(b.instance_view(Y1, i).cpu().numpy(), ref[direction][i]) for i in range(len(hms))]
             _say(f"    worst instance vs oracle {max(errs):.2e}")
             assert max(errs) <= 1e-5
@@ -148,7 +149,7 @@ def test_batch_autotune_picks_launch_form(gpu, case):
     b = hgemm.Batch(hms, nrhs, precisions=("bf16x3",), adjoint=True)
     b.autotune("bf16x3", io="f64", reps=2)
     for d in ("forward", "adjoint"):
-        assert b.get_sweep(d) in (0, 1, 5, 6)
+        assert b.get_sweep(d) in (0, 1, 5, 6, 12)
     _say(f"[batch autotune] sweep forward={b.get_sweep('forward')} adjoint={b.get_sweep('adjoint')}")
     W = _inputs(b, TW)
     Y = _run(b, "forward", W)

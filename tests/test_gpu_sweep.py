@@ -2,9 +2,13 @@
 launch per evaluation, work items started as their producers finish.
 
 * against the oracle (1e-5, the BF16x3 bar) and against the level launches
-  of the same split products (same products; only the place of the u^
-  segment in each sum differs, so within 1e-6), forward and adjoint, for
-  the three tile widths and column counts that are not multiples of them;
+  of the same split products, forward and adjoint, for the four tiles and
+  column counts that are not multiples of them.  The same products summed
+  in the same order up to the place of the u^ segment agree within 1e-6;
+  the split-K tile sums its partial tiles in another order, and the
+  re-split of every skeleton panel to 16 bits turns such last-bit
+  differences into differences at the split's 2^-17 resolution: within
+  5e-6;
 * deterministic: every output element has one writer and a fixed sum
   order, so repeated evaluations are bit-identical;
 * the skeletons it leaves (w^, u^ panels) equal the level launches', and
@@ -31,7 +35,7 @@ CASES = {
     # leaves of 300 points and ranks up to 150: several 128-row tiles per task
     "h2b_tall": dict(n=4800, d=6, leaf=300, rank=150, tol=1e-9, budget=0.3, h=1.0),
 }
-WIDTHS = (1, 5, 6)   # MX_X3_128x128_S2, MX_X3_128x64_S2, MX_X3_128x32_S2
+WIDTHS = (1, 5, 6, 12)   # MX_X3_128x128_S2, MX_X3_128x64_S2, MX_X3_128x32_S2, MX_X3_128x32_K8
 
 
 def _say(*a):
@@ -74,7 +78,7 @@ def test_sweep_matches_oracle_and_levels(gpu, case):
                 e_or = rel(Y1.cpu().numpy(), ref[direction])
                 e_lv = rel(Y1.cpu().numpy(), Yl.cpu().numpy())
                 _say(f"    vs oracle {e_or:.2e}, vs level launches {e_lv:.2e}")
-                assert e_or <= 1e-5 and e_lv <= 1e-6
+                assert e_or <= 1e-5 and e_lv <= (5e-6 if width == 12 else 1e-6)
 
 
 def test_sweep_skeletons_and_backward(gpu):
@@ -121,6 +125,10 @@ def test_sweep_graph_replay(gpu):
             p.set_schedule(d, "bf16x3", "sweep", 5)
     plan.set_graph(True)
     for rep in range(3):
+        if rep == 2:   # the split-K tile in the sweep, captured too
+            for p in (plan, eager):
+                for d in ("forward", "adjoint"):
+                    p.set_schedule(d, "bf16x3", "sweep", 12)
         for nrhs in (64, 130):
             _say(f"[sweep graph] rep={rep} nrhs={nrhs}")
             W = torch.from_numpy(host.normal((hm.N * nrhs,), seed=60 + rep)).cuda()
