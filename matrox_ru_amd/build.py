@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 
 HOST_SRC = [os.path.join(CSRC, "host", "hstruct.cpp")]
 HIP_SRC = [os.path.join(CSRC, "hip", f)
-           for f in ("mx_plan.hip", "mx_dropin.hip", "mx_lists.hip", "mx_generate.hip", "mx_batch.hip", "mx_bf16x3.hip", "mx_grad.hip")]
+           for f in ("mx_plan.hip", "mx_dropin.hip", "mx_lists.hip", "mx_generate.hip", "mx_batch.hip", "mx_bf16x3.hip", "mx_grad.hip", "mx_f64p.hip")]
 
 # Packed FP32 instructions (v_pk_*_f32) are disabled in all device code: some
 # of their forms have given wrong results on this hardware without an error.

@@ -30,6 +30,7 @@
 // C(4*(l/16)+r, l%16).
 #pragma once
 #include "mx_common.h"
+#include "mx_dma.h"
 
 namespace mx {
 

@@ -184,9 +184,6 @@ struct X3Geo {
 template <int WM, int WN, int NJ = 4>
 struct X3Shape : X3Geo<64 * WM, 16 * NJ * WN, WM * WN> {};
 
-typedef __attribute__((address_space(3))) void lds_void_t;
-typedef const __attribute__((address_space(1))) void glb_void_t;
-
 __device__ __forceinline__ void lds_dma16(const uint16_t *src, uint16_t *dst_wave) {
     __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 16, 0, 0);
 }
@@ -242,32 +239,6 @@ __device__ __forceinline__ void x3_issue(X3Loader<S> &L, const X3Args &a, const 
         L.p[t] += f < S::FA ? (int64_t)X3_FRAG : bstep;
     }
     --L.left;
-}
-
-// s_waitcnt immediate for vmcnt(v) with expcnt and lgkmcnt left alone
-constexpr int vmcnt_enc(int v) { return 0xF70 | (v & 15) | ((v >> 4) << 14); }
-
-// wait until at most j * PER of this wave's LDS-DMA loads are outstanding
-// (j <= J, the chunks that may stay in flight; vmcnt takes an immediate, so
-// each j has its own instruction)
-template <int PER, int J>
-__device__ __forceinline__ void wait_dma(int j) {
-    static_assert(J * PER < 64, "vmcnt is 6 bits");
-    if constexpr (J == 0) {
-        __builtin_amdgcn_s_waitcnt(vmcnt_enc(0));
-    } else {
-        if (j >= J) __builtin_amdgcn_s_waitcnt(vmcnt_enc(J * PER));
-        else wait_dma<PER, J - 1>(j);
-    }
-}
-
-// workgroup barrier without the memory fences of __syncthreads (which would
-// wait for every outstanding prefetch); the signal fences keep the compiler
-// from moving LDS accesses across it
-__device__ __forceinline__ void wg_barrier() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_barrier();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
 // The 16 fragments a wave multiplies per chunk (its 4 A row groups and 4 B

@@ -1,0 +1,223 @@
+// FP64 block products on packed A operands (layouts in mx_f64p.h).
+#include "mx_f64p.h"
+#include "mx_internal.h"
+
+namespace mx {
+
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+
+__global__ void pack_f64_blocks(const double *__restrict__ src, const X3Block *__restrict__ blocks,
+                                double *__restrict__ dst) {
+    const X3Block bk = blocks[blockIdx.y];
+    const int nkg = (bk.K + 15) / 16, nrg = (bk.M + 15) / 16;
+    const int64_t n = (int64_t)nrg * nkg * F64P_FRAG;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frag = t >> 8;
+        const int pos = (int)(t & 255);
+        const int rg = (int)(frag / nkg), kg = (int)(frag % nkg);
+        const int l = (pos & 127) >> 1;   // the MFMA lane that reads this element
+        const int row = rg * 16 + (l & 15);
+        const int k = kg * 16 + 4 * (l >> 4) + 2 * (pos >> 7) + (pos & 1);
+        double x = 0.0;
+        if (row < bk.M && k < bk.K)
+            x = bk.trans ? src[bk.src_off + (int64_t)row * bk.lda + k] : src[bk.src_off + (int64_t)k * bk.lda + row];
+        dst[bk.dst_off + t] = x;
+    }
+}
+
+constexpr int F64P_WAVES = 8, F64P_THREADS = 64 * F64P_WAVES;
+// a stage: 8 A fragments (row groups of the tile), then 128 B rows of 16 k
+constexpr int F64P_STAGE = 8 * F64P_FRAG + F64P_BN * F64P_BK;   // doubles (32 KB)
+constexpr int F64P_PER = 2 + 8;   // DMA loads per wave per chunk: one A fragment (2 x 1 KB), 16 B rows (8 x 256 B)
+
+__device__ __forceinline__ void dma16(const double *src, double *dst_wave) {
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 16, 0, 0);
+}
+__device__ __forceinline__ void dma4(const uint32_t *src, double *dst_wave) {
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 4, 0, 0);
+}
+
+__device__ __forceinline__ f64x4_t mfma_f64(double a, double b, f64x4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Half a chunk's operands: the 4 A row groups of the wave (16 bytes per lane
+// at ab + 2048 i) and its 2 B column groups (at bb + 2048 j), read in one asm
+// block that waits for its own reads (LDS reads the compiler can see would
+// wait for every outstanding LDS-DMA load).
+__device__ __forceinline__ void f64p_read(uint32_t ab, uint32_t bb, u32x4_t (&A)[4], u32x4_t (&B)[2]) {
+    asm volatile(
+        "ds_read_b128 %0, %6 offset:0\n\t"
+        "ds_read_b128 %1, %6 offset:2048\n\t"
+        "ds_read_b128 %2, %6 offset:4096\n\t"
+        "ds_read_b128 %3, %6 offset:6144\n\t"
+        "ds_read_b128 %4, %7 offset:0\n\t"
+        "ds_read_b128 %5, %7 offset:2048\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(B[0]), "=&v"(B[1])
+        : "v"(ab), "v"(bb)
+        : "memory");
+}
+
+// The 32 MFMAs of a chunk for a wave of 64 x 32 (row groups 4 wr + i, tile
+// columns 32 wc + 16 j + l%16).  B column n keeps logical slot s (k = 2s,
+// 2s+1) at physical slot s ^ ((n >> 1) & 7).
+__device__ __forceinline__ void f64p_compute(f64x4_t (&acc)[4][2], uint32_t sbase, int lane, int wr, int wc) {
+    const int n = 32 * wc + (lane & 15), g = lane >> 4, f = (n >> 1) & 7;
+    const uint32_t ab = sbase + wr * 4 * (F64P_FRAG * 8) + lane * 16;
+    const uint32_t bb = sbase + 8 * F64P_FRAG * 8 + n * (F64P_BK * 8);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        u32x4_t A[4], B[2];
+        f64p_read(ab + h * 1024, bb + (((2 * g + h) ^ f) << 4), A, B);
+        f64x2_t a[4], b[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = __builtin_bit_cast(f64x2_t, A[i]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = __builtin_bit_cast(f64x2_t, B[j]);
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma_f64(a[i][e], b[j][e], acc[i][j]);
+    }
+}
+
+// Chunk cursor of a wave: segment, 16-deep chunk within it, and the
+// segment's operands (wave-uniform).
+struct F64PLoader {
+    const double *a;    // the wave's A fragment row of the segment (chunk 0)
+    const double *bv;   // B block: column buffer + first row * nrhs
+    int seg, kc, nk, K, b_ld;
+};
+
+// 128 x 128 tile of a task by 8 waves (2 x 4 of 64 x 32).  NST stages of
+// LDS; loads run NST-1 chunks ahead of the MFMAs.
+template <int NST>
+__global__ __launch_bounds__(F64P_THREADS) __attribute__((amdgpu_waves_per_eu(NST == 2 ? 4 : 2))) void f64p_kernel(F64PArgs a) {
+    static_assert(NST * F64P_STAGE >= F64P_WAVES * 64 * 16, "the epilogue staging must fit in the stages");
+    __shared__ __attribute__((aligned(16))) double stg[NST * F64P_STAGE];
+    // the task's segment list, read from LDS inside the pipeline (a global
+    // load there would wait for every outstanding LDS-DMA load)
+    __shared__ Seg sseg[F64P_MAXSEG];
+
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    const int M = task.m;
+    if (M <= 0) return;   // padding tile (the whole workgroup)
+    const int m0 = tl.y * F64P_BM, n0 = nt * F64P_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wr = wv >> 2, wc = wv & 3;
+    const int nrg = (M + 15) / 16;
+    const int arg = min(m0 / 16 + wv, nrg - 1);   // the wave's A row group (rows past M: any valid group)
+    // B loads of the wave: tile columns 16 wv + 2 j + lane/32 (j = 0..7), the
+    // lane's 4 bytes at dword lane%4 of physical 16-byte slot (lane/4)%8,
+    // which holds logical slot ((lane/4)%8) ^ j
+    const int bcol = n0 + 16 * wv + (lane >> 5);
+    const int bps = (lane >> 2) & 7, bkh = (lane >> 1) & 1, bhalf = lane & 1;
+
+    const int nseg = task.seg_end - task.seg_begin;   // <= F64P_MAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += F64P_THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+
+    f64x4_t acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+
+    F64PLoader L;
+    L.seg = -1;
+    L.kc = L.nk = 0;
+    L.K = L.b_ld = 0;
+    L.a = L.bv = nullptr;
+    bool more = true;
+    int issued = 0;
+    auto issue = [&]() {
+        if (!more) return;
+        while (L.kc == L.nk) {
+            if (++L.seg >= nseg) {
+                more = false;
+                return;
+            }
+            const Seg sg = sseg[L.seg];
+            L.K = sg.k;
+            L.nk = (sg.k + F64P_BK - 1) / F64P_BK;
+            L.kc = 0;
+            L.a = a.a[sg.a_buf] + sg.a_off + (int64_t)arg * sg.lda * F64P_FRAG;
+            L.bv = a.vec[sg.b_buf] + sg.b_row * (int64_t)a.nrhs;
+            L.b_ld = sg.b_ld;
+        }
+        double *stage = stg + (issued % NST) * F64P_STAGE;
+        const double *ap = L.a + (int64_t)L.kc * F64P_FRAG + lane * 2;
+        dma16(ap, stage + wv * F64P_FRAG);
+        dma16(ap + 128, stage + wv * F64P_FRAG + 128);
+        const int k0 = L.kc * F64P_BK;
+        double *bst = stage + 8 * F64P_FRAG + 16 * wv * F64P_BK;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int col = min(bcol + 2 * j, a.nrhs - 1);
+            const int k = min(k0 + 2 * (bps ^ j) + bkh, L.K - 1);
+            const uint32_t *p = reinterpret_cast<const uint32_t *>(L.bv + ((int64_t)col * L.b_ld + k)) + bhalf;
+            dma4(p, bst + j * 2 * F64P_BK);
+        }
+        ++L.kc;
+        ++issued;
+    };
+#pragma unroll
+    for (int p = 0; p < NST - 1; ++p) issue();
+    for (int c = 0; c < issued; ++c) {
+        // chunk c has landed (this wave's part, then everyone's), and every
+        // wave is done with the stage chunk c-1 used, which chunk c+NST-1 refills
+        wait_dma<F64P_PER, NST - 1>(issued - c - 1);
+        wg_barrier();
+        issue();
+        f64p_compute(acc, lds_addr(stg + (c % NST) * F64P_STAGE), lane, wr, wc);
+    }
+    __syncthreads();
+
+    // epilogue: per column group, the wave's 64 x 16 block goes through its
+    // own 8 KB of LDS (rows XOR-swizzled by column: the 32 lanes of a write
+    // hit distinct banks) and out as 64 consecutive rows per column
+    double *Cs = stg + wv * 64 * 16;
+    double *C = a.vec[task.c_buf] + task.c_row * (int64_t)a.nrhs;
+    const int rb = m0 + 64 * wr;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = lane & 15, row = 16 * i + (lane >> 4) + 4 * r;
+                Cs[col * 64 + (row ^ (2 * col))] = acc[i][j][r];
+            }
+        __builtin_amdgcn_wave_barrier();
+        const int cb = n0 + 32 * wc + 16 * j;
+#pragma unroll 4
+        for (int e = lane; e < 64 * 16; e += 64) {
+            const int col = e >> 6, row = e & 63;
+            const int gm = rb + row, gn = cb + col;
+            if (gm < M && gn < a.nrhs) {
+                const double v = Cs[col * 64 + (row ^ (2 * col))];
+                double *y = C + (int64_t)gn * M + gm;
+                *y = task.beta ? *y + v : v;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+void launch_f64p(const F64PArgs &a, int nst, hipStream_t st) {
+    const int64_t total = (int64_t)a.ntiles * a.n_ntiles;
+    if (total <= 0) return;
+    if (total > INT32_MAX) throw MxError(MX_ERR_ARG, "launch too large");
+    if (nst == 3) hipLaunchKernelGGL(f64p_kernel<3>, dim3((unsigned)total), dim3(F64P_THREADS), 0, st, a);
+    else hipLaunchKernelGGL(f64p_kernel<2>, dim3((unsigned)total), dim3(F64P_THREADS), 0, st, a);
+}
+
+}  // namespace mx

@@ -7,6 +7,7 @@
 #pragma once
 #include "../../../include/matrox_hgemm.h"
 #include "mx_bf16x3.h"
+#include "mx_f64p.h"
 #include "mx_grad.h"
 #include "mx_seg_gemm.h"
 
@@ -57,10 +58,16 @@ struct Program {
     std::vector<Launch> launches2;
     std::vector<double> flops;   // algorithmic flops per nrhs column of each launch
     int last_write[4] = {-1, -1, -1, -1};   // per VecBuf: the last launch writing it (-1: none)
+    // FP64 programs of plans with FP64: the segments over packed A operands
+    // (f64p_kernel, on tiles2) and the longest task of each launch
+    Seg *segs_p = nullptr;
+    std::vector<int> maxseg;
     bool built = false;
     void release() {
         if (tasks) (void)hipFree(tasks);
         if (segs) (void)hipFree(segs);
+        if (segs_p) (void)hipFree(segs_p);
+        segs_p = nullptr;
         if (tiles) (void)hipFree(tiles);
         if (tiles2) (void)hipFree(tiles2);
         tasks = nullptr;
@@ -153,6 +160,12 @@ struct mx_plan {
     uint16_t *xhi[3] = {nullptr, nullptr, nullptr};   // packed split D, B, VT operands
     uint16_t *xlo[3] = {nullptr, nullptr, nullptr};
     int64_t x3_elems[3] = {0, 0, 0};
+    // FP64 A operands packed in 16 x 16 fragments (plans with FP64), and
+    // their packing lists
+    double *f64p[3] = {nullptr, nullptr, nullptr};
+    int64_t f64p_elems[3] = {0, 0, 0};
+    mx::X3Block *d_fblk[3] = {nullptr, nullptr, nullptr};
+    int64_t n_fblk[3] = {0, 0, 0};
     int64_t D_len = 0, B_len = 0, VT_len = 0;
     uint32_t precisions = 0;
 
@@ -227,7 +240,8 @@ struct mx_plan {
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
                         d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, d_xblk[0], d_xblk[1], d_xblk[2],
-                        wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag};
+                        wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag,
+                        f64p[0], f64p[1], f64p[2], d_fblk[0], d_fblk[1], d_fblk[2]};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);

@@ -59,6 +59,17 @@ static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     }
 }
 
+static void pack_f64_all(mx_plan *P, const double *const src[3], hipStream_t st) {
+    for (int w = 0; w < 3; ++w) {
+        if (!src[w]) continue;
+        for (int64_t b0 = 0; b0 < P->n_fblk[w]; b0 += 65535) {
+            int64_t nb = std::min<int64_t>(65535, P->n_fblk[w] - b0);
+            pack_f64_blocks<<<dim3(32, (unsigned)nb), 256, 0, st>>>(src[w], P->d_fblk[w] + b0, P->f64p[w]);
+            HIPCHK(hipGetLastError());
+        }
+    }
+}
+
 // First padded panel row of every W leaf block and skeleton block (BF16x3
 // split panels), and the rows of each panel buffer.
 static void panel_rows(const mx_structure &s, std::vector<int64_t> &wprow, std::vector<int64_t> &sprow,
@@ -115,6 +126,12 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     const int ndir = P->adjoint ? 2 : 1;
+    // FP64 plans: every A operand of the FP64 programs is also packed in
+    // 16 x 16 fragments for f64p_kernel; each distinct (buffer, offset,
+    // orientation, shape) gets its packed offset, shared by all programs
+    std::vector<X3Block> fblk[3];
+    std::map<std::tuple<int64_t, int, int, int, int>, int64_t> fseen[3];
+    int64_t fnext[3] = {0, 0, 0};
     for (int dir = 0; dir < ndir; ++dir)
         for (int sc = 0; sc < NSCHED; ++sc) {
             TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
@@ -122,6 +139,36 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pair_tiles(L);
             Program &pg = P->prog[dir][0][sc];
             upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
+            if (precisions & MX_PLAN_FP64) {
+                std::vector<Seg> sp(L.segs);
+                for (const Task &tk : L.tasks)
+                    for (int si = tk.seg_begin; si < tk.seg_end; ++si) {
+                        Seg &sg = sp[si];
+                        const int w = sg.a_buf;
+                        auto key = std::make_tuple(sg.a_off, sg.a_trans, sg.lda, tk.m, sg.k);
+                        auto it = fseen[w].find(key);
+                        int64_t dst;
+                        if (it == fseen[w].end()) {
+                            dst = fnext[w];
+                            fseen[w][key] = dst;
+                            fblk[w].push_back(X3Block{sg.a_off, dst, tk.m, sg.k, sg.lda, sg.a_trans});
+                            fnext[w] += f64p_packed_elems(tk.m, sg.k);
+                        } else {
+                            dst = it->second;
+                        }
+                        sg.a_off = dst;
+                        sg.a_trans = 0;
+                        sg.lda = (sg.k + F64P_BK - 1) / F64P_BK;
+                    }
+                pg.segs_p = upload(sp.data(), sp.size());
+                pg.maxseg.assign(L.launches.size(), 0);
+                for (size_t li = 0; li < L.launches.size(); ++li)
+                    for (int ti = L.launches[li].tile_offset; ti < L.launches[li].tile_offset + L.launches[li].ntiles;
+                         ++ti) {
+                        const Task &tk = L.tasks[L.tiles[ti].x];
+                        pg.maxseg[li] = std::max(pg.maxseg[li], tk.seg_end - tk.seg_begin);
+                    }
+            }
             pg.launches = L.launches;
             pg.tiles2 = upload(L.tiles2.data(), L.tiles2.size());
             pg.launches2 = L.launches2;
@@ -312,6 +359,16 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pack_all(P.get(), src, nullptr);
         }
     }
+    if (precisions & MX_PLAN_FP64) {
+        for (int w = 0; w < 3; ++w) {
+            P->f64p[w] = upload<double>(nullptr, (size_t)std::max<int64_t>(fnext[w], 1));
+            P->f64p_elems[w] = fnext[w];
+            P->d_fblk[w] = upload(fblk[w].data(), fblk[w].size());
+            P->n_fblk[w] = (int64_t)fblk[w].size();
+        }
+        const double *src[3] = {P->D64, P->B64, P->VT64};
+        pack_f64_all(P.get(), src, nullptr);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     if (!(precisions & MX_PLAN_FP64)) {   // FP64 blocks were only a source
@@ -431,12 +488,38 @@ void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs) {
 }
 
 // FP64/FP32 kernel variants of a launch: 64 x 64 tiles with one or two
-// chunks of loads in flight, or (FP32) 128 x 128 tiles on the plan's 128-row
-// tile list
-enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3 };
+// chunks of loads in flight, (FP32) 128 x 128 tiles on the plan's 128-row
+// tile list, or (FP64) f64p_kernel on packed A operands with two or three
+// LDS stages.  A launch with a task longer than F64P_MAXSEG segments runs
+// the f64p variants on the 64 x 64 kernel.
+enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3, SEG_F64P2 = 4, SEG_F64P3 = 5 };
+
+static bool seg_variant_ok(mx_precision prec, int v) {
+    return prec == MX_FP64 ? (v == SEG_V64_D1 || v == SEG_V64_D2 || v == SEG_F64P2 || v == SEG_F64P3)
+                           : (v >= SEG_V64_D1 && v <= SEG_V128);
+}
 
 template <typename T>
-static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T> &b, int variant, hipStream_t st) {
+static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T> &b, int variant, hipStream_t st,
+                           const double *const *packed = nullptr) {
+    if (variant == SEG_F64P2 || variant == SEG_F64P3) {
+        if (sizeof(T) == sizeof(double) && packed && pg.segs_p && li < pg.maxseg.size() &&
+            pg.maxseg[li] <= F64P_MAXSEG) {
+            const Launch &ln = pg.launches2[li];
+            F64PArgs fa{};
+            fa.tasks = pg.tasks;
+            fa.segs = pg.segs_p;
+            fa.tiles = pg.tiles2 + ln.tile_offset;
+            fa.ntiles = ln.ntiles;
+            fa.nrhs = nrhs;
+            fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
+            for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
+            for (int v = 0; v < 4; ++v) fa.vec[v] = reinterpret_cast<double *>(b.vec[v]);
+            launch_f64p(fa, variant == SEG_F64P3 ? 3 : 2, st);
+            return;
+        }
+        variant = SEG_V64_D1;
+    }
     const bool wide = variant == SEG_V128 && sizeof(T) == sizeof(float);
     const Launch &ln = wide ? pg.launches2[li] : pg.launches[li];
     const int bn = wide ? SW_BN : SG_BN;
@@ -549,7 +632,7 @@ void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, i
         b.vec[VEC_Y] = (double *)Y;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             mark(MX_KIND_SEG_F64, true);
-            run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st);
+            run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st, P->f64p);
             mark(MX_KIND_SEG_F64, false);
             phase(i);
         }
@@ -893,6 +976,7 @@ static void set_blocks(mx_plan *P, const double *D, const double *B, const doubl
         if (d32[w]) cvt_f64_f32<<<cvt_grid(len[w]), 256, 0, st>>>(src[w], d32[w], len[w]);
     }
     if (P->precisions & MX_PLAN_BF16X3) pack_all(P, src, st);
+    if (P->precisions & MX_PLAN_FP64) pack_f64_all(P, src, st);
     HIPCHK(hipGetLastError());
 }
 
@@ -1033,10 +1117,13 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         if (prec != MX_BF16X3) {
             // per launch: 64 x 64 tiles with prefetch depth 2 for
             // latency-bound launches, depth 1 where more workgroups per CU
-            // pay, 128 x 128 tiles (FP32) for the wide launches
+            // pay, 128 x 128 tiles (FP32; FP64 on packed operands) for the
+            // wide launches
             const int seg_kind = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
             const int pk = prec == MX_FP64 ? 0 : 1;
-            const int nv = prec == MX_FP64 ? SEG_V64_D2 : SEG_V128;
+            const std::vector<int> cand = prec == MX_FP64
+                                              ? std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_F64P2, SEG_F64P3}
+                                              : std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_V128};
             P->profiling = 1;
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][0];
@@ -1044,7 +1131,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                 std::vector<int> &ld = P->lsg[dir][pk][sc];
                 std::vector<int> pick(nl, SEG_V64_D1);
                 std::vector<float> best(nl, 0.f);
-                for (int depth = SEG_V64_D1; depth <= nv; ++depth) {
+                for (int depth : cand) {
                     ld.assign(nl, depth);
                     eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
                     std::vector<float> t(nl, 0.f);
@@ -1688,9 +1775,8 @@ mx_status mx_plan_set_seg_variants(mx_plan *plan, int32_t direction, mx_precisio
             arg_check(variants != nullptr, "null variants");
             arg_check(count == (int32_t)plan->prog[direction][0][sc].launches.size(),
                       "count must equal the number of launches of the current schedule");
-            const int nv = prec == MX_FP64 ? SEG_V64_D2 : SEG_V128;
             for (int32_t i = 0; i < count; ++i)
-                arg_check(variants[i] >= SEG_V64_D1 && variants[i] <= nv, "bad FP64/FP32 kernel variant");
+                arg_check(seg_variant_ok(prec, variants[i]), "bad FP64/FP32 kernel variant");
             lv.assign(variants, variants + count);
         }
         drop_graphs(plan);

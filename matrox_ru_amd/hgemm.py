@@ -389,7 +389,8 @@ class Plan:
 
     def seg_variants(self, precision, direction="forward"):
         """FP64/FP32 kernel variant of each launch (mx_plan_seg_variants):
-        1 = 64x64 tiles, one chunk in flight; 2 = two; 3 = 128x128 (FP32)."""
+        1 = 64x64 tiles, one chunk in flight; 2 = two; 3 = 128x128 (FP32);
+        4 / 5 = 128x128 on packed A operands, two / three LDS stages (FP64)."""
         cap = 4096
         v = (I32 * cap)()
         cnt = I32()

@@ -115,11 +115,13 @@ def test_pinned_launch_variants(gpu):
     assert rel(Yf.cpu().numpy(), Yo) <= 1e-5
 
 
-@pytest.mark.parametrize("case", ["h2b_ragged", "h2b_tall"])
+@pytest.mark.parametrize("case", sorted(CASES))
 def test_fp64_fp32_kernel_variants(gpu, case):
     """Every FP64/FP32 kernel variant (64x64 tiles with one or two chunks of
-    loads in flight, 128x128 tiles for FP32) on every launch, both
-    directions, against the oracle."""
+    loads in flight, 128x128 tiles for FP32, 128x128 tiles on packed A
+    operands with two or three LDS stages for FP64) on every launch, both
+    directions, against the oracle; ragged column counts (150: a partial
+    128-column tile)."""
     import torch
     _say(f"[seg variants] {case}")
     X, hm = make_problem(**CASES[case])
@@ -129,7 +131,7 @@ def test_fp64_fp32_kernel_variants(gpu, case):
     Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
-    for prec, variants in (("fp64", (1, 2)), ("fp32", (1, 2, 3))):
+    for prec, variants in (("fp64", (1, 2, 4, 5)), ("fp32", (1, 2, 3))):
         for v in variants:
             for d in ("forward", "adjoint"):
                 n = len(plan.seg_variants(prec, d))
@@ -147,7 +149,37 @@ def test_fp64_fp32_kernel_variants(gpu, case):
     with pytest.raises(hgemm.MxError):
         plan.set_seg_variants([3] * len(plan.seg_variants("fp64")), "fp64")
     with pytest.raises(hgemm.MxError):
+        plan.set_seg_variants([4] * len(plan.seg_variants("fp32")), "fp32")
+    with pytest.raises(hgemm.MxError):
         plan.set_seg_variants([1], "fp32")
+
+
+def test_fp64_packed_operands_follow_new_blocks(gpu):
+    """The packed FP64 A operands are repacked by mx_plan_set_blocks: after
+    new blocks, the packed-operand kernel (every launch) equals the 64x64
+    kernel reading the stored blocks, forward and adjoint, at nrhs 1 and 300."""
+    import torch
+    _say("[packed operands after set_blocks]")
+    X, hm = make_problem(**CASES["h2b_tall"])
+    plan = hgemm.Plan(hm, precisions=("fp64",), adjoint=True)
+    D, B, VT = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (hm.D, hm.B, hm.VT))
+    plan.set_blocks(D * 1.5, B - 0.25 * B.abs(), VT * -0.75)
+    for nrhs in (1, 300):
+        W = torch.from_numpy(host.normal((hm.N * nrhs,), seed=31 + nrhs)).cuda()
+        out = {}
+        for v in (1, 4, 5):
+            for d in ("forward", "adjoint"):
+                plan.set_seg_variants([v] * len(plan.seg_variants("fp64", d)), "fp64", d)
+            Y = torch.full_like(W, float("nan"))
+            G = torch.full_like(W, float("nan"))
+            plan.eval(W, Y, nrhs, "fp64")
+            plan.eval_adjoint(W, G, nrhs, "fp64")
+            torch.cuda.synchronize()
+            out[v] = (Y.cpu().numpy(), G.cpu().numpy())
+        for v in (4, 5):
+            ey, eg = rel(out[v][0], out[1][0]), rel(out[v][1], out[1][1])
+            _say(f"  nrhs={nrhs} variant {v} vs 1: forward {ey:.2e} adjoint {eg:.2e}")
+            assert ey <= 1e-13 and eg <= 1e-13
 
 
 def test_balanced_schedule_moves_near_work(gpu):
