@@ -510,6 +510,9 @@ static void build_batch_sweep(mx_batch *B, int dir, int var) {
     bs.q = upload(tiles.data(), tiles.size());
     bs.qinfo = upload(qinfo.data(), qinfo.size());
     HIPCHK(hipMalloc(&bs.ctl, (size_t)cb * sizeof(int32_t)));
+    // zero now: mx_batch_sweep_status reads the error word of both
+    // directions, also of one whose sweep has not run yet
+    HIPCHK(hipMemset(bs.ctl, 0, (size_t)cb * sizeof(int32_t)));
     bs.var = var;
 }
 

@@ -359,7 +359,20 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pack_all(P.get(), src, nullptr);
         }
     }
-    if (precisions & MX_PLAN_FP64) {
+    // The packed copies double an FP64 plan's block memory (more with the
+    // adjoint's transposed views).  They are made only when they fit beside
+    // what the plan already holds with room left for the work buffers and
+    // the caller's W and Y (16 GiB + 10% of the device); otherwise every
+    // launch runs the 64 x 64 kernel on the stored blocks.
+    bool pack64 = (precisions & MX_PLAN_FP64) != 0;
+    if (pack64) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const double need = (double)(fnext[0] + fnext[1] + fnext[2]) * sizeof(double);
+        pack64 = need + 16.0 * (1ull << 30) + 0.1 * (double)total_b <= (double)free_b;
+    }
+    if (pack64) {
         for (int w = 0; w < 3; ++w) {
             P->f64p[w] = upload<double>(nullptr, (size_t)std::max<int64_t>(fnext[w], 1));
             P->f64p_elems[w] = fnext[w];
@@ -632,7 +645,8 @@ void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, i
         b.vec[VEC_Y] = (double *)Y;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             mark(MX_KIND_SEG_F64, true);
-            run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st, P->f64p);
+            run_seg_launch<double>(pg0, i, nrhs, b, seg_variant(P, dir, 0, i), st,
+                                   P->f64p[0] ? P->f64p : nullptr);
             mark(MX_KIND_SEG_F64, false);
             phase(i);
         }
@@ -1121,9 +1135,9 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
             // wide launches
             const int seg_kind = prec == MX_FP64 ? MX_KIND_SEG_F64 : MX_KIND_SEG_F32;
             const int pk = prec == MX_FP64 ? 0 : 1;
-            const std::vector<int> cand = prec == MX_FP64
-                                              ? std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_F64P2, SEG_F64P3}
-                                              : std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_V128};
+            std::vector<int> cand = prec == MX_FP64 ? std::vector<int>{SEG_V64_D1, SEG_V64_D2}
+                                                    : std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_V128};
+            if (prec == MX_FP64 && P->f64p[0]) cand.insert(cand.end(), {SEG_F64P2, SEG_F64P3});
             P->profiling = 1;
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][0];
