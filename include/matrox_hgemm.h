@@ -294,7 +294,8 @@ enum {
     MX_X3_256x128_W8S3 = 9, /* 8 waves of 64 x 64, 3 stages, 256-row tiles */
     MX_X3_128x32_S6 = 10,   /* 4 waves of 64 x 16, 6 stages (latency-bound launches) */
     MX_X3_128x64_S5 = 11,   /* 4 waves of 64 x 32, 5 stages */
-    MX_X3_128x32_K8 = 12    /* K split over 8 waves (latency-bound launches) */
+    MX_X3_128x32_K8 = 12,   /* K split over 8 waves (latency-bound launches) */
+    MX_X3_256x256_W8S2 = 13 /* 8 waves of 64 x 128, 2 stages, 256-row tiles */
 };
 mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision prec, int32_t schedule,
                                int32_t x3_variant);

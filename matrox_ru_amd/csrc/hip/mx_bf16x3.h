@@ -118,7 +118,10 @@ void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Row
 //   launches of small nrhs and the upper tree levels;
 //   128 x 32 with the K chunks split over 8 waves (register operands, the
 //   partial tiles summed through LDS in wave order): a task's chunks load
-//   in parallel instead of in a pipeline, for the chains of short tasks.
+//   in parallel instead of in a pipeline, for the chains of short tasks;
+//   256 x 256, 2 stages: 8 waves (4 x 2) of 64 x 128, 128 KB of stages, one
+//   workgroup per CU (two waves per SIMD): half the fragment bytes per flop
+//   of the 128 x 128 tile, and a quarter fewer LDS reads per MFMA.
 enum X3Variant {
     X3_V128x128_S2 = 1,
     X3_V128x128_S4 = 2,
@@ -131,9 +134,10 @@ enum X3Variant {
     X3_V256x128_W8S3 = 9,
     X3_V128x32_S6 = 10,
     X3_V128x64_S5 = 11,
-    X3_V128x32_K8 = 12
+    X3_V128x32_K8 = 12,
+    X3_V256x256_W8S2 = 13
 };
-constexpr int X3_NVARIANT = 12;
+constexpr int X3_NVARIANT = 13;
 int x3_variant_bn(int variant);   // output columns of a tile
 int x3_variant_bm(int variant);   // output rows of a tile (X3_BM or 2 X3_BM)
 void launch_x3(const X3Args &a, int variant, hipStream_t st);

@@ -313,39 +313,92 @@ __device__ __forceinline__ void lds_read_frags(uint32_t abase, uint32_t bbase, u
         : "memory");
 }
 
+// 8 B fragments (the second half of a 64 x 128 wave's columns)
+__device__ __forceinline__ void lds_read_frags(uint32_t bbase, u32x4_t (&B)[8]) {
+    asm volatile(
+        "ds_read_b128 %0, %8 offset:0\n\t"
+        "ds_read_b128 %1, %8 offset:1024\n\t"
+        "ds_read_b128 %2, %8 offset:2048\n\t"
+        "ds_read_b128 %3, %8 offset:3072\n\t"
+        "ds_read_b128 %4, %8 offset:4096\n\t"
+        "ds_read_b128 %5, %8 offset:5120\n\t"
+        "ds_read_b128 %6, %8 offset:6144\n\t"
+        "ds_read_b128 %7, %8 offset:7168\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(B[0]), "=&v"(B[1]), "=&v"(B[2]), "=&v"(B[3]), "=&v"(B[4]), "=&v"(B[5]), "=&v"(B[6]), "=&v"(B[7])
+        : "v"(bbase)
+        : "memory");
+}
+
 __device__ __forceinline__ uint32_t lds_offset(const uint16_t *p) {
     return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) uint16_t *)p;
 }
 
 // Fragments of the current chunk from one stage: 4 A row groups by NJ B
 // column groups, 12 NJ MFMAs.
-template <int NJ>
+// MASK: zero the A fragments of row groups past the block (their DMA loads
+// read a clamped neighbour).  Without it the caller clears those rows of
+// the accumulators once the chunks are done (x3_clear_rows), which gives
+// the same result: the products of clamped fragments are finite and only
+// reach those rows.
+template <int NJ, bool MASK = true>
 __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][NJ], const uint16_t *stage, int fa, int lane, int wr,
                                            int wc, int mtile16, int nfm) {
     const uint32_t base = lds_offset(stage) + lane * 16;
-    u32x4_t A[8], B[2 * NJ];
-    lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * (2 * NJ) * 1024, A, B);
-    bf16x8_t ah[4], al[4], bh[NJ], bl[NJ];
-    const u32x4_t z = u32x4_t{0, 0, 0, 0};
+    if constexpr (NJ == 8) {
+        // waves of 64 x 128: the B fragments in two halves of four column
+        // groups, so that only half of them occupy registers at a time
+        u32x4_t A[8], B[8];
+        const uint32_t bb = base + fa * 1024 + wc * 16 * 1024;
+        lds_read_frags(base + wr * 8192, bb, A, B);
+        bf16x8_t ah[4], al[4];
+        const u32x4_t z = u32x4_t{0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const bool ok = mtile16 + wr * 4 + i < nfm;
-        ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
-        al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
-    }
+        for (int i = 0; i < 4; ++i) {
+            const bool ok = !MASK || mtile16 + wr * 4 + i < nfm;
+            ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
+            al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
+        }
 #pragma unroll
-    for (int j = 0; j < NJ; ++j) {
-        bh[j] = __builtin_bit_cast(bf16x8_t, B[2 * j]);
-        bl[j] = __builtin_bit_cast(bf16x8_t, B[2 * j + 1]);
-    }
+        for (int half = 0; half < 2; ++half) {
+            if (half == 1) lds_read_frags(bb + 8192, B);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, B[2 * j]);
+                    const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, B[2 * j + 1]);
+                    x3_f4 &c = acc[i][half * 4 + j];
+                    c = mfma_bf16(al[i], bh, c);
+                    c = mfma_bf16(ah[i], bl, c);
+                    c = mfma_bf16(ah[i], bh, c);
+                }
+        }
+    } else {
+        u32x4_t A[8], B[2 * NJ];
+        lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * (2 * NJ) * 1024, A, B);
+        bf16x8_t ah[4], al[4], bh[NJ], bl[NJ];
+        const u32x4_t z = u32x4_t{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bool ok = !MASK || mtile16 + wr * 4 + i < nfm;
+            ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
+            al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
+        }
 #pragma unroll
         for (int j = 0; j < NJ; ++j) {
-            acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
-            acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
-            acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+            bh[j] = __builtin_bit_cast(bf16x8_t, B[2 * j]);
+            bl[j] = __builtin_bit_cast(bf16x8_t, B[2 * j + 1]);
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) {
+                acc[i][j] = mfma_bf16(al[i], bh[j], acc[i][j]);
+                acc[i][j] = mfma_bf16(ah[i], bl[j], acc[i][j]);
+                acc[i][j] = mfma_bf16(ah[i], bh[j], acc[i][j]);
+            }
+    }
 }
 
 // Epilogue of a wave's (16 NI) x (16 NJ) accumulator block (NI row groups x
@@ -373,24 +426,27 @@ template <int NI, int NJ, bool DEV = false>
 __device__ __forceinline__ void x3_epilogue(x3_f4 (&acc)[NI][NJ], uint16_t *stg, const X3Args &a, const Task &task,
                                             int lane, int wv, int wr, int wc, int m0, int n0) {
     constexpr int WC = 16 * NJ;   // columns of the wave
+    constexpr int JG = NJ > 4 ? 4 : NJ;   // column groups staged at a time (64 x 16 JG floats per wave)
+    constexpr int GC = 16 * JG;
     const int M = task.m;
     if (task.c_buf == VEC_Y) {
-        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * WC;
+        float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * GC;
         const int64_t base = task.c_row * (int64_t)a.nrhs;
 #pragma unroll
-        for (int h = 0; h < NI / 4; ++h) {
+        for (int hj = 0; hj < (NI / 4) * (NJ / JG); ++hj) {
+            const int h = hj / (NJ / JG), j0 = (hj % (NJ / JG)) * JG;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int j = 0; j < NJ; ++j)
+                for (int j = 0; j < JG; ++j)
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + r;
-                        Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[h * 4 + i][j][r];
+                        Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[h * 4 + i][j0 + j][r];
                     }
             __builtin_amdgcn_wave_barrier();
-            const int rb = m0 + wr * 16 * NI + h * 64, cb = n0 + wc * WC;
-            for (int e = lane; e < 64 * WC; e += 64) {
+            const int rb = m0 + wr * 16 * NI + h * 64, cb = n0 + wc * WC + j0 * 16;
+            for (int e = lane; e < 64 * GC; e += 64) {
                 const int col = e >> 6, row = e & 63;
                 const int gm = rb + row, gn = cb + col;
                 if (gm < M && gn < a.nrhs) {
@@ -521,6 +577,17 @@ __global__ __launch_bounds__(256) void x3r_kernel(X3Args a) {
     x3_epilogue<4, 4>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
 
+// Accumulator rows of row groups past the block (4 per wave from row group
+// mtile16 + 4 wr) set to zero.
+template <int NJ>
+__device__ __forceinline__ void x3_clear_rows(x3_f4 (&acc)[4][NJ], int wr, int mtile16, int nfm) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (mtile16 + wr * 4 + i >= nfm)
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+}
+
 // One output tile (rows m0.., columns n0.. of task): the task's segment
 // list staged in LDS, then the NST-stage pipeline and the epilogue (DEV:
 // device-scope skeleton panel stores).  on_seg as in x3_issue.  Ends with the
@@ -529,11 +596,14 @@ __global__ __launch_bounds__(256) void x3r_kernel(X3Args a) {
 template <typename S, int NST, int NJ, bool DEV = false, typename OnSeg>
 __device__ __forceinline__ void x3_tile(const X3Args &a, const Task &task, int m0, int n0, uint16_t *stg,
                                         Seg *sseg, OnSeg &&on_seg) {
-    static_assert(S::NW * 64 * 16 * NJ * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
+    static_assert(S::NW * 64 * 16 * (NJ > 4 ? 4 : NJ) * 4 <= NST * S::STAGE * 2,
+                  "the epilogue staging must fit in the stages");
     constexpr int WN = S::BN / (16 * NJ);
     const int tid = threadIdx.x;
     const int M = task.m;
-    const int lane = tid & 63, wv = tid >> 6;
+    // the wave index in a scalar register: every per-wave quantity derived
+    // from it (DMA destinations, fragment offsets) stays scalar
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = wv / WN, wc = wv % WN;
     const int nfm = (M + 15) / 16;
     const int mtile16 = m0 / 16;
@@ -568,8 +638,9 @@ __device__ __forceinline__ void x3_tile(const X3Args &a, const Task &task, int m
         wait_dma<S::PER, NST - 1>(issued - c - 1);
         wg_barrier();
         issue();
-        x3_compute<NJ>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+        x3_compute<NJ, false>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
     }
+    x3_clear_rows<NJ>(acc, wr, mtile16, nfm);
     __syncthreads();
     x3_epilogue<4, NJ, DEV>(acc, stg, a, task, lane, wv, wr, wc, m0, n0);
 }
@@ -745,11 +816,14 @@ __global__ __launch_bounds__(64 * X3SK_WAVES) void x3sk_kernel(X3Args a) {
     x3sk_tile<false, true>(a, task, tl.y * X3_BM, nt * 32, part, sseg, [](int) {});
 }
 
-int x3_variant_bm(int variant) { return variant == X3_V256x128_W8S3 ? 2 * X3_BM : X3_BM; }
+int x3_variant_bm(int variant) {
+    return variant == X3_V256x128_W8S3 || variant == X3_V256x256_W8S2 ? 2 * X3_BM : X3_BM;
+}
 
 int x3_variant_bn(int variant) {
     switch (variant) {
     case X3_V128x256_S3: return 256;
+    case X3_V256x256_W8S2: return 256;
     case X3_V128x64_S2: return 64;
     case X3_V128x32_S2: return 32;
     case X3_V128x32_S6: return 32;
@@ -796,6 +870,9 @@ void launch_x3(const X3Args &a, int variant, hipStream_t st) {
         break;
     case X3_V128x32_K8:
         hipLaunchKernelGGL(x3sk_kernel, grid, dim3(64 * X3SK_WAVES), 0, st, a);
+        break;
+    case X3_V256x256_W8S2:
+        hipLaunchKernelGGL((x3_kernel<4, 2, 2, 8>), grid, dim3(512), 0, st, a);
         break;
     default:
         hipLaunchKernelGGL((x3_kernel<2, 2, 2>), grid, dim3(256), 0, st, a);

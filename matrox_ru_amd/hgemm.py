@@ -22,7 +22,7 @@ SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "
 DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5",
-               12: "128x32_k8"}
+               12: "128x32_k8", 13: "256x256_w8s2"}
 IO_F64, IO_F32 = 0, 1
 NOALLOC_SCRATCH_BYTES = 256 << 20   # MX_NOALLOC_SCRATCH_BYTES
 BATCH_X3_GUARD = 8e-6               # MX_BATCH_X3_GUARD
