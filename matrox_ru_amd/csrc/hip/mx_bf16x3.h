@@ -121,7 +121,9 @@ void launch_x3_panels_to_f32(const uint16_t *hi, const uint16_t *lo, const X3Row
 //   in parallel instead of in a pipeline, for the chains of short tasks;
 //   256 x 256, 2 stages: 8 waves (4 x 2) of 64 x 128, 128 KB of stages, one
 //   workgroup per CU (two waves per SIMD): half the fragment bytes per flop
-//   of the 128 x 128 tile, and a quarter fewer LDS reads per MFMA.
+//   of the 128 x 128 tile, and a quarter fewer LDS reads per MFMA; three
+//   quarters of each chunk's MFMAs are issued behind the LDS reads of the
+//   next quarter of B fragments.
 enum X3Variant {
     X3_V128x128_S2 = 1,
     X3_V128x128_S4 = 2,

@@ -336,6 +336,57 @@ __device__ __forceinline__ uint32_t lds_offset(const uint16_t *p) {
 
 // Fragments of the current chunk from one stage: 4 A row groups by NJ B
 // column groups, 12 NJ MFMAs.
+// Waves of 64 x 128, pipelined: the 24 MFMAs of column groups J0, J0+1
+// (acc += al bh, then ah bl, then ah bh, for 4 row groups) issued in one
+// asm block behind the LDS reads of the next four B fragments (N, 1 KB apart
+// from byte offset rd), so that the reads proceed while the MFMAs run; the
+// block waits for its reads before it ends.  The three products of an
+// accumulator keep their order (results are those of the builtins); each
+// accumulator is the source C of the MFMA that wrote it, which needs no
+// wait states.
+template <int J0>
+__device__ __forceinline__ void x3_mma24_read4(x3_f4 (&c)[4][8], const u32x4_t (&A)[8], const u32x4_t (&B)[4],
+                                               uint32_t rd, u32x4_t (&N)[4]) {
+    asm volatile(
+        "ds_read_b128 %0, %24 offset:0\n\t"
+        "ds_read_b128 %1, %24 offset:1024\n\t"
+        "ds_read_b128 %2, %24 offset:2048\n\t"
+        "ds_read_b128 %3, %24 offset:3072\n\t"
+        // al * bh
+        "v_mfma_f32_16x16x32_bf16 %4, %13, %20, %4\n\t"
+        "v_mfma_f32_16x16x32_bf16 %5, %13, %22, %5\n\t"
+        "v_mfma_f32_16x16x32_bf16 %6, %15, %20, %6\n\t"
+        "v_mfma_f32_16x16x32_bf16 %7, %15, %22, %7\n\t"
+        "v_mfma_f32_16x16x32_bf16 %8, %17, %20, %8\n\t"
+        "v_mfma_f32_16x16x32_bf16 %9, %17, %22, %9\n\t"
+        "v_mfma_f32_16x16x32_bf16 %10, %19, %20, %10\n\t"
+        "v_mfma_f32_16x16x32_bf16 %11, %19, %22, %11\n\t"
+        // ah * bl
+        "v_mfma_f32_16x16x32_bf16 %4, %12, %21, %4\n\t"
+        "v_mfma_f32_16x16x32_bf16 %5, %12, %23, %5\n\t"
+        "v_mfma_f32_16x16x32_bf16 %6, %14, %21, %6\n\t"
+        "v_mfma_f32_16x16x32_bf16 %7, %14, %23, %7\n\t"
+        "v_mfma_f32_16x16x32_bf16 %8, %16, %21, %8\n\t"
+        "v_mfma_f32_16x16x32_bf16 %9, %16, %23, %9\n\t"
+        "v_mfma_f32_16x16x32_bf16 %10, %18, %21, %10\n\t"
+        "v_mfma_f32_16x16x32_bf16 %11, %18, %23, %11\n\t"
+        // ah * bh
+        "v_mfma_f32_16x16x32_bf16 %4, %12, %20, %4\n\t"
+        "v_mfma_f32_16x16x32_bf16 %5, %12, %22, %5\n\t"
+        "v_mfma_f32_16x16x32_bf16 %6, %14, %20, %6\n\t"
+        "v_mfma_f32_16x16x32_bf16 %7, %14, %22, %7\n\t"
+        "v_mfma_f32_16x16x32_bf16 %8, %16, %20, %8\n\t"
+        "v_mfma_f32_16x16x32_bf16 %9, %16, %22, %9\n\t"
+        "v_mfma_f32_16x16x32_bf16 %10, %18, %20, %10\n\t"
+        "v_mfma_f32_16x16x32_bf16 %11, %18, %22, %11\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(N[0]), "=&v"(N[1]), "=&v"(N[2]), "=&v"(N[3]), "+v"(c[0][J0]), "+v"(c[0][J0 + 1]), "+v"(c[1][J0]),
+          "+v"(c[1][J0 + 1]), "+v"(c[2][J0]), "+v"(c[2][J0 + 1]), "+v"(c[3][J0]), "+v"(c[3][J0 + 1])
+        : "v"(A[0]), "v"(A[1]), "v"(A[2]), "v"(A[3]), "v"(A[4]), "v"(A[5]), "v"(A[6]), "v"(A[7]), "v"(B[0]),
+          "v"(B[1]), "v"(B[2]), "v"(B[3]), "v"(rd)
+        : "memory");
+}
+
 // MASK: zero the A fragments of row groups past the block (their DMA loads
 // read a clamped neighbour).  Without it the caller clears those rows of
 // the accumulators once the chunks are done (x3_clear_rows), which gives
@@ -346,34 +397,34 @@ __device__ __forceinline__ void x3_compute(x3_f4 (&acc)[4][NJ], const uint16_t *
                                            int wc, int mtile16, int nfm) {
     const uint32_t base = lds_offset(stage) + lane * 16;
     if constexpr (NJ == 8) {
-        // waves of 64 x 128: the B fragments in two halves of four column
-        // groups, so that only half of them occupy registers at a time
-        u32x4_t A[8], B[8];
+        // waves of 64 x 128: A and the first quarter of B (column groups
+        // 0-1), then three quarters of the MFMAs each behind the reads of
+        // the next quarter, the last quarter by builtins (so the compiler
+        // sees the last writes of the chunk)
+        static_assert(!MASK, "the pipelined form clears rows past the block afterwards");
         const uint32_t bb = base + fa * 1024 + wc * 16 * 1024;
-        lds_read_frags(base + wr * 8192, bb, A, B);
+        u32x4_t A[8], B0[4], B1[4];
+        lds_read_frags(base + wr * 8192, bb, A, B0);
+        x3_mma24_read4<0>(acc, A, B0, bb + 4096, B1);
+        x3_mma24_read4<2>(acc, A, B1, bb + 8192, B0);
+        x3_mma24_read4<4>(acc, A, B0, bb + 12288, B1);
         bf16x8_t ah[4], al[4];
-        const u32x4_t z = u32x4_t{0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const bool ok = !MASK || mtile16 + wr * 4 + i < nfm;
-            ah[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i] : z);
-            al[i] = __builtin_bit_cast(bf16x8_t, ok ? A[2 * i + 1] : z);
+            ah[i] = __builtin_bit_cast(bf16x8_t, A[2 * i]);
+            al[i] = __builtin_bit_cast(bf16x8_t, A[2 * i + 1]);
         }
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-            if (half == 1) lds_read_frags(bb + 8192, B);
+        for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, B[2 * j]);
-                    const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, B[2 * j + 1]);
-                    x3_f4 &c = acc[i][half * 4 + j];
-                    c = mfma_bf16(al[i], bh, c);
-                    c = mfma_bf16(ah[i], bl, c);
-                    c = mfma_bf16(ah[i], bh, c);
-                }
-        }
+            for (int j = 0; j < 2; ++j) {
+                const bf16x8_t bh = __builtin_bit_cast(bf16x8_t, B1[2 * j]);
+                const bf16x8_t bl = __builtin_bit_cast(bf16x8_t, B1[2 * j + 1]);
+                x3_f4 &c = acc[i][6 + j];
+                c = mfma_bf16(al[i], bh, c);
+                c = mfma_bf16(ah[i], bl, c);
+                c = mfma_bf16(ah[i], bh, c);
+            }
     } else {
         u32x4_t A[8], B[2 * NJ];
         lds_read_frags(base + wr * 8192, base + fa * 1024 + wc * (2 * NJ) * 1024, A, B);
