@@ -221,6 +221,15 @@ mx_status mx_plan_set_blocks(mx_plan *plan, const double *D, const double *B, co
  * (2*m*n*nrhs per product). */
 mx_status mx_plan_grad_flops(const mx_plan *plan, int32_t nrhs, double *flops);
 
+/* Tile of the split-BF16x3 block-gradient products of mx_backward: 128
+ * (128 x 128, two workgroups per CU), 256 (256 x 256, one workgroup of 8
+ * waves per CU, half the operand bytes per flop), or 0 (default: 256 when
+ * its padded work is within 15% of the 128 tiles', i.e. blocks of about 256
+ * rows and columns).  mx_plan_get_grad_tile gives the tile in use.  No
+ * counterpart in the reference. */
+mx_status mx_plan_set_grad_tile(mx_plan *plan, int32_t tile);
+mx_status mx_plan_get_grad_tile(const mx_plan *plan, int32_t *tile);
+
 /* Device pointers of the skeleton buffers of the last evaluation
  * (wskel = V^T W per node, uskel = coupled + transferred values per node),
  * laid out at skelrow[v]*nrhs with leading dimension slen[v]; element type is

@@ -1450,12 +1450,17 @@ __device__ __forceinline__ void kg_issue(KLoader<S> &L, const GradX3Args<O> &a, 
     --L.left;
 }
 
-template <typename O>
-__global__ __launch_bounds__(256) void x3_grad_kernel(GradX3Args<O> a) {
-    typedef X3Shape<2, 2> S;
+// WM x WN waves of 64 x 16 NJ: 2 x 2 x 4 (128 x 128 tiles, two workgroups
+// per CU) or 4 x 2 x 8 (256 x 256 tiles, one workgroup per CU, the
+// pipelined 64 x 128 wave compute of the evaluation kernel, rows past X
+// cleared after the chunks).
+template <typename O, int WM, int WN, int NJ>
+__global__ __launch_bounds__(64 * WM * WN) void x3_grad_kernel(GradX3Args<O> a) {
+    typedef X3Shape<WM, WN, NJ> S;
     constexpr int NST = 2;
-    static_assert(S::BM == GRAD_X3_TILE && S::BN == GRAD_X3_TILE, "tile size");
-    static_assert(S::NW * 64 * 64 * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
+    constexpr int JG = NJ > 4 ? 4 : NJ;   // column groups staged at a time in the epilogue
+    static_assert(S::BM == S::BN && (S::BM == GRAD_X3_TILE || S::BM == GRAD_X3_WTILE), "tile size");
+    static_assert(S::NW * 64 * 16 * JG * 4 <= NST * S::STAGE * 2, "the epilogue staging must fit in the stages");
     __shared__ __attribute__((aligned(16))) uint16_t stg[NST * S::STAGE];
     __shared__ KSeg sseg[GRAD_KMAXSEG];
 
@@ -1466,8 +1471,8 @@ __global__ __launch_bounds__(256) void x3_grad_kernel(GradX3Args<O> a) {
     if (out == nullptr) return;   // this block type's gradient was not requested (uniform)
     const int M = task.m, N = task.n;
     const int m0 = tl.y * S::BM, n0 = tl.z * S::BN;
-    const int lane = tid & 63, wv = tid >> 6;
-    const int wr = wv >> 1, wc = wv & 1;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wv / WN, wc = wv % WN;
     const int nfm = (M + 15) / 16, nfn = (N + 15) / 16;
     const int mtile16 = m0 / 16, ntile16 = n0 / 16;
     const int nfk = (a.nrhs + 31) / 32;
@@ -1477,11 +1482,11 @@ __global__ __launch_bounds__(256) void x3_grad_kernel(GradX3Args<O> a) {
     __syncthreads();
     const int nch = nseg * nfk;
 
-    x3_f4 acc[4][4];
+    x3_f4 acc[4][NJ];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < NJ; ++j) acc[i][j] = x3_f4{0.f, 0.f, 0.f, 0.f};
 
     KLoader<S> ld;
     ld.seg = -1;
@@ -1499,38 +1504,50 @@ __global__ __launch_bounds__(256) void x3_grad_kernel(GradX3Args<O> a) {
         wait_dma<S::PER, NST - 1>(issued - c - 1);
         wg_barrier();
         issue();
-        x3_compute<4>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
+        x3_compute<NJ, NJ != 8>(acc, stg + (c % NST) * S::STAGE, S::FA, lane, wr, wc, mtile16, nfm);
     }
+    if constexpr (NJ == 8) x3_clear_rows<NJ>(acc, wr, mtile16, nfm);
     __syncthreads();
 
     // C(row, col) of block (i, j), register q: row 16 i + 4 (lane / 16) + q,
-    // column 16 j + lane % 16 of the wave's 64 x 64
-    float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 64;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + q;
-                Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[i][j][q];
-            }
-    __builtin_amdgcn_wave_barrier();
+    // column 16 j + lane % 16 of the wave's 64 x 16 NJ; staged 64 x 16 JG at a time
+    float *Cs = reinterpret_cast<float *>(stg) + wv * 64 * 16 * JG;
     O *C = out + task.c_off;
-    const int rb = m0 + wr * 64, cb = n0 + wc * 64;
-    for (int e = lane; e < 64 * 64; e += 64) {
-        const int col = e >> 6, row = e & 63;
-        const int gm = rb + row, gn = cb + col;
-        if (gm < M && gn < N) C[(int64_t)gn * task.ldc + gm] = (O)Cs[col * 64 + (row ^ ((col & 15) << 2))];
+#pragma unroll
+    for (int g0 = 0; g0 < NJ; g0 += JG) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < JG; ++j)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int col = j * 16 + (lane & 15), row = i * 16 + 4 * (lane >> 4) + q;
+                    Cs[col * 64 + (row ^ ((col & 15) << 2))] = acc[i][g0 + j][q];
+                }
+        __builtin_amdgcn_wave_barrier();
+        const int rb = m0 + wr * 64, cb = n0 + wc * 16 * NJ + g0 * 16;
+        for (int e = lane; e < 64 * 16 * JG; e += 64) {
+            const int col = e >> 6, row = e & 63;
+            const int gm = rb + row, gn = cb + col;
+            if (gm < M && gn < N) C[(int64_t)gn * task.ldc + gm] = (O)Cs[col * 64 + (row ^ ((col & 15) << 2))];
+        }
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 template <typename O>
 void launch_grad_x3(const GradX3Args<O> &a, hipStream_t st) {
     if (a.ntiles <= 0) return;
-    hipLaunchKernelGGL((x3_grad_kernel<O>), dim3((unsigned)a.ntiles), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((x3_grad_kernel<O, 2, 2, 4>), dim3((unsigned)a.ntiles), dim3(256), 0, st, a);
+}
+template <typename O>
+void launch_grad_x3w(const GradX3Args<O> &a, hipStream_t st) {
+    if (a.ntiles <= 0) return;
+    hipLaunchKernelGGL((x3_grad_kernel<O, 4, 2, 8>), dim3((unsigned)a.ntiles), dim3(512), 0, st, a);
 }
 template void launch_grad_x3<float>(const GradX3Args<float> &, hipStream_t);
 template void launch_grad_x3<double>(const GradX3Args<double> &, hipStream_t);
+template void launch_grad_x3w<float>(const GradX3Args<float> &, hipStream_t);
+template void launch_grad_x3w<double>(const GradX3Args<double> &, hipStream_t);
 
 }  // namespace mx

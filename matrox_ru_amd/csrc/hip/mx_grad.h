@@ -65,6 +65,7 @@ void launch_grad(const GradArgs<T, O> &a, hipStream_t st);
 // columns) of Z alike, and a gradient product runs on the LDS-DMA pipeline
 // of the BF16x3 evaluation kernel, 128 x 128 tiles (GRAD_X3_TILE).
 constexpr int GRAD_X3_TILE = 128;
+constexpr int GRAD_X3_WTILE = 256;  // the wide form: 256 x 256 tiles, 8 waves of 64 x 128
 constexpr int GRAD_KMAXSEG = 8;     // segments per gradient task (2 are used)
 
 struct alignas(16) KSeg {
@@ -92,6 +93,9 @@ struct GradX3Args {
 
 template <typename O>
 void launch_grad_x3(const GradX3Args<O> &a, hipStream_t st);
+// the same products on GRAD_X3_WTILE tiles (a.tiles from the wide list)
+template <typename O>
+void launch_grad_x3w(const GradX3Args<O> &a, hipStream_t st);
 
 // Column blocks (T = float or double, element (i, k) of a block at
 // row * nrhs + k * m + i) into K-fragments.

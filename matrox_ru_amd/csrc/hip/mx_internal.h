@@ -176,6 +176,10 @@ struct mx_plan {
     int g_ntiles = 0;
     int4 *g_tiles_x3 = nullptr;         // GRAD_X3_TILE tiles
     int g_ntiles_x3 = 0;
+    int4 *g_tiles_x3w = nullptr;        // GRAD_X3_WTILE tiles
+    int g_ntiles_x3w = 0;
+    int g_x3_tile = 0;                  // BF16x3 gradient tile: 0 = by padded work, 128 or 256
+    bool g_x3w_auto = false;            // the padded work favours the 256 x 256 tiles
     // split-BF16x3 gradient products: segments over K-fragment blocks, the
     // blocks to pack (leaf blocks of W / dY, skeleton blocks in the column
     // layout and as split panels), and the K-fragment buffers of the six
@@ -239,7 +243,7 @@ struct mx_plan {
                 for (auto &ps : pk) ps.release();
         void *ptrs[] = {D64, B64, VT64, D32, B32,
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
-                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, d_xblk[0], d_xblk[1], d_xblk[2],
+                        d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, g_tiles_x3w, d_xblk[0], d_xblk[1], d_xblk[2],
                         wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag,
                         f64p[0], f64p[1], f64p[2], d_fblk[0], d_fblk[1], d_fblk[2]};
         for (void *p : ptrs)
@@ -276,6 +280,8 @@ struct GradLists {
     std::vector<GSeg> segs;
     std::vector<int4> tiles;      // 64 x 64 tiles (FP64 / FP32 products)
     std::vector<int4> tiles_x3;   // GRAD_X3_TILE tiles (split-BF16x3 products)
+    std::vector<int4> tiles_x3w;  // GRAD_X3_WTILE tiles (the same products, 256 x 256)
+    double area_x3 = 0.0, area_x3w = 0.0;   // tile area x segments of each tiling (padded work)
     double flops = 0.0;   // per nrhs column
 };
 

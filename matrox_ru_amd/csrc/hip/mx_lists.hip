@@ -847,9 +847,13 @@ GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
         const GTask &k = G.tasks[tk];
         for (int i = 0; i < (k.m + 63) / 64; ++i)
             for (int j = 0; j < (k.n + 63) / 64; ++j) G.tiles.push_back(make_int4(tk, i, j, 0));
-        constexpr int T = GRAD_X3_TILE;
+        constexpr int T = GRAD_X3_TILE, TW = GRAD_X3_WTILE;
         for (int i = 0; i < (k.m + T - 1) / T; ++i)
             for (int j = 0; j < (k.n + T - 1) / T; ++j) G.tiles_x3.push_back(make_int4(tk, i, j, 0));
+        for (int i = 0; i < (k.m + TW - 1) / TW; ++i)
+            for (int j = 0; j < (k.n + TW - 1) / TW; ++j) G.tiles_x3w.push_back(make_int4(tk, i, j, 0));
+        G.area_x3 += (double)((k.m + T - 1) / T) * ((k.n + T - 1) / T) * T * T * (k.seg_end - k.seg_begin);
+        G.area_x3w += (double)((k.m + TW - 1) / TW) * ((k.n + TW - 1) / TW) * TW * TW * (k.seg_end - k.seg_begin);
     };
     for (int64_t k = 0; k < s.ncount; ++k) {
         int x = s.is_hss ? (int)k : s.npairx[k], y = s.is_hss ? (int)k : s.npairy[k];
@@ -942,6 +946,7 @@ GradLists build_grad_lists(const mx_structure &s, const Topology &t) {
     };
     balance(G.tiles);
     balance(G.tiles_x3);
+    balance(G.tiles_x3w);
     return G;
 }
 

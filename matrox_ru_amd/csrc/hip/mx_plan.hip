@@ -266,6 +266,11 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         P->g_ntiles = (int)G.tiles.size();
         P->g_tiles_x3 = upload(G.tiles_x3.data(), G.tiles_x3.size());
         P->g_ntiles_x3 = (int)G.tiles_x3.size();
+        P->g_tiles_x3w = upload(G.tiles_x3w.data(), G.tiles_x3w.size());
+        P->g_ntiles_x3w = (int)G.tiles_x3w.size();
+        // the 256 x 256 tiles where their padded work is within 15% of the
+        // 128 x 128 tiles' (blocks of about 256 rows and columns)
+        P->g_x3w_auto = G.area_x3w <= 1.15 * G.area_x3;
         P->g_flops = G.flops;
         if (precisions & MX_PLAN_BF16X3) {
             // K-fragment blocks: leaves (W, dY) and skeleton nodes, each
@@ -878,11 +883,12 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
             }
         }
         prof_mark(P, st, MX_KIND_CONVERT, false);
+        const bool wide = P->g_x3_tile == GRAD_X3_WTILE || (P->g_x3_tile == 0 && P->g_x3w_auto);
         auto fill = [&](auto &g) {
             g.tasks = P->g_tasks;
             g.segs = P->g_ksegs;
-            g.tiles = P->g_tiles_x3;
-            g.ntiles = P->g_ntiles_x3;
+            g.tiles = wide ? P->g_tiles_x3w : P->g_tiles_x3;
+            g.ntiles = wide ? P->g_ntiles_x3w : P->g_ntiles_x3;
             g.nrhs = nrhs;
             for (int b = 0; b < GB_COUNT; ++b) {
                 uint16_t *hi, *lo;
@@ -898,14 +904,16 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
             g.out[MAT_D] = (double *)dD;
             g.out[MAT_B] = (double *)dB;
             g.out[MAT_VT] = (double *)dVT;
-            launch_grad_x3(g, st);
+            if (wide) launch_grad_x3w(g, st);
+            else launch_grad_x3(g, st);
         } else {
             GradX3Args<float> g{};
             fill(g);
             g.out[MAT_D] = (float *)dD;
             g.out[MAT_B] = (float *)dB;
             g.out[MAT_VT] = (float *)dVT;
-            launch_grad_x3(g, st);
+            if (wide) launch_grad_x3w(g, st);
+            else launch_grad_x3(g, st);
         }
         prof_mark(P, st, kind, false);
     } else if (prec == MX_FP64) {
@@ -1589,6 +1597,23 @@ mx_status mx_plan_grad_flops(const mx_plan *plan, int32_t nrhs, double *flops) {
         arg_check(plan && flops && nrhs >= 0, "bad argument");
         if (!plan->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
         *flops = plan->g_flops * nrhs;
+        return 0;
+    });
+}
+
+mx_status mx_plan_set_grad_tile(mx_plan *plan, int32_t tile) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr, "null plan");
+        arg_check(tile == 0 || tile == GRAD_X3_TILE || tile == GRAD_X3_WTILE, "gradient tile: 0, 128 or 256");
+        plan->g_x3_tile = tile;
+        return 0;
+    });
+}
+
+mx_status mx_plan_get_grad_tile(const mx_plan *plan, int32_t *tile) {
+    return (mx_status)guard([&]() {
+        arg_check(plan != nullptr && tile != nullptr, "null argument");
+        *tile = plan->g_x3_tile == 0 ? (plan->g_x3w_auto ? GRAD_X3_WTILE : GRAD_X3_TILE) : plan->g_x3_tile;
         return 0;
     });
 }

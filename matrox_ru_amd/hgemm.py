@@ -85,6 +85,10 @@ def lib():
         L.mx_plan_set_blocks.restype = ctypes.c_int
         L.mx_plan_grad_flops.argtypes = [P, I32, ctypes.POINTER(ctypes.c_double)]
         L.mx_plan_grad_flops.restype = ctypes.c_int
+        L.mx_plan_set_grad_tile.argtypes = [P, I32]
+        L.mx_plan_set_grad_tile.restype = ctypes.c_int
+        L.mx_plan_get_grad_tile.argtypes = [P, ctypes.POINTER(I32)]
+        L.mx_plan_get_grad_tile.restype = ctypes.c_int
         L.mx_plan_set_schedule.argtypes = [P, I32, ctypes.c_int, I32, I32]
         L.mx_plan_set_schedule.restype = ctypes.c_int
         L.mx_plan_get_schedule.argtypes = [P, I32, ctypes.c_int, ctypes.POINTER(I32), ctypes.POINTER(I32)]
@@ -408,6 +412,16 @@ class Plan:
         f = ctypes.c_double()
         _check(lib().mx_plan_grad_flops(self._h, nrhs, ctypes.byref(f)))
         return f.value
+
+    def set_grad_tile(self, tile):
+        """Tile of the split-BF16x3 block gradients: 128, 256, or 0 (chosen
+        by padded work; mx_plan_set_grad_tile)."""
+        _check(lib().mx_plan_set_grad_tile(self._h, int(tile)))
+
+    def grad_tile(self):
+        t = I32()
+        _check(lib().mx_plan_get_grad_tile(self._h, ctypes.byref(t)))
+        return t.value
 
     def skeletons(self, nrhs, dtype, stream=None):
         """Copies of (wskel, uskel) of the last evaluation as CUDA tensors."""

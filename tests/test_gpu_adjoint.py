@@ -339,3 +339,35 @@ def test_bf16x3_backward_close_to_fp64(gpu, case, nrhs, io):
         sub = plan.backward(W, dY, nrhs, "bf16x3", skeletons=(ws, us), blocks=flags)
         assert all(sub[1 + i] is None for i in range(3) if i != k)
         assert torch.equal(sub[1 + k], got2[1 + k]), k
+
+
+@pytest.mark.parametrize("case,nrhs,io", [("h2b_ragged", 48, "f32"), ("h2b_wide", 77, "f64"),
+                                          ("h2b_wide", 160, "f32")])
+def test_bf16x3_gradient_tiles_agree(gpu, case, nrhs, io):
+    """The 128 x 128 and 256 x 256 tilings of the split-BF16x3 block
+    gradients give bit-identical results (each output element takes the same
+    products in the same order); 0 picks the tile by padded work: 128 for
+    blocks of about 100 rows, 256 for blocks of 150-200 rows, whose 128 x 128
+    tiles pad to the same 256 x 256 area."""
+    import torch
+    _say(f"[grad tiles] {case} nrhs={nrhs} io={io}")
+    X, hm = make_problem(**CASES[case])
+    plan = hgemm.Plan(hm, precisions=("bf16x3",), adjoint=True)
+    assert plan.grad_tile() == (128 if case == "h2b_ragged" else 256)
+    dt = torch.float64 if io == "f64" else torch.float32
+    W = _dev(host.normal((hm.N * nrhs,), seed=171)).to(dt)
+    dY = _dev(host.normal((hm.N * nrhs,), seed=172)).to(dt)
+    Y = torch.empty_like(W)
+    res = {}
+    for tile in (128, 256):
+        plan.eval(W, Y, nrhs, "bf16x3")   # the skeletons the block gradients read
+        plan.set_grad_tile(tile)
+        assert plan.grad_tile() == tile
+        res[tile] = plan.backward(W, dY, nrhs, "bf16x3")
+        torch.cuda.synchronize()
+    for a, b, nm in zip(res[128], res[256], ["dW", "dD", "dB", "dVT"]):
+        assert torch.isfinite(b).all(), nm
+        assert torch.equal(a, b), nm
+    plan.set_grad_tile(0)
+    with pytest.raises(hgemm.MxError):
+        plan.set_grad_tile(64)
