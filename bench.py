@@ -87,6 +87,9 @@ def parse():
     ap.add_argument("--launch-variants", default="",
                     help="comma-separated BF16x3 kernel variant per forward launch under --schedule, "
                          "replacing the autotuner's choice (to profile a fixed configuration)")
+    ap.add_argument("--fp64-seg-variants", default="",
+                    help="SCHEDULE:v1,v2,... FP64 forward schedule and kernel variant per launch, replacing the "
+                         "autotuner's choice (to profile a fixed configuration)")
     ap.add_argument("--schedule", default="fused", choices=["fused", "split", "fused_balanced", "split_balanced"],
                     help="forward schedule used with --launch-variants")
     ap.add_argument("--no-dropin", action="store_true",
@@ -670,6 +673,13 @@ def main():
         tuning["bf16x3"]["forward"] = plan.get_schedule("forward", "bf16x3")
         tuning["bf16x3"]["launch_variants"]["forward"] = plan.launch_variants("forward")
         tuning["bf16x3"]["pinned"] = True
+    if args.fp64_seg_variants and "fp64" in plan.precisions:
+        sched, vs = args.fp64_seg_variants.split(":")
+        plan.set_schedule("forward", "fp64", sched)
+        plan.set_seg_variants([int(v) for v in vs.split(",")], "fp64", "forward")
+        tuning["fp64"]["forward"] = plan.get_schedule("forward", "fp64")
+        tuning["fp64"]["seg_variants"]["forward"] = plan.seg_variants("fp64", "forward")
+        tuning["fp64"]["pinned"] = True
     log(f"[bench] autotuned: {tuning}")
     plan.set_graph(not args.no_graph)
     ms64 = timer(lambda: plan.eval(Wd, Y64, nrhs, precision="fp64"), max(3, args.steps // 4), 1)
