@@ -325,12 +325,12 @@ mx_status mx_plan_set_launch_variants(mx_plan *plan, int32_t direction, const in
 /* The same for the FP64 / FP32 kernel (prec MX_FP64 or MX_FP32): per launch
  * 1 = 64 x 64 tiles with one chunk of loads in flight, 2 = 64 x 64 with two,
  * 3 = 128 x 128 tiles (FP32 only), 4 / 5 = 128 x 128 tiles on A operands
- * packed when the plan is built, two / three LDS stages (FP64 only; a launch
- * with a task of more than 256 segments runs variant 1 instead, and so does
- * every launch of a plan whose packed copies did not fit in device memory
- * beside its blocks, with 16 GiB + 10% of the device to spare, when it was
- * created).  The autotuner picks them launch by launch.  No counterpart in
- * the reference. */
+ * packed when the plan is built (FP64: two / three LDS stages; FP32: two /
+ * four); a launch with a task of more than 256 segments runs variant 1
+ * instead, and so does every launch of a plan whose packed copies did not
+ * fit in device memory beside its blocks, with 16 GiB + 10% of the device
+ * to spare, when it was created.  The autotuner picks them launch by
+ * launch.  No counterpart in the reference. */
 mx_status mx_plan_seg_variants(const mx_plan *plan, int32_t direction, mx_precision prec, int32_t *variants,
                                int32_t cap, int32_t *count);
 mx_status mx_plan_set_seg_variants(mx_plan *plan, int32_t direction, mx_precision prec, const int32_t *variants,

@@ -1,5 +1,6 @@
-// FP64 block products on packed A operands: 128 x 128 tiles, a pipeline of
-// direct global -> LDS loads, v_mfma_f64_16x16x4_f64.
+// FP64 (and FP32) block products on packed A operands: 128 x 128 tiles, a
+// pipeline of direct global -> LDS loads, v_mfma_f64_16x16x4_f64 (the FP32
+// form, v_mfma_f32_16x16x4_f32, is described at F32PArgs).
 //
 // The same launches as the 64 x 64 FP64 kernel (mx_seg_gemm.h), on the
 // program's 128-row tile list; only the A operands differ:
@@ -54,5 +55,28 @@ __global__ void pack_f64_blocks(const double *__restrict__ src, const X3Block *_
 
 // One launch: nst = 2 (two stages, two workgroups per CU) or 3 (one).
 void launch_f64p(const F64PArgs &a, int nst, hipStream_t st);
+
+// FP32 on packed A operands (v_mfma_f32_16x16x4_f32): the same tiles, tile
+// lists and packed offsets (a 16 x 16 fragment holds 256 elements in either
+// precision).  An FP32 fragment is 1 KB; lane l = r + 16 g of an MFMA reads
+// its k = 4g..4g+3 as one 16-byte piece at byte 16 l, so element (r, k)
+// sits at (r + 16 (k / 4)) * 4 + k % 4.  B chunks (16 k of a column, 64
+// bytes) are moved by 4-byte LDS-DMA loads with their 16-byte slots
+// XOR-swizzled by (column / 4) % 4, so that the 16 lanes of a column group
+// read distinct banks.
+struct F32PArgs {
+    const Task *tasks;
+    const Seg *segs;      // packed segments (as F64PArgs)
+    const int2 *tiles;
+    int ntiles, nrhs, n_ntiles, pad;
+    const float *a[3];    // packed D, B, VT
+    float *vec[4];        // W, wskel, uskel, Y (reference layout)
+};
+
+__global__ void pack_f32_blocks(const float *__restrict__ src, const X3Block *__restrict__ blocks,
+                                float *__restrict__ dst);
+
+// One launch: nst = 2 or 4 stages (16 KB each).
+void launch_f32p(const F32PArgs &a, int nst, hipStream_t st);
 
 }  // namespace mx

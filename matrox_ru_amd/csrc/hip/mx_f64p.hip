@@ -221,3 +221,204 @@ void launch_f64p(const F64PArgs &a, int nst, hipStream_t st) {
 }
 
 }  // namespace mx
+
+// ---- FP32 on packed A operands -------------------------------------------------
+
+namespace mx {
+
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ void pack_f32_blocks(const float *__restrict__ src, const X3Block *__restrict__ blocks,
+                                float *__restrict__ dst) {
+    const X3Block bk = blocks[blockIdx.y];
+    const int nkg = (bk.K + 15) / 16, nrg = (bk.M + 15) / 16;
+    const int64_t n = (int64_t)nrg * nkg * F64P_FRAG;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t frag = t >> 8;
+        const int pos = (int)(t & 255);
+        const int rg = (int)(frag / nkg), kg = (int)(frag % nkg);
+        const int l = pos >> 2;   // the MFMA lane that reads this element
+        const int row = rg * 16 + (l & 15);
+        const int k = kg * 16 + 4 * (l >> 4) + (pos & 3);
+        float x = 0.f;
+        if (row < bk.M && k < bk.K)
+            x = bk.trans ? src[bk.src_off + (int64_t)row * bk.lda + k] : src[bk.src_off + (int64_t)k * bk.lda + row];
+        dst[bk.dst_off + t] = x;
+    }
+}
+
+constexpr int F32P_WAVES = 8, F32P_THREADS = 64 * F32P_WAVES;
+// a stage: 8 A fragments (row groups of the tile), then 128 B columns of 16 k
+constexpr int F32P_STAGE = 8 * F64P_FRAG + F64P_BN * F64P_BK;   // floats (16 KB)
+constexpr int F32P_PER = 1 + 4;   // DMA loads per wave per chunk: one A fragment (1 KB), 16 B columns (4 x 256 B)
+
+__device__ __forceinline__ void dma16f(const float *src, float *dst_wave) {
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 16, 0, 0);
+}
+__device__ __forceinline__ void dma4f(const float *src, float *dst_wave) {
+    __builtin_amdgcn_global_load_lds((glb_void_t *)src, (lds_void_t *)dst_wave, 4, 0, 0);
+}
+
+// A chunk's operands of a wave of 64 x 32: its 4 A row groups (16 bytes
+// per lane at ab + 1024 i) and 2 B column groups (column n of group j at
+// bb[j]), read in one asm block that waits for its own reads.
+__device__ __forceinline__ void f32p_read(uint32_t ab, uint32_t bb0, uint32_t bb1, u32x4_t (&A)[4], u32x4_t (&B)[2]) {
+    asm volatile(
+        "ds_read_b128 %0, %6 offset:0\n\t"
+        "ds_read_b128 %1, %6 offset:1024\n\t"
+        "ds_read_b128 %2, %6 offset:2048\n\t"
+        "ds_read_b128 %3, %6 offset:3072\n\t"
+        "ds_read_b128 %4, %7 offset:0\n\t"
+        "ds_read_b128 %5, %8 offset:0\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(A[0]), "=&v"(A[1]), "=&v"(A[2]), "=&v"(A[3]), "=&v"(B[0]), "=&v"(B[1])
+        : "v"(ab), "v"(bb0), "v"(bb1)
+        : "memory");
+}
+
+// The 32 MFMAs of a chunk for a wave of 64 x 32 (row groups 4 wr + i, tile
+// columns 32 wc + 16 j + l%16).  Step s takes k = 4 (l/16) + s of the chunk
+// in A and B alike.  B column n keeps logical slot g (k = 4g..4g+3) at
+// physical slot g ^ ((n >> 2) & 3).
+__device__ __forceinline__ void f32p_compute(f32x4_t (&acc)[4][2], uint32_t sbase, int lane, int wr, int wc) {
+    const int g = lane >> 4;
+    const uint32_t ab = sbase + wr * 4 * 1024 + lane * 16;
+    uint32_t bb[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int n = 32 * wc + 16 * j + (lane & 15);
+        bb[j] = sbase + 8 * 1024 + n * (F64P_BK * 4) + ((g ^ ((n >> 2) & 3)) << 4);
+    }
+    u32x4_t A[4], B[2];
+    f32p_read(ab, bb[0], bb[1], A, B);
+    f32x4_t a[4], b[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = __builtin_bit_cast(f32x4_t, A[i]);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) b[j] = __builtin_bit_cast(f32x4_t, B[j]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][s], b[j][s], acc[i][j], 0, 0, 0);
+}
+
+// 128 x 128 tile of a task by 8 waves (2 x 4 of 64 x 32).  NST stages of
+// LDS; loads run NST-1 chunks ahead of the MFMAs.
+template <int NST>
+__global__ __launch_bounds__(F32P_THREADS) __attribute__((amdgpu_waves_per_eu(NST == 2 ? 4 : 2))) void f32p_kernel(F32PArgs a) {
+    static_assert(NST * F32P_STAGE >= F32P_WAVES * 64 * 16, "the epilogue staging must fit in the stages");
+    __shared__ __attribute__((aligned(16))) float stg[NST * F32P_STAGE];
+    __shared__ Seg sseg[F64P_MAXSEG];
+
+    const int total = a.ntiles * a.n_ntiles;
+    const int w = xcd_remap(blockIdx.x, total);
+    const int tile_id = w / a.n_ntiles, nt = w % a.n_ntiles;
+    const int2 tl = a.tiles[tile_id];
+    const Task task = a.tasks[tl.x];
+    const int M = task.m;
+    if (M <= 0) return;   // padding tile (the whole workgroup)
+    const int m0 = tl.y * F64P_BM, n0 = nt * F64P_BN;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wr = wv >> 2, wc = wv & 3;
+    const int nrg = (M + 15) / 16;
+    const int arg = min(m0 / 16 + wv, nrg - 1);   // the wave's A row group (rows past M: any valid group)
+    // B loads of the wave: instruction j moves tile columns 16 wv + 4 j + l/16,
+    // the lane's 4 bytes at physical dword l%16 of the column's 64 bytes,
+    // whose 16-byte slot (l%16)/4 holds logical slot ((l%16)/4) ^ j
+    const int bq = lane & 3, bps = (lane >> 2) & 3;
+
+    const int nseg = task.seg_end - task.seg_begin;   // <= F64P_MAXSEG (checked by the plan)
+    for (int i = tid; i < nseg; i += F32P_THREADS) sseg[i] = a.segs[task.seg_begin + i];
+    __syncthreads();
+
+    f32x4_t acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+
+    const float *la = nullptr, *lbv = nullptr;
+    int lseg = -1, lkc = 0, lnk = 0, lK = 0, lbld = 0;
+    bool more = true;
+    int issued = 0;
+    auto issue = [&]() {
+        if (!more) return;
+        while (lkc == lnk) {
+            if (++lseg >= nseg) {
+                more = false;
+                return;
+            }
+            const Seg sg = sseg[lseg];
+            lK = sg.k;
+            lnk = (sg.k + F64P_BK - 1) / F64P_BK;
+            lkc = 0;
+            la = a.a[sg.a_buf] + sg.a_off + (int64_t)arg * sg.lda * F64P_FRAG;
+            lbv = a.vec[sg.b_buf] + sg.b_row * (int64_t)a.nrhs;
+            lbld = sg.b_ld;
+        }
+        float *stage = stg + (issued % NST) * F32P_STAGE;
+        dma16f(la + (int64_t)lkc * F64P_FRAG + lane * 4, stage + wv * F64P_FRAG);
+        const int k0 = lkc * F64P_BK;
+        float *bst = stage + 8 * F64P_FRAG + 16 * wv * F64P_BK;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = min(n0 + 16 * wv + 4 * j + (lane >> 4), a.nrhs - 1);
+            const int k = min(k0 + 4 * (bps ^ j) + bq, lK - 1);
+            dma4f(lbv + ((int64_t)col * lbld + k), bst + j * 4 * F64P_BK);
+        }
+        ++lkc;
+        ++issued;
+    };
+#pragma unroll
+    for (int p = 0; p < NST - 1; ++p) issue();
+    for (int c = 0; c < issued; ++c) {
+        wait_dma<F32P_PER, NST - 1>(issued - c - 1);
+        wg_barrier();
+        issue();
+        f32p_compute(acc, lds_addr(stg + (c % NST) * F32P_STAGE), lane, wr, wc);
+    }
+    __syncthreads();
+
+    // epilogue: per column group, the wave's 64 x 16 block through its own
+    // 4 KB of LDS (rows XOR-swizzled by column) and out as 64 consecutive
+    // rows per column
+    float *Cs = stg + wv * 64 * 16;
+    float *C = a.vec[task.c_buf] + task.c_row * (int64_t)a.nrhs;
+    const int rb = m0 + 64 * wr;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = lane & 15, row = 16 * i + 4 * (lane >> 4) + r;
+                Cs[col * 64 + (row ^ (4 * col))] = acc[i][j][r];
+            }
+        __builtin_amdgcn_wave_barrier();
+        const int cb = n0 + 32 * wc + 16 * j;
+#pragma unroll 4
+        for (int e = lane; e < 64 * 16; e += 64) {
+            const int col = e >> 6, row = e & 63;
+            const int gm = rb + row, gn = cb + col;
+            if (gm < M && gn < a.nrhs) {
+                const float v = Cs[col * 64 + (row ^ (4 * col))];
+                float *y = C + (int64_t)gn * M + gm;
+                *y = task.beta ? *y + v : v;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+void launch_f32p(const F32PArgs &a, int nst, hipStream_t st) {
+    const int64_t total = (int64_t)a.ntiles * a.n_ntiles;
+    if (total <= 0) return;
+    if (total > INT32_MAX) throw MxError(MX_ERR_ARG, "launch too large");
+    if (nst == 4) hipLaunchKernelGGL(f32p_kernel<4>, dim3((unsigned)total), dim3(F32P_THREADS), 0, st, a);
+    else hipLaunchKernelGGL(f32p_kernel<2>, dim3((unsigned)total), dim3(F32P_THREADS), 0, st, a);
+}
+
+}  // namespace mx

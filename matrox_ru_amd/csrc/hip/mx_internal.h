@@ -163,6 +163,7 @@ struct mx_plan {
     // FP64 A operands packed in 16 x 16 fragments (plans with FP64), and
     // their packing lists
     double *f64p[3] = {nullptr, nullptr, nullptr};
+    float *f32p[3] = {nullptr, nullptr, nullptr};   // the same fragments in FP32 (plans with FP32)
     int64_t f64p_elems[3] = {0, 0, 0};
     mx::X3Block *d_fblk[3] = {nullptr, nullptr, nullptr};
     int64_t n_fblk[3] = {0, 0, 0};
@@ -245,7 +246,7 @@ struct mx_plan {
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
                         d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, g_tiles_x3w, d_xblk[0], d_xblk[1], d_xblk[2],
                         wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag,
-                        f64p[0], f64p[1], f64p[2], d_fblk[0], d_fblk[1], d_fblk[2]};
+                        f64p[0], f64p[1], f64p[2], f32p[0], f32p[1], f32p[2], d_fblk[0], d_fblk[1], d_fblk[2]};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);

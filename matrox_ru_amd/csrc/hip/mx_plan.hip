@@ -59,9 +59,20 @@ static void pack_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     }
 }
 
+static void pack_f32_all(mx_plan *P, const float *const src[3], hipStream_t st) {
+    for (int w = 0; w < 3; ++w) {
+        if (!src[w] || !P->f32p[w]) continue;
+        for (int64_t b0 = 0; b0 < P->n_fblk[w]; b0 += 65535) {
+            int64_t nb = std::min<int64_t>(65535, P->n_fblk[w] - b0);
+            pack_f32_blocks<<<dim3(32, (unsigned)nb), 256, 0, st>>>(src[w], P->d_fblk[w] + b0, P->f32p[w]);
+            HIPCHK(hipGetLastError());
+        }
+    }
+}
+
 static void pack_f64_all(mx_plan *P, const double *const src[3], hipStream_t st) {
     for (int w = 0; w < 3; ++w) {
-        if (!src[w]) continue;
+        if (!src[w] || !P->f64p[w]) continue;
         for (int64_t b0 = 0; b0 < P->n_fblk[w]; b0 += 65535) {
             int64_t nb = std::min<int64_t>(65535, P->n_fblk[w] - b0);
             pack_f64_blocks<<<dim3(32, (unsigned)nb), 256, 0, st>>>(src[w], P->d_fblk[w] + b0, P->f64p[w]);
@@ -126,9 +137,10 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
 
     std::vector<int64_t> wrow64(s.wrow, s.wrow + s.nleaf), skel64(s.skelrow, s.skelrow + s.numnodes);
     const int ndir = P->adjoint ? 2 : 1;
-    // FP64 plans: every A operand of the FP64 programs is also packed in
-    // 16 x 16 fragments for f64p_kernel; each distinct (buffer, offset,
-    // orientation, shape) gets its packed offset, shared by all programs
+    // FP64 and FP32 plans: every A operand of the FP64/FP32 programs is also
+    // packed in 16 x 16 fragments for f64p_kernel / f32p_kernel; each
+    // distinct (buffer, offset, orientation, shape) gets its packed offset
+    // (in elements, the same for both precisions), shared by all programs
     std::vector<X3Block> fblk[3];
     std::map<std::tuple<int64_t, int, int, int, int>, int64_t> fseen[3];
     int64_t fnext[3] = {0, 0, 0};
@@ -139,7 +151,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pair_tiles(L);
             Program &pg = P->prog[dir][0][sc];
             upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
-            if (precisions & MX_PLAN_FP64) {
+            if (precisions & (MX_PLAN_FP64 | MX_PLAN_FP32)) {
                 std::vector<Seg> sp(L.segs);
                 for (const Task &tk : L.tasks)
                     for (int si = tk.seg_begin; si < tk.seg_end; ++si) {
@@ -381,11 +393,34 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         for (int w = 0; w < 3; ++w) {
             P->f64p[w] = upload<double>(nullptr, (size_t)std::max<int64_t>(fnext[w], 1));
             P->f64p_elems[w] = fnext[w];
+        }
+    }
+    // the FP32 copies (half the bytes), by the same rule
+    bool pack32 = (precisions & MX_PLAN_FP32) != 0;
+    if (pack32) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const double need = (double)(fnext[0] + fnext[1] + fnext[2]) * sizeof(float);
+        pack32 = need + 16.0 * (1ull << 30) + 0.1 * (double)total_b <= (double)free_b;
+    }
+    if (pack32)
+        for (int w = 0; w < 3; ++w) {
+            P->f32p[w] = upload<float>(nullptr, (size_t)std::max<int64_t>(fnext[w], 1));
+            P->f64p_elems[w] = fnext[w];
+        }
+    if (pack64 || pack32)
+        for (int w = 0; w < 3; ++w) {
             P->d_fblk[w] = upload(fblk[w].data(), fblk[w].size());
             P->n_fblk[w] = (int64_t)fblk[w].size();
         }
+    if (pack64) {
         const double *src[3] = {P->D64, P->B64, P->VT64};
         pack_f64_all(P.get(), src, nullptr);
+    }
+    if (pack32) {
+        const float *src[3] = {P->D32, P->B32, P->VT32};
+        pack_f32_all(P.get(), src, nullptr);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
@@ -510,30 +545,42 @@ void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs) {
 // tile list, or (FP64) f64p_kernel on packed A operands with two or three
 // LDS stages.  A launch with a task longer than F64P_MAXSEG segments runs
 // the f64p variants on the 64 x 64 kernel.
-enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3, SEG_F64P2 = 4, SEG_F64P3 = 5 };
+enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3, SEG_P2 = 4, SEG_P3 = 5 };
 
 static bool seg_variant_ok(mx_precision prec, int v) {
-    return prec == MX_FP64 ? (v == SEG_V64_D1 || v == SEG_V64_D2 || v == SEG_F64P2 || v == SEG_F64P3)
-                           : (v >= SEG_V64_D1 && v <= SEG_V128);
+    return prec == MX_FP64 ? (v == SEG_V64_D1 || v == SEG_V64_D2 || v == SEG_P2 || v == SEG_P3)
+                           : (v >= SEG_V64_D1 && v <= SEG_P3);
 }
 
 template <typename T>
 static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T> &b, int variant, hipStream_t st,
-                           const double *const *packed = nullptr) {
-    if (variant == SEG_F64P2 || variant == SEG_F64P3) {
-        if (sizeof(T) == sizeof(double) && packed && pg.segs_p && li < pg.maxseg.size() &&
-            pg.maxseg[li] <= F64P_MAXSEG) {
+                           const T *const *packed = nullptr) {
+    if (variant == SEG_P2 || variant == SEG_P3) {
+        if (packed && pg.segs_p && li < pg.maxseg.size() && pg.maxseg[li] <= F64P_MAXSEG) {
             const Launch &ln = pg.launches2[li];
-            F64PArgs fa{};
-            fa.tasks = pg.tasks;
-            fa.segs = pg.segs_p;
-            fa.tiles = pg.tiles2 + ln.tile_offset;
-            fa.ntiles = ln.ntiles;
-            fa.nrhs = nrhs;
-            fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
-            for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
-            for (int v = 0; v < 4; ++v) fa.vec[v] = reinterpret_cast<double *>(b.vec[v]);
-            launch_f64p(fa, variant == SEG_F64P3 ? 3 : 2, st);
+            if constexpr (sizeof(T) == sizeof(double)) {
+                F64PArgs fa{};
+                fa.tasks = pg.tasks;
+                fa.segs = pg.segs_p;
+                fa.tiles = pg.tiles2 + ln.tile_offset;
+                fa.ntiles = ln.ntiles;
+                fa.nrhs = nrhs;
+                fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
+                for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
+                for (int v = 0; v < 4; ++v) fa.vec[v] = b.vec[v];
+                launch_f64p(fa, variant == SEG_P3 ? 3 : 2, st);
+            } else {
+                F32PArgs fa{};
+                fa.tasks = pg.tasks;
+                fa.segs = pg.segs_p;
+                fa.tiles = pg.tiles2 + ln.tile_offset;
+                fa.ntiles = ln.ntiles;
+                fa.nrhs = nrhs;
+                fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
+                for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
+                for (int v = 0; v < 4; ++v) fa.vec[v] = b.vec[v];
+                launch_f32p(fa, variant == SEG_P3 ? 4 : 2, st);
+            }
             return;
         }
         variant = SEG_V64_D1;
@@ -678,7 +725,7 @@ void eval_core(mx_plan *P, int dir, mx_precision prec, const void *W, void *Y, i
         b.vec[VEC_Y] = Yf;
         for (size_t i = 0; i < pg0.launches.size(); ++i) {
             mark(MX_KIND_SEG_F32, true);
-            run_seg_launch<float>(pg0, i, nrhs, b, seg_variant(P, dir, 1, i), st);
+            run_seg_launch<float>(pg0, i, nrhs, b, seg_variant(P, dir, 1, i), st, P->f32p[0] ? P->f32p : nullptr);
             mark(MX_KIND_SEG_F32, false);
             phase(i);
         }
@@ -999,6 +1046,10 @@ static void set_blocks(mx_plan *P, const double *D, const double *B, const doubl
     }
     if (P->precisions & MX_PLAN_BF16X3) pack_all(P, src, st);
     if (P->precisions & MX_PLAN_FP64) pack_f64_all(P, src, st);
+    if (P->precisions & MX_PLAN_FP32) {   // from the FP32 copies just converted (src[w] null: unchanged)
+        const float *s32[3] = {src[0] ? d32[0] : nullptr, src[1] ? d32[1] : nullptr, src[2] ? d32[2] : nullptr};
+        pack_f32_all(P, s32, st);
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -1145,7 +1196,8 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
             const int pk = prec == MX_FP64 ? 0 : 1;
             std::vector<int> cand = prec == MX_FP64 ? std::vector<int>{SEG_V64_D1, SEG_V64_D2}
                                                     : std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_V128};
-            if (prec == MX_FP64 && P->f64p[0]) cand.insert(cand.end(), {SEG_F64P2, SEG_F64P3});
+            if ((prec == MX_FP64 && P->f64p[0]) || (prec == MX_FP32 && P->f32p[0]))
+                cand.insert(cand.end(), {SEG_P2, SEG_P3});
             P->profiling = 1;
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][0];
