@@ -154,8 +154,8 @@ def launch_profile(plan, run, prec, nrhs, parts, nprof=3):
     dom_avg_ms = kind_total[dom] / max(1, kind_launches[dom])
     dom_flops_per_launch = dom_flops / max(1, kind_launches[dom])
     achieved = dom_flops_per_launch / (dom_avg_ms * 1e-3) / 1e12
-    peak = {"bf16x3": MI355X_PEAK["bf16"], "seg_gemm_f32": MI355X_PEAK["fp32"],
-            "seg_gemm_f64": MI355X_PEAK["fp64"]}[dom]
+    peak = {"bf16x3": MI355X_PEAK["bf16"], "gemm_f32": MI355X_PEAK["fp32"],
+            "gemm_f64": MI355X_PEAK["fp64"]}[dom]
     roofline = {"bound": "mfma", "kernel": dom, "achieved": round(achieved, 3), "peak": peak,
                 "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": None,
                 "launches_per_eval": kind_launches[dom], "avg_launch_ms": round(dom_avg_ms, 5),

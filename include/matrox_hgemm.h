@@ -362,7 +362,9 @@ mx_status mx_dense_rows(mx_plan *plan, const int32_t *rows, int32_t nrows, const
  * and mx_backward is bracketed
  * by HIP events on the evaluation stream.  mx_plan_launch_times waits for the
  * last evaluation and returns, per launch, its duration in ms and its kernel
- * kind (MX_KIND_*); *count is the number of launches (<= cap filled). */
+ * kind (MX_KIND_*); *count is the number of launches (<= cap filled).
+ * MX_KIND_SEG_F64 / MX_KIND_SEG_F32 are every FP64 / FP32 evaluation launch,
+ * on the 64 x 64 kernel or on the packed-operand kernels alike. */
 enum {
     MX_KIND_SEG_F64 = 0,
     MX_KIND_SEG_F32 = 1,

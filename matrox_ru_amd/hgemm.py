@@ -15,7 +15,9 @@ import numpy as np
 from . import _native
 
 FP64, FP32, BF16X3 = 0, 1, 2
-KIND_NAMES = {0: "seg_gemm_f64", 1: "seg_gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32",
+# launch kinds: the FP64 / FP32 kinds cover both seg_gemm_kernel (64 x 64
+# tiles on the stored blocks) and f64p_kernel / f32p_kernel (packed operands)
+KIND_NAMES = {0: "gemm_f64", 1: "gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32",
               6: "grad_bf16x3"}
 PLAN_ADJOINT = 1 << 8
 SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4}
