@@ -326,7 +326,8 @@ mx_status mx_plan_set_launch_variants(mx_plan *plan, int32_t direction, const in
  * 1 = 64 x 64 tiles with one chunk of loads in flight, 2 = 64 x 64 with two,
  * 3 = 128 x 128 tiles (FP32 only), 4 / 5 = 128 x 128 tiles on A operands
  * packed when the plan is built (FP64: two / three LDS stages; FP32: two /
- * four); a launch with a task of more than 256 segments runs variant 1
+ * four), 6 / 7 = the same with 128 x 256 tiles (each stored block read
+ * half as often); a launch with a task of more than 256 segments runs variant 1
  * instead, and so does every launch of a plan whose packed copies did not
  * fit in device memory beside its blocks, with 16 GiB + 10% of the device
  * to spare, when it was created.  The autotuner picks them launch by

@@ -1,5 +1,5 @@
-// FP64 (and FP32) block products on packed A operands: 128 x 128 tiles, a
-// pipeline of direct global -> LDS loads, v_mfma_f64_16x16x4_f64 (the FP32
+// FP64 (and FP32) block products on packed A operands: 128 x 128 or
+// 128 x 256 tiles, a pipeline of direct global -> LDS loads, v_mfma_f64_16x16x4_f64 (the FP32
 // form, v_mfma_f32_16x16x4_f32, is described at F32PArgs).
 //
 // The same launches as the 64 x 64 FP64 kernel (mx_seg_gemm.h), on the
@@ -14,10 +14,10 @@
 //   so that lane l = r + 16 g of an MFMA reads its operands for k = 4g..4g+3
 //   as two 16-byte pieces, 1 KB apart.
 // * B operands (W, wskel, uskel) stay in the reference layout (k fastest in
-//   each column, leading dimension = rows of the block): the 16 x 128 chunk
-//   is moved by 4-byte LDS-DMA loads (any alignment), one 128-byte row of
-//   16 k per tile column, its 16-byte slots XOR-swizzled by the column so
-//   that the MFMA reads hit distinct banks.  Addresses are clamped to the
+//   each column, leading dimension = rows of the block): the 16 x 128 (or
+//   16 x 256) chunk is moved by 4-byte LDS-DMA loads (any alignment), one
+//   128-byte row of 16 k per tile column, its 16-byte slots XOR-swizzled by
+//   the column so that the MFMA reads hit distinct banks.  Addresses are clamped to the
 //   block (k <= K-1, column <= nrhs-1); the A fragments' zero padding makes
 //   the clamped k terms vanish, and clamped columns only reach outputs that
 //   are not stored.
@@ -53,8 +53,12 @@ struct F64PArgs {
 __global__ void pack_f64_blocks(const double *__restrict__ src, const X3Block *__restrict__ blocks,
                                 double *__restrict__ dst);
 
-// One launch: nst = 2 (two stages, two workgroups per CU) or 3 (one).
-void launch_f64p(const F64PArgs &a, int nst, hipStream_t st);
+// Columns per tile: F64P_BN, or F64P_BNW for the wide tile.
+constexpr int F64P_BNW = 256;
+
+// One launch: nst = 2 (two stages; two workgroups per CU for 128 columns)
+// or 3; wide: 128 x 256 tiles (8 waves of 64 x 64, one workgroup per CU).
+void launch_f64p(const F64PArgs &a, int nst, int wide, hipStream_t st);
 
 // FP32 on packed A operands (v_mfma_f32_16x16x4_f32): the same tiles, tile
 // lists and packed offsets (a 16 x 16 fragment holds 256 elements in either
@@ -76,7 +80,7 @@ struct F32PArgs {
 __global__ void pack_f32_blocks(const float *__restrict__ src, const X3Block *__restrict__ blocks,
                                 float *__restrict__ dst);
 
-// One launch: nst = 2 or 4 stages (16 KB each).
-void launch_f32p(const F32PArgs &a, int nst, hipStream_t st);
+// One launch: nst = 2 or 4 stages (16 KB each; 24 KB for the wide tile).
+void launch_f32p(const F32PArgs &a, int nst, int wide, hipStream_t st);
 
 }  // namespace mx

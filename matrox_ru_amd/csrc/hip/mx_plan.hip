@@ -542,22 +542,26 @@ void note_eval(mx_plan *P, int dir, mx_precision prec, int nrhs) {
 
 // FP64/FP32 kernel variants of a launch: 64 x 64 tiles with one or two
 // chunks of loads in flight, (FP32) 128 x 128 tiles on the plan's 128-row
-// tile list, or (FP64) f64p_kernel on packed A operands with two or three
-// LDS stages.  A launch with a task longer than F64P_MAXSEG segments runs
-// the f64p variants on the 64 x 64 kernel.
-enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3, SEG_P2 = 4, SEG_P3 = 5 };
+// tile list, or f64p_kernel / f32p_kernel on packed A operands: 128 x 128
+// tiles with two or three (FP32: four) LDS stages, and 128 x 256 tiles
+// likewise.  A launch with a task longer than F64P_MAXSEG segments, or a
+// plan without packed copies, runs the packed variants on the 64 x 64
+// kernel.
+enum { SEG_V64_D1 = 1, SEG_V64_D2 = 2, SEG_V128 = 3, SEG_P2 = 4, SEG_P3 = 5, SEG_PW2 = 6, SEG_PW3 = 7 };
 
 static bool seg_variant_ok(mx_precision prec, int v) {
-    return prec == MX_FP64 ? (v == SEG_V64_D1 || v == SEG_V64_D2 || v == SEG_P2 || v == SEG_P3)
-                           : (v >= SEG_V64_D1 && v <= SEG_P3);
+    return prec == MX_FP64 ? (v == SEG_V64_D1 || v == SEG_V64_D2 || (v >= SEG_P2 && v <= SEG_PW3))
+                           : (v >= SEG_V64_D1 && v <= SEG_PW3);
 }
 
 template <typename T>
 static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T> &b, int variant, hipStream_t st,
                            const T *const *packed = nullptr) {
-    if (variant == SEG_P2 || variant == SEG_P3) {
+    if (variant >= SEG_P2 && variant <= SEG_PW3) {
         if (packed && pg.segs_p && li < pg.maxseg.size() && pg.maxseg[li] <= F64P_MAXSEG) {
             const Launch &ln = pg.launches2[li];
+            const int wide = variant >= SEG_PW2, deep = variant == SEG_P3 || variant == SEG_PW3;
+            const int bn = wide ? F64P_BNW : F64P_BN;
             if constexpr (sizeof(T) == sizeof(double)) {
                 F64PArgs fa{};
                 fa.tasks = pg.tasks;
@@ -565,10 +569,10 @@ static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T>
                 fa.tiles = pg.tiles2 + ln.tile_offset;
                 fa.ntiles = ln.ntiles;
                 fa.nrhs = nrhs;
-                fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
+                fa.n_ntiles = (nrhs + bn - 1) / bn;
                 for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
                 for (int v = 0; v < 4; ++v) fa.vec[v] = b.vec[v];
-                launch_f64p(fa, variant == SEG_P3 ? 3 : 2, st);
+                launch_f64p(fa, deep ? 3 : 2, wide, st);
             } else {
                 F32PArgs fa{};
                 fa.tasks = pg.tasks;
@@ -576,10 +580,10 @@ static void run_seg_launch(const Program &pg, size_t li, int nrhs, const Bufs<T>
                 fa.tiles = pg.tiles2 + ln.tile_offset;
                 fa.ntiles = ln.ntiles;
                 fa.nrhs = nrhs;
-                fa.n_ntiles = (nrhs + F64P_BN - 1) / F64P_BN;
+                fa.n_ntiles = (nrhs + bn - 1) / bn;
                 for (int w = 0; w < 3; ++w) fa.a[w] = packed[w];
                 for (int v = 0; v < 4; ++v) fa.vec[v] = b.vec[v];
-                launch_f32p(fa, variant == SEG_P3 ? 4 : 2, st);
+                launch_f32p(fa, deep ? 4 : 2, wide, st);
             }
             return;
         }
@@ -1197,7 +1201,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
             std::vector<int> cand = prec == MX_FP64 ? std::vector<int>{SEG_V64_D1, SEG_V64_D2}
                                                     : std::vector<int>{SEG_V64_D1, SEG_V64_D2, SEG_V128};
             if ((prec == MX_FP64 && P->f64p[0]) || (prec == MX_FP32 && P->f32p[0]))
-                cand.insert(cand.end(), {SEG_P2, SEG_P3});
+                cand.insert(cand.end(), {SEG_P2, SEG_P3, SEG_PW2, SEG_PW3});
             P->profiling = 1;
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][0];

@@ -118,10 +118,10 @@ def test_pinned_launch_variants(gpu):
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_fp64_fp32_kernel_variants(gpu, case):
     """Every FP64/FP32 kernel variant (64x64 tiles with one or two chunks of
-    loads in flight, 128x128 tiles for FP32, 128x128 tiles on packed A
-    operands with two or three LDS stages for FP64, two or four for FP32) on
-    every launch, both directions, against the oracle; ragged column counts
-    (150: a partial 128-column tile)."""
+    loads in flight, 128x128 tiles for FP32, 128x128 and 128x256 tiles on
+    packed A operands with two or three LDS stages for FP64, two or four for
+    FP32) on every launch, both directions, against the oracle; ragged
+    column counts (150: a partial 128- and 256-column tile)."""
     import torch
     _say(f"[seg variants] {case}")
     X, hm = make_problem(**CASES[case])
@@ -131,7 +131,7 @@ def test_fp64_fp32_kernel_variants(gpu, case):
     Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
-    for prec, variants in (("fp64", (1, 2, 4, 5)), ("fp32", (1, 2, 3, 4, 5))):
+    for prec, variants in (("fp64", (1, 2, 4, 5, 6, 7)), ("fp32", (1, 2, 3, 4, 5, 6, 7))):
         for v in variants:
             for d in ("forward", "adjoint"):
                 n = len(plan.seg_variants(prec, d))
@@ -149,7 +149,7 @@ def test_fp64_fp32_kernel_variants(gpu, case):
     with pytest.raises(hgemm.MxError):
         plan.set_seg_variants([3] * len(plan.seg_variants("fp64")), "fp64")
     with pytest.raises(hgemm.MxError):
-        plan.set_seg_variants([6] * len(plan.seg_variants("fp32")), "fp32")
+        plan.set_seg_variants([8] * len(plan.seg_variants("fp32")), "fp32")
     with pytest.raises(hgemm.MxError):
         plan.set_seg_variants([1], "fp32")
 
@@ -160,7 +160,8 @@ def test_packed_operands_follow_new_blocks(gpu, prec):
     after new blocks, the packed-operand kernel (every launch) agrees with
     the 64x64 kernel reading the stored blocks, forward and adjoint, at nrhs
     1 and 300 (FP64 to 1e-13; FP32 to 2e-6: the two kernels sum the same
-    products in another order)."""
+    products in another order).  The packed variants (128 or 256 columns,
+    any stage count) sum every output in the same order: bit-identical."""
     import torch
     _say(f"[packed operands after set_blocks] {prec}")
     X, hm = make_problem(**CASES["h2b_tall"])
@@ -172,7 +173,7 @@ def test_packed_operands_follow_new_blocks(gpu, prec):
     for nrhs in (1, 300):
         W = torch.from_numpy(host.normal((hm.N * nrhs,), seed=31 + nrhs)).cuda().to(dt)
         out = {}
-        for v in (1, 4, 5):
+        for v in (1, 4, 5, 6, 7):
             for d in ("forward", "adjoint"):
                 plan.set_seg_variants([v] * len(plan.seg_variants(prec, d)), prec, d)
             Y = torch.full_like(W, float("nan"))
@@ -181,10 +182,11 @@ def test_packed_operands_follow_new_blocks(gpu, prec):
             plan.eval_adjoint(W, G, nrhs, prec)
             torch.cuda.synchronize()
             out[v] = (Y.double().cpu().numpy(), G.double().cpu().numpy())
-        for v in (4, 5):
+        for v in (4, 5, 6, 7):
             ey, eg = rel(out[v][0], out[1][0]), rel(out[v][1], out[1][1])
             _say(f"  nrhs={nrhs} variant {v} vs 1: forward {ey:.2e} adjoint {eg:.2e}")
             assert ey <= tol and eg <= tol
+            assert np.array_equal(out[v][0], out[4][0]) and np.array_equal(out[v][1], out[4][1]), v
 
 
 def test_balanced_schedule_moves_near_work(gpu):
