@@ -296,8 +296,11 @@ void split_long_tasks(TaskLists &L, int maxseg);
 // tiles2 / launches2: every launch's tiles merged in row pairs, in the same
 // order and XCD groups
 void pair_tiles(TaskLists &L);
+// kind 0: FP64/FP32 lists (64-row tiles), kind 1: BF16x3 (128-row tiles).
+// The tiles of big launches are grouped by shared B operands for kind 1, or
+// with group set; otherwise kept in tree order (order_tiles).
 TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir, int sched,
-                      const int64_t *wrow_in, const int64_t *skel_in);
+                      const int64_t *wrow_in, const int64_t *skel_in, bool group = false);
 GradLists build_grad_lists(const mx_structure &s, const Topology &t);
 
 // The dependency-driven sweep of one direction (protocol in mx_bf16x3.h),

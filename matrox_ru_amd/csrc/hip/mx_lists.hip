@@ -426,7 +426,7 @@ void split_long_tasks(TaskLists &L, int maxseg) {
 // the buffers the kernel reads (reference layout for kind 0, split panels for
 // kind 1); Y always uses the reference layout.
 TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int beta_y, int kind, int dir,
-                      int sched, const int64_t *wrow_in, const int64_t *skel_in) {
+                      int sched, const int64_t *wrow_in, const int64_t *skel_in, bool group) {
     TaskLists L;
     const int nn = s.numnodes;
     // far pair k couples row node fx(k) with column node fy(k) through B_k
@@ -662,7 +662,7 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
-    order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1);
+    order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1 || group);
     return L;
 }
 

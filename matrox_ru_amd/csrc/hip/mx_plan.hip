@@ -148,11 +148,23 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         for (int sc = 0; sc < NSCHED; ++sc) {
             TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
             compute_launch_flops(L);
-            pair_tiles(L);
+            // The 128-row tiles (packed-operand kernels, FP32 128 x 128
+            // kernel) of the big launches go in groups of tasks that share B
+            // operands, with each group's segments aligned (as the BF16x3
+            // lists): at the headline the FP64 evaluation fetches 15.7
+            // instead of 19.0 GB and runs 1.5% faster.  The 64-row tiles
+            // keep the tree order and the segments their natural order: the
+            // 64 x 64 kernel ran 2.8% slower on grouped lists (N = 524,288).
+            // Both lists hold the same tasks, each with the same segments.
+            TaskLists Lg = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data(), true);
+            if (Lg.tasks.size() != L.tasks.size() || Lg.segs.size() != L.segs.size() ||
+                Lg.launches.size() != L.launches.size())
+                throw MxError(MX_ERR_STATE, "grouped task lists differ from the tree-order lists");
+            pair_tiles(Lg);
             Program &pg = P->prog[dir][0][sc];
             upload_lists(L, &pg.tasks, &pg.segs, &pg.tiles);
             if (precisions & (MX_PLAN_FP64 | MX_PLAN_FP32)) {
-                std::vector<Seg> sp(L.segs);
+                std::vector<Seg> sp(Lg.segs);
                 for (const Task &tk : L.tasks)
                     for (int si = tk.seg_begin; si < tk.seg_end; ++si) {
                         Seg &sg = sp[si];
@@ -182,8 +194,8 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                     }
             }
             pg.launches = L.launches;
-            pg.tiles2 = upload(L.tiles2.data(), L.tiles2.size());
-            pg.launches2 = L.launches2;
+            pg.tiles2 = upload(Lg.tiles2.data(), Lg.tiles2.size());
+            pg.launches2 = Lg.launches2;
             pg.flops = L.flops;
             std::copy(L.last_write, L.last_write + 4, pg.last_write);
             pg.built = true;
