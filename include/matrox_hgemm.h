@@ -270,6 +270,18 @@ enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1, MX_SCHED_FUSED_BALANCED = 2, MX_S
  * faster; any other schedule turns it off.  No counterpart in the reference,
  * whose sweeps synchronise per level group (symGen/HGEMM_gen.h:142-236). */
 enum { MX_SCHED_SWEEP = 4 };
+/* MX_SCHED_COARSE (MX_BF16X3 only; plans of stored blocks): the fused
+ * schedule with consecutive launches merged in pairs where the second reads
+ * the first's skeletons through few, small products.  A segment A_c x_P over
+ * a skeleton block x_P = sum_s A_s B_s written one launch earlier becomes the
+ * segments (A_c A_s) B_s, over composite blocks A_c A_s formed in FP64 when
+ * the plan is built (and again by mx_plan_set_blocks); a pair is merged when
+ * that adds at most 2.5 MFLOP per right-hand side column.  Fewer launches
+ * for the latency-bound top of the tree, every skeleton block still written.
+ * The plan keeps FP64 copies of B and VT for it.  Like the reference's
+ * coarsened level groups (symGen/HGEMM_gen.h:142-178, 196-236), with the
+ * group boundary removed by re-association instead of a barrier. */
+enum { MX_SCHED_COARSE = 5 };
 
 /* Error word of the last sweep this plan enqueued (after synchronising
  * `stream`): 0 = none or clean; 1 = a wait for a producer, 2 = claiming
@@ -277,6 +289,22 @@ enum { MX_SCHED_SWEEP = 4 };
  * waiting (its outputs are then incomplete).  No counterpart in the
  * reference. */
 mx_status mx_plan_sweep_status(const mx_plan *plan, void *stream, int32_t *error);
+
+/* Host-only: the BF16x3 evaluation's task lists of one direction and
+ * schedule (MX_SCHED_FUSED ... MX_SCHED_SPLIT_BALANCED or MX_SCHED_COARSE)
+ * of a structure, in terms of the stored blocks and the reference layouts
+ * (no device work, for checks of the schedules).  With launches == NULL it
+ * returns the counts.  launches: nlaunch + 1 offsets into ltasks, the tasks
+ * of each launch in order; tasks: 6 per task (c_row, m, c_buf, seg_begin,
+ * seg_end, beta); segs: 8 per segment (a_off, b_row, a_buf, a_trans, lda,
+ * k, b_buf, b_ld), a_buf 3 naming composite a_off; comps: 11 per composite
+ * (a1_off, a2_off, a1_buf, a1_trans, a1_lda, a2_buf, a2_trans, a2_lda, m,
+ * kmid, n).  Buffers: a_buf 0/1/2 = D/B/VT, c_buf/b_buf 0/1/2/3 = W, w^,
+ * u^, Y; a block of rows r at row offset c_row (b_row) is r x nrhs
+ * column-major at element c_row * nrhs.  No counterpart in the reference. */
+mx_status mx_lists_describe(const mx_structure *s, int32_t direction, int32_t schedule, int32_t *nlaunch,
+                            int32_t *nltask, int32_t *ntask, int32_t *nseg, int32_t *ncomp, int64_t *launches,
+                            int64_t *ltasks, int64_t *tasks, int64_t *segs, int64_t *comps);
 
 /* Host-only: the sweep's work lists of one direction of a structure (no
  * device work, for checks of the scheduling protocol).  With tiles == NULL
@@ -384,7 +412,9 @@ mx_status mx_plan_launch_times(mx_plan *plan, float *ms, int32_t *kind, int32_t 
 mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, double *total);
 
 /* Algorithmic flops of each launch of one evaluation (in launch order, the
- * order mx_plan_launch_times reports kernel launches after the conversion). */
+ * order mx_plan_launch_times reports kernel launches after the conversion).
+ * Under MX_SCHED_COARSE: the flops each launch performs, composite products
+ * included (mx_plan_flops still gives the algorithmic total). */
 mx_status mx_plan_launch_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, double *flops, int32_t cap,
                                int32_t *count);
 

@@ -77,6 +77,24 @@ struct Program {
     }
 };
 
+// A composite A operand of the coarse schedule: op(A1) (m x kmid) times
+// op(A2) (kmid x n), both stored blocks (MAT_B or MAT_VT; op = transposed
+// when *_trans), formed in FP64 when the plan is built.  In host lists a
+// segment over it has a_buf = MAT_COMP and a_off = its index in comps.
+constexpr int32_t MAT_COMP = 3;
+struct Composite {
+    int64_t a1_off, a2_off;
+    int32_t a1_buf, a1_trans, a1_lda, a2_buf, a2_trans, a2_lda;
+    int32_t m, kmid, n, pad;
+};
+
+// A composite as formed on the device: its FP64 product goes to c_off (m x
+// n, column-major) of a scratch buffer, packed from there.
+struct CompDesc {
+    Composite c;
+    int64_t c_off;
+};
+
 enum { DIR_FWD = 0, DIR_ADJ = 1 };
 
 // Schedules of the coupling and downward phases:
@@ -86,8 +104,11 @@ enum { DIR_FWD = 0, DIR_ADJ = 1 };
 //      from the parent (beta = 1).  Long coupling lists at the top of the
 //      tree (the adjoint's column-grouped lists) then run beside everything
 //      else instead of in a nearly empty launch.
-// coupling/downward schedule (bit 0: split) and NEAR+LEAF balancing (bit 1)
-enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, SCHED_FUSED_BAL = 2, SCHED_SPLIT_BAL = 3, NSCHED = 4 };
+// coupling/downward schedule (bit 0: split) and NEAR+LEAF balancing (bit 1);
+//  SCHED_COARSE (BF16x3 only): the fused lists with pairs of consecutive
+//      launches merged where the second reads the first's skeletons through
+//      few, small products (coarsen in mx_lists.hip).
+enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, SCHED_FUSED_BAL = 2, SCHED_SPLIT_BAL = 3, SCHED_COARSE = 4, NSCHED = 5 };
 
 }  // namespace mx
 
@@ -195,6 +216,12 @@ struct mx_plan {
     // packing lists of the split-BF16 operands, kept to repack new blocks
     mx::X3Block *d_xblk[3] = {nullptr, nullptr, nullptr};
     int64_t n_xblk[3] = {0, 0, 0};
+    // coarse schedule (BF16x3): its composite A operands (formed in FP64
+    // from B64 / VT64, which the plan then keeps) and their packing list
+    // (into the packed VT operands)
+    mx::CompDesc *comp_desc = nullptr;
+    mx::X3Block *comp_xblk = nullptr;
+    int64_t n_comp = 0, comp_elems = 0, comp_tiles = 1;
 
     // work buffers: skeletons of the forward (w^, u^) and of the adjoint
     // (y^ = dL/du^, g^ = dL/dw^) evaluations
@@ -246,7 +273,8 @@ struct mx_plan {
                         VT32, xhi[0], xhi[1], xhi[2], xlo[0], xlo[1], xlo[2], wskel, uskel, io_w, io_y, X_leaf,
                         d_wrows, d_srows, panels, g_tasks, g_segs, g_tiles, g_tiles_x3, g_tiles_x3w, d_xblk[0], d_xblk[1], d_xblk[2],
                         wskel_adj, uskel_adj, io_g, g_ksegs, kb_leaf, kb_skel, kb_panel, kfrag,
-                        f64p[0], f64p[1], f64p[2], f32p[0], f32p[1], f32p[2], d_fblk[0], d_fblk[1], d_fblk[2]};
+                        f64p[0], f64p[1], f64p[2], f32p[0], f32p[1], f32p[2], d_fblk[0], d_fblk[1], d_fblk[2],
+                        comp_desc, comp_xblk};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -257,6 +285,7 @@ struct mx_plan {
 namespace mx {
 
 struct TaskLists {
+    std::vector<Composite> comps;   // coarse schedule: the composite A operands
     std::vector<Task> tasks;
     std::vector<Seg> segs;
     std::vector<int2> tiles;

@@ -364,6 +364,149 @@ static void order_tiles(TaskLists &L, int bk, bool group_by_operand) {
     L.launches.swap(lns);
 }
 
+// The coarse schedule: consecutive launches (j, j+1) of the fused lists
+// merge into one launch when launch j+1's segments that read skeleton
+// blocks written in launch j can be rewritten over launch j's inputs.  A
+// consumer segment A_c x_P, where x_P = sum_s A_s B_s is the output of
+// producer task P (beta = 0, its only writer), becomes the segments
+// (A_c A_s) B_s, one per segment of P, with the composite A_c A_s (formed in
+// FP64 when the plan is built).  Every skeleton block is still written (the
+// block gradients read them all), and no composite is made of composites
+// (pairs are disjoint).  A pair is taken when it adds at most max_extra
+// flops per right-hand side column (2 m (sum_s k_s - k_c) per inlined
+// segment): the UP launches of the lowest two levels merge for nothing when
+// the leaf blocks are as tall as the ranks, the top-of-tree UP and
+// coupling/downward launches for little.  The HSS evaluation at N = 16,384
+// (15 launches in the fused lists) runs in 8.
+static void coarsen(TaskLists &L, double max_extra) {
+    const int n = (int)L.launches.size();
+    std::vector<int> launch_of(L.tasks.size(), -1);
+    std::vector<std::vector<int>> tasks_of(n);
+    for (int li = 0; li < n; ++li)
+        for (int t = L.launches[li].tile_offset; t < L.launches[li].tile_offset + L.launches[li].ntiles; ++t) {
+            const int task = L.tiles[t].x;
+            if (launch_of[task] < 0) {
+                launch_of[task] = li;
+                tasks_of[li].push_back(task);
+            }
+        }
+    // the producer of every skeleton block (-1: several writers)
+    std::map<std::pair<int, int64_t>, int> prod;
+    for (size_t t = 0; t < L.tasks.size(); ++t) {
+        const Task &tk = L.tasks[t];
+        if (launch_of[t] < 0 || (tk.c_buf != VEC_WSKEL && tk.c_buf != VEC_USKEL)) continue;
+        auto key = std::make_pair((int)tk.c_buf, tk.c_row);
+        auto it = prod.find(key);
+        if (it == prod.end()) prod[key] = (int)t;
+        else it->second = -1;
+    }
+    // the producer in launch j of segment s (-1: none there)
+    auto producer_in = [&](const Seg &sg, int j) {
+        if (sg.b_buf != VEC_WSKEL && sg.b_buf != VEC_USKEL) return -1;
+        auto it = prod.find(std::make_pair((int)sg.b_buf, sg.b_row));
+        if (it == prod.end() || it->second < 0) return -1;
+        return launch_of[it->second] == j ? it->second : -1;
+    };
+    // pairs, top of the launch order down
+    std::vector<char> consumer(n, 0);
+    for (int j = 0; j + 1 < n;) {
+        double extra = 0.0;
+        bool ok = true;
+        for (int t : tasks_of[j + 1]) {
+            const Task &tk = L.tasks[t];
+            for (int si = tk.seg_begin; si < tk.seg_end && ok; ++si) {
+                const Seg &sg = L.segs[si];
+                if (sg.b_buf != VEC_WSKEL && sg.b_buf != VEC_USKEL) continue;
+                auto it = prod.find(std::make_pair((int)sg.b_buf, sg.b_row));
+                if (it != prod.end() && it->second < 0) {
+                    ok = false;   // a block with several writers
+                    break;
+                }
+                const int P = producer_in(sg, j);
+                if (P < 0) continue;
+                const Task &pt = L.tasks[P];
+                if (pt.beta != 0 || pt.m != sg.k || sg.a_buf == MAT_COMP) {
+                    ok = false;
+                    break;
+                }
+                double kp = 0.0;
+                for (int ps = pt.seg_begin; ps < pt.seg_end; ++ps) {
+                    if (L.segs[ps].a_buf == MAT_COMP) ok = false;
+                    kp += L.segs[ps].k;
+                }
+                extra += 2.0 * tk.m * (kp - sg.k);
+            }
+            if (!ok) break;
+        }
+        if (ok && extra <= max_extra) {
+            consumer[j + 1] = 1;
+            j += 2;
+        } else {
+            j += 1;
+        }
+    }
+    // segments: each consumer segment replaced by its producer's, over
+    // composites (the producers' segment ranges as they were)
+    const std::vector<Task> old = L.tasks;
+    std::vector<Seg> segs;
+    segs.reserve(L.segs.size());
+    for (size_t t = 0; t < L.tasks.size(); ++t) {
+        Task &tk = L.tasks[t];
+        const int b = (int)segs.size();
+        const int li = launch_of[t];
+        for (int si = old[t].seg_begin; si < old[t].seg_end; ++si) {
+            const Seg sg = L.segs[si];
+            const int P = li > 0 && consumer[li] ? producer_in(sg, li - 1) : -1;
+            if (P < 0) {
+                segs.push_back(sg);
+                continue;
+            }
+            const Task &pt = old[P];
+            for (int ps = pt.seg_begin; ps < pt.seg_end; ++ps) {
+                const Seg &inner = L.segs[ps];
+                Composite c{};
+                c.a1_off = sg.a_off;
+                c.a1_buf = sg.a_buf;
+                c.a1_trans = sg.a_trans;
+                c.a1_lda = sg.lda;
+                c.a2_off = inner.a_off;
+                c.a2_buf = inner.a_buf;
+                c.a2_trans = inner.a_trans;
+                c.a2_lda = inner.lda;
+                c.m = tk.m;
+                c.kmid = sg.k;
+                c.n = inner.k;
+                Seg ns = inner;   // the producer's B operand
+                ns.a_buf = MAT_COMP;
+                ns.a_off = (int64_t)L.comps.size();
+                ns.a_trans = 0;
+                ns.lda = tk.m;
+                segs.push_back(ns);
+                L.comps.push_back(c);
+            }
+        }
+        tk.seg_begin = b;
+        tk.seg_end = (int)segs.size();
+    }
+    L.segs.swap(segs);
+    // launches: a consumer's tiles join its producer launch
+    std::vector<int2> tiles;
+    std::vector<Launch> lns;
+    std::vector<int> kinds;
+    for (int li = 0; li < n; ++li) {
+        if (!consumer[li]) {
+            lns.push_back(Launch{(int)tiles.size(), 0});
+            kinds.push_back(L.kinds[li]);
+        }
+        const Launch &ln = L.launches[li];
+        tiles.insert(tiles.end(), L.tiles.begin() + ln.tile_offset, L.tiles.begin() + ln.tile_offset + ln.ntiles);
+        lns.back().ntiles = (int)tiles.size() - lns.back().tile_offset;
+    }
+    L.tiles.swap(tiles);
+    L.launches.swap(lns);
+    L.kinds.swap(kinds);
+}
+
 // A task with more than maxseg segments keeps its first maxseg; the rest go
 // to continuation tasks (beta = 1, same output block and row tiles), the
 // k-th continuation in a launch of its own inserted right after the task's
@@ -662,6 +805,8 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
+    // at most 2.5 MFLOP per right-hand side column for each merge
+    if (sched == SCHED_COARSE) coarsen(L, 2.5e6);
     order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1 || group);
     return L;
 }

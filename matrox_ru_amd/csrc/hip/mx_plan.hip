@@ -81,6 +81,76 @@ static void pack_f64_all(mx_plan *P, const double *const src[3], hipStream_t st)
     }
 }
 
+// The coarse schedule's composite operands op(A1) op(A2) in FP64: one 32 x
+// 32 tile of composite blockIdx.y per workgroup (blockIdx.x: tile, column
+// of tiles major), written column-major (ld = m) at its c_off of out.
+__global__ void comp_gemm(const double *__restrict__ B, const double *__restrict__ VT,
+                          const CompDesc *__restrict__ cd, double *__restrict__ out) {
+    const CompDesc d = cd[blockIdx.y];
+    const Composite &c = d.c;
+    const int tm = (c.m + 31) / 32, tn = (c.n + 31) / 32;
+    if ((int)blockIdx.x >= tm * tn) return;
+    const int i0 = ((int)blockIdx.x % tm) * 32, j0 = ((int)blockIdx.x / tm) * 32;
+    const double *A1 = (c.a1_buf == MAT_B ? B : VT) + c.a1_off;
+    const double *A2 = (c.a2_buf == MAT_B ? B : VT) + c.a2_off;
+    __shared__ double sa[32][33], sb[32][33];   // sa[l][i] = op(A1)(i0+i, l0+l), sb[l][j] = op(A2)(l0+l, j0+j)
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int l0 = 0; l0 < c.kmid; l0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int r = ty + 8 * q;
+            const int i = i0 + tx, l = l0 + r;
+            double a = 0.0, b = 0.0;
+            if (i < c.m && l < c.kmid)
+                a = c.a1_trans ? A1[(int64_t)i * c.a1_lda + l] : A1[(int64_t)l * c.a1_lda + i];
+            const int j = j0 + tx;
+            if (l < c.kmid && j < c.n)
+                b = c.a2_trans ? A2[(int64_t)l * c.a2_lda + j] : A2[(int64_t)j * c.a2_lda + l];
+            sa[r][tx] = a;
+            sb[r][tx] = b;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int l = 0; l < 32; ++l) {
+            const double a = sa[l][tx];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = fma(a, sb[l][ty + 8 * q], acc[q]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int i = i0 + tx, j = j0 + ty + 8 * q;
+        if (i < c.m && j < c.n) out[d.c_off + (int64_t)j * c.m + i] = acc[q];
+    }
+}
+
+// Form the coarse schedule's composites from FP64 B and VT (device arrays)
+// and pack them into the split VT operands.
+static void form_composites(mx_plan *P, const double *B, const double *VT, hipStream_t st) {
+    if (P->n_comp == 0) return;
+    // (a structure without far blocks has no composite over B)
+    if (!VT) throw MxError(MX_ERR_STATE, "composite operands need the FP64 VT blocks");
+    double *tmp = nullptr;
+    HIPCHK(hipMalloc(&tmp, (size_t)std::max<int64_t>(P->comp_elems, 1) * sizeof(double)));
+    try {
+        for (int64_t c0 = 0; c0 < P->n_comp; c0 += 65535) {
+            const int64_t nc = std::min<int64_t>(65535, P->n_comp - c0);
+            comp_gemm<<<dim3((unsigned)P->comp_tiles, (unsigned)nc), 256, 0, st>>>(B, VT, P->comp_desc + c0, tmp);
+            HIPCHK(hipGetLastError());
+            pack_x3_blocks<<<dim3(64, (unsigned)nc), 256, 0, st>>>(tmp, P->comp_xblk + c0, P->xhi[MAT_VT],
+                                                                    P->xlo[MAT_VT]);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipStreamSynchronize(st));
+    } catch (...) {
+        (void)hipFree(tmp);
+        throw;
+    }
+    HIPCHK(hipFree(tmp));
+}
+
 // First padded panel row of every W leaf block and skeleton block (BF16x3
 // split panels), and the rows of each panel buffer.
 static void panel_rows(const mx_structure &s, std::vector<int64_t> &wprow, std::vector<int64_t> &sprow,
@@ -146,6 +216,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     int64_t fnext[3] = {0, 0, 0};
     for (int dir = 0; dir < ndir; ++dir)
         for (int sc = 0; sc < NSCHED; ++sc) {
+            if (sc == SCHED_COARSE) continue;   // BF16x3 only
             TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
             compute_launch_flops(L);
             // The 128-row tiles (packed-operand kernels, FP32 128 x 128
@@ -201,6 +272,9 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             pg.built = true;
         }
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
+    std::vector<CompDesc> comps;     // the coarse schedule's composites and their packing list
+    std::vector<X3Block> comp_xblk;
+    int64_t comp_elems = 0;
     if (precisions & MX_PLAN_BF16X3) {
         // split panels: padded row offsets of W leaf blocks and skeleton blocks
         panel_rows(s, P->wprow, P->sprow, &P->wp_rows, &P->sp_rows);
@@ -217,8 +291,14 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         // shared by the forward and adjoint lists
         std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
         int64_t next[3] = {0, 0, 0};
+        // composites of the coarse schedule: each distinct one formed once,
+        // packed into the VT operands
+        std::map<std::tuple<int, int64_t, int, int, int, int64_t, int, int, int, int, int>, int64_t> cseen;
         for (int dsc = 0; dsc < ndir * NSCHED; ++dsc) {
             const int dir = dsc / NSCHED, sc = dsc % NSCHED;
+            // composites need FP64 B and VT, which a plan of generated blocks
+            // may never hold
+            if (sc == SCHED_COARSE && gen) continue;
             TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, sc, P->wprow.data(), P->sprow.data());
             split_long_tasks(X, X3_MAXSEG);
             for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
@@ -226,6 +306,35 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
                     throw MxError(MX_ERR_ARG, "internal: a BF16x3 task longer than its segment window");
                 for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
                     Seg &sg = X.segs[si];
+                    if (sg.a_buf == MAT_COMP) {
+                        const Composite &c = X.comps[sg.a_off];
+                        auto key = std::make_tuple(c.a1_buf, c.a1_off, c.a1_trans, c.a1_lda, c.a2_buf, c.a2_off,
+                                                   c.a2_trans, c.a2_lda, c.m, c.kmid, c.n);
+                        auto it = cseen.find(key);
+                        int64_t dst;
+                        if (it == cseen.end()) {
+                            dst = next[MAT_VT];
+                            cseen[key] = dst;
+                            comps.push_back(CompDesc{c, comp_elems});
+                            X3Block xb{};
+                            xb.src_off = comp_elems;
+                            xb.dst_off = dst;
+                            xb.M = c.m;
+                            xb.K = c.n;
+                            xb.lda = c.m;
+                            xb.trans = 0;
+                            comp_xblk.push_back(xb);
+                            comp_elems += (int64_t)c.m * c.n;
+                            next[MAT_VT] += x3_packed_elems(c.m, c.n);
+                        } else {
+                            dst = it->second;
+                        }
+                        sg.a_buf = MAT_VT;
+                        sg.a_off = dst;
+                        sg.a_trans = 0;
+                        sg.lda = (sg.k + 31) / 32;
+                        continue;
+                    }
                     const int w = sg.a_buf, m = X.tasks[ti].m;
                     auto key = std::make_tuple(sg.a_off, sg.a_trans, sg.lda, m, sg.k);
                     auto it = seen[w].find(key);
@@ -386,6 +495,16 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         } else {
             const double *src[3] = {P->D64, P->B64, P->VT64};
             pack_all(P.get(), src, nullptr);
+            if (!comps.empty()) {
+                P->comp_desc = upload(comps.data(), comps.size());
+                P->comp_xblk = upload(comp_xblk.data(), comp_xblk.size());
+                P->n_comp = (int64_t)comps.size();
+                P->comp_elems = comp_elems;
+                for (const CompDesc &cd : comps)   // 32 x 32 tiles of the largest composite
+                    P->comp_tiles = std::max<int64_t>(P->comp_tiles, (int64_t)((cd.c.m + 31) / 32) * ((cd.c.n + 31) / 32));
+                HIPCHK(hipDeviceSynchronize());
+                form_composites(P.get(), P->B64, P->VT64, nullptr);
+            }
         }
     }
     // The packed copies double an FP64 plan's block memory (more with the
@@ -437,9 +556,12 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     if (!(precisions & MX_PLAN_FP64)) {   // FP64 blocks were only a source
+        // (B and VT stay while composites are formed from them)
         for (double **p : {&P->D64, &P->B64, &P->VT64}) {
-            if (*p) HIPCHK(hipFree(*p));
-            *p = nullptr;
+            if (*p && (P->n_comp == 0 || p == &P->D64)) {
+                HIPCHK(hipFree(*p));
+                *p = nullptr;
+            }
         }
     }
     return P.release();
@@ -1060,7 +1182,11 @@ static void set_blocks(mx_plan *P, const double *D, const double *B, const doubl
         if (d64[w]) HIPCHK(hipMemcpyAsync(d64[w], src[w], (size_t)len[w] * sizeof(double), hipMemcpyDeviceToDevice, st));
         if (d32[w]) cvt_f64_f32<<<cvt_grid(len[w]), 256, 0, st>>>(src[w], d32[w], len[w]);
     }
-    if (P->precisions & MX_PLAN_BF16X3) pack_all(P, src, st);
+    if (P->precisions & MX_PLAN_BF16X3) {
+        pack_all(P, src, st);
+        // composites: from the kept FP64 copies, updated above
+        if (P->n_comp && (src[MAT_B] || src[MAT_VT])) form_composites(P, P->B64, P->VT64, st);
+    }
     if (P->precisions & MX_PLAN_FP64) pack_f64_all(P, src, st);
     if (P->precisions & MX_PLAN_FP32) {   // from the FP32 copies just converted (src[w] null: unchanged)
         const float *s32[3] = {src[0] ? d32[0] : nullptr, src[1] ? d32[1] : nullptr, src[2] ? d32[2] : nullptr};
@@ -1096,7 +1222,8 @@ static void drop_graphs(mx_plan *P) {
 
 static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int variant) {
     arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
-    arg_check((sched >= 0 && sched < NSCHED) || sched == MX_SCHED_SWEEP, "bad schedule");
+    arg_check((sched >= 0 && sched < SCHED_COARSE) || sched == MX_SCHED_SWEEP || sched == MX_SCHED_COARSE,
+              "bad schedule");
     arg_check(variant >= 0 && variant <= X3_NVARIANT, "bad BF16x3 kernel variant (0 keeps)");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
     const int kd = prec == MX_BF16X3 ? 1 : 0;
@@ -1111,6 +1238,12 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
         if (variant) P->sweep_var[dir] = variant;
         drop_graphs(P);
         return;
+    }
+    if (sched == MX_SCHED_COARSE) {
+        if (prec != MX_BF16X3) throw MxError(MX_ERR_UNSUPPORTED, "the coarse schedule is for MX_BF16X3");
+        if (!P->prog[dir][1][SCHED_COARSE].built)
+            throw MxError(MX_ERR_UNSUPPORTED, "no coarse schedule in this plan (BF16x3 lists of stored blocks)");
+        sched = SCHED_COARSE;
     }
     if (kd == 1) P->sweep_on[dir] = 0;
     P->sched[dir][kd] = sched;
@@ -1173,6 +1306,7 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                 for (int si2 = 0; si2 < NSCHED; ++si2) {
                     // the current schedule first
                     const int sc = si2 == 0 ? cur_sched[dir] : (si2 <= cur_sched[dir] ? si2 - 1 : si2);
+                    if (!P->prog[dir][kd][sc].built) continue;
                     P->sched[dir][kd] = sc;
                     eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up (sizes work buffers)
                     HIPCHK(hipEventRecord(e0, st));
@@ -1755,6 +1889,63 @@ mx_status mx_plan_set_schedule(mx_plan *plan, int32_t direction, mx_precision pr
     });
 }
 
+mx_status mx_lists_describe(const mx_structure *s, int32_t direction, int32_t schedule, int32_t *nlaunch,
+                            int32_t *nltask, int32_t *ntask, int32_t *nseg, int32_t *ncomp, int64_t *launches,
+                            int64_t *ltasks, int64_t *tasks, int64_t *segs, int64_t *comps) {
+    return (mx_status)guard([&]() {
+        arg_check(s && nlaunch && nltask && ntask && nseg && ncomp && (direction == 0 || direction == 1),
+                  "bad argument");
+        arg_check((schedule >= 0 && schedule < SCHED_COARSE) || schedule == MX_SCHED_COARSE, "bad schedule");
+        const int sc = schedule == MX_SCHED_COARSE ? SCHED_COARSE : schedule;
+        Topology t = topology(*s);
+        validate(*s, t);
+        std::vector<int64_t> wrow(s->wrow, s->wrow + s->nleaf), skel(s->skelrow, s->skelrow + s->numnodes);
+        TaskLists X = build_lists(*s, t, 0, 0, 1, direction, sc, wrow.data(), skel.data());
+        // the tasks of each launch, once each, in tile order
+        std::vector<int64_t> off{0}, lt;
+        std::vector<int> seen(X.tasks.size(), -1);
+        for (size_t li = 0; li < X.launches.size(); ++li) {
+            for (int ti = X.launches[li].tile_offset; ti < X.launches[li].tile_offset + X.launches[li].ntiles; ++ti) {
+                const int task = X.tiles[ti].x;
+                if (seen[task] == (int)li || X.tasks[task].m == 0) continue;
+                seen[task] = (int)li;
+                lt.push_back(task);
+            }
+            off.push_back((int64_t)lt.size());
+        }
+        const bool fill = launches != nullptr;
+        arg_check(!fill || (*nlaunch >= (int32_t)X.launches.size() && *nltask >= (int32_t)lt.size() &&
+                            *ntask >= (int32_t)X.tasks.size() && *nseg >= (int32_t)X.segs.size() &&
+                            *ncomp >= (int32_t)X.comps.size() && ltasks && tasks && segs && (comps || X.comps.empty())),
+                  "arrays too small");
+        *nlaunch = (int32_t)X.launches.size();
+        *nltask = (int32_t)lt.size();
+        *ntask = (int32_t)X.tasks.size();
+        *nseg = (int32_t)X.segs.size();
+        *ncomp = (int32_t)X.comps.size();
+        if (!fill) return 0;
+        std::copy(off.begin(), off.end(), launches);
+        std::copy(lt.begin(), lt.end(), ltasks);
+        for (size_t i = 0; i < X.tasks.size(); ++i) {
+            const Task &k = X.tasks[i];
+            const int64_t v[6] = {k.c_row, k.m, k.c_buf, k.seg_begin, k.seg_end, k.beta};
+            std::copy(v, v + 6, tasks + 6 * i);
+        }
+        for (size_t i = 0; i < X.segs.size(); ++i) {
+            const Seg &g = X.segs[i];
+            const int64_t v[8] = {g.a_off, g.b_row, g.a_buf, g.a_trans, g.lda, g.k, g.b_buf, g.b_ld};
+            std::copy(v, v + 8, segs + 8 * i);
+        }
+        for (size_t i = 0; i < X.comps.size(); ++i) {
+            const Composite &c = X.comps[i];
+            const int64_t v[11] = {c.a1_off, c.a2_off, c.a1_buf, c.a1_trans, c.a1_lda, c.a2_buf,
+                                   c.a2_trans, c.a2_lda, c.m, c.kmid, c.n};
+            std::copy(v, v + 11, comps + 11 * i);
+        }
+        return 0;
+    });
+}
+
 mx_status mx_sweep_describe(const mx_structure *s, int32_t direction, int32_t *ntiles, int32_t *ntasks,
                             int32_t *nwaits, int32_t *tiles, int32_t *qinfo, int32_t *tasks, int32_t *waits) {
     return (mx_status)guard([&]() {
@@ -1808,7 +1999,8 @@ mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precis
     return (mx_status)guard([&]() {
         arg_check(plan && schedule && x3_variant && (direction == 0 || direction == 1), "bad argument");
         const bool sw = prec == MX_BF16X3 && plan->sweep_on[direction];
-        *schedule = sw ? MX_SCHED_SWEEP : plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
+        const int sc = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
+        *schedule = sw ? MX_SCHED_SWEEP : sc == SCHED_COARSE ? MX_SCHED_COARSE : sc;
         *x3_variant = sw ? plan->sweep_var[direction] : plan->x3_variant;
         return 0;
     });
@@ -1930,7 +2122,11 @@ mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, do
         arg_check(plan && flops && total && nrhs >= 0, "bad argument");
         for (int i = 0; i < 4; ++i) flops[i] = 0.0;
         if (prec == MX_BF16X3) {
-            const Program &pg = plan->prog[DIR_FWD][1][plan->sched[DIR_FWD][1]];
+            // algorithmic flops: the coarse schedule's composites add work the
+            // reference's evaluation does not have, so its total is the fused
+            // lists' (the same for every other schedule)
+            const int sc = plan->sched[DIR_FWD][1] == SCHED_COARSE ? SCHED_FUSED : plan->sched[DIR_FWD][1];
+            const Program &pg = plan->prog[DIR_FWD][1][sc];
             if (!pg.built) throw MxError(MX_ERR_UNSUPPORTED, "no BF16x3 lists in the plan");
             for (double f : pg.flops) flops[MX_KIND_BF16X3] += f * nrhs;
         } else {

@@ -20,7 +20,7 @@ FP64, FP32, BF16X3 = 0, 1, 2
 KIND_NAMES = {0: "gemm_f64", 1: "gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32",
               6: "grad_bf16x3"}
 PLAN_ADJOINT = 1 << 8
-SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4}
+SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4, "coarse": 5}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5",
@@ -97,6 +97,8 @@ def lib():
         L.mx_plan_get_schedule.restype = ctypes.c_int
         L.mx_sweep_describe.argtypes = [P, I32] + [ctypes.POINTER(I32)] * 3 + [P] * 4
         L.mx_sweep_describe.restype = ctypes.c_int
+        L.mx_lists_describe.argtypes = [P, I32, I32] + [ctypes.POINTER(I32)] * 5 + [P] * 5
+        L.mx_lists_describe.restype = ctypes.c_int
         L.mx_plan_sweep_status.argtypes = [P, P, ctypes.POINTER(I32)]
         L.mx_plan_sweep_status.restype = ctypes.c_int
         L.mx_plan_autotune.argtypes = [P, ctypes.c_int, I32, ctypes.c_int, I32, P, P]
@@ -197,6 +199,30 @@ def sweep_describe(hm, direction="forward"):
                                    out["tiles"].ctypes.data, out["qinfo"].ctypes.data, out["tasks"].ctypes.data,
                                    out["waits"].ctypes.data))
     out["waits"] = out["waits"][:nw.value]
+    del keep
+    return out
+
+
+def lists_describe(hm, direction="forward", schedule="fused"):
+    """Host-only description of the BF16x3 task lists of one direction and
+    schedule (mx_lists_describe), over the stored blocks and the reference
+    layouts: dict of launches (nlaunch + 1 offsets into ltasks), ltasks,
+    tasks (n x 6: c_row, m, c_buf, seg_begin, seg_end, beta), segs (n x 8:
+    a_off, b_row, a_buf, a_trans, lda, k, b_buf, b_ld; a_buf 3 = composite)
+    and comps (n x 11: a1_off, a2_off, a1_buf, a1_trans, a1_lda, a2_buf,
+    a2_trans, a2_lda, m, kmid, n), int64."""
+    st, keep = structure_of(hm)
+    n = [I32(0) for _ in range(5)]
+    args = [ctypes.byref(st), DIRECTIONS[direction], SCHEDULES[schedule]] + [ctypes.byref(x) for x in n]
+    _check(lib().mx_lists_describe(*args, None, None, None, None, None))
+    nl, nlt, nt, ns, nc = (x.value for x in n)
+    out = dict(launches=np.zeros(nl + 1, np.int64), ltasks=np.zeros(max(nlt, 1), np.int64),
+               tasks=np.zeros((max(nt, 1), 6), np.int64), segs=np.zeros((max(ns, 1), 8), np.int64),
+               comps=np.zeros((max(nc, 1), 11), np.int64))
+    _check(lib().mx_lists_describe(*args, out["launches"].ctypes.data, out["ltasks"].ctypes.data,
+                                   out["tasks"].ctypes.data, out["segs"].ctypes.data, out["comps"].ctypes.data))
+    out["ltasks"], out["tasks"] = out["ltasks"][:nlt], out["tasks"][:nt]
+    out["segs"], out["comps"] = out["segs"][:ns], out["comps"][:nc]
     del keep
     return out
 

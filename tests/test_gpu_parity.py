@@ -100,8 +100,8 @@ def test_smallest_trees(gpu, n):
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
     for prec in ("fp64", "fp32", "bf16x3"):
-        # the dependency-driven sweep is a BF16x3 schedule
-        for sc in sorted(s for s in hgemm.SCHEDULES if prec == "bf16x3" or s != "sweep"):
+        # the dependency-driven sweep and the coarse lists are BF16x3 schedules
+        for sc in sorted(s for s in hgemm.SCHEDULES if prec == "bf16x3" or s not in ("sweep", "coarse")):
             for d in ("forward", "adjoint"):
                 plan.set_schedule(d, prec, sc)
             Y = torch.full_like(W, float("nan"))

@@ -1,6 +1,8 @@
-"""Both coupling/downward schedules (fused per level, split: all couplings
-in one launch then per-level transfers accumulated with beta = 1), both
-directions and every precision, against the oracle; and the autotuner."""
+"""The coupling/downward schedules (fused per level, split: all couplings
+in one launch then per-level transfers accumulated with beta = 1, their
+balanced forms, and the coarse lists: BF16x3 launches merged in pairs over
+composite blocks), both directions and every precision, against the
+oracle; the FP64/FP32 kernel variants; and the autotuner."""
 import sys
 
 import numpy as np
@@ -39,7 +41,12 @@ def test_schedules_match_oracle(gpu, case, schedule):
     Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
-    for prec, stages in [("fp64", 0), ("fp32", 0)] + [("bf16x3", v) for v in sorted(hgemm.X3_VARIANTS)]:
+    precs = [("fp64", 0), ("fp32", 0)] if schedule != "coarse" else []   # coarse: BF16x3 only
+    if schedule == "coarse":
+        for prec in ("fp64", "fp32"):
+            with pytest.raises(hgemm.MxError):
+                plan.set_schedule("forward", prec, schedule)
+    for prec, stages in precs + [("bf16x3", v) for v in sorted(hgemm.X3_VARIANTS)]:
         for direction in ("forward", "adjoint"):
             plan.set_schedule(direction, prec, schedule, stages)
             assert plan.get_schedule(direction, prec)[0] == schedule
@@ -244,3 +251,69 @@ def test_long_interaction_lists_are_split(gpu):
         ey, eg = rel(Y.cpu().numpy(), Yo), rel(G.cpu().numpy(), Go)
         _say(f"  {prec}: forward {ey:.2e} adjoint {eg:.2e}")
         assert ey <= tol and eg <= tol, prec
+
+
+@pytest.mark.parametrize("case", ["h2b_small", "hss_small", "h2b_tall"])
+def test_coarse_schedule_backward_and_graphs(gpu, case):
+    """The coarse schedule writes every skeleton block: the block gradients
+    after a coarse forward evaluation are within 5e-5 of the FP64 ones, as
+    after a fused one (the bound of the split-BF16x3 gradients: operands
+    split to 16 significant bits); its graph replays equal its eager
+    evaluations bit for bit."""
+    import torch
+    _say(f"[coarse backward] {case}")
+    X, hm = make_problem(**CASES[case])
+    nrhs = 40
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"), adjoint=True)
+    W = torch.from_numpy(host.normal((hm.N * nrhs,), seed=61)).cuda()
+    dY = torch.from_numpy(host.normal((hm.N * nrhs,), seed=62)).cuda()
+    Y = torch.empty_like(W)
+    plan.eval(W, Y, nrhs, "fp64")
+    ref = [g.clone() for g in plan.backward(W, dY, nrhs, "fp64")]
+    res = {}
+    for sc in ("fused", "coarse"):
+        for d in ("forward", "adjoint"):
+            plan.set_schedule(d, "bf16x3", sc)
+        plan.eval(W, Y, nrhs, "bf16x3")
+        res[sc] = [g.clone() for g in plan.backward(W, dY, nrhs, "bf16x3")]
+        torch.cuda.synchronize()
+    for i, nm in enumerate(["dW", "dD", "dB", "dVT"]):
+        r = ref[i].double().cpu().numpy()
+        ef = rel(res["fused"][i].double().cpu().numpy(), r)
+        ec = rel(res["coarse"][i].double().cpu().numpy(), r)
+        _say(f"  {nm}: vs FP64 fused {ef:.2e} coarse {ec:.2e}")
+        assert ec <= 5e-5 and ef <= 5e-5, nm
+    eager = torch.empty_like(W)
+    plan.set_graph(False)
+    plan.eval(W, eager, nrhs, "bf16x3")
+    plan.set_graph(True)
+    for _ in range(3):
+        Yg = torch.full_like(W, float("nan"))
+        plan.eval(W, Yg, nrhs, "bf16x3")
+        torch.cuda.synchronize()
+        assert torch.equal(Yg, eager)
+    plan.set_graph(False)
+
+
+def test_coarse_composites_follow_new_blocks(gpu):
+    """mx_plan_set_blocks forms the coarse schedule's composites again, from
+    the FP64 B and VT the plan keeps: after all new blocks, and after new B
+    alone, the coarse BF16x3 evaluation agrees with the FP64 one to 1e-5."""
+    import torch
+    _say("[coarse after set_blocks]")
+    X, hm = make_problem(**CASES["h2b_small"])
+    nrhs = 33
+    plan = hgemm.Plan(hm, precisions=("fp64", "bf16x3"))
+    plan.set_schedule("forward", "bf16x3", "coarse")
+    W = torch.from_numpy(host.normal((hm.N * nrhs,), seed=63)).cuda()
+    D, B, VT = (torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in (hm.D, hm.B, hm.VT))
+    for blocks in ((D * 1.5, B - 0.25 * B.abs(), VT * -0.75), (None, B * 0.5, None)):
+        plan.set_blocks(*blocks)
+        Y64 = torch.empty_like(W)
+        Yc = torch.empty_like(W)
+        plan.eval(W, Y64, nrhs, "fp64")
+        plan.eval(W, Yc, nrhs, "bf16x3")
+        torch.cuda.synchronize()
+        e = rel(Yc.cpu().numpy(), Y64.cpu().numpy())
+        _say(f"  coarse vs fp64: {e:.2e}")
+        assert e <= 1e-5
