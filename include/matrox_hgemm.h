@@ -276,7 +276,7 @@ enum { MX_SCHED_SWEEP = 4 };
  * a skeleton block x_P = sum_s A_s B_s written one launch earlier becomes the
  * segments (A_c A_s) B_s, over composite blocks A_c A_s formed in FP64 when
  * the plan is built (and again by mx_plan_set_blocks); a pair is merged when
- * that adds at most 2.5 MFLOP per right-hand side column.  Fewer launches
+ * that adds at most 4.5 MFLOP per right-hand side column.  Fewer launches
  * for the latency-bound top of the tree, every skeleton block still written.
  * The plan keeps FP64 copies of B and VT for it.  Like the reference's
  * coarsened level groups (symGen/HGEMM_gen.h:142-178, 196-236), with the

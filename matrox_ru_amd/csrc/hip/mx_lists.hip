@@ -805,8 +805,8 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
         add_tiles(L, tk, s.Ddim[lf]);
     }
     end_launch();
-    // at most 2.5 MFLOP per right-hand side column for each merge
-    if (sched == SCHED_COARSE) coarsen(L, 2.5e6);
+    // at most 4.5 MFLOP per right-hand side column for each merge
+    if (sched == SCHED_COARSE) coarsen(L, 4.5e6);
     order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1 || group);
     return L;
 }

@@ -85,7 +85,7 @@ def test_coarse_lists_match_oracle(case, direction):
 
 
 def test_coarse_lists_keep_pairs_cheap():
-    """A merge adds at most 2.5 MFLOP per right-hand side column: the coarse
+    """A merge adds at most 4.5 MFLOP per right-hand side column: the coarse
     lists' work per column exceeds the fused lists' by no more than that per
     merged launch."""
     X, hm = make_problem(**CASES["h2b_small"])
@@ -101,4 +101,4 @@ def test_coarse_lists_keep_pairs_cheap():
     Lc = hgemm.lists_describe(hm, "forward", "coarse")
     merged = len(Lf["launches"]) - len(Lc["launches"])
     assert merged >= 1
-    assert work(Lf) <= work(Lc) <= work(Lf) + 2.5e6 * merged
+    assert work(Lf) <= work(Lc) <= work(Lf) + 4.5e6 * merged
