@@ -1382,14 +1382,17 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         }
         if (prec == MX_BF16X3) {
             // per launch: the deep top-of-tree launches and the wide leaf
-            // launches favour different variants
-            P->profiling = 1;
-            for (int dir = 0; dir < ndir; ++dir) {
-                const int sc = P->sched[dir][1];
+            // launches favour different variants.  Tuned for the schedule
+            // picked above and for the coarse lists, whose merged launches
+            // favour other variants than the plan-wide one it was timed
+            // with; the faster of the two tuned evaluations is kept.
+            auto tune_launches = [&](int dir, int sc) {
+                P->sched[dir][1] = sc;
                 const size_t nl = P->prog[dir][1][sc].launches.size();
                 std::vector<int> &lv = P->lvar[dir][sc];
                 std::vector<int> pick(nl, P->x3_variant);
                 std::vector<float> best(nl, 0.f);
+                P->profiling = 1;
                 for (int vi = 0; vi < X3_NVARIANT; ++vi) {
                     const int v = vi == 0 ? P->x3_variant : (vi <= P->x3_variant - 1 ? vi : vi + 1);
                     lv.assign(nl, v);
@@ -1412,29 +1415,40 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
                         }
                 }
                 lv = pick;
+                P->profiling = 0;
+            };
+            auto time_eval = [&](int dir) {
+                eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
+                HIPCHK(hipEventRecord(e0, st));
+                for (int r = 0; r < reps; ++r) eval(P, dir, prec, W, Y, nrhs, io, st);
+                HIPCHK(hipEventRecord(e1, st));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                return ms / reps;
+            };
+            for (int dir = 0; dir < ndir; ++dir) {
+                const int sc = P->sched[dir][1];
+                tune_launches(dir, sc);
+                if (sc != SCHED_COARSE && P->prog[dir][1][SCHED_COARSE].built) {
+                    const float t_sc = time_eval(dir);
+                    tune_launches(dir, SCHED_COARSE);
+                    const float t_co = time_eval(dir);
+                    P->sched[dir][1] = t_co < 0.98f * t_sc ? SCHED_COARSE : sc;
+                    best_dir[dir] = std::min(t_co, t_sc);
+                }
             }
-            P->profiling = 0;
             // the dependency-driven sweep (three tile widths) against the
             // tuned level launches
             for (int dir = 0; dir < ndir; ++dir) {
                 if (!P->sweep[dir].ok) continue;
-                auto time_eval = [&]() {
-                    eval(P, dir, prec, W, Y, nrhs, io, st);   // warm-up
-                    HIPCHK(hipEventRecord(e0, st));
-                    for (int r = 0; r < reps; ++r) eval(P, dir, prec, W, Y, nrhs, io, st);
-                    HIPCHK(hipEventRecord(e1, st));
-                    HIPCHK(hipEventSynchronize(e1));
-                    float ms = 0.f;
-                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-                    return ms / reps;
-                };
                 P->sweep_on[dir] = 0;
-                float best = time_eval();
+                float best = time_eval(dir);
                 int pick = 0;
                 for (int v : {X3_V128x128_S2, X3_V128x64_S2, X3_V128x32_S2, X3_V128x32_K8}) {
                     P->sweep_on[dir] = 1;
                     P->sweep_var[dir] = v;
-                    const float ms = time_eval();
+                    const float ms = time_eval(dir);
                     if (ms < 0.98f * best) {
                         best = ms;
                         pick = v;
