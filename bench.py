@@ -698,10 +698,12 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "traffic_h2b_n65536.txt")
     if args.workload == "h2b_n65536" and nrhs == wl["nrhs"] and os.path.exists(tfile):
         kv = dict(l.strip().split("=", 1) for l in open(tfile) if "=" in l and not l.startswith("#"))
-        if kv.get("kernel") == roofline["kernel"]:
-            roofline["traffic"] = float(kv["fetch_bytes_per_launch"]) + float(kv["write_bytes_per_launch"])
+        nl = roofline["launches_per_eval"]
+        if kv.get("kernel") == roofline["kernel"] and f"fetch_bytes_per_eval_{nl}" in kv:
+            roofline["traffic"] = round((float(kv[f"fetch_bytes_per_eval_{nl}"]) +
+                                         float(kv[f"write_bytes_per_eval_{nl}"])) / nl, -6)
             roofline["traffic_source"] = "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/x3_kernel_h2b_n65536.md"
-            roofline["algorithmic_bytes_per_launch"] = round(4.67e9 / roofline["launches_per_eval"], -6)
+            roofline["algorithmic_bytes_per_launch"] = round(4.67e9 / nl, -6)
 
     d = Ymx - Y64
     err_vs64 = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(Y64))
