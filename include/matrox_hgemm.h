@@ -271,16 +271,18 @@ enum { MX_SCHED_FUSED = 0, MX_SCHED_SPLIT = 1, MX_SCHED_FUSED_BALANCED = 2, MX_S
  * whose sweeps synchronise per level group (symGen/HGEMM_gen.h:142-236). */
 enum { MX_SCHED_SWEEP = 4 };
 /* MX_SCHED_COARSE (MX_BF16X3 only; plans of stored blocks): the fused
- * schedule with consecutive launches merged in pairs where the second reads
- * the first's skeletons through few, small products.  A segment A_c x_P over
- * a skeleton block x_P = sum_s A_s B_s written one launch earlier becomes the
- * segments (A_c A_s) B_s, over composite blocks A_c A_s formed in FP64 when
- * the plan is built (and again by mx_plan_set_blocks); a pair is merged when
- * that adds at most 4.5 MFLOP per right-hand side column.  Fewer launches
- * for the latency-bound top of the tree, every skeleton block still written.
- * The plan keeps FP64 copies of B and VT for it.  Like the reference's
- * coarsened level groups (symGen/HGEMM_gen.h:142-178, 196-236), with the
- * group boundary removed by re-association instead of a barrier. */
+ * schedule with runs of consecutive launches merged into one.  A segment
+ * A_c x_P over a skeleton block x_P = sum_s A_s B_s written earlier in the
+ * same run becomes the segments (A_c A_s) B_s, over composite blocks A_c A_s
+ * (A_s itself a composite when P was rewritten so) formed in FP64 when the
+ * plan is built (and again by mx_plan_set_blocks).  A launch joins the run
+ * when that adds at most 4.5 MFLOP per right-hand side column and does not
+ * lengthen the run's longest task by more than its own longest task plus 4
+ * chunks of 32.  Fewer launches for the latency-bound top of the tree,
+ * every skeleton block still written.  The plan keeps FP64 copies of B and
+ * VT for it.  Like the reference's coarsened level groups
+ * (symGen/HGEMM_gen.h:142-178, 196-236), with the group boundaries removed
+ * by re-association instead of barriers. */
 enum { MX_SCHED_COARSE = 5 };
 
 /* Error word of the last sweep this plan enqueued (after synchronising
@@ -366,11 +368,13 @@ mx_status mx_plan_set_seg_variants(mx_plan *plan, int32_t direction, mx_precisio
                                    int32_t count);
 
 /* Autotuner: times every schedule (and, for MX_BF16X3, every kernel
- * variant, first for the whole evaluation, then launch by launch) of each
- * direction built into the plan with `reps` evaluations of
- * nrhs columns on scratch buffers it allocates and frees, and keeps the
- * fastest.  best_ms (optional, 2 entries) receives the chosen per-direction
- * times.  Synchronises `stream`; drops captured graphs. */
+ * variant, first for the whole evaluation, then launch by launch, for the
+ * schedule picked and for MX_SCHED_COARSE) of each direction built into the
+ * plan with at least `reps` evaluations of nrhs columns (more for short
+ * evaluations: about 2 ms per measurement, at most 20) on scratch buffers
+ * it allocates and frees, and keeps the fastest.  best_ms (optional, 2
+ * entries) receives the chosen per-direction times.  Synchronises
+ * `stream`; drops captured graphs. */
 mx_status mx_plan_autotune(mx_plan *plan, mx_precision prec, int32_t nrhs, mx_io_type io, int32_t reps,
                            float *best_ms, void *stream);
 

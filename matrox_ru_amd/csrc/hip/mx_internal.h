@@ -222,6 +222,7 @@ struct mx_plan {
     mx::CompDesc *comp_desc = nullptr;
     mx::X3Block *comp_xblk = nullptr;
     int64_t n_comp = 0, comp_elems = 0, comp_tiles = 1;
+    std::vector<int64_t> comp_gen;   // descriptor ranges of the generations (a nested composite after its factor)
 
     // work buffers: skeletons of the forward (w^, u^) and of the adjoint
     // (y^ = dL/du^, g^ = dL/dw^) evaluations

@@ -364,21 +364,24 @@ static void order_tiles(TaskLists &L, int bk, bool group_by_operand) {
     L.launches.swap(lns);
 }
 
-// The coarse schedule: consecutive launches (j, j+1) of the fused lists
-// merge into one launch when launch j+1's segments that read skeleton
-// blocks written in launch j can be rewritten over launch j's inputs.  A
-// consumer segment A_c x_P, where x_P = sum_s A_s B_s is the output of
-// producer task P (beta = 0, its only writer), becomes the segments
-// (A_c A_s) B_s, one per segment of P, with the composite A_c A_s (formed in
-// FP64 when the plan is built).  Every skeleton block is still written (the
-// block gradients read them all), and no composite is made of composites
-// (pairs are disjoint).  A pair is taken when it adds at most max_extra
-// flops per right-hand side column (2 m (sum_s k_s - k_c) per inlined
-// segment): the UP launches of the lowest two levels merge for nothing when
-// the leaf blocks are as tall as the ranks, the top-of-tree UP and
-// coupling/downward launches for little.  The HSS evaluation at N = 16,384
-// (15 launches in the fused lists) runs in 8.
-static void coarsen(TaskLists &L, double max_extra) {
+// The coarse schedule: runs of consecutive launches of the fused lists merge
+// into one launch.  A segment A_c x_P of a launch, over a skeleton block
+// x_P = sum_s A_s B_s written by producer task P (beta = 0, its only writer)
+// in an earlier launch of the same run, becomes the segments (A_c A_s) B_s,
+// one per segment of P, with the composite A_c A_s formed in FP64 when the
+// plan is built.  P's own segments are already over blocks written before
+// the run, so a composite's second factor is a stored block or an earlier
+// composite (chains as long as the run).  Every skeleton block is still
+// written (the block gradients read them all).  Launch j+1 joins the run
+// of launch j when that adds at most max_extra flops per right-hand side
+// column (2 m (sum_s k_s - k_c) per rewritten segment), leaves every task
+// within maxseg segments, and does not lengthen the run's longest task by
+// more than launch j+1's own longest task plus launch_chunks (the top of the
+// tree is latency-bound: a launch lasts about as long as its longest task
+// plus a fixed cost); otherwise it starts a run of its own.  The UP launches
+// of the lowest two levels merge for nothing when the leaf blocks are as
+// tall as the ranks, the top-of-tree launches for little.
+static void coarsen(TaskLists &L, double max_extra, int maxseg, int bk, int launch_chunks) {
     const int n = (int)L.launches.size();
     std::vector<int> launch_of(L.tasks.size(), -1);
     std::vector<std::vector<int>> tasks_of(n);
@@ -400,101 +403,129 @@ static void coarsen(TaskLists &L, double max_extra) {
         if (it == prod.end()) prod[key] = (int)t;
         else it->second = -1;
     }
-    // the producer in launch j of segment s (-1: none there)
-    auto producer_in = [&](const Seg &sg, int j) {
-        if (sg.b_buf != VEC_WSKEL && sg.b_buf != VEC_USKEL) return -1;
-        auto it = prod.find(std::make_pair((int)sg.b_buf, sg.b_row));
-        if (it == prod.end() || it->second < 0) return -1;
-        return launch_of[it->second] == j ? it->second : -1;
+    // every task's segments as they will be (over blocks written before its run)
+    std::vector<std::vector<Seg>> segs_of(L.tasks.size());
+    for (size_t t = 0; t < L.tasks.size(); ++t)
+        segs_of[t].assign(L.segs.begin() + L.tasks[t].seg_begin, L.segs.begin() + L.tasks[t].seg_end);
+    auto composite_of = [&](const Seg &outer, const Seg &inner, int m) {
+        Composite c{};
+        c.a1_off = outer.a_off;
+        c.a1_buf = outer.a_buf;
+        c.a1_trans = outer.a_trans;
+        c.a1_lda = outer.lda;
+        c.a2_off = inner.a_off;   // a stored block, or (MAT_COMP) an earlier composite
+        c.a2_buf = inner.a_buf;
+        c.a2_trans = inner.a_trans;
+        c.a2_lda = inner.lda;
+        c.m = m;
+        c.kmid = outer.k;
+        c.n = inner.k;
+        return c;
     };
-    // pairs, top of the launch order down
-    std::vector<char> consumer(n, 0);
-    for (int j = 0; j + 1 < n;) {
+    // a segment of a joining launch: kept (P < 0), or the composite of it
+    // with segment q of its producer P
+    struct Piece {
+        Seg sg;
+        int P, q;
+    };
+    // a task's length in K chunks: a launch of latency-bound tasks lasts about
+    // as long as its longest task, plus a fixed cost of some chunks' worth
+    auto chunks = [&](const std::vector<Seg> &sg) {
+        int c = 0;
+        for (const Seg &x : sg) c += (x.k + bk - 1) / bk;
+        return c;
+    };
+    std::vector<char> joins(n, 0);   // launch joins the run of the launch before it
+    int run0 = 0;                    // first launch of the current run
+    int run_len = 0;                 // longest task of the current run, in chunks
+    for (int t : tasks_of[0]) run_len = std::max(run_len, chunks(segs_of[t]));
+    for (int j = 1; j < n; ++j) {
         double extra = 0.0;
         bool ok = true;
-        for (int t : tasks_of[j + 1]) {
+        int own_len = 0, new_len = 0;   // launch j's longest task before and after rewriting
+        std::vector<std::pair<int, std::vector<Piece>>> rewritten;
+        for (int t : tasks_of[j]) {
             const Task &tk = L.tasks[t];
-            for (int si = tk.seg_begin; si < tk.seg_end && ok; ++si) {
-                const Seg &sg = L.segs[si];
-                if (sg.b_buf != VEC_WSKEL && sg.b_buf != VEC_USKEL) continue;
-                auto it = prod.find(std::make_pair((int)sg.b_buf, sg.b_row));
-                if (it != prod.end() && it->second < 0) {
-                    ok = false;   // a block with several writers
-                    break;
+            std::vector<Piece> out;
+            for (const Seg &sg : segs_of[t]) {
+                int P = -1;
+                if (sg.b_buf == VEC_WSKEL || sg.b_buf == VEC_USKEL) {
+                    auto it = prod.find(std::make_pair((int)sg.b_buf, sg.b_row));
+                    if (it != prod.end()) {
+                        if (it->second < 0) ok = false;   // a block with several writers
+                        else if (launch_of[it->second] >= run0 && launch_of[it->second] < j) P = it->second;
+                    }
                 }
-                const int P = producer_in(sg, j);
-                if (P < 0) continue;
+                if (P < 0) {
+                    out.push_back(Piece{sg, -1, -1});
+                    continue;
+                }
                 const Task &pt = L.tasks[P];
-                if (pt.beta != 0 || pt.m != sg.k || sg.a_buf == MAT_COMP) {
+                if (pt.beta != 0 || pt.m != sg.k) {
                     ok = false;
                     break;
                 }
                 double kp = 0.0;
-                for (int ps = pt.seg_begin; ps < pt.seg_end; ++ps) {
-                    if (L.segs[ps].a_buf == MAT_COMP) ok = false;
-                    kp += L.segs[ps].k;
+                for (size_t q = 0; q < segs_of[P].size(); ++q) {
+                    out.push_back(Piece{sg, P, (int)q});
+                    kp += segs_of[P][q].k;
                 }
                 extra += 2.0 * tk.m * (kp - sg.k);
             }
-            if (!ok) break;
+            if (!ok || (int)out.size() > maxseg) {
+                ok = false;
+                break;
+            }
+            own_len = std::max(own_len, chunks(segs_of[t]));
+            int c = 0;
+            for (const Piece &pc : out) c += ((pc.P < 0 ? pc.sg.k : segs_of[pc.P][pc.q].k) + bk - 1) / bk;
+            new_len = std::max(new_len, c);
+            rewritten.push_back({t, std::move(out)});
         }
-        if (ok && extra <= max_extra) {
-            consumer[j + 1] = 1;
-            j += 2;
-        } else {
-            j += 1;
+        // the merged launch must not outlast the two it replaces: its
+        // longest task against the run's and launch j's, with a launch's
+        // fixed cost taken as launch_chunks chunks
+        if (!ok || extra > max_extra || std::max(run_len, new_len) > run_len + own_len + launch_chunks) {
+            run0 = j;
+            run_len = own_len;
+            continue;
+        }
+        joins[j] = 1;
+        run_len = std::max(run_len, new_len);
+        for (auto &tr : rewritten) {
+            const int t = tr.first, m = L.tasks[t].m;
+            std::vector<Seg> ns;
+            for (const Piece &pc : tr.second) {
+                if (pc.P < 0) {
+                    ns.push_back(pc.sg);
+                    continue;
+                }
+                const Seg &inner = segs_of[pc.P][pc.q];
+                Seg s2 = inner;   // the producer segment's B operand, over the composite
+                s2.a_buf = MAT_COMP;
+                s2.a_off = (int64_t)L.comps.size();
+                s2.a_trans = 0;
+                s2.lda = m;
+                L.comps.push_back(composite_of(pc.sg, inner, m));
+                ns.push_back(s2);
+            }
+            segs_of[t].swap(ns);
         }
     }
-    // segments: each consumer segment replaced by its producer's, over
-    // composites (the producers' segment ranges as they were)
-    const std::vector<Task> old = L.tasks;
+    // segments in task order
     std::vector<Seg> segs;
-    segs.reserve(L.segs.size());
     for (size_t t = 0; t < L.tasks.size(); ++t) {
-        Task &tk = L.tasks[t];
-        const int b = (int)segs.size();
-        const int li = launch_of[t];
-        for (int si = old[t].seg_begin; si < old[t].seg_end; ++si) {
-            const Seg sg = L.segs[si];
-            const int P = li > 0 && consumer[li] ? producer_in(sg, li - 1) : -1;
-            if (P < 0) {
-                segs.push_back(sg);
-                continue;
-            }
-            const Task &pt = old[P];
-            for (int ps = pt.seg_begin; ps < pt.seg_end; ++ps) {
-                const Seg &inner = L.segs[ps];
-                Composite c{};
-                c.a1_off = sg.a_off;
-                c.a1_buf = sg.a_buf;
-                c.a1_trans = sg.a_trans;
-                c.a1_lda = sg.lda;
-                c.a2_off = inner.a_off;
-                c.a2_buf = inner.a_buf;
-                c.a2_trans = inner.a_trans;
-                c.a2_lda = inner.lda;
-                c.m = tk.m;
-                c.kmid = sg.k;
-                c.n = inner.k;
-                Seg ns = inner;   // the producer's B operand
-                ns.a_buf = MAT_COMP;
-                ns.a_off = (int64_t)L.comps.size();
-                ns.a_trans = 0;
-                ns.lda = tk.m;
-                segs.push_back(ns);
-                L.comps.push_back(c);
-            }
-        }
-        tk.seg_begin = b;
-        tk.seg_end = (int)segs.size();
+        L.tasks[t].seg_begin = (int)segs.size();
+        segs.insert(segs.end(), segs_of[t].begin(), segs_of[t].end());
+        L.tasks[t].seg_end = (int)segs.size();
     }
     L.segs.swap(segs);
-    // launches: a consumer's tiles join its producer launch
+    // launches: a joining launch's tiles go to its run's launch
     std::vector<int2> tiles;
     std::vector<Launch> lns;
     std::vector<int> kinds;
     for (int li = 0; li < n; ++li) {
-        if (!consumer[li]) {
+        if (!joins[li]) {
             lns.push_back(Launch{(int)tiles.size(), 0});
             kinds.push_back(L.kinds[li]);
         }
@@ -806,7 +837,7 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
     }
     end_launch();
     // at most 4.5 MFLOP per right-hand side column for each merge
-    if (sched == SCHED_COARSE) coarsen(L, 4.5e6);
+    if (sched == SCHED_COARSE) coarsen(L, 4.5e6, X3_MAXSEG, X3_BK, 4);
     order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1 || group);
     return L;
 }

@@ -19,9 +19,11 @@
 // tasks gives the same sums with each output block written once per launch.
 #include "mx_internal.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <tuple>
 
@@ -83,16 +85,19 @@ static void pack_f64_all(mx_plan *P, const double *const src[3], hipStream_t st)
 
 // The coarse schedule's composite operands op(A1) op(A2) in FP64: one 32 x
 // 32 tile of composite blockIdx.y per workgroup (blockIdx.x: tile, column
-// of tiles major), written column-major (ld = m) at its c_off of out.
+// of tiles major), written column-major (ld = m) at its c_off of out.  A2
+// is a stored block, or a composite of an earlier generation (in out, at
+// a2_off, column-major with ld a2_lda).
 __global__ void comp_gemm(const double *__restrict__ B, const double *__restrict__ VT,
-                          const CompDesc *__restrict__ cd, double *__restrict__ out) {
+                          const CompDesc *__restrict__ cd, double *out) {
     const CompDesc d = cd[blockIdx.y];
     const Composite &c = d.c;
     const int tm = (c.m + 31) / 32, tn = (c.n + 31) / 32;
     if ((int)blockIdx.x >= tm * tn) return;
     const int i0 = ((int)blockIdx.x % tm) * 32, j0 = ((int)blockIdx.x / tm) * 32;
     const double *A1 = (c.a1_buf == MAT_B ? B : VT) + c.a1_off;
-    const double *A2 = (c.a2_buf == MAT_B ? B : VT) + c.a2_off;
+    // a second factor that is a composite itself was formed by an earlier launch
+    const double *A2 = (c.a2_buf == MAT_COMP ? out : c.a2_buf == MAT_B ? B : VT) + c.a2_off;
     __shared__ double sa[32][33], sb[32][33];   // sa[l][i] = op(A1)(i0+i, l0+l), sb[l][j] = op(A2)(l0+l, j0+j)
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -135,10 +140,15 @@ static void form_composites(mx_plan *P, const double *B, const double *VT, hipSt
     double *tmp = nullptr;
     HIPCHK(hipMalloc(&tmp, (size_t)std::max<int64_t>(P->comp_elems, 1) * sizeof(double)));
     try {
+        // generation by generation (the descriptors are in generation order)
+        for (size_t g = 0; g + 1 < P->comp_gen.size(); ++g)
+            for (int64_t c0 = P->comp_gen[g]; c0 < P->comp_gen[g + 1]; c0 += 65535) {
+                const int64_t nc = std::min<int64_t>(65535, P->comp_gen[g + 1] - c0);
+                comp_gemm<<<dim3((unsigned)P->comp_tiles, (unsigned)nc), 256, 0, st>>>(B, VT, P->comp_desc + c0, tmp);
+                HIPCHK(hipGetLastError());
+            }
         for (int64_t c0 = 0; c0 < P->n_comp; c0 += 65535) {
             const int64_t nc = std::min<int64_t>(65535, P->n_comp - c0);
-            comp_gemm<<<dim3((unsigned)P->comp_tiles, (unsigned)nc), 256, 0, st>>>(B, VT, P->comp_desc + c0, tmp);
-            HIPCHK(hipGetLastError());
             pack_x3_blocks<<<dim3(64, (unsigned)nc), 256, 0, st>>>(tmp, P->comp_xblk + c0, P->xhi[MAT_VT],
                                                                     P->xlo[MAT_VT]);
             HIPCHK(hipGetLastError());
@@ -274,6 +284,8 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     std::vector<X3Block> xblk[3];   // blocks to pack per stored buffer (D, B, VT)
     std::vector<CompDesc> comps;     // the coarse schedule's composites and their packing list
     std::vector<X3Block> comp_xblk;
+    std::vector<int> cgen;           // per composite: generation (1: both factors stored blocks)
+    std::vector<int64_t> cdst;       // per composite: packed offset in the VT operands
     int64_t comp_elems = 0;
     if (precisions & MX_PLAN_BF16X3) {
         // split panels: padded row offsets of W leaf blocks and skeleton blocks
@@ -291,9 +303,10 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
         // shared by the forward and adjoint lists
         std::map<std::tuple<int64_t, int, int, int, int>, int64_t> seen[3];
         int64_t next[3] = {0, 0, 0};
-        // composites of the coarse schedule: each distinct one formed once,
-        // packed into the VT operands
-        std::map<std::tuple<int, int64_t, int, int, int, int64_t, int, int, int, int, int>, int64_t> cseen;
+        // composites of the coarse schedule: each distinct one formed once
+        // (a nested one after the composite it multiplies), packed into the
+        // VT operands; keyed with the global id of a composite factor
+        std::map<std::tuple<int, int64_t, int, int, int, int64_t, int, int, int, int, int>, int> cid;
         for (int dsc = 0; dsc < ndir * NSCHED; ++dsc) {
             const int dir = dsc / NSCHED, sc = dsc % NSCHED;
             // composites need FP64 B and VT, which a plan of generated blocks
@@ -301,36 +314,41 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             if (sc == SCHED_COARSE && gen) continue;
             TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, sc, P->wprow.data(), P->sprow.data());
             split_long_tasks(X, X3_MAXSEG);
+            std::vector<int> local(X.comps.size(), -1);   // this program's composites -> global ids
+            std::function<int(int)> resolve = [&](int li) -> int {
+                if (local[li] >= 0) return local[li];
+                Composite c = X.comps[li];
+                int g = 1;
+                if (c.a2_buf == MAT_COMP) {
+                    const int inner = resolve((int)c.a2_off);
+                    c.a2_off = inner;
+                    g = cgen[inner] + 1;
+                }
+                auto key = std::make_tuple(c.a1_buf, c.a1_off, c.a1_trans, c.a1_lda, c.a2_buf, c.a2_off,
+                                           c.a2_trans, c.a2_lda, c.m, c.kmid, c.n);
+                auto it = cid.find(key);
+                if (it != cid.end()) return local[li] = it->second;
+                const int id = (int)comps.size();
+                cid[key] = id;
+                comps.push_back(CompDesc{c, comp_elems});
+                comp_elems += (int64_t)c.m * c.n;
+                cgen.push_back(g);
+                cdst.push_back(-1);
+                return local[li] = id;
+            };
             for (size_t ti = 0; ti < X.tasks.size(); ++ti) {
                 if (X.tasks[ti].seg_end - X.tasks[ti].seg_begin > X3_MAXSEG)
                     throw MxError(MX_ERR_ARG, "internal: a BF16x3 task longer than its segment window");
                 for (int si = X.tasks[ti].seg_begin; si < X.tasks[ti].seg_end; ++si) {
                     Seg &sg = X.segs[si];
                     if (sg.a_buf == MAT_COMP) {
-                        const Composite &c = X.comps[sg.a_off];
-                        auto key = std::make_tuple(c.a1_buf, c.a1_off, c.a1_trans, c.a1_lda, c.a2_buf, c.a2_off,
-                                                   c.a2_trans, c.a2_lda, c.m, c.kmid, c.n);
-                        auto it = cseen.find(key);
-                        int64_t dst;
-                        if (it == cseen.end()) {
-                            dst = next[MAT_VT];
-                            cseen[key] = dst;
-                            comps.push_back(CompDesc{c, comp_elems});
-                            X3Block xb{};
-                            xb.src_off = comp_elems;
-                            xb.dst_off = dst;
-                            xb.M = c.m;
-                            xb.K = c.n;
-                            xb.lda = c.m;
-                            xb.trans = 0;
-                            comp_xblk.push_back(xb);
-                            comp_elems += (int64_t)c.m * c.n;
-                            next[MAT_VT] += x3_packed_elems(c.m, c.n);
-                        } else {
-                            dst = it->second;
+                        const int id = resolve((int)sg.a_off);
+                        if (cdst[id] < 0) {
+                            cdst[id] = next[MAT_VT];
+                            next[MAT_VT] += x3_packed_elems(comps[id].c.m, comps[id].c.n);
                         }
                         sg.a_buf = MAT_VT;
-                        sg.a_off = dst;
+                        sg.a_off = cdst[id];
                         sg.a_trans = 0;
                         sg.lda = (sg.k + 31) / 32;
                         continue;
@@ -496,12 +514,36 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             const double *src[3] = {P->D64, P->B64, P->VT64};
             pack_all(P.get(), src, nullptr);
             if (!comps.empty()) {
-                P->comp_desc = upload(comps.data(), comps.size());
+                // descriptors in generation order, a composite factor named by
+                // its offset in the FP64 buffer; every composite is read by a
+                // segment, so each has its packed place
+                std::vector<int> order(comps.size());
+                for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+                std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cgen[a] < cgen[b]; });
+                std::vector<CompDesc> cd;
+                for (int i : order) {
+                    if (cdst[i] < 0) throw MxError(MX_ERR_STATE, "internal: a composite no segment reads");
+                    CompDesc d = comps[i];
+                    if (d.c.a2_buf == MAT_COMP) d.c.a2_off = comps[d.c.a2_off].c_off;
+                    cd.push_back(d);
+                    X3Block xb{};
+                    xb.src_off = d.c_off;
+                    xb.dst_off = cdst[i];
+                    xb.M = d.c.m;
+                    xb.K = d.c.n;
+                    xb.lda = d.c.m;
+                    xb.trans = 0;
+                    comp_xblk.push_back(xb);
+                }
+                P->comp_gen.assign(1, 0);
+                for (size_t k = 1; k <= order.size(); ++k)
+                    if (k == order.size() || cgen[order[k]] != cgen[order[k - 1]]) P->comp_gen.push_back((int64_t)k);
+                P->comp_desc = upload(cd.data(), cd.size());
                 P->comp_xblk = upload(comp_xblk.data(), comp_xblk.size());
-                P->n_comp = (int64_t)comps.size();
+                P->n_comp = (int64_t)cd.size();
                 P->comp_elems = comp_elems;
-                for (const CompDesc &cd : comps)   // 32 x 32 tiles of the largest composite
-                    P->comp_tiles = std::max<int64_t>(P->comp_tiles, (int64_t)((cd.c.m + 31) / 32) * ((cd.c.n + 31) / 32));
+                for (const CompDesc &d : cd)   // 32 x 32 tiles of the largest composite
+                    P->comp_tiles = std::max<int64_t>(P->comp_tiles, (int64_t)((d.c.m + 31) / 32) * ((d.c.n + 31) / 32));
                 HIPCHK(hipDeviceSynchronize());
                 form_composites(P.get(), P->B64, P->VT64, nullptr);
             }
@@ -1290,6 +1332,19 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
         const int ndir = P->adjoint ? 2 : 1;
         const int nvar = prec == MX_BF16X3 ? X3_NVARIANT : 1;
         if (prec == MX_BF16X3) P->sweep_on[0] = P->sweep_on[1] = 0;   // level launches first
+        {
+            // short evaluations are timed over more repetitions (about 2 ms
+            // per measurement, at most 20): their launch-to-launch jitter is
+            // a larger share of their time
+            eval(P, DIR_FWD, prec, W, Y, nrhs, io, st);   // warm-up
+            HIPCHK(hipEventRecord(e0, st));
+            eval(P, DIR_FWD, prec, W, Y, nrhs, io, st);
+            HIPCHK(hipEventRecord(e1, st));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms > 0.f) reps = std::max(reps, std::min(20, (int)std::ceil(2.0f / ms)));
+        }
         const int cur_variant = P->x3_variant;
         const int cur_sched[2] = {P->sched[0][kd], P->sched[1][kd]};
         int best_variant = cur_variant;
