@@ -108,5 +108,7 @@ def test_coarse_lists_keep_merges_cheap():
     Lc = hgemm.lists_describe(hm, "forward", "coarse")
     merged = len(Lf["launches"]) - len(Lc["launches"])
     assert merged >= 1
-    assert work(Lf) <= work(Lc) <= work(Lf) + 4.5e6 * merged
+    # (it can also be less: a composite is narrower than the block it
+    # replaces when the producer's segments are shorter than its rows)
+    assert work(Lc) <= work(Lf) + 4.5e6 * merged
     assert any(int(c[5]) == 3 for c in Lc["comps"])
