@@ -1,6 +1,6 @@
 """The coupling/downward schedules (fused per level, split: all couplings
 in one launch then per-level transfers accumulated with beta = 1, their
-balanced forms, and the coarse lists: BF16x3 launches merged in pairs over
+balanced forms, and the coarse lists: runs of BF16x3 launches merged over
 composite blocks), both directions and every precision, against the
 oracle; the FP64/FP32 kernel variants; and the autotuner."""
 import sys
