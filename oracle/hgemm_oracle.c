@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Only the tests, the smoke check and the CPU
  * baseline leg of the benchmark may load this library, and only as the
- * checker (or the timed CPU baseline); the product path never calls it.
+ * reference results are compared with (or the timed CPU baseline); the
+ * product path never calls it.
  *
  * Contents (each function cites the reference code it restates):
  *   orc_hgemm_h2      symGen/HGEMM_gen.h:117-238      H2-b evaluation (blocked near/far,

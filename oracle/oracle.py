@@ -1,8 +1,8 @@
 """ctypes view of the CPU oracle (oracle/hgemm_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker or the timed CPU
-baseline, never as part of the product path.
+CPU-baseline leg of bench.py, always as the reference results are
+compared with or the timed CPU baseline, never as part of the product path.
 
 Functions take an ``HMatrix`` (matrox_ru_amd.host) and numpy arrays and call
 the C restatements:
