@@ -90,7 +90,8 @@ def parse():
     ap.add_argument("--fp64-seg-variants", default="",
                     help="SCHEDULE:v1,v2,... FP64 forward schedule and kernel variant per launch, replacing the "
                          "autotuner's choice (to profile a fixed configuration)")
-    ap.add_argument("--schedule", default="fused", choices=["fused", "split", "fused_balanced", "split_balanced", "coarse"],
+    ap.add_argument("--schedule", default="fused", choices=["fused", "split", "fused_balanced", "split_balanced", "coarse",
+                                                            "coarse_split"],
                     help="forward schedule used with --launch-variants")
     ap.add_argument("--no-dropin", action="store_true",
                     help="skip timing the drop-in HGEMM entry point on host arrays (PCIe-inclusive)")

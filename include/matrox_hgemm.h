@@ -282,8 +282,10 @@ enum { MX_SCHED_SWEEP = 4 };
  * every skeleton block still written.  The plan keeps FP64 copies of B and
  * VT for it.  Like the reference's coarsened level groups
  * (symGen/HGEMM_gen.h:142-178, 196-236), with the group boundaries removed
- * by re-association instead of barriers. */
-enum { MX_SCHED_COARSE = 5 };
+ * by re-association instead of barriers.  MX_SCHED_COARSE_SPLIT: the same
+ * over the MX_SCHED_SPLIT lists (whose per-level transfers accumulate, so
+ * only the UP launches and the couplings merge). */
+enum { MX_SCHED_COARSE = 5, MX_SCHED_COARSE_SPLIT = 6 };
 
 /* Error word of the last sweep this plan enqueued (after synchronising
  * `stream`): 0 = none or clean; 1 = a wait for a producer, 2 = claiming
@@ -293,7 +295,8 @@ enum { MX_SCHED_COARSE = 5 };
 mx_status mx_plan_sweep_status(const mx_plan *plan, void *stream, int32_t *error);
 
 /* Host-only: the BF16x3 evaluation's task lists of one direction and
- * schedule (MX_SCHED_FUSED ... MX_SCHED_SPLIT_BALANCED or MX_SCHED_COARSE)
+ * schedule (MX_SCHED_FUSED ... MX_SCHED_SPLIT_BALANCED, MX_SCHED_COARSE or
+ * MX_SCHED_COARSE_SPLIT)
  * of a structure, in terms of the stored blocks and the reference layouts
  * (no device work, for checks of the schedules).  With launches == NULL it
  * returns the counts.  launches: nlaunch + 1 offsets into ltasks, the tasks
@@ -369,7 +372,7 @@ mx_status mx_plan_set_seg_variants(mx_plan *plan, int32_t direction, mx_precisio
 
 /* Autotuner: times every schedule (and, for MX_BF16X3, every kernel
  * variant, first for the whole evaluation, then launch by launch, for the
- * schedule picked and for MX_SCHED_COARSE) of each direction built into the
+ * schedule picked and for the coarse schedules) of each direction built into the
  * plan with at least `reps` evaluations of nrhs columns (more for short
  * evaluations: about 2 ms per measurement, at most 20) on scratch buffers
  * it allocates and frees, and keeps the fastest.  best_ms (optional, 2

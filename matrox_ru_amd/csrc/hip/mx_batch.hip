@@ -998,7 +998,7 @@ mx_status mx_batch_flops(const mx_batch *batch, mx_precision prec, double *total
             const mx_plan *P = batch->plans[g];
             const int kd = group_prec(batch, (int)g, prec) == MX_BF16X3 ? 1 : 0;
             // algorithmic: the coarse schedule's composite work is not counted
-            const int sc = P->sched[DIR_FWD][kd] == SCHED_COARSE ? SCHED_FUSED : P->sched[DIR_FWD][kd];
+            const int sc = sched_coarse(P->sched[DIR_FWD][kd]) ? SCHED_FUSED : P->sched[DIR_FWD][kd];
             for (double f : P->prog[DIR_FWD][kd][sc].flops) *total += f * batch->group_nrhs[g];
         }
         return 0;

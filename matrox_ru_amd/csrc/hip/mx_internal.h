@@ -105,10 +105,20 @@ enum { DIR_FWD = 0, DIR_ADJ = 1 };
 //      tree (the adjoint's column-grouped lists) then run beside everything
 //      else instead of in a nearly empty launch.
 // coupling/downward schedule (bit 0: split) and NEAR+LEAF balancing (bit 1);
-//  SCHED_COARSE (BF16x3 only): the fused lists with pairs of consecutive
-//      launches merged where the second reads the first's skeletons through
-//      few, small products (coarsen in mx_lists.hip).
-enum { SCHED_FUSED = 0, SCHED_SPLIT = 1, SCHED_FUSED_BAL = 2, SCHED_SPLIT_BAL = 3, SCHED_COARSE = 4, NSCHED = 5 };
+//  SCHED_COARSE, SCHED_COARSE_SPLIT (BF16x3 only): the fused or the split
+//      lists with runs of consecutive launches merged where the later ones
+//      read the earlier ones' skeletons through few, small products
+//      (coarsen in mx_lists.hip).
+enum {
+    SCHED_FUSED = 0,
+    SCHED_SPLIT = 1,
+    SCHED_FUSED_BAL = 2,
+    SCHED_SPLIT_BAL = 3,
+    SCHED_COARSE = 4,
+    SCHED_COARSE_SPLIT = 5,
+    NSCHED = 6
+};
+inline bool sched_coarse(int sc) { return sc == SCHED_COARSE || sc == SCHED_COARSE_SPLIT; }
 
 }  // namespace mx
 

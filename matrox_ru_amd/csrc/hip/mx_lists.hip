@@ -378,7 +378,9 @@ static void order_tiles(TaskLists &L, int bk, bool group_by_operand) {
 // within maxseg segments, and does not lengthen the run's longest task by
 // more than launch j+1's own longest task plus launch_chunks (the top of the
 // tree is latency-bound: a launch lasts about as long as its longest task
-// plus a fixed cost); otherwise it starts a run of its own.  The UP launches
+// plus a fixed cost), and none of its tasks writes a block the run writes
+// or reads (the split lists' per-level transfers accumulate onto the
+// couplings' blocks); otherwise it starts a run of its own.  The UP launches
 // of the lowest two levels merge for nothing when the leaf blocks are as
 // tall as the ranks, the top-of-tree launches for little.
 static void coarsen(TaskLists &L, double max_extra, int maxseg, int bk, int launch_chunks) {
@@ -438,13 +440,26 @@ static void coarsen(TaskLists &L, double max_extra, int maxseg, int bk, int laun
     std::vector<char> joins(n, 0);   // launch joins the run of the launch before it
     int run0 = 0;                    // first launch of the current run
     int run_len = 0;                 // longest task of the current run, in chunks
+    // blocks the run's tasks write and read: a launch whose tasks write one
+    // of them cannot join (its tasks would run beside the run's, in any order)
+    std::set<std::pair<int, int64_t>> run_io;
+    auto note_run = [&](int li) {
+        for (int t : tasks_of[li]) {
+            run_io.insert(std::make_pair((int)L.tasks[t].c_buf, L.tasks[t].c_row));
+            for (const Seg &sg : segs_of[t]) run_io.insert(std::make_pair((int)sg.b_buf, sg.b_row));
+        }
+    };
     for (int t : tasks_of[0]) run_len = std::max(run_len, chunks(segs_of[t]));
+    note_run(0);
     for (int j = 1; j < n; ++j) {
         double extra = 0.0;
         bool ok = true;
         int own_len = 0, new_len = 0;   // launch j's longest task before and after rewriting
         std::vector<std::pair<int, std::vector<Piece>>> rewritten;
+        for (int t : tasks_of[j])
+            if (run_io.count(std::make_pair((int)L.tasks[t].c_buf, L.tasks[t].c_row))) ok = false;
         for (int t : tasks_of[j]) {
+            if (!ok) break;
             const Task &tk = L.tasks[t];
             std::vector<Piece> out;
             for (const Seg &sg : segs_of[t]) {
@@ -487,7 +502,10 @@ static void coarsen(TaskLists &L, double max_extra, int maxseg, int bk, int laun
         // fixed cost taken as launch_chunks chunks
         if (!ok || extra > max_extra || std::max(run_len, new_len) > run_len + own_len + launch_chunks) {
             run0 = j;
-            run_len = own_len;
+            run_len = 0;
+            for (int t : tasks_of[j]) run_len = std::max(run_len, chunks(segs_of[t]));
+            run_io.clear();
+            note_run(j);
             continue;
         }
         joins[j] = 1;
@@ -511,6 +529,7 @@ static void coarsen(TaskLists &L, double max_extra, int maxseg, int bk, int laun
             }
             segs_of[t].swap(ns);
         }
+        note_run(j);
     }
     // segments in task order
     std::vector<Seg> segs;
@@ -837,7 +856,7 @@ TaskLists build_lists(const mx_structure &s, const Topology &t, int beta_u, int 
     }
     end_launch();
     // at most 4.5 MFLOP per right-hand side column for each merge
-    if (sched == SCHED_COARSE) coarsen(L, 4.5e6, X3_MAXSEG, X3_BK, 4);
+    if (sched_coarse(sched)) coarsen(L, 4.5e6, X3_MAXSEG, X3_BK, 4);
     order_tiles(L, kind == 1 ? X3_BK : SG_BK, kind == 1 || group);
     return L;
 }

@@ -226,7 +226,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
     int64_t fnext[3] = {0, 0, 0};
     for (int dir = 0; dir < ndir; ++dir)
         for (int sc = 0; sc < NSCHED; ++sc) {
-            if (sc == SCHED_COARSE) continue;   // BF16x3 only
+            if (sched_coarse(sc)) continue;   // BF16x3 only
             TaskLists L = build_lists(s, t, beta_u, beta_y, 0, dir, sc, wrow64.data(), skel64.data());
             compute_launch_flops(L);
             // The 128-row tiles (packed-operand kernels, FP32 128 x 128
@@ -311,7 +311,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             const int dir = dsc / NSCHED, sc = dsc % NSCHED;
             // composites need FP64 B and VT, which a plan of generated blocks
             // may never hold
-            if (sc == SCHED_COARSE && gen) continue;
+            if (sched_coarse(sc) && gen) continue;
             TaskLists X = build_lists(s, t, beta_u, beta_y, 1, dir, sc, P->wprow.data(), P->sprow.data());
             split_long_tasks(X, X3_MAXSEG);
             std::vector<int> local(X.comps.size(), -1);   // this program's composites -> global ids
@@ -1264,7 +1264,8 @@ static void drop_graphs(mx_plan *P) {
 
 static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int variant) {
     arg_check(dir == DIR_FWD || dir == DIR_ADJ, "bad direction");
-    arg_check((sched >= 0 && sched < SCHED_COARSE) || sched == MX_SCHED_SWEEP || sched == MX_SCHED_COARSE,
+    arg_check((sched >= 0 && sched < SCHED_COARSE) || sched == MX_SCHED_SWEEP || sched == MX_SCHED_COARSE ||
+                  sched == MX_SCHED_COARSE_SPLIT,
               "bad schedule");
     arg_check(variant >= 0 && variant <= X3_NVARIANT, "bad BF16x3 kernel variant (0 keeps)");
     if (dir == DIR_ADJ && !P->adjoint) throw MxError(MX_ERR_UNSUPPORTED, "plan was created without MX_PLAN_ADJOINT");
@@ -1281,11 +1282,11 @@ static void set_schedule(mx_plan *P, int dir, mx_precision prec, int sched, int 
         drop_graphs(P);
         return;
     }
-    if (sched == MX_SCHED_COARSE) {
-        if (prec != MX_BF16X3) throw MxError(MX_ERR_UNSUPPORTED, "the coarse schedule is for MX_BF16X3");
-        if (!P->prog[dir][1][SCHED_COARSE].built)
+    if (sched == MX_SCHED_COARSE || sched == MX_SCHED_COARSE_SPLIT) {
+        sched = sched == MX_SCHED_COARSE ? SCHED_COARSE : SCHED_COARSE_SPLIT;
+        if (prec != MX_BF16X3) throw MxError(MX_ERR_UNSUPPORTED, "the coarse schedules are for MX_BF16X3");
+        if (!P->prog[dir][1][sched].built)
             throw MxError(MX_ERR_UNSUPPORTED, "no coarse schedule in this plan (BF16x3 lists of stored blocks)");
-        sched = SCHED_COARSE;
     }
     if (kd == 1) P->sweep_on[dir] = 0;
     P->sched[dir][kd] = sched;
@@ -1485,13 +1486,19 @@ void autotune(mx_plan *P, mx_precision prec, int nrhs, mx_io_type io, int reps, 
             for (int dir = 0; dir < ndir; ++dir) {
                 const int sc = P->sched[dir][1];
                 tune_launches(dir, sc);
-                if (sc != SCHED_COARSE && P->prog[dir][1][SCHED_COARSE].built) {
-                    const float t_sc = time_eval(dir);
-                    tune_launches(dir, SCHED_COARSE);
-                    const float t_co = time_eval(dir);
-                    P->sched[dir][1] = t_co < 0.98f * t_sc ? SCHED_COARSE : sc;
-                    best_dir[dir] = std::min(t_co, t_sc);
+                float best = time_eval(dir);
+                int pick = sc;
+                for (int co : {SCHED_COARSE, SCHED_COARSE_SPLIT}) {
+                    if (co == sc || !P->prog[dir][1][co].built) continue;
+                    tune_launches(dir, co);
+                    const float t = time_eval(dir);
+                    if (t < 0.98f * best) {
+                        best = t;
+                        pick = co;
+                    }
                 }
+                P->sched[dir][1] = pick;
+                best_dir[dir] = best;
             }
             // the dependency-driven sweep (three tile widths) against the
             // tuned level launches
@@ -1964,8 +1971,11 @@ mx_status mx_lists_describe(const mx_structure *s, int32_t direction, int32_t sc
     return (mx_status)guard([&]() {
         arg_check(s && nlaunch && nltask && ntask && nseg && ncomp && (direction == 0 || direction == 1),
                   "bad argument");
-        arg_check((schedule >= 0 && schedule < SCHED_COARSE) || schedule == MX_SCHED_COARSE, "bad schedule");
-        const int sc = schedule == MX_SCHED_COARSE ? SCHED_COARSE : schedule;
+        arg_check((schedule >= 0 && schedule < SCHED_COARSE) || schedule == MX_SCHED_COARSE ||
+                      schedule == MX_SCHED_COARSE_SPLIT,
+                  "bad schedule");
+        const int sc = schedule == MX_SCHED_COARSE ? SCHED_COARSE
+                       : schedule == MX_SCHED_COARSE_SPLIT ? SCHED_COARSE_SPLIT : schedule;
         Topology t = topology(*s);
         validate(*s, t);
         std::vector<int64_t> wrow(s->wrow, s->wrow + s->nleaf), skel(s->skelrow, s->skelrow + s->numnodes);
@@ -2069,7 +2079,9 @@ mx_status mx_plan_get_schedule(const mx_plan *plan, int32_t direction, mx_precis
         arg_check(plan && schedule && x3_variant && (direction == 0 || direction == 1), "bad argument");
         const bool sw = prec == MX_BF16X3 && plan->sweep_on[direction];
         const int sc = plan->sched[direction][prec == MX_BF16X3 ? 1 : 0];
-        *schedule = sw ? MX_SCHED_SWEEP : sc == SCHED_COARSE ? MX_SCHED_COARSE : sc;
+        *schedule = sw ? MX_SCHED_SWEEP
+                    : sc == SCHED_COARSE ? MX_SCHED_COARSE
+                    : sc == SCHED_COARSE_SPLIT ? MX_SCHED_COARSE_SPLIT : sc;
         *x3_variant = sw ? plan->sweep_var[direction] : plan->x3_variant;
         return 0;
     });
@@ -2194,7 +2206,7 @@ mx_status mx_plan_flops(const mx_plan *plan, mx_precision prec, int32_t nrhs, do
             // algorithmic flops: the coarse schedule's composites add work the
             // reference's evaluation does not have, so its total is the fused
             // lists' (the same for every other schedule)
-            const int sc = plan->sched[DIR_FWD][1] == SCHED_COARSE ? SCHED_FUSED : plan->sched[DIR_FWD][1];
+            const int sc = sched_coarse(plan->sched[DIR_FWD][1]) ? SCHED_FUSED : plan->sched[DIR_FWD][1];
             const Program &pg = plan->prog[DIR_FWD][1][sc];
             if (!pg.built) throw MxError(MX_ERR_UNSUPPORTED, "no BF16x3 lists in the plan");
             for (double f : pg.flops) flops[MX_KIND_BF16X3] += f * nrhs;

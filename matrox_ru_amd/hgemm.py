@@ -20,7 +20,8 @@ FP64, FP32, BF16X3 = 0, 1, 2
 KIND_NAMES = {0: "gemm_f64", 1: "gemm_f32", 2: "bf16x3", 3: "convert", 4: "grad_f64", 5: "grad_f32",
               6: "grad_bf16x3"}
 PLAN_ADJOINT = 1 << 8
-SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4, "coarse": 5}
+SCHEDULES = {"fused": 0, "split": 1, "fused_balanced": 2, "split_balanced": 3, "sweep": 4, "coarse": 5,
+             "coarse_split": 6}
 DIRECTIONS = {"forward": 0, "adjoint": 1}
 X3_VARIANTS = {1: "128x128_s2", 2: "128x128_s4", 3: "128x256_s3", 4: "128x128_reg", 5: "128x64_s2", 6: "128x32_s2",
                7: "128x128_w8s3", 8: "128x128_w8s4", 9: "256x128_w8s3", 10: "128x32_s6", 11: "128x64_s5",

@@ -72,11 +72,12 @@ def test_coarse_lists_match_oracle(case, direction):
     TW = host.normal((hm.N * nrhs,), seed=41)
     ref = oracle.hgemm(hm if direction == "forward" else hm.transposed(), TW, nrhs)[0]
     out = {}
-    for sc in ("fused", "coarse"):
+    for sc in ("fused", "split", "coarse", "coarse_split"):
         L = hgemm.lists_describe(hm, direction, sc)
         vec = run_lists(hm, L, TW, nrhs)
         assert rel(vec[3], ref) <= 1e-12, sc
         out[sc] = (L, vec)
+    assert len(out["coarse_split"][0]["launches"]) < len(out["split"][0]["launches"])
     Lf, vf = out["fused"]
     Lc, vc = out["coarse"]
     assert len(Lc["launches"]) < len(Lf["launches"]) and len(Lc["comps"]) > 0

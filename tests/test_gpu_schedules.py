@@ -41,8 +41,9 @@ def test_schedules_match_oracle(gpu, case, schedule):
     Go, _, _ = oracle.hgemm(hm.transposed(), TW, nrhs)
     plan = hgemm.Plan(hm, precisions=("fp64", "fp32", "bf16x3"), adjoint=True)
     W = torch.from_numpy(TW).cuda()
-    precs = [("fp64", 0), ("fp32", 0)] if schedule != "coarse" else []   # coarse: BF16x3 only
-    if schedule == "coarse":
+    coarse = schedule.startswith("coarse")
+    precs = [("fp64", 0), ("fp32", 0)] if not coarse else []   # coarse lists: BF16x3 only
+    if coarse:
         for prec in ("fp64", "fp32"):
             with pytest.raises(hgemm.MxError):
                 plan.set_schedule("forward", prec, schedule)
