@@ -232,7 +232,7 @@ mx_plan *create_plan(const mx_structure &s, uint32_t precisions, int beta_u, int
             // The 128-row tiles (packed-operand kernels, FP32 128 x 128
             // kernel) of the big launches go in groups of tasks that share B
             // operands, with each group's segments aligned (as the BF16x3
-            // lists): at the headline the FP64 evaluation fetches 15.7
+            // lists): at the headline the FP64 evaluation fetches 15.6
             // instead of 19.0 GB and runs 1.5% faster.  The 64-row tiles
             // keep the tree order and the segments their natural order: the
             // 64 x 64 kernel ran 2.8% slower on grouped lists (N = 524,288).
