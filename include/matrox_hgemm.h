@@ -205,8 +205,9 @@ mx_status mx_eval_adjoint(mx_plan *plan, mx_precision prec, const void *dY, void
  * plan's own, valid when the plan's last forward evaluation was of this W in
  * this precision.  prec is MX_FP64 (FP64 I/O), MX_FP32 or MX_BF16X3 (either
  * I/O type; with MX_BF16X3 the evaluations and the block products are
- * split-BF16x3, each operand split once into a plan work buffer).  Needs
- * MX_PLAN_ADJOINT. */
+ * split-BF16x3, each operand split once into a plan work buffer).  With
+ * block gradients requested, dW must not alias W or dY (MX_ERR_ARG): the
+ * products read them after dW is written.  Needs MX_PLAN_ADJOINT. */
 mx_status mx_backward(mx_plan *plan, mx_precision prec, const void *W, const void *dY, void *dW, int32_t nrhs,
                       mx_io_type io, const void *wskel, const void *uskel, void *dD, void *dB, void *dVT,
                       void *stream);

@@ -1066,6 +1066,8 @@ void backward(mx_plan *P, mx_precision prec, const void *W, const void *dY, void
     // operands of the requested products: dD = dY W^T, dB = y^ w^^T,
     // dVT = g^ [W | w^]^T + u^ [dY | y^]^T
     const bool need_w_dy = dD || dVT, need_skel_w = dB || dVT, need_skel_u = dVT != nullptr;
+    // the products read W and dY after the adjoint has written dW
+    arg_check(!grads || (dW != W && dW != dY), "dW must not alias W or dY when block gradients are requested");
     if (grads && x3) {
         ensure_kfrag(P, nrhs);
         if (wskel && uskel) {   // caller FP32 skeletons

@@ -292,6 +292,14 @@ def test_backward_errors(gpu):
     with pytest.raises(hgemm.MxError) as e:   # not built into the plan
         plan.backward(W, W, nrhs, "fp32")
     assert e.value.code == -4
+    # dW aliasing W or dY with block gradients requested: the products would
+    # read the overwritten operand
+    dD = torch.empty(plan.block_lens[0], dtype=W.dtype, device=W.device)
+    dY = W.clone()
+    for dw in (W, dY):
+        rc = hgemm.lib().mx_backward(plan._h, hgemm.PRECISIONS["fp64"], W.data_ptr(), dY.data_ptr(),
+                                     dw.data_ptr(), nrhs, hgemm.IO_F64, None, None, dD.data_ptr(), None, None, None)
+        assert rc == -1
     D_host = np.ascontiguousarray(hm.D)
     import ctypes
     rc = hgemm.lib().mx_plan_set_blocks(plan._h, D_host.ctypes.data, None, None, None)
